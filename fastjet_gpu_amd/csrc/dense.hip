@@ -1,0 +1,595 @@
+/*
+ * dense.hip — the dense nearest-neighbour clustering engine for gfx950.
+ *
+ * One workgroup clusters one event at a time; persistent workgroups pull event
+ * indices from a queue (LaunchArgs::queue/head), so the same kernel serves the
+ * one-event-per-launch path (grid of 1, queue of 1) and the batched path (a
+ * persistent grid over a multiplicity bucket).
+ *
+ * Semantics: FastJet's N2Plain (the method the reference approximates with its
+ * all-pairs triangle, tri_matrix.cu:307-357, and its single-block recompute loop,
+ * n_array.cu:129-200), restated so that every decision, including exact ties, is
+ * the same as the CPU oracle's:
+ *   - jets live in slots 0..n-1; each has its nearest neighbour (NN) within R in
+ *     (y, phi) (lowest slot on equal distance), NN distance and
+ *     diJ = NN_dist * min(w_i, w_NN) (R^2 w_i without a NN);
+ *   - each step takes the lowest slot with the smallest diJ (the reference's
+ *     reduction_min/reduction_blocks argmin, tri_matrix.cu:182-257, with an
+ *     index tie-break instead of CUB's tree order);
+ *   - a pair merges (E-scheme, tri_matrix.cu:269-272) into the lower slot and the
+ *     last slot moves into the higher one; a beam step removes the jet and the
+ *     last slot moves into its place (tri_matrix.cu:262-276);
+ *   - after a step only the merged jet's row is evaluated against every jet, and
+ *     jets that had the removed or merged jet as NN rescan (incremental update,
+ *     instead of the reference's full-triangle rescan, tri_matrix.cu:337-349).
+ *
+ * Layout (MI355X-first):
+ *   - slot s is owned by thread s % T and lives in that thread's registers at
+ *     index s / T: y, phi, w, NN distance, diJ, NN slot (11 VGPRs per slot);
+ *   - LDS holds only what other threads need: the four-momenta and history
+ *     indices of the slots (or, for the largest events, the four-momenta live in
+ *     global memory and LDS only stages coordinates for the initial NN pass);
+ *   - every argmin is a 64-lane DPP reduction (value, then lowest slot among the
+ *     minima), combined across waves through LDS;
+ *   - the merge (including atan2/log of the new jet) is computed redundantly by
+ *     every lane from broadcast LDS reads, so no lane waits for a single writer.
+ * Per step: one barrier after the merge reads, one after the update pass, and
+ * one per round of up to MO orphan rescans (only when jets lost their NN).
+ */
+#include <hip/hip_runtime.h>
+
+#include "fjmi_internal.h"
+#include "fjmi_math.h"
+
+namespace fjmi {
+
+constexpr int BIG = 0x7fffffff;
+constexpr int OCH = 64; /* orphan list entries per chunk */
+constexpr int MO = 4;   /* orphans rescanned per reduction round */
+
+struct Red {
+  double d;
+  int s;
+  int pi;
+  double pd;
+};
+
+struct Orph {
+  double rap, phi, w;
+  int label;
+  int pad;
+};
+
+struct Mail {
+  double rap, phi, w, nnd, dij;
+  int nn;
+  int pend;
+};
+
+template <int W>
+struct Shared {
+  Red ru[W];
+  Red rb[W];
+  Red ro[2][MO][W];
+  Orph orph[2][OCH];
+  int norph[2];
+  Mail mail;
+  int ev;
+};
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_d(double v) {
+  int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, RM, 0xF, false);
+  int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, RM, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, RM, 0xF, false);
+}
+
+/* Minimum over the 64 lanes, returned to every lane.  quad_perm xor1, xor2,
+ * row_half_mirror, row_mirror give the row-of-16 minimum in every lane;
+ * row_bcast15 (rows 1,3) and row_bcast31 (rows 2,3) fold the rows into lane 63. */
+__device__ __forceinline__ double wave_min_d(double v) {
+  v = __builtin_fmin(v, dpp_d<0xB1, 0xF>(v));
+  v = __builtin_fmin(v, dpp_d<0x4E, 0xF>(v));
+  v = __builtin_fmin(v, dpp_d<0x141, 0xF>(v));
+  v = __builtin_fmin(v, dpp_d<0x140, 0xF>(v));
+  v = __builtin_fmin(v, dpp_d<0x142, 0xA>(v));
+  v = __builtin_fmin(v, dpp_d<0x143, 0xC>(v));
+  return readlane_d(v, 63);
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+  v = min(v, dpp_i<0xB1, 0xF>(v));
+  v = min(v, dpp_i<0x4E, 0xF>(v));
+  v = min(v, dpp_i<0x141, 0xF>(v));
+  v = min(v, dpp_i<0x140, 0xF>(v));
+  v = min(v, dpp_i<0x142, 0xA>(v));
+  v = min(v, dpp_i<0x143, 0xC>(v));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+/* Lexicographic (d, s) minimum over the wave with the winner's payload. */
+__device__ __forceinline__ Red wave_argmin(double d, int s, int pi, double pd) {
+  Red r;
+  r.d = wave_min_d(d);
+  const int key = (d == r.d) ? s : BIG;
+  r.s = wave_min_i(key);
+  const unsigned long long bal = __ballot(key == r.s && r.s != BIG);
+  if (bal) {
+    const int ln = __ffsll(bal) - 1;
+    r.pi = __builtin_amdgcn_readlane(pi, ln);
+    r.pd = readlane_d(pd, ln);
+  } else {
+    r.pi = -1;
+    r.pd = 0.0;
+  }
+  return r;
+}
+
+__device__ __forceinline__ bool lex_less(double d1, int s1, double d2, int s2) {
+  return d1 < d2 || (d1 == d2 && s1 < s2);
+}
+
+template <int W>
+__device__ __forceinline__ Red combine(const Red *r) {
+  Red b = r[0];
+#pragma unroll
+  for (int i = 1; i < W; ++i) {
+    Red x = r[i];
+    if (lex_less(x.d, x.s, b.d, b.s)) b = x;
+  }
+  return b;
+}
+
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+/* Writes one PseudoJet record (int index; bool isJet; double px,py,pz,E). */
+__device__ __forceinline__ void write_pseudojet(unsigned char *rec, int index, bool is_jet, double4 q) {
+  int *ip = reinterpret_cast<int *>(rec);
+  ip[0] = index;
+  ip[1] = is_jet ? 1 : 0;
+  double *dp = reinterpret_cast<double *>(rec + 8);
+  dp[0] = q.x;
+  dp[1] = q.y;
+  dp[2] = q.z;
+  dp[3] = q.w;
+}
+
+template <int T, int K, bool P4G>
+__global__ __launch_bounds__(T) void dense_kernel(LaunchArgs a) {
+  constexpr int W = T / 64;
+  constexpr int CAP = T * K;
+  static_assert(T % 64 == 0 && T <= 1024, "workgroup must be whole waves");
+  static_assert(K >= 1 && K <= 32, "slot mask is 32 bits");
+  __shared__ Shared<W> sh;
+  extern __shared__ __align__(16) unsigned char smem[];
+
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wv = t >> 6;
+  const int CH = P4G ? a.stage_chunk : CAP; /* staging entries */
+  double4 *stage = reinterpret_cast<double4 *>(smem);
+  double4 *lp4 = reinterpret_cast<double4 *>(smem); /* aliases the stage when !P4G */
+  int *lhist = reinterpret_cast<int *>(smem + sizeof(double4) * (size_t)CH);
+
+  double rap[K], phi[K], w[K], nnd[K], dij[K];
+  int nn[K];
+
+  for (;;) {
+    __syncthreads(); /* previous event (or queue fetch) fully consumed */
+    if (t == 0) sh.ev = (int)atomicAdd(a.head, 1u);
+    __syncthreads();
+    const int qi = uni(sh.ev);
+    if (qi >= a.n_queue) break;
+    const int e = uni(a.queue[qi]);
+    const int64_t base = a.offsets[e];
+    const int N = uni((int)(a.offsets[e + 1] - base));
+    if (N > CAP) {
+      if (t == 0) atomicOr(a.status, (unsigned)STATUS_CAPACITY);
+      continue;
+    }
+    if (N <= 0) {
+      if (t == 0) a.njets[e] = 0;
+      continue;
+    }
+    const int p = a.p_arr ? a.p_arr[e] : a.p_uni;
+    const double R = a.R_arr ? a.R_arr[e] : a.R_uni;
+    const double R2 = R * R;
+    const double invR2 = 1.0 / R2;
+    double4 *gp4 = P4G ? reinterpret_cast<double4 *>(a.p4_scratch) + base : nullptr;
+
+    /* ---- prologue: coordinates of the own slots ---- */
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int s = k * T + t;
+      rap[k] = 0.0;
+      phi[k] = 0.0;
+      w[k] = 0.0;
+      nnd[k] = R2;
+      dij[k] = 0.0;
+      nn[k] = -1;
+      if (s < N) {
+        const double *q = a.in + (size_t)(base + s) * a.in_stride + a.in_off;
+        const double px = q[0], py = q[1], pz = q[2], E = q[3];
+        const double kt2 = px * px + py * py;
+        phi[k] = fjmi_phi(px, py, kt2);
+        rap[k] = fjmi_rap(pz, E, kt2);
+        w[k] = fjmi_weight(kt2, p);
+        if (P4G) gp4[s] = make_double4(px, py, pz, E);
+      }
+    }
+
+    /* ---- initial nearest neighbours: all pairs, staged through LDS in chunks ---- */
+    double wnn[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) wnn[k] = 0.0;
+    for (int c0 = 0; c0 < N; c0 += CH) {
+      const int cn = min(CH, N - c0);
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int s = k * T + t;
+        if (s >= c0 && s < c0 + cn) stage[s - c0] = make_double4(rap[k], phi[k], w[k], 0.0);
+      }
+      __syncthreads();
+      for (int j = 0; j < cn; ++j) {
+        const double4 q = stage[j];
+        const int sj = c0 + j;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (k * T >= N) break;
+          const int s = k * T + t;
+          const double d = fjmi_dr2(rap[k], phi[k], q.x, q.y);
+          if (s < N && s != sj && d < nnd[k]) {
+            nnd[k] = d;
+            nn[k] = sj;
+            wnn[k] = q.z;
+          }
+        }
+      }
+    }
+    double uD = __builtin_inf();
+    int uS = BIG, uNN = -1;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int s = k * T + t;
+      if (s < N) {
+        const double kt2 = (nn[k] >= 0 && wnn[k] < w[k]) ? wnn[k] : w[k];
+        dij[k] = nnd[k] * kt2;
+        if (lex_less(dij[k], s, uD, uS)) {
+          uD = dij[k];
+          uS = s;
+          uNN = nn[k];
+        }
+      }
+    }
+    __syncthreads(); /* staging consumed */
+    for (int s = t; s < N; s += T) {
+      if (!P4G) {
+        const double *q = a.in + (size_t)(base + s) * a.in_stride + a.in_off;
+        lp4[s] = make_double4(q[0], q[1], q[2], q[3]);
+      }
+      lhist[s] = s;
+    }
+    if (t == 0) {
+      sh.norph[0] = 0;
+      sh.norph[1] = 0;
+    }
+    Red r0 = wave_argmin(uD, uS, uNN, 0.0);
+    if (W > 1 && lane == 0) sh.ru[wv] = r0;
+    __syncthreads();
+    if (W > 1) r0 = combine<W>(sh.ru);
+
+    double dmin = r0.d;
+    int A = uni(r0.s), nnA = uni(r0.pi);
+    int nj = 0;
+
+    /* ---- the N recombination steps ---- */
+    for (int step = 0; step < N; ++step) {
+      const int n_old = N - step;
+      const int hi = nnA >= 0 ? max(A, nnA) : A;
+      const int lo = nnA >= 0 ? min(A, nnA) : -1;
+      const int tail = n_old - 1;
+      const int par = step & 1;
+
+      const double4 qhi = P4G ? gp4[hi] : lp4[hi];
+      const double4 qtail = P4G ? gp4[tail] : lp4[tail];
+      const int hhi = lhist[hi];
+      const int htail = lhist[tail];
+      double4 qn = make_double4(0.0, 0.0, 0.0, 0.0);
+      double nrap = 0.0, nphi = 0.0, nw = 0.0;
+      if (lo >= 0) {
+        const double4 qlo = P4G ? gp4[lo] : lp4[lo];
+        const int hlo = lhist[lo];
+        qn = make_double4(qhi.x + qlo.x, qhi.y + qlo.y, qhi.z + qlo.z, qhi.w + qlo.w);
+        const double kt2 = qn.x * qn.x + qn.y * qn.y;
+        nphi = fjmi_phi(qn.x, qn.y, kt2);
+        nrap = fjmi_rap(qn.z, qn.w, kt2);
+        nw = fjmi_weight(kt2, p);
+        if (t == 0) {
+          a.hist_p1[base + step] = min(hhi, hlo);
+          a.hist_p2[base + step] = max(hhi, hlo);
+          a.hist_dij[base + step] = dmin * invR2;
+        }
+      } else {
+        if (t == 0) {
+          a.hist_p1[base + step] = hhi;
+          a.hist_p2[base + step] = -1;
+          a.hist_dij[base + step] = dmin * invR2;
+          double *jo = a.jets + 4 * (base + nj);
+          jo[0] = qhi.x;
+          jo[1] = qhi.y;
+          jo[2] = qhi.z;
+          jo[3] = qhi.w;
+        }
+        ++nj;
+      }
+      if (a.pj_out && t == 0)
+        write_pseudojet(a.pj_out + 40 * (size_t)(base + tail), tail, lo < 0, lo < 0 ? qhi : qtail);
+      __syncthreads(); /* every lane has read the merge inputs */
+      if (t == 0) {
+        if (lo >= 0) {
+          if (P4G)
+            gp4[lo] = qn;
+          else
+            lp4[lo] = qn;
+          lhist[lo] = N + step;
+        }
+        if (hi != tail) {
+          if (P4G)
+            gp4[hi] = qtail;
+          else
+            lp4[hi] = qtail;
+          lhist[hi] = htail;
+        }
+      }
+      const int n = n_old - 1;
+      if (n == 0) break;
+
+      /* ---- update pass over the own slots ---- */
+      double bD = R2, bW = 0.0;
+      int bS = BIG;
+      uD = __builtin_inf();
+      uS = BIG;
+      uNN = -1;
+      unsigned pend = 0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (k * T >= n_old) break;
+        const int s = k * T + t;
+        if (s < n_old && s != hi) {
+          const int label = (s == tail) ? hi : s;
+          if (label == lo) {
+            rap[k] = nrap;
+            phi[k] = nphi;
+            w[k] = nw;
+          } else {
+            int m = nn[k];
+            const bool orph = (m == hi) || (lo >= 0 && m == lo);
+            if (!orph && m == tail) m = hi;
+            if (lo >= 0) {
+              const double d = fjmi_dr2(rap[k], phi[k], nrap, nphi);
+              if (!orph && d < nnd[k]) {
+                nnd[k] = d;
+                m = lo;
+                dij[k] = d * (nw < w[k] ? nw : w[k]);
+              }
+              if (d < R2 && lex_less(d, label, bD, bS)) {
+                bD = d;
+                bS = label;
+                bW = w[k];
+              }
+            }
+            nn[k] = m;
+            bool pending = false;
+            if (orph) {
+              const int idx = atomicAdd(&sh.norph[par], 1);
+              if (idx < OCH) {
+                Orph o;
+                o.rap = rap[k];
+                o.phi = phi[k];
+                o.w = w[k];
+                o.label = label;
+                o.pad = 0;
+                sh.orph[par][idx] = o;
+              } else {
+                pending = true;
+              }
+            } else if (lex_less(dij[k], label, uD, uS)) {
+              uD = dij[k];
+              uS = label;
+              uNN = m;
+            }
+            if (s == tail) { /* hi != tail here: hand the moved jet to slot hi's owner */
+              Mail ml;
+              ml.rap = rap[k];
+              ml.phi = phi[k];
+              ml.w = w[k];
+              ml.nnd = nnd[k];
+              ml.dij = dij[k];
+              ml.nn = m;
+              ml.pend = pending ? 1 : 0;
+              sh.mail = ml;
+            } else if (pending) {
+              pend |= 1u << k;
+            }
+          }
+        }
+      }
+      Red ru = wave_argmin(uD, uS, uNN, 0.0);
+      Red rb = wave_argmin(bD, bS, 0, bW);
+      if (W > 1 && lane == 0) {
+        sh.ru[wv] = ru;
+        sh.rb[wv] = rb;
+      }
+      __syncthreads();
+      if (W > 1) {
+        ru = combine<W>(sh.ru);
+        rb = combine<W>(sh.rb);
+      }
+      if (t == 0) sh.norph[par ^ 1] = 0; /* the other parity's list was last read before this barrier */
+      int norph = uni(sh.norph[par]);
+
+      /* the moved jet changes owner */
+      if (hi != tail && (hi % T) == t) {
+        const int kh = hi / T;
+        const Mail ml = sh.mail;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (k == kh) {
+            rap[k] = ml.rap;
+            phi[k] = ml.phi;
+            w[k] = ml.w;
+            nnd[k] = ml.nnd;
+            dij[k] = ml.dij;
+            nn[k] = ml.nn;
+            if (ml.pend) pend |= 1u << k;
+          }
+      }
+
+      /* candidates of the next argmin: non-orphans (ru), the merged jet, the orphans */
+      double bd = ru.d;
+      int bs = ru.s, bnn = ru.pi;
+      if (lo >= 0) {
+        const int nnB = rb.s == BIG ? -1 : rb.s;
+        const double nndB = rb.d;
+        const double dijB = nndB * ((nnB >= 0 && rb.pd < nw) ? rb.pd : nw);
+        if (lo % T == t) {
+          const int kl = lo / T;
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            if (k == kl) {
+              nn[k] = nnB;
+              nnd[k] = nndB;
+              dij[k] = dijB;
+            }
+        }
+        if (lex_less(dijB, lo, bd, bs)) {
+          bd = dijB;
+          bs = lo;
+          bnn = nnB;
+        }
+      }
+
+      /* ---- orphan rescans ---- */
+      int oround = 0;
+      while (norph > 0) {
+        const int chunk = min(norph, OCH);
+        for (int o0 = 0; o0 < chunk; o0 += MO) {
+          const int mo = min(MO, chunk - o0);
+          double orap[MO], ophi[MO], od[MO], ow[MO];
+          int olab[MO], os[MO];
+#pragma unroll
+          for (int j = 0; j < MO; ++j) {
+            const Orph o = sh.orph[par][o0 + (j < mo ? j : 0)];
+            orap[j] = o.rap;
+            ophi[j] = o.phi;
+            olab[j] = j < mo ? o.label : -1;
+            od[j] = R2;
+            os[j] = BIG;
+            ow[j] = 0.0;
+          }
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            if (k * T >= n) break;
+            const int s = k * T + t;
+            if (s < n) {
+#pragma unroll
+              for (int j = 0; j < MO; ++j) {
+                const double d = fjmi_dr2(rap[k], phi[k], orap[j], ophi[j]);
+                if (s != olab[j] && d < R2 && lex_less(d, s, od[j], os[j])) {
+                  od[j] = d;
+                  os[j] = s;
+                  ow[j] = w[k];
+                }
+              }
+            }
+          }
+          Red ro[MO];
+#pragma unroll
+          for (int j = 0; j < MO; ++j) ro[j] = wave_argmin(od[j], os[j], 0, ow[j]);
+          if (W > 1) {
+            const int rp = oround & 1;
+            if (lane == 0) {
+#pragma unroll
+              for (int j = 0; j < MO; ++j) sh.ro[rp][j][wv] = ro[j];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < MO; ++j) ro[j] = combine<W>(sh.ro[rp][j]);
+          }
+          ++oround;
+#pragma unroll
+          for (int j = 0; j < MO; ++j) {
+            if (j < mo) {
+              const int lab = uni(olab[j]);
+              const int nno = ro[j].s == BIG ? -1 : ro[j].s;
+              const double nndo = ro[j].d;
+              const double wo = sh.orph[par][o0 + j].w;
+              const double dijo = nndo * ((nno >= 0 && ro[j].pd < wo) ? ro[j].pd : wo);
+              if (lab % T == t) {
+                const int kk = lab / T;
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                  if (k == kk) {
+                    nn[k] = nno;
+                    nnd[k] = nndo;
+                    dij[k] = dijo;
+                  }
+              }
+              if (lex_less(dijo, lab, bd, bs)) {
+                bd = dijo;
+                bs = lab;
+                bnn = nno;
+              }
+            }
+          }
+        }
+        norph -= chunk;
+        if (norph <= 0) break;
+        /* more orphans than one list holds: refill the list from the pending ones */
+        __syncthreads();
+        if (t == 0) sh.norph[par] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (pend & (1u << k)) {
+            const int idx = atomicAdd(&sh.norph[par], 1);
+            if (idx < OCH) {
+              Orph o;
+              o.rap = rap[k];
+              o.phi = phi[k];
+              o.w = w[k];
+              o.label = k * T + t;
+              o.pad = 0;
+              sh.orph[par][idx] = o;
+              pend &= ~(1u << k);
+            }
+          }
+        }
+        __syncthreads();
+        norph = uni(sh.norph[par]); /* every pending orphan counted once */
+      }
+      dmin = bd;
+      A = uni(bs);
+      nnA = uni(bnn);
+    }
+    if (t == 0) a.njets[e] = nj;
+  }
+}
+
+/* Per-step launch floor (the analogue of the reference's tri_matrix_empty.cu):
+ * an empty kernel launched as often as a per-step-launch design would. */
+__global__ void empty_kernel(int) {}
+
+}  // namespace fjmi

@@ -1,0 +1,295 @@
+/*
+ * fjmi_math.h — deterministic FP64 kinematics shared by the HIP kernels (device)
+ * and by host code (C or C++), so that both sides produce bit-identical rapidity,
+ * azimuth and clustering weights for the same four-momentum.
+ *
+ * Why a private atan2/log: the clustering decisions are compared bit for bit
+ * between the GPU and the CPU.  ROCm's device libm and glibc are different
+ * implementations and may differ in the last ulp, which could flip a near-tie.
+ * Everything here uses only IEEE-754 basic operations (+ - * /) and explicit
+ * fused multiply-add (fma), which are correctly rounded on x86-64 SSE2 and on
+ * gfx950, so the results are identical wherever this header is compiled —
+ * provided the compiler does not contract a*b+c on its own: every translation
+ * unit that includes this header must be built with -ffp-contract=off.
+ *
+ * Accuracy: log and atan2 are evaluated in double-double arithmetic
+ * (~2^-70 relative error before the final rounding), so the returned double is
+ * the correctly rounded result except in rare hard cases.  That makes them agree
+ * with glibc's log/atan2 (which FastJet uses) on all but a tiny fraction of
+ * inputs; the test-suite measures that fraction against high-precision decimal
+ * values and against glibc.
+ *
+ * The kinematic conventions follow FastJet's PseudoJet (rapidity with the
+ * effective-mass clamp, phi in [0, 2pi), MaxRap = 1e5 for zero-pt momenta) as
+ * restated by the reference: tri_matrix.cu:77-121 (constants and _set_jet),
+ * grid.cu:110-166 (weights per algorithm), tri_matrix.cu:123-130 (distance).
+ */
+#ifndef FJMI_MATH_H
+#define FJMI_MATH_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define FJMI_HD __host__ __device__ static inline
+#else
+#define FJMI_HD static inline
+#endif
+
+/* Constants (FastJet conventions; tri_matrix.cu:77-79). */
+#define FJMI_PI 0x1.921fb54442d18p+1
+#define FJMI_TWOPI 0x1.921fb54442d18p+2
+#define FJMI_MAXRAP 1e5
+
+typedef struct {
+  double hi, lo;
+} fjmi_dd;
+
+FJMI_HD uint64_t fjmi_d2u(double x) {
+  uint64_t u;
+  __builtin_memcpy(&u, &x, sizeof u);
+  return u;
+}
+
+FJMI_HD double fjmi_u2d(uint64_t u) {
+  double x;
+  __builtin_memcpy(&x, &u, sizeof x);
+  return x;
+}
+
+/* ---- double-double building blocks (Dekker/Knuth error-free transforms) ---- */
+
+FJMI_HD fjmi_dd fjmi_two_sum(double a, double b) {
+  fjmi_dd r;
+  r.hi = a + b;
+  double bb = r.hi - a;
+  r.lo = (a - (r.hi - bb)) + (b - bb);
+  return r;
+}
+
+/* requires |a| >= |b| (or a == 0) */
+FJMI_HD fjmi_dd fjmi_fast_two_sum(double a, double b) {
+  fjmi_dd r;
+  r.hi = a + b;
+  r.lo = b - (r.hi - a);
+  return r;
+}
+
+FJMI_HD fjmi_dd fjmi_two_prod(double a, double b) {
+  fjmi_dd r;
+  r.hi = a * b;
+  r.lo = __builtin_fma(a, b, -r.hi);
+  return r;
+}
+
+FJMI_HD fjmi_dd fjmi_dd_add(fjmi_dd a, fjmi_dd b) {
+  fjmi_dd s = fjmi_two_sum(a.hi, b.hi);
+  fjmi_dd t = fjmi_two_sum(a.lo, b.lo);
+  s.lo += t.hi;
+  s = fjmi_fast_two_sum(s.hi, s.lo);
+  s.lo += t.lo;
+  return fjmi_fast_two_sum(s.hi, s.lo);
+}
+
+FJMI_HD fjmi_dd fjmi_dd_add_d(fjmi_dd a, double b) {
+  fjmi_dd s = fjmi_two_sum(a.hi, b);
+  s.lo += a.lo;
+  return fjmi_fast_two_sum(s.hi, s.lo);
+}
+
+FJMI_HD fjmi_dd fjmi_dd_mul(fjmi_dd a, fjmi_dd b) {
+  fjmi_dd p = fjmi_two_prod(a.hi, b.hi);
+  p.lo += a.hi * b.lo + a.lo * b.hi;
+  return fjmi_fast_two_sum(p.hi, p.lo);
+}
+
+FJMI_HD fjmi_dd fjmi_dd_mul_d(fjmi_dd a, double b) {
+  fjmi_dd p = fjmi_two_prod(a.hi, b);
+  p.lo += a.lo * b;
+  return fjmi_fast_two_sum(p.hi, p.lo);
+}
+
+FJMI_HD fjmi_dd fjmi_dd_neg(fjmi_dd a) {
+  a.hi = -a.hi;
+  a.lo = -a.lo;
+  return a;
+}
+
+FJMI_HD fjmi_dd fjmi_dd_make(double hi, double lo) {
+  fjmi_dd r;
+  r.hi = hi;
+  r.lo = lo;
+  return r;
+}
+
+/* ---- natural logarithm ----
+ * x = 2^e * m, m in (sqrt(1/2), sqrt(2)];  log(m) = 2 atanh(s), s = (m-1)/(m+1).
+ * s, s^2 and the two leading series terms are carried in double-double; the
+ * remaining terms (relative size < 2^-18) in double. */
+FJMI_HD double fjmi_log(double x) {
+  if (!(x > 0.0)) {
+    if (x == 0.0) return -__builtin_inf();
+    return __builtin_nan(""); /* negative or NaN */
+  }
+  if (x == __builtin_inf()) return x;
+  int e = 0;
+  uint64_t u = fjmi_d2u(x);
+  if (u < 0x0010000000000000ULL) { /* subnormal */
+    x *= 0x1p54;
+    u = fjmi_d2u(x);
+    e = -54;
+  }
+  e += (int)(u >> 52) - 1023;
+  double m = fjmi_u2d((u & 0x000FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL);
+  if (m > 0x1.6a09e667f3bcdp+0) {
+    m *= 0.5;
+    e += 1;
+  }
+  double f = m - 1.0; /* exact */
+  /* s = f / (2 + f) in double-double */
+  fjmi_dd den = fjmi_two_sum(2.0, f);
+  double sh = f / den.hi;
+  double r = __builtin_fma(-sh, den.hi, f);
+  r = __builtin_fma(-sh, den.lo, r);
+  fjmi_dd s = fjmi_fast_two_sum(sh, r / den.hi);
+  fjmi_dd z = fjmi_dd_mul(s, s);
+  double zh = z.hi;
+  /* tail: 2/7 + 2z/9 + ... + 2 z^12/31 */
+  double c = 2.0 / 31.0;
+  c = c * zh + 2.0 / 29.0;
+  c = c * zh + 2.0 / 27.0;
+  c = c * zh + 2.0 / 25.0;
+  c = c * zh + 2.0 / 23.0;
+  c = c * zh + 2.0 / 21.0;
+  c = c * zh + 2.0 / 19.0;
+  c = c * zh + 2.0 / 17.0;
+  c = c * zh + 2.0 / 15.0;
+  c = c * zh + 2.0 / 13.0;
+  c = c * zh + 2.0 / 11.0;
+  c = c * zh + 2.0 / 9.0;
+  c = c * zh + 2.0 / 7.0;
+  /* q = 2/5 + z*c ; q2 = 2/3 + z*q */
+  fjmi_dd q = fjmi_dd_add(fjmi_dd_make(0x1.999999999999ap-2, -0x1.999999999999ap-56), fjmi_dd_mul_d(z, c));
+  fjmi_dd q2 = fjmi_dd_add(fjmi_dd_make(0x1.5555555555555p-1, 0x1.5555555555555p-55), fjmi_dd_mul(z, q));
+  fjmi_dd s3 = fjmi_dd_mul(z, s);
+  fjmi_dd L = fjmi_dd_add(fjmi_dd_make(2.0 * s.hi, 2.0 * s.lo), fjmi_dd_mul(s3, q2));
+  if (e != 0) {
+    fjmi_dd el = fjmi_dd_mul_d(fjmi_dd_make(0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56), (double)e);
+    L = fjmi_dd_add(el, L);
+  }
+  return L.hi;
+}
+
+/* ---- atan2 ----
+ * t = min(|x|,|y|)/max(|x|,|y|) in double-double; atan(t) = atan(k/8) + atan(u),
+ * u = (t - k/8)/(1 + t k/8), |u| <= 1/16; then octant/quadrant reconstruction in
+ * double-double.  IEEE special cases follow C99 Annex F. */
+FJMI_HD double fjmi_atan2(double y, double x) {
+  if (x != x || y != y) return x + y;
+  const int xneg = (int)(fjmi_d2u(x) >> 63);
+  const int yneg = (int)(fjmi_d2u(y) >> 63);
+  double ax = __builtin_fabs(x), ay = __builtin_fabs(y);
+  const double inf = __builtin_inf();
+  if (ay == 0.0) {
+    double r = xneg ? FJMI_PI : 0.0;
+    return yneg ? -r : r;
+  }
+  if (ax == 0.0) return yneg ? -0x1.921fb54442d18p+0 : 0x1.921fb54442d18p+0;
+  if (ax == inf || ay == inf) {
+    double r;
+    if (ax == inf && ay == inf)
+      r = xneg ? 0x1.2d97c7f3321d2p+1 /* 3pi/4 */ : 0x1.921fb54442d18p-1 /* pi/4 */;
+    else if (ax == inf)
+      r = xneg ? FJMI_PI : 0.0;
+    else
+      r = 0x1.921fb54442d18p+0;
+    return yneg ? -r : r;
+  }
+  const int swap = ay > ax;
+  const double num = swap ? ax : ay;
+  const double den = swap ? ay : ax;
+  const double th = num / den;
+  const double tl = __builtin_fma(-th, den, num) / den;
+  const int k = (int)(th * 8.0 + 0.5);
+  const double c = (double)k * 0.125;
+  /* u = (t - c) / (1 + t c) */
+  fjmi_dd nu = fjmi_two_sum(th - c, tl); /* th - c is exact (Sterbenz) */
+  fjmi_dd du = fjmi_dd_add_d(fjmi_dd_mul_d(fjmi_dd_make(th, tl), c), 1.0);
+  double uh = nu.hi / du.hi;
+  double rr = __builtin_fma(-uh, du.hi, nu.hi);
+  rr = rr + nu.lo;
+  rr = __builtin_fma(-uh, du.lo, rr);
+  fjmi_dd uu = fjmi_fast_two_sum(uh, rr / du.hi);
+  fjmi_dd w = fjmi_dd_mul(uu, uu);
+  double wh = w.hi;
+  /* S(w) = 1/5 - w/7 + w^2/9 - ... + w^8/21 */
+  double g = 1.0 / 21.0;
+  g = g * wh - 1.0 / 19.0;
+  g = g * wh + 1.0 / 17.0;
+  g = g * wh - 1.0 / 15.0;
+  g = g * wh + 1.0 / 13.0;
+  g = g * wh - 1.0 / 11.0;
+  g = g * wh + 1.0 / 9.0;
+  g = g * wh - 1.0 / 7.0;
+  g = g * wh + 1.0 / 5.0;
+  fjmi_dd inner = fjmi_dd_add(fjmi_dd_make(-0x1.5555555555555p-2, -0x1.5555555555555p-56), fjmi_dd_mul_d(w, g));
+  fjmi_dd corr = fjmi_dd_mul(fjmi_dd_mul(uu, w), inner);
+  fjmi_dd at = fjmi_dd_add(uu, corr);
+  fjmi_dd tab;
+  switch (k) {
+    case 0: tab = fjmi_dd_make(0.0, 0.0); break;
+    case 1: tab = fjmi_dd_make(0x1.fd5ba9aac2f6ep-4, -0x1.cd37686760c17p-59); break;
+    case 2: tab = fjmi_dd_make(0x1.f5b75f92c80ddp-3, 0x1.8ab6e3cf7afbdp-57); break;
+    case 3: tab = fjmi_dd_make(0x1.6f61941e4def1p-2, -0x1.c63aae6f6e918p-56); break;
+    case 4: tab = fjmi_dd_make(0x1.dac670561bb4fp-2, 0x1.a2b7f222f65e2p-56); break;
+    case 5: tab = fjmi_dd_make(0x1.1e00babdefeb4p-1, -0x1.928df287a668fp-58); break;
+    case 6: tab = fjmi_dd_make(0x1.4978fa3269ee1p-1, 0x1.2419a87f2a458p-56); break;
+    case 7: tab = fjmi_dd_make(0x1.700a7c5784634p-1, -0x1.8c34d25aadef6p-56); break;
+    default: tab = fjmi_dd_make(0x1.921fb54442d18p-1, 0x1.1a62633145c07p-55); break;
+  }
+  fjmi_dd th2 = k ? fjmi_dd_add(tab, at) : at;
+  if (swap) th2 = fjmi_dd_add(fjmi_dd_make(0x1.921fb54442d18p+0, 0x1.1a62633145c07p-54), fjmi_dd_neg(th2));
+  if (xneg) th2 = fjmi_dd_add(fjmi_dd_make(0x1.921fb54442d18p+1, 0x1.1a62633145c07p-53), fjmi_dd_neg(th2));
+  return yneg ? -th2.hi : th2.hi;
+}
+
+/* ---- FastJet PseudoJet kinematics (tri_matrix.cu:82-121, grid.cu:112-166) ---- */
+
+/* Azimuth in [0, 2pi); 0 for zero transverse momentum. */
+FJMI_HD double fjmi_phi(double px, double py, double kt2) {
+  double phi = (kt2 == 0.0) ? 0.0 : fjmi_atan2(py, px);
+  if (phi < 0.0) phi += FJMI_TWOPI;
+  if (phi >= FJMI_TWOPI) phi -= FJMI_TWOPI;
+  return phi;
+}
+
+/* Rapidity with the effective-mass clamp; +-(MaxRap + |pz|) for E == |pz|, kt2 == 0. */
+FJMI_HD double fjmi_rap(double pz, double E, double kt2) {
+  const double apz = __builtin_fabs(pz);
+  if (E == apz && kt2 == 0.0) {
+    double maxraphere = FJMI_MAXRAP + apz;
+    return (pz >= 0.0) ? maxraphere : -maxraphere;
+  }
+  double m2 = (E + pz) * (E - pz) - kt2;
+  double effective_m2 = m2 > 0.0 ? m2 : 0.0;
+  double E_plus_pz = E + apz;
+  double rap = 0.5 * fjmi_log((kt2 + effective_m2) / (E_plus_pz * E_plus_pz));
+  if (pz > 0) rap = -rap;
+  return rap;
+}
+
+/* Generalised-kt weight kt^(2p) for p in {1, 0, -1} (cluster.h:4; tri_matrix.cu:61-74, :84). */
+FJMI_HD double fjmi_weight(double kt2, int p) {
+  if (p > 0) return kt2;
+  if (p == 0) return 1.0;
+  return kt2 > 1e-300 ? 1.0 / kt2 : 1e300;
+}
+
+/* Squared (rapidity, phi) distance with the phi wrap at pi (tri_matrix.cu:123-130). */
+FJMI_HD double fjmi_dr2(double rap1, double phi1, double rap2, double phi2) {
+  double dphi = __builtin_fabs(phi1 - phi2);
+  if (dphi > FJMI_PI) dphi = FJMI_TWOPI - dphi;
+  double drap = rap1 - rap2;
+  return dphi * dphi + drap * drap;
+}
+
+#endif /* FJMI_MATH_H */

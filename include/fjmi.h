@@ -1,0 +1,129 @@
+/*
+ * fjmi.h — C ABI of libfjmi, the MI355X (gfx950) jet-clustering engine.
+ *
+ * Plain C: pointers, sizes and status codes; no HIP or C++ types in the
+ * signatures (a stream is passed as `void *`, a hipStream_t; NULL is the null
+ * stream).  Every function returns FJMI_OK (0) or a negative fjmi_status and
+ * sets a thread-local message readable with fjmi_last_error().
+ *
+ * Output convention (all entry points): an event of N particles has exactly N
+ * clustering steps.  Step s writes hist_p1[s], hist_p2[s], hist_dij[s]:
+ *   pair merge:  (min, max) of the parents' history indices, dij = d_ij / R^2;
+ *   beam step:   (parent, -1), dij = d_iB / R^2;
+ * history indices are FastJet's: particles 0..N-1, the jet made at step s is N+s.
+ * jets[4j..4j+3] = (px, py, pz, E) of the j-th jet promoted to the beam, and
+ * njets = number of jets.  In the batched calls every per-particle array is
+ * indexed like the input particles (event e starts at offsets[e]), and njets has
+ * one entry per event.
+ *
+ * The reference has one entry point, the C++ function cluster() (cluster.h:6 of
+ * the reference), bound at link time to one engine (Makefile:31-50).  The
+ * functions below replace it and the launch/allocation code inside each of its
+ * engines; each cites what it replaces.
+ */
+#ifndef FJMI_H
+#define FJMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fjmi_context fjmi_context;
+typedef struct fjmi_batch fjmi_batch;
+
+enum fjmi_status {
+  FJMI_OK = 0,
+  FJMI_EINVAL = -1,      /* bad argument */
+  FJMI_EHIP = -2,        /* a HIP runtime call failed */
+  FJMI_ECAPACITY = -3,   /* an event is larger than any configured kernel */
+  FJMI_ENOMEM = -4,      /* allocation failed / over the memory budget */
+  FJMI_EUNSUPPORTED = -5 /* engine not available */
+};
+
+enum fjmi_engine {
+  FJMI_ENGINE_AUTO = 0,  /* dense for every multiplicity the tuning table covers */
+  FJMI_ENGINE_DENSE = 1, /* replaces tri_matrix.cu / n_array.cu */
+  FJMI_ENGINE_TILED = 2, /* replaces grid.cu */
+  FJMI_ENGINE_EMPTY = 3  /* replaces tri_matrix_empty.cu: per-step launch floor, no result */
+};
+
+/* One row of the multiplicity-bucket table: events with N <= max_n (and above
+ * the previous row's max_n) run on the dense kernel with `threads` threads per
+ * workgroup, `slots` slots per thread (capacity threads*slots >= max_n), and
+ * four-momenta in LDS (p4_global = 0) or global memory (p4_global = 1).
+ * Replaces the reference's occupancy-derived launch shapes (launch.h:11-60). */
+typedef struct {
+  int32_t max_n;
+  int32_t threads;
+  int32_t slots;
+  int32_t p4_global;
+} fjmi_bucket;
+
+/* Last error message of the calling thread ("" if none). */
+const char *fjmi_last_error(void);
+
+/* Library version, major*10000 + minor*100 + patch. */
+int fjmi_version(void);
+
+/* Context on HIP device `device`: device buffers, kernels, tuning table.
+ * Replaces the per-call cudaMalloc/cudaFree of every engine
+ * (tri_matrix.cu:308-320, :353-356; grid.cu:586-593, :636-639). */
+int fjmi_create(int device, fjmi_context **out);
+int fjmi_destroy(fjmi_context *ctx);
+
+/* Kernel configurations compiled into the library: writes up to `cap` rows
+ * (max_n = capacity of the configuration) and returns how many exist. */
+int fjmi_kernel_configs(fjmi_bucket *out, int cap);
+
+/* Replace / read the multiplicity-bucket table (rows sorted by max_n). */
+int fjmi_set_buckets(fjmi_context *ctx, const fjmi_bucket *rows, int n_rows);
+int fjmi_get_buckets(fjmi_context *ctx, fjmi_bucket *rows, int cap, int *n_rows);
+
+/* Single event, host memory in and out, synchronous.  Replaces one call of the
+ * reference's cluster() with the CLI's H2D/D2H around it (main.cc:291-298,
+ * tri_matrix.cu:307-357).  hist_* and jets hold n and 4n entries. */
+int fjmi_cluster_event(fjmi_context *ctx, const double *p4, int n, int p, double R, int engine, int32_t *hist_p1,
+                       int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets);
+
+/* Single event, device memory in and out, asynchronous on `stream`: one kernel
+ * launch (the "plain" one-event-per-launch path).  d_p4 holds 4n doubles. */
+int fjmi_cluster_event_device(fjmi_context *ctx, const double *d_p4, int n, int p, double R, int engine,
+                              int32_t *d_hist_p1, int32_t *d_hist_p2, double *d_hist_dij, double *d_jets,
+                              int32_t *d_njets, void *stream);
+
+/* Drop-in: in-place clustering of `n` PseudoJet records (40 bytes each) in device
+ * memory, the contract of the reference's cluster() (cluster.h:6; output layout
+ * of tri_matrix.cu:293-305): afterwards records with isJet hold the jets.
+ * Asynchronous on `stream`. */
+int fjmi_cluster_pseudojets(fjmi_context *ctx, void *d_particles, int n, int p, double R, int engine, void *stream);
+
+/* Batched clustering of many events in ONE graph launch.  Not in the reference
+ * (it clusters one event per call, main.cc:269-368).  The plan sorts events into
+ * multiplicity buckets (fjmi_set_buckets), orders each bucket's queue by cost,
+ * and captures one hipGraph whose kernels are persistent workgroups pulling
+ * events from those queues.  All pointers are device memory and stay bound to
+ * the plan: h_offsets (host) and d_offsets (device) both hold n_events+1 entries.
+ * d_p / d_R give per-event exponent and radius, or NULL for p_uni / R_uni. */
+int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const double *d_p4,
+                      const int64_t *d_offsets, const int32_t *d_p, const double *d_R, int p_uni, double R_uni,
+                      int32_t *d_hist_p1, int32_t *d_hist_p2, double *d_hist_dij, double *d_jets, int32_t *d_njets,
+                      fjmi_batch **out);
+/* Enqueue the captured graph on `stream` (asynchronous). */
+int fjmi_batch_launch(fjmi_batch *b, void *stream);
+/* Synchronise `stream` and check the kernels' status word. */
+int fjmi_batch_sync(fjmi_batch *b, void *stream);
+int fjmi_batch_destroy(fjmi_batch *b);
+
+/* Host-memory convenience for a batch: copies in, plans, launches, copies out,
+ * synchronises.  Per-particle outputs are sized like p4/4 (total particles). */
+int fjmi_cluster_batch(fjmi_context *ctx, int n_events, const int64_t *offsets, const double *p4, const int32_t *p,
+                       const double *R, int p_uni, double R_uni, int32_t *hist_p1, int32_t *hist_p2, double *hist_dij,
+                       double *jets, int32_t *njets);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FJMI_H */

@@ -1,0 +1,424 @@
+/*
+ * fj_oracle.c — CPU oracle (TEST INFRASTRUCTURE ONLY; see fj_oracle.h).
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off).  The shared kinematics
+ * header is the product's fjmi_math.h: both sides must use the same bits for
+ * rapidity and phi; that header is pinned separately against high-precision
+ * decimal arithmetic and glibc in tests/test_math.py.
+ */
+#include "fj_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "../fastjet_gpu_amd/csrc/fjmi_math.h"
+
+double fjo_log(double x) { return fjmi_log(x); }
+double fjo_atan2(double y, double x) { return fjmi_atan2(y, x); }
+
+/* PseudoJet rap/phi and algorithm weight (tri_matrix.cu:82-121, grid.cu:153-166). */
+static void set_jet(const double *q, int p, int math_mode, double *rap, double *phi, double *w) {
+  const double px = q[0], py = q[1], pz = q[2], E = q[3];
+  const double kt2 = px * px + py * py;
+  if (math_mode == FJO_MATH_LIBM) {
+    double ph = (kt2 == 0.0) ? 0.0 : atan2(py, px);
+    if (ph < 0.0) ph += FJMI_TWOPI;
+    if (ph >= FJMI_TWOPI) ph -= FJMI_TWOPI;
+    *phi = ph;
+    if (E == fabs(pz) && kt2 == 0.0) {
+      double mr = FJMI_MAXRAP + fabs(pz);
+      *rap = (pz >= 0.0) ? mr : -mr;
+    } else {
+      double m2 = (E + pz) * (E - pz) - kt2;
+      double em2 = m2 > 0.0 ? m2 : 0.0;
+      double epz = E + fabs(pz);
+      double r = 0.5 * log((kt2 + em2) / (epz * epz));
+      *rap = (pz > 0) ? -r : r;
+    }
+  } else {
+    *phi = fjmi_phi(px, py, kt2);
+    *rap = fjmi_rap(pz, E, kt2);
+  }
+  *w = fjmi_weight(kt2, p);
+}
+
+void fjo_set_jet(const double *p4, int p, int math_mode, double *rap, double *phi, double *w) {
+  set_jet(p4, p, math_mode, rap, phi, w);
+}
+
+/* ---------------------------------------------------------------------------
+ * Shared bookkeeping: the FastJet jets/history arrays.
+ * jets p4 storage holds the N inputs followed by one entry per merge.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  double *p4;      /* 4 * (2N) */
+  int32_t *hist;   /* history index of each jet */
+  int njets_store; /* number of jets stored */
+  int N;
+  int step;
+  int32_t *hp1, *hp2;
+  double *hdij;
+  double *out_jets;
+  int32_t nout;
+} book_t;
+
+static int book_init(book_t *b, const double *p4, int n, int32_t *hp1, int32_t *hp2, double *hdij, double *jets) {
+  b->p4 = (double *)malloc(sizeof(double) * 4 * (size_t)(2 * n + 1));
+  b->hist = (int32_t *)malloc(sizeof(int32_t) * (size_t)(2 * n + 1));
+  if (!b->p4 || !b->hist) return -1;
+  memcpy(b->p4, p4, sizeof(double) * 4 * (size_t)n);
+  for (int i = 0; i < n; ++i) b->hist[i] = i;
+  b->njets_store = n;
+  b->N = n;
+  b->step = 0;
+  b->hp1 = hp1;
+  b->hp2 = hp2;
+  b->hdij = hdij;
+  b->out_jets = jets;
+  b->nout = 0;
+  return 0;
+}
+
+static void book_free(book_t *b) {
+  free(b->p4);
+  free(b->hist);
+}
+
+/* E-scheme recombination (tri_matrix.cu:269-272): returns the new jet index. */
+static int book_merge(book_t *b, int ja, int jb, double dij) {
+  int k = b->njets_store++;
+  for (int c = 0; c < 4; ++c) b->p4[4 * k + c] = b->p4[4 * ja + c] + b->p4[4 * jb + c];
+  int ha = b->hist[ja], hb = b->hist[jb];
+  b->hist[k] = b->N + b->step;
+  b->hp1[b->step] = ha < hb ? ha : hb;
+  b->hp2[b->step] = ha < hb ? hb : ha;
+  b->hdij[b->step] = dij;
+  b->step++;
+  return k;
+}
+
+static void book_beam(book_t *b, int ja, double dij) {
+  b->hp1[b->step] = b->hist[ja];
+  b->hp2[b->step] = -1;
+  b->hdij[b->step] = dij;
+  b->step++;
+  memcpy(b->out_jets + 4 * (size_t)b->nout, b->p4 + 4 * (size_t)ja, sizeof(double) * 4);
+  b->nout++;
+}
+
+/* ---------------------------------------------------------------------------
+ * N2Plain restatement.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  double eta, phi, kt2; /* kt2 holds the algorithm weight w = kt^(2p) */
+  double nn_dist;
+  int nn; /* slot of the nearest neighbour within R, or -1 */
+  int jet;   /* index into book_t jets */
+} bj_t;
+
+static inline double bj_dist(const bj_t *a, const bj_t *b) { return fjmi_dr2(a->eta, a->phi, b->eta, b->phi); }
+
+static inline double bj_diJ(const bj_t *B, int i) {
+  double kt2 = B[i].kt2;
+  if (B[i].nn >= 0 && B[B[i].nn].kt2 < kt2) kt2 = B[B[i].nn].kt2;
+  return B[i].nn_dist * kt2;
+}
+
+static void bj_set(bj_t *x, const book_t *b, int jet, int p, int mm, double R2) {
+  set_jet(b->p4 + 4 * (size_t)jet, p, mm, &x->eta, &x->phi, &x->kt2);
+  x->jet = jet;
+  x->nn_dist = R2;
+  x->nn = -1;
+}
+
+/* nearest neighbour among all other slots in [0,n), lowest slot on ties */
+static void bj_set_nn_nocross(bj_t *B, int i, int n, double R2) {
+  double best = R2;
+  int nn = -1;
+  for (int j = 0; j < n; ++j) {
+    if (j == i) continue;
+    double d = bj_dist(&B[i], &B[j]);
+    if (d < best) {
+      best = d;
+      nn = j;
+    }
+  }
+  B[i].nn = nn;
+  B[i].nn_dist = best;
+}
+
+int fjo_cluster_n2plain(const double *p4, int n, int p, double R, int mm, int32_t *hp1, int32_t *hp2, double *hdij,
+                        double *jets, int32_t *njets) {
+  if (n < 0 || !(R > 0.0) || p < -1 || p > 1) return -1;
+  *njets = 0;
+  if (n == 0) return 0;
+  const double R2 = R * R, invR2 = 1.0 / R2;
+  book_t bk;
+  if (book_init(&bk, p4, n, hp1, hp2, hdij, jets)) return -2;
+  bj_t *B = (bj_t *)malloc(sizeof(bj_t) * (size_t)n);
+  double *diJ = (double *)malloc(sizeof(double) * (size_t)n);
+  if (!B || !diJ) {
+    free(B);
+    free(diJ);
+    book_free(&bk);
+    return -2;
+  }
+  for (int i = 0; i < n; ++i) bj_set(&B[i], &bk, i, p, mm, R2);
+  /* initial NN by cross-checking each slot against the lower slots */
+  for (int a = 1; a < n; ++a) {
+    double best = R2;
+    int nn = -1;
+    for (int b = 0; b < a; ++b) {
+      double d = bj_dist(&B[a], &B[b]);
+      if (d < best) {
+        best = d;
+        nn = b;
+      }
+      if (d < B[b].nn_dist) {
+        B[b].nn_dist = d;
+        B[b].nn = a;
+      }
+    }
+    B[a].nn = nn;
+    B[a].nn_dist = best;
+  }
+  for (int i = 0; i < n; ++i) diJ[i] = bj_diJ(B, i);
+
+  while (n > 0) {
+    double dmin = diJ[0];
+    int amin = 0;
+    for (int i = 0; i < n; ++i)
+      if (diJ[i] < dmin) {
+        amin = i;
+        dmin = diJ[i];
+      }
+    int ja = amin;
+    int jb = B[ja].nn;
+    double dij = dmin * invR2;
+    if (jb >= 0) {
+      if (ja < jb) {
+        int t = ja;
+        ja = jb;
+        jb = t;
+      }
+      int k = book_merge(&bk, B[ja].jet, B[jb].jet, dij);
+      bj_set(&B[jb], &bk, k, p, mm, R2);
+    } else {
+      book_beam(&bk, B[ja].jet, dij);
+    }
+    const int tail = n - 1;
+    n--;
+    B[ja] = B[tail];
+    diJ[ja] = diJ[tail];
+    for (int i = 0; i < n; ++i) {
+      if (B[i].nn == ja || (jb >= 0 && B[i].nn == jb)) {
+        bj_set_nn_nocross(B, i, n, R2);
+        diJ[i] = bj_diJ(B, i);
+      }
+      if (jb >= 0) {
+        double d = bj_dist(&B[i], &B[jb]);
+        if (d < B[i].nn_dist && i != jb) {
+          B[i].nn_dist = d;
+          B[i].nn = jb;
+          diJ[i] = bj_diJ(B, i);
+        }
+        if (d < B[jb].nn_dist && i != jb) {
+          B[jb].nn_dist = d;
+          B[jb].nn = i;
+        }
+      }
+      if (B[i].nn == tail) B[i].nn = ja;
+    }
+    if (jb >= 0) diJ[jb] = bj_diJ(B, jb);
+  }
+  *njets = bk.nout;
+  free(B);
+  free(diJ);
+  book_free(&bk);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Brute force: same slot semantics, NN recomputed from scratch every step.
+ * ------------------------------------------------------------------------- */
+int fjo_cluster_brute(const double *p4, int n, int p, double R, int mm, int32_t *hp1, int32_t *hp2, double *hdij,
+                      double *jets, int32_t *njets) {
+  if (n < 0 || !(R > 0.0) || p < -1 || p > 1) return -1;
+  *njets = 0;
+  if (n == 0) return 0;
+  const double R2 = R * R, invR2 = 1.0 / R2;
+  book_t bk;
+  if (book_init(&bk, p4, n, hp1, hp2, hdij, jets)) return -2;
+  bj_t *B = (bj_t *)malloc(sizeof(bj_t) * (size_t)n);
+  if (!B) {
+    book_free(&bk);
+    return -2;
+  }
+  for (int i = 0; i < n; ++i) bj_set(&B[i], &bk, i, p, mm, R2);
+  while (n > 0) {
+    for (int i = 0; i < n; ++i) bj_set_nn_nocross(B, i, n, R2);
+    double dmin = bj_diJ(B, 0);
+    int amin = 0;
+    for (int i = 1; i < n; ++i) {
+      double d = bj_diJ(B, i);
+      if (d < dmin) {
+        dmin = d;
+        amin = i;
+      }
+    }
+    int ja = amin, jb = B[ja].nn;
+    double dij = dmin * invR2;
+    if (jb >= 0) {
+      if (ja < jb) {
+        int t = ja;
+        ja = jb;
+        jb = t;
+      }
+      int k = book_merge(&bk, B[ja].jet, B[jb].jet, dij);
+      bj_set(&B[jb], &bk, k, p, mm, R2);
+    } else {
+      book_beam(&bk, B[ja].jet, dij);
+    }
+    B[ja] = B[n - 1];
+    n--;
+  }
+  *njets = bk.nout;
+  free(B);
+  book_free(&bk);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Textbook pairwise generalised-kt (no NN bookkeeping).
+ * ------------------------------------------------------------------------- */
+int fjo_cluster_pairwise(const double *p4, int n, int p, double R, int mm, int32_t *hp1, int32_t *hp2, double *hdij,
+                         double *jets, int32_t *njets) {
+  if (n < 0 || !(R > 0.0) || p < -1 || p > 1) return -1;
+  *njets = 0;
+  if (n == 0) return 0;
+  const double R2 = R * R, invR2 = 1.0 / R2;
+  book_t bk;
+  if (book_init(&bk, p4, n, hp1, hp2, hdij, jets)) return -2;
+  int *live = (int *)malloc(sizeof(int) * (size_t)n);
+  double *rap = (double *)malloc(sizeof(double) * (size_t)(2 * n));
+  double *phi = (double *)malloc(sizeof(double) * (size_t)(2 * n));
+  double *w = (double *)malloc(sizeof(double) * (size_t)(2 * n));
+  if (!live || !rap || !phi || !w) {
+    free(live);
+    free(rap);
+    free(phi);
+    free(w);
+    book_free(&bk);
+    return -2;
+  }
+  for (int i = 0; i < n; ++i) {
+    live[i] = i;
+    set_jet(bk.p4 + 4 * (size_t)i, p, mm, &rap[i], &phi[i], &w[i]);
+  }
+  int nl = n;
+  while (nl > 0) {
+    double best = 0.0;
+    int bi = -1, bj = -1;
+    for (int a = 0; a < nl; ++a) {
+      int i = live[a];
+      double d = w[i] * R2;
+      if (bi < 0 || d < best) {
+        best = d;
+        bi = a;
+        bj = -1;
+      }
+      for (int c = a + 1; c < nl; ++c) {
+        int j = live[c];
+        double dr2 = fjmi_dr2(rap[i], phi[i], rap[j], phi[j]);
+        if (!(dr2 < R2)) continue;
+        double wm = w[i] < w[j] ? w[i] : w[j];
+        double dd = dr2 * wm;
+        if (dd < best) {
+          best = dd;
+          bi = a;
+          bj = c;
+        }
+      }
+    }
+    double dij = best * invR2;
+    if (bj >= 0) {
+      int k = book_merge(&bk, live[bi], live[bj], dij);
+      set_jet(bk.p4 + 4 * (size_t)k, p, mm, &rap[k], &phi[k], &w[k]);
+      live[bi] = k;
+      for (int c = bj; c < nl - 1; ++c) live[c] = live[c + 1];
+    } else {
+      book_beam(&bk, live[bi], dij);
+      for (int c = bi; c < nl - 1; ++c) live[c] = live[c + 1];
+    }
+    nl--;
+  }
+  *njets = bk.nout;
+  free(live);
+  free(rap);
+  free(phi);
+  free(w);
+  book_free(&bk);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * CPU baseline timing: events spread over threads, n2plain, shared math.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  const double *p4;
+  const int64_t *off;
+  int n_events, p, nthreads, tid;
+  double R;
+  int err;
+} tb_arg_t;
+
+static void *tb_worker(void *vp) {
+  tb_arg_t *a = (tb_arg_t *)vp;
+  int64_t maxn = 0;
+  for (int e = 0; e < a->n_events; ++e) {
+    int64_t m = a->off[e + 1] - a->off[e];
+    if (m > maxn) maxn = m;
+  }
+  int32_t *h1 = (int32_t *)malloc(sizeof(int32_t) * (size_t)(maxn + 1));
+  int32_t *h2 = (int32_t *)malloc(sizeof(int32_t) * (size_t)(maxn + 1));
+  double *hd = (double *)malloc(sizeof(double) * (size_t)(maxn + 1));
+  double *jt = (double *)malloc(sizeof(double) * 4 * (size_t)(maxn + 1));
+  if (!h1 || !h2 || !hd || !jt) a->err = 1;
+  for (int e = a->tid; e < a->n_events && !a->err; e += a->nthreads) {
+    int32_t nj;
+    if (fjo_cluster_n2plain(a->p4 + 4 * a->off[e], (int)(a->off[e + 1] - a->off[e]), a->p, a->R, FJO_MATH_SHARED, h1,
+                            h2, hd, jt, &nj))
+      a->err = 1;
+  }
+  free(h1);
+  free(h2);
+  free(hd);
+  free(jt);
+  return NULL;
+}
+
+double fjo_time_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R, int nthreads) {
+  if (nthreads < 1 || nthreads > 1024) return -1.0;
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+  tb_arg_t *args = (tb_arg_t *)malloc(sizeof(tb_arg_t) * (size_t)nthreads);
+  if (!th || !args) return -1.0;
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  for (int t = 0; t < nthreads; ++t) {
+    args[t] = (tb_arg_t){p4, offsets, n_events, p, nthreads, t, R, 0};
+    pthread_create(&th[t], NULL, tb_worker, &args[t]);
+  }
+  int err = 0;
+  for (int t = 0; t < nthreads; ++t) {
+    pthread_join(th[t], NULL);
+    err |= args[t].err;
+  }
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  free(th);
+  free(args);
+  if (err) return -1.0;
+  return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}

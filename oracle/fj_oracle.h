@@ -1,0 +1,69 @@
+/*
+ * fj_oracle.h — CPU oracle for sequential-recombination jet clustering.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library, the CLI,
+ * the drop-in cluster()) may include, link or call this code.  It is used by
+ * tests/ as the parity checker, by __graft_entry__.smoke(), and by bench.py's
+ * CPU-baseline leg.
+ *
+ * What it restates:
+ *  - FastJet 3.3.2's N2Plain strategy (ClusterSequence::_simple_N2_cluster, the
+ *    published nearest-neighbour method of Cacciari & Salam, hep-ph/0512210),
+ *    which is the algorithm the reference engines approximate on the GPU
+ *    (README.md:1-4; tri_matrix.cu:307-357 all-pairs form; n_array.cu:129-200).
+ *    Kinematics and distances follow tri_matrix.cu:82-134 and grid.cu:110-201.
+ *    Recombination / compaction follow tri_matrix.cu:262-276 (beam: the jet is
+ *    removed and the last slot moves into its place; pair: the merged jet takes
+ *    the lower slot and the last slot moves into the higher one).
+ *  - The FastJet history numbering: particles are 0..N-1, the jet made at step s
+ *    has history index N+s; a step records (min parent, max parent) or
+ *    (parent, -1) for a beam step, and dij = min diJ / R^2.
+ *
+ * Output convention shared with the GPU library (include/fjmi.h):
+ *   hist_p1[s], hist_p2[s], hist_dij[s] for s = 0..N-1 (there are exactly N steps);
+ *   jets[4*j .. 4*j+3] = (px,py,pz,E) of the j-th jet promoted to the beam;
+ *   *njets = number of jets.
+ */
+#ifndef FJ_ORACLE_H
+#define FJ_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* math_mode: 0 = the shared correctly-rounded fjmi_math.h functions (what the GPU
+ * uses), 1 = glibc atan2/log (what FastJet uses). */
+enum { FJO_MATH_SHARED = 0, FJO_MATH_LIBM = 1 };
+
+/* FastJet N2Plain restatement. Returns 0 on success, negative on bad arguments. */
+int fjo_cluster_n2plain(const double *p4, int n, int p, double R, int math_mode, int32_t *hist_p1,
+                        int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets);
+
+/* Same semantics, but every step recomputes every nearest neighbour from scratch
+ * (O(N^3)).  Equal to n2plain except on exact nearest-neighbour distance ties. */
+int fjo_cluster_brute(const double *p4, int n, int p, double R, int math_mode, int32_t *hist_p1, int32_t *hist_p2,
+                      double *hist_dij, double *jets, int32_t *njets);
+
+/* Textbook generalised-kt: each step takes the smallest of all pair distances
+ * min(w_i,w_j) dR^2 (dR^2 < R^2) and beam distances w_i R^2, with no nearest-
+ * neighbour bookkeeping and no slot compaction (live jets kept in creation order,
+ * ties -> first found).  Equal to n2plain except on exact ties. */
+int fjo_cluster_pairwise(const double *p4, int n, int p, double R, int math_mode, int32_t *hist_p1,
+                         int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets);
+
+/* Kinematics exported for tests. */
+double fjo_log(double x);
+double fjo_atan2(double y, double x);
+void fjo_set_jet(const double *p4, int p, int math_mode, double *rap, double *phi, double *w);
+
+/* CPU baseline: clusters n_events events (p4 packed, offsets[n_events+1] in
+ * particles) with n2plain on nthreads threads; returns wall seconds, or -1. */
+double fjo_time_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
