@@ -1,0 +1,243 @@
+"""fastjet_gpu_amd — MI355X (gfx950) sequential-recombination jet clustering.
+
+Python binding (ctypes) over the C ABI of lib/libfjmi.so (include/fjmi.h) and the
+event generator lib/libfjmi_gen.so (include/fjmi_gen.h).  The clustering itself
+runs only in the HIP kernels: if the library is missing, importing the engine
+raises instead of falling back to anything else.
+
+Algorithm exponents follow the reference's Algorithm enum (cluster.h:4):
+kt = 1, Cambridge/Aachen = 0, anti-kt = -1.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["Engine", "FjmiError", "KT", "CAMBRIDGE_AACHEN", "ANTIKT", "ENGINE_AUTO", "ENGINE_DENSE",
+           "ENGINE_TILED", "ENGINE_EMPTY", "gen_event", "gen_workload", "lib_path", "load_lib"]
+
+KT, CAMBRIDGE_AACHEN, ANTIKT = 1, 0, -1
+ENGINE_AUTO, ENGINE_DENSE, ENGINE_TILED, ENGINE_EMPTY = 0, 1, 2, 3
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_PKG, "lib")
+
+_c_int_p = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+
+
+class FjmiError(RuntimeError):
+    pass
+
+
+class Bucket(ctypes.Structure):
+    _fields_ = [("max_n", ctypes.c_int32), ("threads", ctypes.c_int32), ("slots", ctypes.c_int32),
+                ("p4_global", ctypes.c_int32)]
+
+
+def lib_path(name="libfjmi.so"):
+    return os.path.join(_LIB, name)
+
+
+_libs = {}
+
+
+def load_lib(name="libfjmi.so"):
+    """Loads a native library of the package; raises if it has not been built."""
+    if name in _libs:
+        return _libs[name]
+    path = lib_path(name)
+    if not os.path.exists(path):
+        raise FjmiError(f"{path} is not built; run `python -m fastjet_gpu_amd.build`")
+    lib = ctypes.CDLL(path)
+    if name == "libfjmi.so":
+        _declare(lib)
+    elif name == "libfjmi_gen.so":
+        lib.fjmi_gen_event_size.argtypes = [ctypes.c_int, ctypes.c_int64]
+        lib.fjmi_gen_event_size.restype = ctypes.c_int
+        lib.fjmi_gen_event.argtypes = [ctypes.c_int, ctypes.c_int64, _vp, ctypes.c_int]
+        lib.fjmi_gen_event.restype = ctypes.c_int
+        lib.fjmi_gen_serving_attrs.argtypes = [ctypes.c_int64, _c_int_p, ctypes.POINTER(ctypes.c_double),
+                                               ctypes.POINTER(ctypes.c_double)]
+        lib.fjmi_gen_serving_attrs.restype = ctypes.c_int
+    _libs[name] = lib
+    return lib
+
+
+def _declare(lib):
+    d = ctypes.c_double
+    i = ctypes.c_int
+    sig = {
+        "fjmi_last_error": ([], ctypes.c_char_p),
+        "fjmi_version": ([], i),
+        "fjmi_create": ([i, ctypes.POINTER(_vp)], i),
+        "fjmi_destroy": ([_vp], i),
+        "fjmi_kernel_configs": ([_vp, i], i),
+        "fjmi_set_buckets": ([_vp, _vp, i], i),
+        "fjmi_get_buckets": ([_vp, _vp, i, _c_int_p], i),
+        "fjmi_cluster_event": ([_vp, _vp, i, i, d, i, _vp, _vp, _vp, _vp, _vp], i),
+        "fjmi_cluster_event_device": ([_vp, _vp, i, i, d, i, _vp, _vp, _vp, _vp, _vp, _vp], i),
+        "fjmi_cluster_pseudojets": ([_vp, _vp, i, i, d, i, _vp], i),
+        "fjmi_batch_create": ([_vp, i, _vp, _vp, _vp, _vp, _vp, i, d, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)],
+                              i),
+        "fjmi_batch_launch": ([_vp, _vp], i),
+        "fjmi_batch_sync": ([_vp, _vp], i),
+        "fjmi_batch_destroy": ([_vp], i),
+        "fjmi_cluster_batch": ([_vp, i, _vp, _vp, _vp, _vp, i, d, _vp, _vp, _vp, _vp, _vp], i),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp) if a is not None else None
+
+
+def _check(lib, rc, what):
+    if rc != 0:
+        msg = lib.fjmi_last_error().decode(errors="replace")
+        raise FjmiError(f"{what} failed ({rc}): {msg}")
+
+
+class Engine:
+    """A libfjmi context on one HIP device."""
+
+    def __init__(self, device=0):
+        self.lib = load_lib()
+        h = _vp()
+        _check(self.lib, self.lib.fjmi_create(device, ctypes.byref(h)), "fjmi_create")
+        self.ctx = h
+
+    def close(self):
+        if self.ctx:
+            self.lib.fjmi_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- configuration --
+    def kernel_configs(self):
+        n = self.lib.fjmi_kernel_configs(None, 0)
+        arr = (Bucket * n)()
+        self.lib.fjmi_kernel_configs(ctypes.cast(arr, _vp), n)
+        return [(b.max_n, b.threads, b.slots, b.p4_global) for b in arr]
+
+    def buckets(self):
+        arr = (Bucket * 64)()
+        n = ctypes.c_int32()
+        _check(self.lib, self.lib.fjmi_get_buckets(self.ctx, ctypes.cast(arr, _vp), 64, ctypes.byref(n)),
+               "fjmi_get_buckets")
+        return [(arr[k].max_n, arr[k].threads, arr[k].slots, arr[k].p4_global) for k in range(n.value)]
+
+    def set_buckets(self, rows):
+        arr = (Bucket * len(rows))(*[Bucket(*r) for r in rows])
+        _check(self.lib, self.lib.fjmi_set_buckets(self.ctx, ctypes.cast(arr, _vp), len(rows)), "fjmi_set_buckets")
+
+    # -- host-memory entry points --
+    def cluster_event(self, p4, p, R, engine=ENGINE_AUTO):
+        """Clusters one event (N x 4 array of px,py,pz,E). Returns (hist_p1, hist_p2, hist_dij, jets)."""
+        p4 = np.ascontiguousarray(p4, dtype=np.float64).reshape(-1, 4)
+        n = len(p4)
+        h1 = np.zeros(n, np.int32)
+        h2 = np.zeros(n, np.int32)
+        hd = np.zeros(n, np.float64)
+        jets = np.zeros((max(n, 1), 4), np.float64)
+        nj = np.zeros(1, np.int32)
+        _check(self.lib, self.lib.fjmi_cluster_event(self.ctx, _ptr(p4), n, int(p), float(R), engine, _ptr(h1),
+                                                     _ptr(h2), _ptr(hd), _ptr(jets), _ptr(nj)), "fjmi_cluster_event")
+        return h1, h2, hd, jets[:nj[0]].copy()
+
+    def cluster_batch(self, p4, offsets, p, R):
+        """Clusters many events in one graph launch. p and R are scalars or per-event arrays.
+        Returns (hist_p1, hist_p2, hist_dij, jets, njets) with per-particle arrays."""
+        p4 = np.ascontiguousarray(p4, dtype=np.float64).reshape(-1, 4)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        ne = len(offsets) - 1
+        tot = len(p4)
+        h1 = np.zeros(tot, np.int32)
+        h2 = np.zeros(tot, np.int32)
+        hd = np.zeros(tot, np.float64)
+        jets = np.zeros((max(tot, 1), 4), np.float64)
+        nj = np.zeros(max(ne, 1), np.int32)
+        pa = np.ascontiguousarray(p, dtype=np.int32) if np.ndim(p) else None
+        Ra = np.ascontiguousarray(R, dtype=np.float64) if np.ndim(R) else None
+        _check(self.lib, self.lib.fjmi_cluster_batch(
+            self.ctx, ne, _ptr(offsets), _ptr(p4), _ptr(pa), _ptr(Ra), int(p) if pa is None else 0,
+            float(R) if Ra is None else 0.0, _ptr(h1), _ptr(h2), _ptr(hd), _ptr(jets), _ptr(nj)),
+            "fjmi_cluster_batch")
+        return h1, h2, hd, jets, nj[:ne]
+
+    # -- device-memory entry points (pointers as ints, e.g. torch tensor data_ptr()) --
+    def cluster_event_device(self, d_p4, n, p, R, d_h1, d_h2, d_hd, d_jets, d_nj, stream=0, engine=ENGINE_AUTO):
+        _check(self.lib, self.lib.fjmi_cluster_event_device(self.ctx, d_p4, n, int(p), float(R), engine, d_h1, d_h2,
+                                                            d_hd, d_jets, d_nj, stream or None),
+               "fjmi_cluster_event_device")
+
+    def cluster_pseudojets(self, d_particles, n, p, R, engine=ENGINE_AUTO, stream=0):
+        _check(self.lib, self.lib.fjmi_cluster_pseudojets(self.ctx, d_particles, n, int(p), float(R), engine,
+                                                          stream or None), "fjmi_cluster_pseudojets")
+
+    def batch_create(self, h_offsets, d_p4, d_offsets, d_p, d_R, p_uni, R_uni, d_h1, d_h2, d_hd, d_jets, d_nj):
+        h_offsets = np.ascontiguousarray(h_offsets, dtype=np.int64)
+        b = _vp()
+        _check(self.lib, self.lib.fjmi_batch_create(self.ctx, len(h_offsets) - 1, _ptr(h_offsets), d_p4, d_offsets,
+                                                    d_p, d_R, int(p_uni), float(R_uni), d_h1, d_h2, d_hd, d_jets,
+                                                    d_nj, ctypes.byref(b)), "fjmi_batch_create")
+        return Batch(self, b)
+
+
+class Batch:
+    def __init__(self, eng, handle):
+        self.eng = eng
+        self.h = handle
+
+    def launch(self, stream=0):
+        _check(self.eng.lib, self.eng.lib.fjmi_batch_launch(self.h, stream or None), "fjmi_batch_launch")
+
+    def sync(self, stream=0):
+        _check(self.eng.lib, self.eng.lib.fjmi_batch_sync(self.h, stream or None), "fjmi_batch_sync")
+
+    def close(self):
+        if self.h:
+            self.eng.lib.fjmi_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# -- generator --
+def gen_event(workload, event):
+    """(N, 4) float64 array of event `event` of workload 1..5 (see csrc/fjmi_gen.c)."""
+    g = load_lib("libfjmi_gen.so")
+    n = g.fjmi_gen_event_size(workload, event)
+    if n < 0:
+        raise FjmiError(f"unknown workload {workload}")
+    out = np.zeros((max(n, 1), 4), np.float64)
+    rc = g.fjmi_gen_event(workload, event, out.ctypes.data_as(_vp), n)
+    if rc != n:
+        raise FjmiError(f"generator failed ({rc})")
+    return out[:n]
+
+
+def gen_workload(workload, n_events, first=0):
+    """Packed (total, 4) array and int64 offsets[n_events+1] for events first..first+n_events-1."""
+    g = load_lib("libfjmi_gen.so")
+    sizes = np.array([g.fjmi_gen_event_size(workload, first + e) for e in range(n_events)], np.int64)
+    offsets = np.zeros(n_events + 1, np.int64)
+    np.cumsum(sizes, out=offsets[1:])
+    p4 = np.zeros((max(int(offsets[-1]), 1), 4), np.float64)
+    for e in range(n_events):
+        rc = g.fjmi_gen_event(workload, first + e, p4[offsets[e]:].ctypes.data_as(_vp), int(sizes[e]))
+        if rc != sizes[e]:
+            raise FjmiError(f"generator failed ({rc})")
+    return p4[:offsets[-1]], offsets

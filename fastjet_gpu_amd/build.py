@@ -1,0 +1,93 @@
+"""Builds the native parts of fastjet_gpu_amd in-tree.
+
+  lib/libfjmi.so      HIP engine + C ABI (include/fjmi.h), gfx950 code objects
+  lib/libfjmi_gen.so  seeded event generator (include/fjmi_gen.h), host C
+  bin/tri_matrix, bin/n_array, bin/grid, bin/tri_matrix_empty
+                      the drop-in CLI linked with one engine each
+  oracle/_build/libfj_oracle.so   CPU oracle (test infrastructure, via oracle/Makefile)
+
+Usage: python -m fastjet_gpu_amd.build [--jobs N] [--no-cli] [--no-oracle]
+"""
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "lib")
+BIN = os.path.join(PKG, "bin")
+OBJ = os.path.join(PKG, "_build")
+INC = os.path.join(ROOT, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("FJMI_OFFLOAD_ARCH", "gfx950")
+# -ffp-contract=off: every a*b+c must round like the CPU oracle (bit-exact decisions).
+# -packed-fp32-ops: no packed FP32 instructions are generated.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result", f"-I{INC}", f"-I{CSRC}",
+]
+CC_FLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", f"-I{INC}"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-ffp-contract=off", "-Wall", f"-I{INC}"]
+
+
+def run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def hip_obj(src, extra=()):
+    out = os.path.join(OBJ, os.path.basename(src) + ".o")
+    run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", out])
+    return out
+
+
+def build(jobs=4, cli=True, oracle=True):
+    for d in (LIB, BIN, OBJ):
+        os.makedirs(d, exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        f_dense = ex.submit(hip_obj, "dense.hip")
+        f_rt = ex.submit(hip_obj, "runtime.hip")
+        f_gen = ex.submit(run, ["gcc", *CC_FLAGS, "-shared", "-o", os.path.join(LIB, "libfjmi_gen.so"),
+                                os.path.join(CSRC, "fjmi_gen.c"), "-lm"])
+        f_or = ex.submit(run, ["make", "-s", "-C", os.path.join(ROOT, "oracle")]) if oracle else None
+        objs = [f_dense.result(), f_rt.result()]
+        f_gen.result()
+        if f_or:
+            f_or.result()
+    lib = os.path.join(LIB, "libfjmi.so")
+    tmp = lib + ".tmp"
+    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, lib)
+    if cli and os.path.exists(os.path.join(CSRC, "cli_main.cpp")):
+        build_cli()
+
+
+def build_cli():
+    main_o = os.path.join(OBJ, "cli_main.o")
+    run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "cli_main.cpp"), "-o", main_o])
+    for name, engine in (("tri_matrix", "FJMI_ENGINE_DENSE"), ("n_array", "FJMI_ENGINE_DENSE"),
+                         ("grid", "FJMI_ENGINE_TILED"), ("tri_matrix_empty", "FJMI_ENGINE_EMPTY")):
+        eo = os.path.join(OBJ, f"engine_{name}.o")
+        run(["g++", *CXX_FLAGS, f"-DFJMI_LINKED_ENGINE={engine}", f'-DFJMI_ENGINE_NAME="{name}"', "-c",
+             os.path.join(CSRC, "cluster_dropin.cpp"), "-o", eo])
+        run(["g++", "-o", os.path.join(BIN, name), main_o, eo, f"-L{LIB}", "-lfjmi",
+             f"-Wl,-rpath,$ORIGIN/../lib", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--no-cli", action="store_true")
+    ap.add_argument("--no-oracle", action="store_true")
+    a = ap.parse_args()
+    build(a.jobs, cli=not a.no_cli, oracle=not a.no_oracle)
+
+
+if __name__ == "__main__":
+    main()

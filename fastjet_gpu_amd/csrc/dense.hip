@@ -43,9 +43,17 @@
 
 namespace fjmi {
 
+/* (y, phi, w) of a four-momentum.  Not inlined: the double-double atan2/log are
+ * long, and one out-of-line copy keeps them from inflating the register
+ * allocation of the clustering loop. */
+__device__ __noinline__ double4 kin_coords(double px, double py, double pz, double E, int p) {
+  const double kt2 = px * px + py * py;
+  return make_double4(fjmi_rap(pz, E, kt2), fjmi_phi(px, py, kt2), fjmi_weight(kt2, p), 0.0);
+}
+
 constexpr int BIG = 0x7fffffff;
 constexpr int OCH = 64; /* orphan list entries per chunk */
-constexpr int MO = 4;   /* orphans rescanned per reduction round */
+constexpr int MO = 2;   /* orphans rescanned per reduction round */
 
 struct Red {
   double d;
@@ -165,8 +173,8 @@ __device__ __forceinline__ void write_pseudojet(unsigned char *rec, int index, b
   dp[3] = q.w;
 }
 
-template <int T, int K, bool P4G>
-__global__ __launch_bounds__(T) void dense_kernel(LaunchArgs a) {
+template <int T, int K, bool P4G, int OCC>
+__global__ __launch_bounds__(T, OCC) void dense_kernel(LaunchArgs a) {
   constexpr int W = T / 64;
   constexpr int CAP = T * K;
   static_assert(T % 64 == 0 && T <= 1024, "workgroup must be whole waves");
@@ -187,13 +195,13 @@ __global__ __launch_bounds__(T) void dense_kernel(LaunchArgs a) {
 
   for (;;) {
     __syncthreads(); /* previous event (or queue fetch) fully consumed */
-    if (t == 0) sh.ev = (int)atomicAdd(a.head, 1u);
+    if (t == 0) sh.ev = (int)(atomicAdd(a.head, 1u) - a.head_base);
     __syncthreads();
     const int qi = uni(sh.ev);
     if (qi >= a.n_queue) break;
-    const int e = uni(a.queue[qi]);
-    const int64_t base = a.offsets[e];
-    const int N = uni((int)(a.offsets[e + 1] - base));
+    const int e = a.queue ? uni(a.queue[qi]) : qi;
+    const int64_t base = a.offsets ? a.offsets[e] : 0;
+    const int N = a.offsets ? uni((int)(a.offsets[e + 1] - base)) : a.n_single;
     if (N > CAP) {
       if (t == 0) atomicOr(a.status, (unsigned)STATUS_CAPACITY);
       continue;
@@ -208,40 +216,48 @@ __global__ __launch_bounds__(T) void dense_kernel(LaunchArgs a) {
     const double invR2 = 1.0 / R2;
     double4 *gp4 = P4G ? reinterpret_cast<double4 *>(a.p4_scratch) + base : nullptr;
 
-    /* ---- prologue: coordinates of the own slots ---- */
+    /* ---- prologue: coordinates of every particle (one loop, not unrolled per
+     * slot, so the atan2/log code exists once) into the LDS stage, or into global
+     * scratch when the event is larger than the stage ---- */
+    double4 *gcoord = P4G ? reinterpret_cast<double4 *>(a.coord_scratch) + base : nullptr;
+#pragma unroll 1
+    for (int s = t; s < N; s += T) {
+      const double *q = a.in + (size_t)(base + s) * a.in_stride + a.in_off;
+      const double px = q[0], py = q[1], pz = q[2], E = q[3];
+      const double4 c = kin_coords(px, py, pz, E, p);
+      if (P4G) {
+        gcoord[s] = c;
+        gp4[s] = make_double4(px, py, pz, E);
+      } else {
+        stage[s] = c;
+      }
+    }
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int s = k * T + t;
-      rap[k] = 0.0;
-      phi[k] = 0.0;
-      w[k] = 0.0;
       nnd[k] = R2;
       dij[k] = 0.0;
       nn[k] = -1;
-      if (s < N) {
-        const double *q = a.in + (size_t)(base + s) * a.in_stride + a.in_off;
-        const double px = q[0], py = q[1], pz = q[2], E = q[3];
-        const double kt2 = px * px + py * py;
-        phi[k] = fjmi_phi(px, py, kt2);
-        rap[k] = fjmi_rap(pz, E, kt2);
-        w[k] = fjmi_weight(kt2, p);
-        if (P4G) gp4[s] = make_double4(px, py, pz, E);
-      }
+      double4 c = make_double4(0.0, 0.0, 0.0, 0.0);
+      if (s < N) c = P4G ? gcoord[s] : stage[s];
+      rap[k] = c.x;
+      phi[k] = c.y;
+      w[k] = c.z;
     }
 
-    /* ---- initial nearest neighbours: all pairs, staged through LDS in chunks ---- */
+    /* ---- initial nearest neighbours: all pairs, read from the LDS stage (in
+     * chunks when the coordinates live in global memory) ---- */
     double wnn[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) wnn[k] = 0.0;
     for (int c0 = 0; c0 < N; c0 += CH) {
       const int cn = min(CH, N - c0);
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const int s = k * T + t;
-        if (s >= c0 && s < c0 + cn) stage[s - c0] = make_double4(rap[k], phi[k], w[k], 0.0);
+      if (P4G) {
+        __syncthreads();
+        for (int s = t; s < cn; s += T) stage[s] = gcoord[c0 + s];
+        __syncthreads();
       }
-      __syncthreads();
       for (int j = 0; j < cn; ++j) {
         const double4 q = stage[j];
         const int sj = c0 + j;
@@ -312,10 +328,10 @@ __global__ __launch_bounds__(T) void dense_kernel(LaunchArgs a) {
         const double4 qlo = P4G ? gp4[lo] : lp4[lo];
         const int hlo = lhist[lo];
         qn = make_double4(qhi.x + qlo.x, qhi.y + qlo.y, qhi.z + qlo.z, qhi.w + qlo.w);
-        const double kt2 = qn.x * qn.x + qn.y * qn.y;
-        nphi = fjmi_phi(qn.x, qn.y, kt2);
-        nrap = fjmi_rap(qn.z, qn.w, kt2);
-        nw = fjmi_weight(kt2, p);
+        const double4 c = kin_coords(qn.x, qn.y, qn.z, qn.w, p);
+        nrap = c.x;
+        nphi = c.y;
+        nw = c.z;
         if (t == 0) {
           a.hist_p1[base + step] = min(hhi, hlo);
           a.hist_p2[base + step] = max(hhi, hlo);
@@ -591,5 +607,29 @@ __global__ __launch_bounds__(T) void dense_kernel(LaunchArgs a) {
 /* Per-step launch floor (the analogue of the reference's tri_matrix_empty.cu):
  * an empty kernel launched as often as a per-step-launch design would. */
 __global__ void empty_kernel(int) {}
+
+}  // namespace fjmi
+
+/* ---- registry of the compiled configurations (see fjmi_kernels.h) ---- */
+#include "fjmi_kernels.h"
+
+namespace fjmi {
+
+#define FJMI_DENSE(T, K, G, O) {T, K, G, O, reinterpret_cast<const void *>(&dense_kernel<T, K, G, O>)}
+const KernelDesc kDenseKernels[] = {
+#ifdef FJMI_KERNEL_LIST
+    FJMI_KERNEL_LIST
+#else
+#include "dense_configs.inc"
+#endif
+};
+#undef FJMI_DENSE
+const int kNumDenseKernels = sizeof(kDenseKernels) / sizeof(kDenseKernels[0]);
+const void *const kEmptyKernel = reinterpret_cast<const void *>(&empty_kernel);
+
+__global__ void zero_u32_kernel(unsigned int *p, int n) {
+  for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < n; i += blockDim.x * gridDim.x) p[i] = 0u;
+}
+const void *const kZeroKernel = reinterpret_cast<const void *>(&zero_u32_kernel);
 
 }  // namespace fjmi

@@ -17,11 +17,13 @@ struct LaunchArgs {
   const double *in;
   int in_stride; /* in doubles: 4 for packed double4, 5 for the PseudoJet record */
   int in_off;    /* in doubles: 0 for packed double4, 1 for the PseudoJet record */
-  const int64_t *offsets;
+  const int64_t *offsets; /* nullptr: a single event of n_single particles */
+  int n_single;
   /* event queue */
-  const int32_t *queue; /* event indices, processed in order */
+  const int32_t *queue; /* event indices, processed in order; nullptr: identity */
   int n_queue;
-  unsigned int *head; /* queue head counter (zeroed before the launch) */
+  unsigned int *head;      /* queue head counter */
+  unsigned int head_base;  /* value of *head when the launch starts */
   /* per-event parameters (nullptr -> uniform values) */
   const int32_t *p_arr;
   const double *R_arr;
@@ -37,6 +39,8 @@ struct LaunchArgs {
   unsigned char *pj_out;
   /* global scratch for the four-momenta when they are not kept in LDS: 4 doubles per particle */
   double *p4_scratch;
+  /* global scratch for (y, phi, w) when the event does not fit the LDS stage: 4 doubles per particle */
+  double *coord_scratch;
   /* error word: bit 0 = event larger than the kernel's capacity */
   unsigned int *status;
   int stage_chunk; /* number of staging entries in LDS when four-momenta live in global memory */

@@ -1,0 +1,29 @@
+/*
+ * fjmi_kernels.h — the kernels compiled into libfjmi, as seen by the host runtime.
+ */
+#ifndef FJMI_KERNELS_H
+#define FJMI_KERNELS_H
+
+namespace fjmi {
+
+/* One compiled configuration of the dense kernel: T threads per workgroup, K slots
+ * per thread (capacity T*K particles), four-momenta in global memory (p4g) or LDS. */
+struct KernelDesc {
+  int T;
+  int K;
+  bool p4g;
+  int occ; /* minimum waves per SIMD the register allocation was built for */
+  const void *fn; /* __global__ void(LaunchArgs) */
+};
+
+extern const KernelDesc kDenseKernels[];
+extern const int kNumDenseKernels;
+extern const void *const kEmptyKernel; /* __global__ void(int) */
+extern const void *const kZeroKernel;  /* __global__ void(unsigned *, int) */
+
+/* LDS staging entries used by the p4g kernels for the initial nearest-neighbour pass. */
+constexpr int kStageChunk = 2048;
+
+}  // namespace fjmi
+
+#endif

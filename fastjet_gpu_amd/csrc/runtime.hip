@@ -1,0 +1,616 @@
+/*
+ * runtime.hip — host side of libfjmi: the C ABI declared in include/fjmi.h.
+ *
+ * Holds the per-device context (preallocated device buffers, the multiplicity
+ * bucket table, kernel attributes), the one-event-per-launch path, the batched
+ * path (bucketed persistent queues captured into one hipGraph) and the
+ * host-memory conveniences used by tests and tools.  Replaces the reference's
+ * per-call allocation and launch sequences (tri_matrix.cu:307-357,
+ * n_array.cu:229-243, grid.cu:580-640) and its launch-shape helpers
+ * (launch.h:11-60).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fjmi.h"
+#include "fjmi_internal.h"
+#include "fjmi_kernels.h"
+
+using fjmi::KernelDesc;
+using fjmi::LaunchArgs;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess)                                                                              \
+      return fail(FJMI_EHIP, "%s:%d: %s: %s: %s", __FILE__, __LINE__, #call, hipGetErrorName(e_),      \
+                  hipGetErrorString(e_));                                                              \
+  } while (0)
+
+/* Growable device buffer. */
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need) {
+    if (need <= bytes) return FJMI_OK;
+    if (ptr) HIPCHK(hipFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+    size_t b = std::max(need, (size_t)256);
+    HIPCHK(hipMalloc(&ptr, b));
+    bytes = b;
+    return FJMI_OK;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(ptr);
+  }
+};
+
+const fjmi_bucket kDefaultBuckets[] = {
+#include "tune_table.inc"
+};
+
+const KernelDesc *find_kernel(int T, int K, int p4g) {
+  for (int i = 0; i < fjmi::kNumDenseKernels; ++i) {
+    const KernelDesc &d = fjmi::kDenseKernels[i];
+    if (d.T == T && d.K == K && (int)d.p4g == p4g) return &d;
+  }
+  return nullptr;
+}
+
+size_t dyn_lds_bytes(const KernelDesc &d) {
+  const size_t cap = (size_t)d.T * d.K;
+  const size_t ch = d.p4g ? (size_t)fjmi::kStageChunk : cap;
+  return 32 * ch + 4 * cap;
+}
+
+/* Control block of the one-event-per-launch path: the queue head (monotonic;
+ * each launch consumes 2 counts) and the status word. */
+struct Ctrl {
+  uint32_t head;
+  uint32_t status;
+};
+
+}  // namespace
+
+struct fjmi_context {
+  int device = 0;
+  int n_cu = 0;
+  std::vector<fjmi_bucket> buckets;
+  std::mutex mtx;
+  /* one-event path */
+  DevBuf ctrl;
+  uint32_t head_base = 0;
+  DevBuf hist1, hist2, hdij, jets, njets, p4s, coords, p4in;
+  int max_lds = 0;
+};
+
+struct BucketRun {
+  const KernelDesc *kd;
+  int grid;
+  size_t lds;
+  LaunchArgs args;
+  int n_events;
+};
+
+struct fjmi_batch {
+  fjmi_context *ctx = nullptr;
+  int n_events = 0;
+  DevBuf queue, heads, p4s, coords;
+  std::vector<BucketRun> runs;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t *d_status = nullptr;
+};
+
+namespace {
+
+int set_device(fjmi_context *ctx) {
+  HIPCHK(hipSetDevice(ctx->device));
+  return FJMI_OK;
+}
+
+int check_table(const fjmi_bucket *rows, int n) {
+  if (n <= 0) return fail(FJMI_EINVAL, "empty bucket table");
+  for (int i = 0; i < n; ++i) {
+    const fjmi_bucket &b = rows[i];
+    if (i > 0 && b.max_n <= rows[i - 1].max_n) return fail(FJMI_EINVAL, "bucket table not sorted by max_n");
+    const KernelDesc *kd = find_kernel(b.threads, b.slots, b.p4_global);
+    if (!kd)
+      return fail(FJMI_EINVAL, "no compiled dense kernel with threads=%d slots=%d p4_global=%d", b.threads, b.slots,
+                  b.p4_global);
+    if ((int64_t)b.threads * b.slots < b.max_n)
+      return fail(FJMI_EINVAL, "bucket max_n=%d exceeds capacity %d", b.max_n, b.threads * b.slots);
+  }
+  return FJMI_OK;
+}
+
+const fjmi_bucket *bucket_for(const fjmi_context *ctx, int n) {
+  for (const fjmi_bucket &b : ctx->buckets)
+    if (n <= b.max_n) return &b;
+  return nullptr;
+}
+
+LaunchArgs base_args() {
+  LaunchArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.in_stride = 4;
+  a.in_off = 0;
+  a.stage_chunk = fjmi::kStageChunk;
+  return a;
+}
+
+int launch_dense(const KernelDesc &kd, int grid, size_t lds, const LaunchArgs &a, hipStream_t stream) {
+  LaunchArgs copy = a;
+  void *params[] = {&copy};
+  HIPCHK(hipLaunchKernel(kd.fn, dim3(grid), dim3(kd.T), params, lds, stream));
+  return FJMI_OK;
+}
+
+/* One event on the device: one kernel launch with one workgroup. */
+int cluster_one_device(fjmi_context *ctx, const double *d_in, int in_stride, int in_off, int n, int p, double R,
+                       int32_t *h1, int32_t *h2, double *hd, double *jets, int32_t *nj, unsigned char *pj,
+                       hipStream_t stream) {
+  const fjmi_bucket *b = bucket_for(ctx, n);
+  if (!b) return fail(FJMI_ECAPACITY, "event with %d particles exceeds the largest bucket", n);
+  const KernelDesc *kd = find_kernel(b->threads, b->slots, b->p4_global);
+  int rc;
+  if ((rc = ctx->ctrl.ensure(sizeof(Ctrl)))) return rc;
+  if (kd->p4g) {
+    if ((rc = ctx->p4s.ensure(sizeof(double) * 4 * (size_t)n))) return rc;
+    if ((rc = ctx->coords.ensure(sizeof(double) * 4 * (size_t)n))) return rc;
+  }
+  Ctrl *c = ctx->ctrl.as<Ctrl>();
+  LaunchArgs a = base_args();
+  a.in = d_in;
+  a.in_stride = in_stride;
+  a.in_off = in_off;
+  a.offsets = nullptr; /* single event: particles [0, n) */
+  a.n_single = n;
+  a.queue = nullptr;
+  a.n_queue = 1;
+  a.head = &c->head;
+  a.head_base = ctx->head_base;
+  a.p_uni = p;
+  a.R_uni = R;
+  a.hist_p1 = h1;
+  a.hist_p2 = h2;
+  a.hist_dij = hd;
+  a.jets = jets;
+  a.njets = nj;
+  a.pj_out = pj;
+  a.p4_scratch = ctx->p4s.as<double>();
+  a.coord_scratch = ctx->coords.as<double>();
+  a.status = &c->status;
+  if ((rc = launch_dense(*kd, 1, dyn_lds_bytes(*kd), a, stream))) return rc;
+  ctx->head_base += 2; /* one successful and one failing queue pop */
+  return FJMI_OK;
+}
+
+int check_args(int n, int p, double R) {
+  if (n < 0) return fail(FJMI_EINVAL, "negative particle count");
+  if (p < -1 || p > 1) return fail(FJMI_EINVAL, "exponent p must be -1, 0 or 1");
+  if (!(R > 0.0) || !(R < 1e150)) return fail(FJMI_EINVAL, "radius must be positive and finite");
+  return FJMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fjmi_last_error(void) { return g_err.c_str(); }
+
+int fjmi_version(void) { return 100; }
+
+int fjmi_kernel_configs(fjmi_bucket *out, int cap) {
+  for (int i = 0; i < fjmi::kNumDenseKernels && i < cap; ++i) {
+    const KernelDesc &d = fjmi::kDenseKernels[i];
+    out[i].max_n = d.T * d.K;
+    out[i].threads = d.T;
+    out[i].slots = d.K;
+    out[i].p4_global = d.p4g ? 1 : 0;
+  }
+  return fjmi::kNumDenseKernels;
+}
+
+int fjmi_create(int device, fjmi_context **out) {
+  if (!out) return fail(FJMI_EINVAL, "null output pointer");
+  *out = nullptr;
+  int count = 0;
+  HIPCHK(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return fail(FJMI_EINVAL, "device %d not present (%d devices)", device, count);
+  auto *ctx = new fjmi_context();
+  ctx->device = device;
+  int rc = set_device(ctx);
+  if (rc) {
+    delete ctx;
+    return rc;
+  }
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) {
+    delete ctx;
+    return fail(FJMI_EHIP, "hipGetDeviceProperties: %s", hipGetErrorString(e));
+  }
+  ctx->n_cu = prop.multiProcessorCount;
+  ctx->max_lds = (int)prop.sharedMemPerBlock;
+  for (int i = 0; i < fjmi::kNumDenseKernels; ++i) {
+    const KernelDesc &d = fjmi::kDenseKernels[i];
+    size_t lds = dyn_lds_bytes(d);
+    e = hipFuncSetAttribute(d.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) {
+      delete ctx;
+      return fail(FJMI_EHIP, "hipFuncSetAttribute(%d,%d): %s", d.T, d.K, hipGetErrorString(e));
+    }
+  }
+  const int nb = (int)(sizeof kDefaultBuckets / sizeof kDefaultBuckets[0]);
+  if ((rc = check_table(kDefaultBuckets, nb))) {
+    delete ctx;
+    return rc;
+  }
+  ctx->buckets.assign(kDefaultBuckets, kDefaultBuckets + nb);
+  if ((rc = ctx->ctrl.ensure(sizeof(Ctrl)))) {
+    delete ctx;
+    return rc;
+  }
+  e = hipMemset(ctx->ctrl.ptr, 0, sizeof(Ctrl));
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    delete ctx;
+    return fail(FJMI_EHIP, "control block init: %s", hipGetErrorString(e));
+  }
+  *out = ctx;
+  return FJMI_OK;
+}
+
+int fjmi_destroy(fjmi_context *ctx) {
+  if (!ctx) return FJMI_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  for (DevBuf *b : {&ctx->ctrl, &ctx->hist1, &ctx->hist2, &ctx->hdij, &ctx->jets, &ctx->njets, &ctx->p4s,
+                    &ctx->coords, &ctx->p4in})
+    b->release();
+  delete ctx;
+  return FJMI_OK;
+}
+
+int fjmi_set_buckets(fjmi_context *ctx, const fjmi_bucket *rows, int n_rows) {
+  if (!ctx || !rows) return fail(FJMI_EINVAL, "null argument");
+  int rc = check_table(rows, n_rows);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  ctx->buckets.assign(rows, rows + n_rows);
+  return FJMI_OK;
+}
+
+int fjmi_get_buckets(fjmi_context *ctx, fjmi_bucket *rows, int cap, int *n_rows) {
+  if (!ctx || !n_rows) return fail(FJMI_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  *n_rows = (int)ctx->buckets.size();
+  for (int i = 0; i < *n_rows && i < cap; ++i) rows[i] = ctx->buckets[i];
+  return FJMI_OK;
+}
+
+int fjmi_cluster_event_device(fjmi_context *ctx, const double *d_p4, int n, int p, double R, int engine,
+                              int32_t *d_h1, int32_t *d_h2, double *d_hd, double *d_jets, int32_t *d_nj,
+                              void *stream) {
+  if (!ctx) return fail(FJMI_EINVAL, "null context");
+  int rc = check_args(n, p, R);
+  if (rc) return rc;
+  if (engine != FJMI_ENGINE_AUTO && engine != FJMI_ENGINE_DENSE)
+    return fail(FJMI_EUNSUPPORTED, "engine %d not available for this entry point", engine);
+  if (n > 0 && (!d_p4 || !d_h1 || !d_h2 || !d_hd || !d_jets || !d_nj)) return fail(FJMI_EINVAL, "null buffer");
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  if ((rc = set_device(ctx))) return rc;
+  if (n == 0) {
+    HIPCHK(hipMemsetAsync(d_nj, 0, sizeof(int32_t), (hipStream_t)stream));
+    return FJMI_OK;
+  }
+  return cluster_one_device(ctx, d_p4, 4, 0, n, p, R, d_h1, d_h2, d_hd, d_jets, d_nj, nullptr, (hipStream_t)stream);
+}
+
+int fjmi_cluster_pseudojets(fjmi_context *ctx, void *d_particles, int n, int p, double R, int engine, void *stream) {
+  if (!ctx) return fail(FJMI_EINVAL, "null context");
+  int rc = check_args(n, p, R);
+  if (rc) return rc;
+  if (n == 0) return FJMI_OK;
+  if (!d_particles) return fail(FJMI_EINVAL, "null particle buffer");
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (engine == FJMI_ENGINE_EMPTY) {
+    /* the per-step launch schedule of the reference's tri_matrix_empty.cu:257-292:
+     * init + set_jets + 2 launches per step + output, all empty */
+    const int launches = 2 * n + 3;
+    for (int i = 0; i < launches; ++i) {
+      int arg = i;
+      void *params[] = {&arg};
+      HIPCHK(hipLaunchKernel(fjmi::kEmptyKernel, dim3(1), dim3(64), params, 0, s));
+    }
+    return FJMI_OK;
+  }
+  if (engine != FJMI_ENGINE_AUTO && engine != FJMI_ENGINE_DENSE)
+    return fail(FJMI_EUNSUPPORTED, "engine %d not available", engine);
+  const size_t nn = (size_t)n;
+  if ((rc = ctx->hist1.ensure(4 * nn)) || (rc = ctx->hist2.ensure(4 * nn)) || (rc = ctx->hdij.ensure(8 * nn)) ||
+      (rc = ctx->jets.ensure(32 * nn)) || (rc = ctx->njets.ensure(4)))
+    return rc;
+  return cluster_one_device(ctx, static_cast<const double *>(d_particles), 5, 1, n, p, R, ctx->hist1.as<int32_t>(),
+                            ctx->hist2.as<int32_t>(), ctx->hdij.as<double>(), ctx->jets.as<double>(),
+                            ctx->njets.as<int32_t>(), static_cast<unsigned char *>(d_particles), s);
+}
+
+int fjmi_cluster_event(fjmi_context *ctx, const double *p4, int n, int p, double R, int engine, int32_t *h1,
+                       int32_t *h2, double *hd, double *jets, int32_t *nj) {
+  if (!ctx) return fail(FJMI_EINVAL, "null context");
+  int rc = check_args(n, p, R);
+  if (rc) return rc;
+  if (!nj) return fail(FJMI_EINVAL, "null njets");
+  if (n == 0) {
+    *nj = 0;
+    return FJMI_OK;
+  }
+  if (!p4 || !h1 || !h2 || !hd || !jets) return fail(FJMI_EINVAL, "null buffer");
+  if (engine != FJMI_ENGINE_AUTO && engine != FJMI_ENGINE_DENSE)
+    return fail(FJMI_EUNSUPPORTED, "engine %d not available", engine);
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  if ((rc = set_device(ctx))) return rc;
+  const size_t nn = (size_t)n;
+  if ((rc = ctx->p4in.ensure(32 * nn)) || (rc = ctx->hist1.ensure(4 * nn)) || (rc = ctx->hist2.ensure(4 * nn)) ||
+      (rc = ctx->hdij.ensure(8 * nn)) || (rc = ctx->jets.ensure(32 * nn)) || (rc = ctx->njets.ensure(4)))
+    return rc;
+  HIPCHK(hipMemcpy(ctx->p4in.ptr, p4, 32 * nn, hipMemcpyHostToDevice));
+  Ctrl *c = ctx->ctrl.as<Ctrl>();
+  HIPCHK(hipMemsetAsync(&c->status, 0, sizeof(uint32_t), nullptr));
+  if ((rc = cluster_one_device(ctx, ctx->p4in.as<double>(), 4, 0, n, p, R, ctx->hist1.as<int32_t>(),
+                               ctx->hist2.as<int32_t>(), ctx->hdij.as<double>(), ctx->jets.as<double>(),
+                               ctx->njets.as<int32_t>(), nullptr, nullptr)))
+    return rc;
+  HIPCHK(hipStreamSynchronize(nullptr));
+  uint32_t status = 0;
+  HIPCHK(hipMemcpy(&status, &c->status, sizeof status, hipMemcpyDeviceToHost));
+  if (status) return fail(FJMI_ECAPACITY, "kernel reported status 0x%x", status);
+  HIPCHK(hipMemcpy(h1, ctx->hist1.ptr, 4 * nn, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(h2, ctx->hist2.ptr, 4 * nn, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(hd, ctx->hdij.ptr, 8 * nn, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(nj, ctx->njets.ptr, 4, hipMemcpyDeviceToHost));
+  if (*nj < 0 || *nj > n) return fail(FJMI_EHIP, "corrupt jet count %d", *nj);
+  HIPCHK(hipMemcpy(jets, ctx->jets.ptr, 32 * (size_t)(*nj), hipMemcpyDeviceToHost));
+  return FJMI_OK;
+}
+
+/* ---- batched path ---- */
+
+int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const double *d_p4,
+                      const int64_t *d_offsets, const int32_t *d_p, const double *d_R, int p_uni, double R_uni,
+                      int32_t *d_h1, int32_t *d_h2, double *d_hd, double *d_jets, int32_t *d_nj, fjmi_batch **out) {
+  if (!ctx || !out || !h_offsets || n_events < 0) return fail(FJMI_EINVAL, "bad argument");
+  *out = nullptr;
+  if (!d_p && (p_uni < -1 || p_uni > 1)) return fail(FJMI_EINVAL, "exponent p must be -1, 0 or 1");
+  if (!d_R && !(R_uni > 0.0)) return fail(FJMI_EINVAL, "radius must be positive");
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  const int nb = (int)ctx->buckets.size();
+  std::vector<std::vector<int32_t>> lists(nb);
+  int64_t max_p4g_total = 0;
+  for (int e = 0; e < n_events; ++e) {
+    const int64_t n = h_offsets[e + 1] - h_offsets[e];
+    if (n < 0 || h_offsets[e] < 0) return fail(FJMI_EINVAL, "offsets not monotonic at event %d", e);
+    int b = 0;
+    while (b < nb && n > ctx->buckets[b].max_n) ++b;
+    if (b == nb) return fail(FJMI_ECAPACITY, "event %d has %lld particles, above the largest bucket", e, (long long)n);
+    lists[b].push_back(e);
+    if (ctx->buckets[b].p4_global) max_p4g_total = h_offsets[n_events];
+  }
+  auto *bt = new fjmi_batch();
+  bt->ctx = ctx;
+  bt->n_events = n_events;
+  auto cleanup = [&](int code) {
+    fjmi_batch_destroy(bt);
+    return code;
+  };
+  /* cost-ordered queues (longest first): the last events to start are the cheapest */
+  std::vector<int32_t> flat;
+  std::vector<size_t> start(nb);
+  for (int b = 0; b < nb; ++b) {
+    std::stable_sort(lists[b].begin(), lists[b].end(), [&](int32_t x, int32_t y) {
+      return h_offsets[x + 1] - h_offsets[x] > h_offsets[y + 1] - h_offsets[y];
+    });
+    start[b] = flat.size();
+    flat.insert(flat.end(), lists[b].begin(), lists[b].end());
+  }
+  if ((rc = bt->queue.ensure(sizeof(int32_t) * std::max<size_t>(flat.size(), 1)))) return cleanup(rc);
+  if ((rc = bt->heads.ensure(sizeof(uint32_t) * (nb + 1)))) return cleanup(rc);
+  if (max_p4g_total > 0) {
+    if ((rc = bt->p4s.ensure(32 * (size_t)max_p4g_total))) return cleanup(rc);
+    if ((rc = bt->coords.ensure(32 * (size_t)max_p4g_total))) return cleanup(rc);
+  }
+  if (!flat.empty()) {
+    hipError_t e = hipMemcpy(bt->queue.ptr, flat.data(), sizeof(int32_t) * flat.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return cleanup(fail(FJMI_EHIP, "queue upload: %s", hipGetErrorString(e)));
+  }
+  uint32_t *heads = bt->heads.as<uint32_t>();
+  bt->d_status = heads + nb;
+  for (int b = nb - 1; b >= 0; --b) { /* largest multiplicities first */
+    if (lists[b].empty()) continue;
+    const fjmi_bucket &bk = ctx->buckets[b];
+    BucketRun r;
+    r.kd = find_kernel(bk.threads, bk.slots, bk.p4_global);
+    r.lds = dyn_lds_bytes(*r.kd);
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, r.kd->fn, r.kd->T, r.lds);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    r.n_events = (int)lists[b].size();
+    r.grid = (int)std::min<int64_t>(r.n_events, (int64_t)per_cu * ctx->n_cu);
+    LaunchArgs a = base_args();
+    a.in = d_p4;
+    a.offsets = d_offsets;
+    a.queue = bt->queue.as<int32_t>() + start[b];
+    a.n_queue = r.n_events;
+    a.head = heads + b;
+    a.head_base = 0;
+    a.p_arr = d_p;
+    a.R_arr = d_R;
+    a.p_uni = p_uni;
+    a.R_uni = R_uni;
+    a.hist_p1 = d_h1;
+    a.hist_p2 = d_h2;
+    a.hist_dij = d_hd;
+    a.jets = d_jets;
+    a.njets = d_nj;
+    a.p4_scratch = bt->p4s.as<double>();
+    a.coord_scratch = bt->coords.as<double>();
+    a.status = bt->d_status;
+    r.args = a;
+    bt->runs.push_back(r);
+  }
+  /* capture: zero the heads and status, then every bucket's persistent kernel on
+   * its own forked stream so the buckets run concurrently */
+  hipStream_t s0 = nullptr;
+  std::vector<hipStream_t> ss(bt->runs.size(), nullptr);
+  std::vector<hipEvent_t> evs(bt->runs.size() + 1, nullptr);
+  hipError_t e = hipStreamCreateWithFlags(&s0, hipStreamNonBlocking);
+  for (size_t i = 0; e == hipSuccess && i < ss.size(); ++i) e = hipStreamCreateWithFlags(&ss[i], hipStreamNonBlocking);
+  for (size_t i = 0; e == hipSuccess && i < evs.size(); ++i) e = hipEventCreateWithFlags(&evs[i], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    unsigned *hp = heads;
+    int cnt = nb + 1;
+    void *zp[] = {&hp, &cnt};
+    e = hipLaunchKernel(fjmi::kZeroKernel, dim3(1), dim3(64), zp, 0, s0);
+    if (e == hipSuccess) e = hipEventRecord(evs[0], s0);
+    for (size_t i = 0; e == hipSuccess && i < bt->runs.size(); ++i) {
+      e = hipStreamWaitEvent(ss[i], evs[0], 0);
+      BucketRun &r = bt->runs[i];
+      void *params[] = {&r.args};
+      if (e == hipSuccess) e = hipLaunchKernel(r.kd->fn, dim3(r.grid), dim3(r.kd->T), params, r.lds, ss[i]);
+      if (e == hipSuccess) e = hipEventRecord(evs[i + 1], ss[i]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s0, evs[i + 1], 0);
+    }
+    hipError_t e2 = hipStreamEndCapture(s0, &bt->graph);
+    if (e == hipSuccess) e = e2;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0);
+  for (hipEvent_t ev : evs)
+    if (ev) (void)hipEventDestroy(ev);
+  for (hipStream_t s : ss)
+    if (s) (void)hipStreamDestroy(s);
+  if (s0) (void)hipStreamDestroy(s0);
+  if (e != hipSuccess) return cleanup(fail(FJMI_EHIP, "graph capture: %s", hipGetErrorString(e)));
+  *out = bt;
+  return FJMI_OK;
+}
+
+int fjmi_batch_launch(fjmi_batch *b, void *stream) {
+  if (!b || !b->exec) return fail(FJMI_EINVAL, "null batch");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  HIPCHK(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return FJMI_OK;
+}
+
+int fjmi_batch_sync(fjmi_batch *b, void *stream) {
+  if (!b) return fail(FJMI_EINVAL, "null batch");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  uint32_t st = 0;
+  HIPCHK(hipMemcpy(&st, b->d_status, sizeof st, hipMemcpyDeviceToHost));
+  if (st) return fail(FJMI_ECAPACITY, "kernel reported status 0x%x", st);
+  return FJMI_OK;
+}
+
+int fjmi_batch_destroy(fjmi_batch *b) {
+  if (!b) return FJMI_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipDeviceSynchronize();
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->queue.release();
+  b->heads.release();
+  b->p4s.release();
+  b->coords.release();
+  delete b;
+  return FJMI_OK;
+}
+
+int fjmi_cluster_batch(fjmi_context *ctx, int n_events, const int64_t *offsets, const double *p4, const int32_t *p,
+                       const double *R, int p_uni, double R_uni, int32_t *h1, int32_t *h2, double *hd, double *jets,
+                       int32_t *nj) {
+  if (!ctx || !offsets || n_events < 0) return fail(FJMI_EINVAL, "bad argument");
+  if (n_events == 0) return FJMI_OK;
+  const size_t total = (size_t)offsets[n_events];
+  if (total > 0 && (!p4 || !h1 || !h2 || !hd || !jets)) return fail(FJMI_EINVAL, "null buffer");
+  if (!nj) return fail(FJMI_EINVAL, "null njets");
+  HIPCHK(hipSetDevice(ctx->device));
+  DevBuf dp4, doff, dpp, dR, d1, d2, dd, dj, dn;
+  auto rel = [&]() {
+    for (DevBuf *x : {&dp4, &doff, &dpp, &dR, &d1, &d2, &dd, &dj, &dn}) x->release();
+  };
+  int rc = 0;
+  const size_t tb = std::max<size_t>(total, 1);
+  if ((rc = dp4.ensure(32 * tb)) || (rc = doff.ensure(8 * (size_t)(n_events + 1))) || (rc = d1.ensure(4 * tb)) ||
+      (rc = d2.ensure(4 * tb)) || (rc = dd.ensure(8 * tb)) || (rc = dj.ensure(32 * tb)) ||
+      (rc = dn.ensure(4 * (size_t)n_events)) || (p && (rc = dpp.ensure(4 * (size_t)n_events))) ||
+      (R && (rc = dR.ensure(8 * (size_t)n_events)))) {
+    rel();
+    return rc;
+  }
+  hipError_t e = hipSuccess;
+  if (total) e = hipMemcpy(dp4.ptr, p4, 32 * total, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(doff.ptr, offsets, 8 * (size_t)(n_events + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess && p) e = hipMemcpy(dpp.ptr, p, 4 * (size_t)n_events, hipMemcpyHostToDevice);
+  if (e == hipSuccess && R) e = hipMemcpy(dR.ptr, R, 8 * (size_t)n_events, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    rel();
+    return fail(FJMI_EHIP, "upload: %s", hipGetErrorString(e));
+  }
+  fjmi_batch *b = nullptr;
+  rc = fjmi_batch_create(ctx, n_events, offsets, dp4.as<double>(), doff.as<int64_t>(), p ? dpp.as<int32_t>() : nullptr,
+                         R ? dR.as<double>() : nullptr, p_uni, R_uni, d1.as<int32_t>(), d2.as<int32_t>(),
+                         dd.as<double>(), dj.as<double>(), dn.as<int32_t>(), &b);
+  if (!rc) rc = fjmi_batch_launch(b, nullptr);
+  if (!rc) rc = fjmi_batch_sync(b, nullptr);
+  if (b) fjmi_batch_destroy(b);
+  if (!rc) {
+    e = hipSuccess;
+    if (total) {
+      e = hipMemcpy(h1, d1.ptr, 4 * total, hipMemcpyDeviceToHost);
+      if (e == hipSuccess) e = hipMemcpy(h2, d2.ptr, 4 * total, hipMemcpyDeviceToHost);
+      if (e == hipSuccess) e = hipMemcpy(hd, dd.ptr, 8 * total, hipMemcpyDeviceToHost);
+      if (e == hipSuccess) e = hipMemcpy(jets, dj.ptr, 32 * total, hipMemcpyDeviceToHost);
+    }
+    if (e == hipSuccess) e = hipMemcpy(nj, dn.ptr, 4 * (size_t)n_events, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(FJMI_EHIP, "download: %s", hipGetErrorString(e));
+  }
+  rel();
+  return rc;
+}
+
+}  // extern "C"

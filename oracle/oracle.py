@@ -1,0 +1,102 @@
+"""ctypes binding of the CPU oracle (oracle/_build/libfj_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker.  The product never imports it.
+Build with `make -C oracle` (also done by `python -m fastjet_gpu_amd.build`).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PATH = os.path.join(_HERE, "_build", "libfj_oracle.so")
+_lib = None
+_vp = ctypes.c_void_p
+
+MATH_SHARED, MATH_LIBM = 0, 1
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_PATH):
+            raise RuntimeError(f"{_PATH} missing; run `make -C oracle`")
+        L = ctypes.CDLL(_PATH)
+        for name in ("fjo_cluster_n2plain", "fjo_cluster_brute", "fjo_cluster_pairwise"):
+            f = getattr(L, name)
+            f.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
+            f.restype = ctypes.c_int
+        L.fjo_log.argtypes = [ctypes.c_double]
+        L.fjo_log.restype = ctypes.c_double
+        L.fjo_atan2.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.fjo_atan2.restype = ctypes.c_double
+        L.fjo_set_jet.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
+        L.fjo_set_jet.restype = None
+        L.fjo_time_batch.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.fjo_time_batch.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _run(fname, p4, p, R, math_mode):
+    p4 = np.ascontiguousarray(p4, dtype=np.float64).reshape(-1, 4)
+    n = len(p4)
+    h1 = np.zeros(max(n, 1), np.int32)
+    h2 = np.zeros(max(n, 1), np.int32)
+    hd = np.zeros(max(n, 1), np.float64)
+    jets = np.zeros((max(n, 1), 4), np.float64)
+    nj = np.zeros(1, np.int32)
+    P = lambda a: a.ctypes.data_as(_vp)
+    rc = getattr(lib(), fname)(P(p4), n, int(p), float(R), math_mode, P(h1), P(h2), P(hd), P(jets), P(nj))
+    if rc != 0:
+        raise ValueError(f"{fname} returned {rc}")
+    return h1[:n], h2[:n], hd[:n], jets[:nj[0]].copy()
+
+
+def n2plain(p4, p, R, math_mode=MATH_SHARED):
+    """FastJet N2Plain restatement: (hist_p1, hist_p2, hist_dij, jets)."""
+    return _run("fjo_cluster_n2plain", p4, p, R, math_mode)
+
+
+def brute(p4, p, R, math_mode=MATH_SHARED):
+    return _run("fjo_cluster_brute", p4, p, R, math_mode)
+
+
+def pairwise(p4, p, R, math_mode=MATH_SHARED):
+    return _run("fjo_cluster_pairwise", p4, p, R, math_mode)
+
+
+def time_batch(p4, offsets, p, R, nthreads):
+    p4 = np.ascontiguousarray(p4, dtype=np.float64)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    return lib().fjo_time_batch(p4.ctypes.data_as(_vp), offsets.ctypes.data_as(_vp), len(offsets) - 1, int(p),
+                                float(R), int(nthreads))
+
+
+def set_jet(p4row, p, math_mode=MATH_SHARED):
+    q = np.ascontiguousarray(p4row, dtype=np.float64)
+    r, f, w = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    lib().fjo_set_jet(q.ctypes.data_as(_vp), int(p), math_mode, ctypes.byref(r), ctypes.byref(f), ctypes.byref(w))
+    return r.value, f.value, w.value
+
+
+def read_events(path):
+    """Events of a reference-format text file (main.cc:34-70): 'px py pz E' lines;
+    a '#' or blank line ends an event."""
+    events, cur = [], []
+    with open(path) as fh:
+        for line in fh:
+            parts = line.split()
+            if len(parts) >= 4 and not line.startswith("#"):
+                try:
+                    cur.append([float(x) for x in parts[:4]])
+                    continue
+                except ValueError:
+                    pass
+            if cur:
+                events.append(np.array(cur, np.float64))
+                cur = []
+    if cur:
+        events.append(np.array(cur, np.float64))
+    return events

@@ -1,0 +1,167 @@
+"""GPU parity: the HIP dense engine against the CPU oracle, through the C ABI.
+
+Bar: bit-exact merge history (parents and dij, compared as raw bits) and
+bit-exact jets, on the reference's own input files (tests/golden/*.dat), on the
+seeded workloads and on edge cases.  Every test prints the case it runs before
+touching the GPU.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.conftest import GOLDEN
+
+ALGOS = [(1, "kt"), (0, "cam"), (-1, "antikt")]
+
+
+def same(a, b):
+    """Bitwise equality of two oracle/engine results (hist_p1, hist_p2, hist_dij, jets)."""
+    h1, h2, hd, j = a
+    g1, g2, gd, gj = b
+    return (np.array_equal(h1, g1) and np.array_equal(h2, g2) and hd.tobytes() == gd.tobytes()
+            and np.asarray(j).tobytes() == np.asarray(gj).tobytes())
+
+
+def first_diff(a, b):
+    for s in range(len(a[0])):
+        if a[0][s] != b[0][s] or a[1][s] != b[1][s] or a[2][s].tobytes() != b[2][s].tobytes():
+            return s, (a[0][s], a[1][s], a[2][s]), (b[0][s], b[1][s], b[2][s])
+    return None
+
+
+def check_event(engine, p4, p, R, label):
+    print(f"[gpu parity] {label} N={len(p4)} p={p} R={R}", flush=True)
+    want = orc.n2plain(p4, p, R)
+    got = engine.cluster_event(p4, p, R)
+    assert same(want, got), f"{label}: first differing step {first_diff(want, got)}"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fname", ["354_fastjet.dat", "808_cms.dat"])
+@pytest.mark.parametrize("p,alg", ALGOS)
+@pytest.mark.parametrize("R", [0.4, 1.0])
+def test_reference_files(engine, fname, p, alg, R):
+    ev = orc.read_events(os.path.join(GOLDEN, fname))[0]
+    check_event(engine, ev, p, R, f"{fname}/{alg}")
+
+
+@pytest.mark.gpu
+def test_multi_event_file(engine):
+    events = orc.read_events(os.path.join(GOLDEN, "multi-event.dat"))
+    assert [len(e) for e in events] == [354, 713, 676, 552, 520, 294, 582, 515, 417, 837]
+    for k, ev in enumerate(events):
+        for p, alg in ALGOS:
+            check_event(engine, ev, p, 0.4, f"multi-event[{k}]/{alg}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("workload", [1, 2])
+@pytest.mark.parametrize("p,alg", ALGOS)
+def test_generated_single(engine, workload, p, alg):
+    import fastjet_gpu_amd as fj
+    ev = fj.gen_event(workload, 0)
+    for R in (0.4, 1.0):
+        check_event(engine, ev, p, R, f"workload{workload}/{alg}")
+
+
+@pytest.mark.gpu
+def test_every_bucket_size(engine):
+    """One event per multiplicity bucket boundary (and just above it)."""
+    import fastjet_gpu_amd as fj
+    rng = np.random.default_rng(7)
+    big = fj.gen_event(4, 3)
+    for (max_n, threads, slots, g) in engine.buckets():
+        for n in sorted({max(1, max_n - 1), max_n, max_n + 1}):
+            if n > len(big) or n > engine.buckets()[-1][0]:
+                continue
+            sel = big[rng.permutation(len(big))[:n]]
+            check_event(engine, sel, -1, 0.4, f"bucket<= {max_n} ({threads}x{slots},g{g})")
+
+
+@pytest.mark.gpu
+def test_edge_cases(engine):
+    one = np.array([[1.0, 2.0, 3.0, 10.0]])
+    check_event(engine, one, 1, 0.4, "single particle")
+    two = np.array([[1.0, 0.0, 0.0, 1.0], [1.0, 0.01, 0.0, 1.0001]])
+    for p, alg in ALGOS:
+        check_event(engine, two, p, 0.4, f"two particles/{alg}")
+    # exact duplicates: many exact distance ties and orphans
+    base = np.array([[3.0, 1.0, 0.5, 4.0], [0.5, 2.0, -1.0, 3.0], [-1.0, -1.0, 0.2, 2.0]])
+    dup = np.repeat(base, 40, axis=0)
+    for p, alg in ALGOS:
+        check_event(engine, dup, p, 0.7, f"duplicates/{alg}")
+    # zero-pt and beam-like momenta (rapidity +-(1e5+|pz|))
+    zp = np.array([[0.0, 0.0, 5.0, 5.0], [0.0, 0.0, -7.0, 7.0], [1.0, 1.0, 0.0, 2.0], [0.0, 0.0, 1.0, 2.0],
+                   [1e-3, 0.0, 0.0, 1e-3], [0.0, 0.0, 0.0, 0.0]])
+    for p, alg in ALGOS:
+        check_event(engine, zp, p, 0.4, f"zero pt/{alg}")
+    # a regular lattice: many exact geometric ties
+    g = []
+    for i in range(12):
+        for j in range(12):
+            eta, phi = -1.0 + 0.25 * i, 0.25 * j
+            pt = 1.0 + 0.125 * ((i * 7 + j * 3) % 5)
+            g.append([pt * np.cos(phi), pt * np.sin(phi), pt * np.sinh(eta), pt * np.cosh(eta)])
+    for p, alg in ALGOS:
+        check_event(engine, np.array(g), p, 0.3, f"lattice/{alg}")
+
+
+@pytest.mark.gpu
+def test_empty_event(engine):
+    h1, h2, hd, jets = engine.cluster_event(np.zeros((0, 4)), -1, 0.4)
+    assert len(h1) == 0 and len(jets) == 0
+
+
+@pytest.mark.gpu
+def test_batch_matches_oracle(engine):
+    """Mixed multiplicities, one graph launch, per-event algorithm and radius."""
+    import fastjet_gpu_amd as fj
+    p4, off = fj.gen_workload(3, 96)
+    ne = len(off) - 1
+    ps = np.array([(-1, 0, 1)[e % 3] for e in range(ne)], np.int32)
+    Rs = np.array([(0.4, 0.8, 1.0, 0.6)[e % 4] for e in range(ne)], np.float64)
+    print(f"[gpu parity] batch workload3 x{ne} events, total {len(p4)} particles", flush=True)
+    h1, h2, hd, jets, nj = engine.cluster_batch(p4, off, ps, Rs)
+    for e in range(ne):
+        a, b = off[e], off[e + 1]
+        want = orc.n2plain(p4[a:b], int(ps[e]), float(Rs[e]))
+        got = (h1[a:b], h2[a:b], hd[a:b], jets[a:a + nj[e]])
+        assert same(want, got), f"batch event {e} (N={b - a}): {first_diff(want, got)}"
+
+
+@pytest.mark.gpu
+def test_batch_alone_equals_in_mix(engine):
+    """Each event's result is the same clustered alone and inside a batch."""
+    import fastjet_gpu_amd as fj
+    p4, off = fj.gen_workload(3, 24, first=500)
+    print(f"[gpu parity] batch vs alone, {len(off) - 1} events", flush=True)
+    h1, h2, hd, jets, nj = engine.cluster_batch(p4, off, -1, 0.4)
+    for e in range(len(off) - 1):
+        a, b = off[e], off[e + 1]
+        alone = engine.cluster_event(p4[a:b], -1, 0.4)
+        assert same(alone, (h1[a:b], h2[a:b], hd[a:b], jets[a:a + nj[e]])), f"event {e}"
+
+
+@pytest.mark.gpu
+def test_pseudojet_dropin_layout(engine):
+    """cluster.h contract: in place, isJet slots hold the jets, jet promoted with n live sits in slot n-1."""
+    import torch
+    ev = orc.read_events(os.path.join(GOLDEN, "808_cms.dat"))[0]
+    n = len(ev)
+    rec = np.zeros(n, dtype=[("index", "<i4"), ("isJet", "u1"), ("pad", "u1", 3), ("p4", "<f8", 4)])
+    assert rec.itemsize == 40
+    rec["index"] = np.arange(n)
+    rec["p4"] = ev
+    raw = torch.from_numpy(rec.view(np.uint8).copy()).cuda()
+    print(f"[gpu parity] drop-in PseudoJet N={n}", flush=True)
+    engine.cluster_pseudojets(raw.data_ptr(), n, -1, 0.4)
+    torch.cuda.synchronize()
+    out = raw.cpu().numpy().view(rec.dtype)
+    h1, h2, hd, jets = orc.n2plain(ev, -1, 0.4)
+    assert np.array_equal(out["index"], np.arange(n))
+    jet_slots = [n - 1 - s for s in range(n) if h2[s] == -1]  # beam step s had n-s live jets
+    assert sorted(np.nonzero(out["isJet"])[0].tolist()) == sorted(jet_slots)
+    for j, slot in enumerate(jet_slots):
+        assert out["p4"][slot].tobytes() == jets[j].tobytes()
