@@ -35,6 +35,11 @@ class Bucket(ctypes.Structure):
                 ("p4_global", ctypes.c_int32)]
 
 
+class BucketStats(ctypes.Structure):
+    _fields_ = [("bucket", Bucket), ("n_events", ctypes.c_int32), ("grid", ctypes.c_int32),
+                ("particles", ctypes.c_int64), ("sum_n2", ctypes.c_double), ("ms", ctypes.c_float)]
+
+
 def lib_path(name="libfjmi.so"):
     return os.path.join(_LIB, name)
 
@@ -83,6 +88,7 @@ def _declare(lib):
         "fjmi_batch_launch": ([_vp, _vp], i),
         "fjmi_batch_sync": ([_vp, _vp], i),
         "fjmi_batch_destroy": ([_vp], i),
+        "fjmi_batch_stats": ([_vp, _vp, i, _c_int_p], i),
         "fjmi_cluster_batch": ([_vp, i, _vp, _vp, _vp, _vp, i, d, _vp, _vp, _vp, _vp, _vp], i),
     }
     for name, (args, res) in sig.items():
@@ -202,6 +208,20 @@ class Batch:
 
     def sync(self, stream=0):
         _check(self.eng.lib, self.eng.lib.fjmi_batch_sync(self.h, stream or None), "fjmi_batch_sync")
+
+    def stats(self):
+        """Per-bucket dicts: max_n, threads, slots, p4_global, n_events, grid, particles, sum_n2, ms."""
+        arr = (BucketStats * 64)()
+        n = ctypes.c_int32()
+        _check(self.eng.lib, self.eng.lib.fjmi_batch_stats(self.h, ctypes.cast(arr, _vp), 64, ctypes.byref(n)),
+               "fjmi_batch_stats")
+        out = []
+        for k in range(n.value):
+            b = arr[k]
+            out.append(dict(max_n=b.bucket.max_n, threads=b.bucket.threads, slots=b.bucket.slots,
+                            p4_global=b.bucket.p4_global, n_events=b.n_events, grid=b.grid,
+                            particles=b.particles, sum_n2=b.sum_n2, ms=b.ms))
+        return out
 
     def close(self):
         if self.h:

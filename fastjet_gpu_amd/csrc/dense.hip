@@ -31,9 +31,10 @@
  *     global memory and LDS only stages coordinates for the initial NN pass);
  *   - every argmin is a 64-lane DPP reduction (value, then lowest slot among the
  *     minima), combined across waves through LDS;
- *   - the merge (including atan2/log of the new jet) is computed redundantly by
- *     every lane from broadcast LDS reads, so no lane waits for a single writer.
- * Per step: one barrier after the merge reads, one after the update pass, and
+ *   - the merge (including atan2/log of the new jet) is computed by wave 0 only
+ *     and broadcast through LDS; per-slot updates are branch-free selects, so the
+ *     register arrays are never copied at control-flow joins.
+ * Per step: one barrier after the merge (multi-wave groups), one after the update pass, and
  * one per round of up to MO orphan rescans (only when jets lost their NN).
  */
 #include <hip/hip_runtime.h>
@@ -82,6 +83,7 @@ struct Shared {
   Orph orph[2][OCH];
   int norph[2];
   Mail mail;
+  double4 newc; /* (y, phi, w) of the jet made by the current step */
   int ev;
 };
 
@@ -263,14 +265,13 @@ __global__ __launch_bounds__(T, OCC) void dense_kernel(LaunchArgs a) {
         const int sj = c0 + j;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-          if (k * T >= N) break;
+          if (k * T >= N) continue;
           const int s = k * T + t;
           const double d = fjmi_dr2(rap[k], phi[k], q.x, q.y);
-          if (s < N && s != sj && d < nnd[k]) {
-            nnd[k] = d;
-            nn[k] = sj;
-            wnn[k] = q.z;
-          }
+          const bool c = s < N && s != sj && d < nnd[k];
+          nnd[k] = c ? d : nnd[k];
+          nn[k] = c ? sj : nn[k];
+          wnn[k] = c ? q.z : wnn[k];
         }
       }
     }
@@ -318,27 +319,32 @@ __global__ __launch_bounds__(T, OCC) void dense_kernel(LaunchArgs a) {
       const int tail = n_old - 1;
       const int par = step & 1;
 
-      const double4 qhi = P4G ? gp4[hi] : lp4[hi];
-      const double4 qtail = P4G ? gp4[tail] : lp4[tail];
-      const int hhi = lhist[hi];
-      const int htail = lhist[tail];
-      double4 qn = make_double4(0.0, 0.0, 0.0, 0.0);
+      /* The merge: wave 0 reads the two four-momenta, records the step and
+       * computes the new jet's (y, phi, w) once; the other waves pick the three
+       * numbers up from LDS after the barrier.  Only wave 0 reads the four-momenta
+       * and history indices in LDS, so lane 0 updates them in program order. */
       double nrap = 0.0, nphi = 0.0, nw = 0.0;
-      if (lo >= 0) {
-        const double4 qlo = P4G ? gp4[lo] : lp4[lo];
-        const int hlo = lhist[lo];
-        qn = make_double4(qhi.x + qlo.x, qhi.y + qlo.y, qhi.z + qlo.z, qhi.w + qlo.w);
-        const double4 c = kin_coords(qn.x, qn.y, qn.z, qn.w, p);
-        nrap = c.x;
-        nphi = c.y;
-        nw = c.z;
-        if (t == 0) {
-          a.hist_p1[base + step] = min(hhi, hlo);
-          a.hist_p2[base + step] = max(hhi, hlo);
-          a.hist_dij[base + step] = dmin * invR2;
-        }
-      } else {
-        if (t == 0) {
+      if (wv == 0) {
+        const double4 qhi = P4G ? gp4[hi] : lp4[hi];
+        const double4 qtail = P4G ? gp4[tail] : lp4[tail];
+        const int hhi = lhist[hi];
+        const int htail = lhist[tail];
+        double4 qn = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (lo >= 0) {
+          const double4 qlo = P4G ? gp4[lo] : lp4[lo];
+          const int hlo = lhist[lo];
+          qn = make_double4(qhi.x + qlo.x, qhi.y + qlo.y, qhi.z + qlo.z, qhi.w + qlo.w);
+          const double4 c = kin_coords(qn.x, qn.y, qn.z, qn.w, p);
+          nrap = c.x;
+          nphi = c.y;
+          nw = c.z;
+          if (t == 0) {
+            a.hist_p1[base + step] = min(hhi, hlo);
+            a.hist_p2[base + step] = max(hhi, hlo);
+            a.hist_dij[base + step] = dmin * invR2;
+            if (W > 1) sh.newc = c;
+          }
+        } else if (t == 0) {
           a.hist_p1[base + step] = hhi;
           a.hist_p2[base + step] = -1;
           a.hist_dij[base + step] = dmin * invR2;
@@ -348,25 +354,32 @@ __global__ __launch_bounds__(T, OCC) void dense_kernel(LaunchArgs a) {
           jo[2] = qhi.z;
           jo[3] = qhi.w;
         }
-        ++nj;
-      }
-      if (a.pj_out && t == 0)
-        write_pseudojet(a.pj_out + 40 * (size_t)(base + tail), tail, lo < 0, lo < 0 ? qhi : qtail);
-      __syncthreads(); /* every lane has read the merge inputs */
-      if (t == 0) {
-        if (lo >= 0) {
-          if (P4G)
-            gp4[lo] = qn;
-          else
-            lp4[lo] = qn;
-          lhist[lo] = N + step;
+        if (t == 0) {
+          if (a.pj_out) write_pseudojet(a.pj_out + 40 * (size_t)(base + tail), tail, lo < 0, lo < 0 ? qhi : qtail);
+          if (lo >= 0) {
+            if (P4G)
+              gp4[lo] = qn;
+            else
+              lp4[lo] = qn;
+            lhist[lo] = N + step;
+          }
+          if (hi != tail) {
+            if (P4G)
+              gp4[hi] = qtail;
+            else
+              lp4[hi] = qtail;
+            lhist[hi] = htail;
+          }
         }
-        if (hi != tail) {
-          if (P4G)
-            gp4[hi] = qtail;
-          else
-            lp4[hi] = qtail;
-          lhist[hi] = htail;
+      }
+      if (lo < 0) ++nj;
+      if (W > 1) {
+        __syncthreads(); /* the new jet's coordinates are in LDS */
+        if (wv != 0 && lo >= 0) {
+          const double4 c = sh.newc;
+          nrap = c.x;
+          nphi = c.y;
+          nw = c.z;
         }
       }
       const int n = n_old - 1;
@@ -379,68 +392,61 @@ __global__ __launch_bounds__(T, OCC) void dense_kernel(LaunchArgs a) {
       uS = BIG;
       uNN = -1;
       unsigned pend = 0;
+      /* Branch-free per slot (selects only): divergent branches around updates
+       * of the register arrays make the compiler copy whole arrays at the joins. */
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        if (k * T >= n_old) break;
+        if (k * T >= n_old) continue; /* uniform */
         const int s = k * T + t;
-        if (s < n_old && s != hi) {
-          const int label = (s == tail) ? hi : s;
-          if (label == lo) {
-            rap[k] = nrap;
-            phi[k] = nphi;
-            w[k] = nw;
-          } else {
-            int m = nn[k];
-            const bool orph = (m == hi) || (lo >= 0 && m == lo);
-            if (!orph && m == tail) m = hi;
-            if (lo >= 0) {
-              const double d = fjmi_dr2(rap[k], phi[k], nrap, nphi);
-              if (!orph && d < nnd[k]) {
-                nnd[k] = d;
-                m = lo;
-                dij[k] = d * (nw < w[k] ? nw : w[k]);
-              }
-              if (d < R2 && lex_less(d, label, bD, bS)) {
-                bD = d;
-                bS = label;
-                bW = w[k];
-              }
-            }
-            nn[k] = m;
-            bool pending = false;
-            if (orph) {
-              const int idx = atomicAdd(&sh.norph[par], 1);
-              if (idx < OCH) {
-                Orph o;
-                o.rap = rap[k];
-                o.phi = phi[k];
-                o.w = w[k];
-                o.label = label;
-                o.pad = 0;
-                sh.orph[par][idx] = o;
-              } else {
-                pending = true;
-              }
-            } else if (lex_less(dij[k], label, uD, uS)) {
-              uD = dij[k];
-              uS = label;
-              uNN = m;
-            }
-            if (s == tail) { /* hi != tail here: hand the moved jet to slot hi's owner */
-              Mail ml;
-              ml.rap = rap[k];
-              ml.phi = phi[k];
-              ml.w = w[k];
-              ml.nnd = nnd[k];
-              ml.dij = dij[k];
-              ml.nn = m;
-              ml.pend = pending ? 1 : 0;
-              sh.mail = ml;
-            } else if (pending) {
-              pend |= 1u << k;
-            }
-          }
+        const bool live = s < n_old && s != hi;
+        const int label = (s == tail) ? hi : s;
+        const bool isB = live && label == lo;
+        const bool normal = live && label != lo;
+        rap[k] = isB ? nrap : rap[k];
+        phi[k] = isB ? nphi : phi[k];
+        w[k] = isB ? nw : w[k];
+        int m = nn[k];
+        const bool orph = normal && ((m == hi) || (lo >= 0 && m == lo));
+        m = (normal && !orph && m == tail) ? hi : m;
+        const double d = fjmi_dr2(rap[k], phi[k], nrap, nphi);
+        const bool closer = normal && lo >= 0 && !orph && d < nnd[k];
+        const double wmin = nw < w[k] ? nw : w[k];
+        dij[k] = closer ? d * wmin : dij[k];
+        nnd[k] = closer ? d : nnd[k];
+        m = closer ? lo : m;
+        nn[k] = m;
+        const bool bc = normal && lo >= 0 && d < R2 && lex_less(d, label, bD, bS);
+        bD = bc ? d : bD;
+        bS = bc ? label : bS;
+        bW = bc ? w[k] : bW;
+        const bool uc = normal && !orph && lex_less(dij[k], label, uD, uS);
+        uD = uc ? dij[k] : uD;
+        uS = uc ? label : uS;
+        uNN = uc ? m : uNN;
+        int idx = 0;
+        if (orph) idx = atomicAdd(&sh.norph[par], 1);
+        if (orph && idx < OCH) {
+          Orph o;
+          o.rap = rap[k];
+          o.phi = phi[k];
+          o.w = w[k];
+          o.label = label;
+          o.pad = 0;
+          sh.orph[par][idx] = o;
         }
+        const bool pending = orph && idx >= OCH;
+        if (normal && s == tail) { /* hi != tail here: hand the moved jet to slot hi's owner */
+          Mail ml;
+          ml.rap = rap[k];
+          ml.phi = phi[k];
+          ml.w = w[k];
+          ml.nnd = nnd[k];
+          ml.dij = dij[k];
+          ml.nn = m;
+          ml.pend = pending ? 1 : 0;
+          sh.mail = ml;
+        }
+        pend |= (pending && s != tail) ? (1u << k) : 0u;
       }
       Red ru = wave_argmin(uD, uS, uNN, 0.0);
       Red rb = wave_argmin(bD, bS, 0, bW);
@@ -457,19 +463,20 @@ __global__ __launch_bounds__(T, OCC) void dense_kernel(LaunchArgs a) {
       int norph = uni(sh.norph[par]);
 
       /* the moved jet changes owner */
-      if (hi != tail && (hi % T) == t) {
+      if (hi != tail) { /* uniform */
         const int kh = hi / T;
+        const bool own = (hi % T) == t;
         const Mail ml = sh.mail;
 #pragma unroll
         for (int k = 0; k < K; ++k)
-          if (k == kh) {
-            rap[k] = ml.rap;
-            phi[k] = ml.phi;
-            w[k] = ml.w;
-            nnd[k] = ml.nnd;
-            dij[k] = ml.dij;
-            nn[k] = ml.nn;
-            if (ml.pend) pend |= 1u << k;
+          if (k == kh) { /* uniform */
+            rap[k] = own ? ml.rap : rap[k];
+            phi[k] = own ? ml.phi : phi[k];
+            w[k] = own ? ml.w : w[k];
+            nnd[k] = own ? ml.nnd : nnd[k];
+            dij[k] = own ? ml.dij : dij[k];
+            nn[k] = own ? ml.nn : nn[k];
+            pend |= (own && ml.pend) ? (1u << k) : 0u;
           }
       }
 
@@ -480,16 +487,15 @@ __global__ __launch_bounds__(T, OCC) void dense_kernel(LaunchArgs a) {
         const int nnB = rb.s == BIG ? -1 : rb.s;
         const double nndB = rb.d;
         const double dijB = nndB * ((nnB >= 0 && rb.pd < nw) ? rb.pd : nw);
-        if (lo % T == t) {
-          const int kl = lo / T;
+        const int kl = lo / T;
+        const bool own = (lo % T) == t;
 #pragma unroll
-          for (int k = 0; k < K; ++k)
-            if (k == kl) {
-              nn[k] = nnB;
-              nnd[k] = nndB;
-              dij[k] = dijB;
-            }
-        }
+        for (int k = 0; k < K; ++k)
+          if (k == kl) { /* uniform */
+            nn[k] = own ? nnB : nn[k];
+            nnd[k] = own ? nndB : nnd[k];
+            dij[k] = own ? dijB : dij[k];
+          }
         if (lex_less(dijB, lo, bd, bs)) {
           bd = dijB;
           bs = lo;
@@ -517,18 +523,15 @@ __global__ __launch_bounds__(T, OCC) void dense_kernel(LaunchArgs a) {
           }
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            if (k * T >= n) break;
+            if (k * T >= n) continue;
             const int s = k * T + t;
-            if (s < n) {
 #pragma unroll
-              for (int j = 0; j < MO; ++j) {
-                const double d = fjmi_dr2(rap[k], phi[k], orap[j], ophi[j]);
-                if (s != olab[j] && d < R2 && lex_less(d, s, od[j], os[j])) {
-                  od[j] = d;
-                  os[j] = s;
-                  ow[j] = w[k];
-                }
-              }
+            for (int j = 0; j < MO; ++j) {
+              const double d = fjmi_dr2(rap[k], phi[k], orap[j], ophi[j]);
+              const bool c = s < n && s != olab[j] && d < R2 && lex_less(d, s, od[j], os[j]);
+              od[j] = c ? d : od[j];
+              os[j] = c ? s : os[j];
+              ow[j] = c ? w[k] : ow[j];
             }
           }
           Red ro[MO];
@@ -553,16 +556,15 @@ __global__ __launch_bounds__(T, OCC) void dense_kernel(LaunchArgs a) {
               const double nndo = ro[j].d;
               const double wo = sh.orph[par][o0 + j].w;
               const double dijo = nndo * ((nno >= 0 && ro[j].pd < wo) ? ro[j].pd : wo);
-              if (lab % T == t) {
-                const int kk = lab / T;
+              const int kk = lab / T;
+              const bool own = (lab % T) == t;
 #pragma unroll
-                for (int k = 0; k < K; ++k)
-                  if (k == kk) {
-                    nn[k] = nno;
-                    nnd[k] = nndo;
-                    dij[k] = dijo;
-                  }
-              }
+              for (int k = 0; k < K; ++k)
+                if (k == kk) { /* uniform */
+                  nn[k] = own ? nno : nn[k];
+                  nnd[k] = own ? nndo : nnd[k];
+                  dij[k] = own ? dijo : dij[k];
+                }
               if (lex_less(dijo, lab, bd, bs)) {
                 bd = dijo;
                 bs = lab;

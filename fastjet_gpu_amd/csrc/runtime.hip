@@ -118,6 +118,9 @@ struct BucketRun {
   size_t lds;
   LaunchArgs args;
   int n_events;
+  int max_n;
+  int64_t particles;
+  double sum_n2;
 };
 
 struct fjmi_batch {
@@ -128,6 +131,8 @@ struct fjmi_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint32_t *d_status = nullptr;
+  /* timing events recorded inside the graph: [0] after the queue reset, [i+1] after bucket i */
+  std::vector<hipEvent_t> tev;
 };
 
 namespace {
@@ -469,6 +474,14 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, r.kd->fn, r.kd->T, r.lds);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     r.n_events = (int)lists[b].size();
+    r.max_n = bk.max_n;
+    r.particles = 0;
+    r.sum_n2 = 0.0;
+    for (int32_t ev : lists[b]) {
+      const double nev = (double)(h_offsets[ev + 1] - h_offsets[ev]);
+      r.particles += (int64_t)nev;
+      r.sum_n2 += nev * nev;
+    }
     r.grid = (int)std::min<int64_t>(r.n_events, (int64_t)per_cu * ctx->n_cu);
     LaunchArgs a = base_args();
     a.in = d_p4;
@@ -496,31 +509,29 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
    * its own forked stream so the buckets run concurrently */
   hipStream_t s0 = nullptr;
   std::vector<hipStream_t> ss(bt->runs.size(), nullptr);
-  std::vector<hipEvent_t> evs(bt->runs.size() + 1, nullptr);
+  bt->tev.assign(bt->runs.size() + 1, nullptr);
   hipError_t e = hipStreamCreateWithFlags(&s0, hipStreamNonBlocking);
   for (size_t i = 0; e == hipSuccess && i < ss.size(); ++i) e = hipStreamCreateWithFlags(&ss[i], hipStreamNonBlocking);
-  for (size_t i = 0; e == hipSuccess && i < evs.size(); ++i) e = hipEventCreateWithFlags(&evs[i], hipEventDisableTiming);
+  for (size_t i = 0; e == hipSuccess && i < bt->tev.size(); ++i) e = hipEventCreate(&bt->tev[i]);
   if (e == hipSuccess) e = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
     unsigned *hp = heads;
     int cnt = nb + 1;
     void *zp[] = {&hp, &cnt};
     e = hipLaunchKernel(fjmi::kZeroKernel, dim3(1), dim3(64), zp, 0, s0);
-    if (e == hipSuccess) e = hipEventRecord(evs[0], s0);
+    if (e == hipSuccess) e = hipEventRecord(bt->tev[0], s0);
     for (size_t i = 0; e == hipSuccess && i < bt->runs.size(); ++i) {
-      e = hipStreamWaitEvent(ss[i], evs[0], 0);
+      e = hipStreamWaitEvent(ss[i], bt->tev[0], 0);
       BucketRun &r = bt->runs[i];
       void *params[] = {&r.args};
       if (e == hipSuccess) e = hipLaunchKernel(r.kd->fn, dim3(r.grid), dim3(r.kd->T), params, r.lds, ss[i]);
-      if (e == hipSuccess) e = hipEventRecord(evs[i + 1], ss[i]);
-      if (e == hipSuccess) e = hipStreamWaitEvent(s0, evs[i + 1], 0);
+      if (e == hipSuccess) e = hipEventRecord(bt->tev[i + 1], ss[i]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s0, bt->tev[i + 1], 0);
     }
     hipError_t e2 = hipStreamEndCapture(s0, &bt->graph);
     if (e == hipSuccess) e = e2;
   }
   if (e == hipSuccess) e = hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0);
-  for (hipEvent_t ev : evs)
-    if (ev) (void)hipEventDestroy(ev);
   for (hipStream_t s : ss)
     if (s) (void)hipStreamDestroy(s);
   if (s0) (void)hipStreamDestroy(s0);
@@ -546,12 +557,36 @@ int fjmi_batch_sync(fjmi_batch *b, void *stream) {
   return FJMI_OK;
 }
 
+int fjmi_batch_stats(fjmi_batch *b, fjmi_bucket_stats *out, int cap, int *n) {
+  if (!b || !n) return fail(FJMI_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  *n = (int)b->runs.size();
+  for (int i = 0; i < *n && i < cap; ++i) {
+    const BucketRun &r = b->runs[i];
+    fjmi_bucket_stats &o = out[i];
+    o.bucket.max_n = r.max_n;
+    o.bucket.threads = r.kd->T;
+    o.bucket.slots = r.kd->K;
+    o.bucket.p4_global = r.kd->p4g ? 1 : 0;
+    o.n_events = r.n_events;
+    o.grid = r.grid;
+    o.particles = r.particles;
+    o.sum_n2 = r.sum_n2;
+    float ms = -1.0f;
+    if (hipEventElapsedTime(&ms, b->tev[0], b->tev[i + 1]) != hipSuccess) ms = -1.0f;
+    o.ms = ms;
+  }
+  return FJMI_OK;
+}
+
 int fjmi_batch_destroy(fjmi_batch *b) {
   if (!b) return FJMI_OK;
   (void)hipSetDevice(b->ctx->device);
   (void)hipDeviceSynchronize();
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (hipEvent_t ev : b->tev)
+    if (ev) (void)hipEventDestroy(ev);
   b->queue.release();
   b->heads.release();
   b->p4s.release();

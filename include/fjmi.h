@@ -116,6 +116,19 @@ int fjmi_batch_launch(fjmi_batch *b, void *stream);
 int fjmi_batch_sync(fjmi_batch *b, void *stream);
 int fjmi_batch_destroy(fjmi_batch *b);
 
+/* Per-bucket statistics of a batch plan, with each bucket kernel's completion
+ * time (ms after the graph's queue reset) in the last synchronised launch.
+ * Not in the reference (its CLI times a whole cluster() call, main.cc:281-306). */
+typedef struct {
+  fjmi_bucket bucket; /* the table row (max_n) and the kernel configuration */
+  int32_t n_events;
+  int32_t grid; /* persistent workgroups launched */
+  int64_t particles;
+  double sum_n2; /* sum over the bucket's events of N^2 */
+  float ms;
+} fjmi_bucket_stats;
+int fjmi_batch_stats(fjmi_batch *b, fjmi_bucket_stats *out, int cap, int *n_buckets);
+
 /* Host-memory convenience for a batch: copies in, plans, launches, copies out,
  * synchronises.  Per-particle outputs are sized like p4/4 (total particles). */
 int fjmi_cluster_batch(fjmi_context *ctx, int n_events, const int64_t *offsets, const double *p4, const int32_t *p,

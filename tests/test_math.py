@@ -1,0 +1,158 @@
+"""Pins the shared kinematics (fastjet_gpu_amd/csrc/fjmi_math.h) that both the HIP
+kernels and the oracle use.
+
+- log and atan2 against correctly rounded values from high-precision decimal
+  arithmetic (independent of every C library);
+- their agreement rate with glibc (which FastJet uses) is measured and bounded;
+- special values follow C99 Annex F;
+- rapidity/phi conventions of FastJet's PseudoJet as restated by the reference
+  (tri_matrix.cu:82-121): closed-form cases, symmetries, zero-pt handling.
+"""
+import math
+import random
+from decimal import Decimal as D, getcontext
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+getcontext().prec = 60
+
+
+def dec_atan(x):
+    x = D(x)
+    n = 0
+    while abs(x) > D("0.001"):
+        x = x / (1 + (1 + x * x).sqrt())
+        n += 1
+    s, t, k, x2 = D(0), x, 0, x * x
+    while True:
+        term = t / (2 * k + 1)
+        if abs(term) < D(10) ** -58:
+            break
+        s += term if k % 2 == 0 else -term
+        t *= x2
+        k += 1
+    return s * (2 ** n)
+
+
+PI = dec_atan(1) * 4
+
+
+def dec_atan2(y, x):
+    y, x = D(y), D(x)
+    if x > 0:
+        return dec_atan(y / x)
+    if x == 0:
+        return PI / 2 if y > 0 else -PI / 2
+    return dec_atan(y / x) + (PI if y >= 0 else -PI)
+
+
+def test_log_correctly_rounded():
+    rnd = random.Random(11)
+    L = orc.lib()
+    bad = 0
+    xs = [rnd.uniform(1e-6, 1.0) for _ in range(3000)] + [math.exp(rnd.uniform(-700, 700)) for _ in range(2000)]
+    xs += [1 + rnd.uniform(-1e-6, 1e-6) for _ in range(500)] + [5e-324, 2.2250738585072014e-308, 1.7976931348623157e308]
+    for x in xs:
+        if L.fjo_log(x) != float(D(x).ln()):
+            bad += 1
+    assert bad == 0
+
+
+def test_atan2_correctly_rounded():
+    rnd = random.Random(12)
+    L = orc.lib()
+    bad = 0
+    for k in range(3000):
+        y, x = rnd.uniform(-50, 50), rnd.uniform(-50, 50)
+        if k % 5 == 0:
+            y *= 1e-9
+        if k % 7 == 0:
+            x *= 1e-9
+        if L.fjo_atan2(y, x) != float(dec_atan2(y, x)):
+            bad += 1
+    assert bad == 0
+
+
+def test_agreement_with_glibc_is_high():
+    """glibc's log/atan2 are not always correctly rounded; the shared functions are.
+    The two agree on all but a small fraction of inputs (measured ~1e-3)."""
+    rnd = random.Random(13)
+    L = orc.lib()
+    n = 20000
+    dl = sum(L.fjo_log(x) != math.log(x) for x in (rnd.uniform(1e-4, 1) for _ in range(n)))
+    da = sum(L.fjo_atan2(y, x) != math.atan2(y, x)
+             for y, x in ((rnd.uniform(-9, 9), rnd.uniform(-9, 9)) for _ in range(n)))
+    assert dl / n < 5e-3 and da / n < 5e-3, (dl, da)
+
+
+@pytest.mark.parametrize("y,x", [(0.0, 1.0), (-0.0, 1.0), (0.0, -1.0), (-0.0, -1.0), (0.0, 0.0), (-0.0, -0.0),
+                                 (1.0, 0.0), (-1.0, -0.0), (math.inf, 1.0), (1.0, -math.inf), (-math.inf, math.inf),
+                                 (math.inf, -math.inf), (1e-300, 1e300), (-1e300, 1e-300)])
+def test_atan2_special(y, x):
+    got = orc.lib().fjo_atan2(y, x)
+    want = math.atan2(y, x)
+    assert got == want and math.copysign(1, got) == math.copysign(1, want)
+
+
+def test_log_special():
+    L = orc.lib()
+    assert L.fjo_log(0.0) == -math.inf and L.fjo_log(math.inf) == math.inf
+    assert math.isnan(L.fjo_log(-1.0)) and math.isnan(L.fjo_log(math.nan))
+    assert L.fjo_log(1.0) == 0.0
+
+
+def kin(px, py, pz, E, p=1):
+    return orc.set_jet(np.array([px, py, pz, E]), p)
+
+
+def test_phi_conventions():
+    assert kin(1.0, 0.0, 0.0, 1.0)[1] == 0.0            # phi(px>0, py=0) = 0
+    assert kin(-1.0, 0.0, 0.0, 1.0)[1] == math.pi       # phi on the negative x axis
+    assert kin(-1.0, -0.0, 0.0, 1.0)[1] == math.pi      # atan2(-0,-1) = -pi, wrapped to pi
+    assert kin(0.0, 0.0, 1.0, 2.0)[1] == 0.0            # zero pt -> phi 0
+    for a in np.linspace(0.05, 6.2, 40):
+        phi = kin(math.cos(a), math.sin(a), 0.3, 2.0)[1]
+        assert 0.0 <= phi < 2 * math.pi and abs(phi - a) < 1e-14
+        if math.sin(a) < 0:
+            assert phi > math.pi
+
+
+def test_rapidity_conventions():
+    assert kin(1.0, 1.0, 0.0, 3.0)[0] == 0.0            # pz = 0 -> y = 0 (log(1) exactly)
+    rnd = random.Random(5)
+    for _ in range(500):
+        pt, eta = rnd.uniform(0.1, 100), rnd.uniform(-6, 6)
+        phi = rnd.uniform(0, 2 * math.pi)
+        q = (pt * math.cos(phi), pt * math.sin(phi), pt * math.sinh(eta), pt * math.cosh(eta))
+        y = kin(*q)[0]
+        assert abs(y - eta) < 1e-9 * max(1, abs(eta))   # massless: y = eta
+        ym = kin(q[0], q[1], -q[2], q[3])[0]
+        assert ym == -y                                  # y(-pz) = -y(pz), bitwise
+    assert kin(0.0, 0.0, 5.0, 5.0)[0] == 1e5 + 5.0       # E == |pz|, pt = 0
+    assert kin(0.0, 0.0, -7.0, 7.0)[0] == -(1e5 + 7.0)
+    assert kin(0.0, 0.0, 0.0, 0.0)[0] == 1e5
+
+
+def test_weights():
+    assert kin(3.0, 4.0, 0.0, 6.0, 1)[2] == 25.0
+    assert kin(3.0, 4.0, 0.0, 6.0, 0)[2] == 1.0
+    assert kin(3.0, 4.0, 0.0, 6.0, -1)[2] == 1.0 / 25.0
+    assert kin(0.0, 0.0, 1.0, 2.0, -1)[2] == 1e300       # anti-kt guard for kt2 <= 1e-300
+
+
+def test_shared_and_libm_modes_agree_on_inputs():
+    """On the reference's input files the two math modes give the same rap/phi
+    bits for all but a handful of particles (and identical histories; see
+    test_oracle.py)."""
+    import os
+    from tests.conftest import GOLDEN
+    ev = orc.read_events(os.path.join(GOLDEN, "808_cms.dat"))[0]
+    diff = 0
+    for q in ev:
+        a = orc.set_jet(q, -1, orc.MATH_SHARED)
+        b = orc.set_jet(q, -1, orc.MATH_LIBM)
+        diff += a != b
+    assert diff <= 5, diff
