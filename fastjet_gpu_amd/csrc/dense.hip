@@ -132,9 +132,15 @@ __device__ __forceinline__ int wave_min_i(int v) {
 __device__ __forceinline__ Red wave_argmin(double d, int s, int pi, double pd) {
   Red r;
   r.d = wave_min_d(d);
-  const int key = (d == r.d) ? s : BIG;
-  r.s = wave_min_i(key);
-  const unsigned long long bal = __ballot(key == r.s && r.s != BIG);
+  unsigned long long bal = __ballot(d == r.d);
+  if (__popcll(bal) != 1) { /* ties across lanes (rare): lowest slot among the minima */
+    const int key = (d == r.d) ? s : BIG;
+    r.s = wave_min_i(key);
+    bal = __ballot(key == r.s && r.s != BIG);
+  } else {
+    r.s = __builtin_amdgcn_readlane(s, __ffsll(bal) - 1);
+    if (r.s == BIG) bal = 0;
+  }
   if (bal) {
     const int ln = __ffsll(bal) - 1;
     r.pi = __builtin_amdgcn_readlane(pi, ln);
@@ -392,61 +398,89 @@ __global__ __launch_bounds__(T, OCC) void dense_kernel(LaunchArgs a) {
       uS = BIG;
       uNN = -1;
       unsigned pend = 0;
-      /* Branch-free per slot (selects only): divergent branches around updates
-       * of the register arrays make the compiler copy whole arrays at the joins. */
+      /* Per slot: selects and bitwise predicates only (divergent branches
+       * around updates of the register arrays make the compiler copy whole
+       * arrays at the joins).  Slots are visited in increasing order, so a
+       * strict < keeps the lowest slot on ties; only the moved jet (visited
+       * last, relabelled hi) needs the explicit slot comparison. */
+      const bool hasB = lo >= 0;
+      unsigned omask = 0;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         if (k * T >= n_old) continue; /* uniform */
         const int s = k * T + t;
-        const bool live = s < n_old && s != hi;
-        const int label = (s == tail) ? hi : s;
-        const bool isB = live && label == lo;
-        const bool normal = live && label != lo;
+        const bool live = (s < n_old) & (s != hi);
+        const bool isT = s == tail;
+        const int label = isT ? hi : s;
+        const bool isB = live & (label == lo);
+        const bool normal = live & (label != lo);
         rap[k] = isB ? nrap : rap[k];
         phi[k] = isB ? nphi : phi[k];
         w[k] = isB ? nw : w[k];
         int m = nn[k];
-        const bool orph = normal && ((m == hi) || (lo >= 0 && m == lo));
-        m = (normal && !orph && m == tail) ? hi : m;
+        const bool orph = normal & ((m == hi) | (hasB & (m == lo)));
+        m = (normal & !orph & (m == tail)) ? hi : m;
         const double d = fjmi_dr2(rap[k], phi[k], nrap, nphi);
-        const bool closer = normal && lo >= 0 && !orph && d < nnd[k];
+        const bool closer = normal & hasB & !orph & (d < nnd[k]);
         const double wmin = nw < w[k] ? nw : w[k];
         dij[k] = closer ? d * wmin : dij[k];
         nnd[k] = closer ? d : nnd[k];
         m = closer ? lo : m;
         nn[k] = m;
-        const bool bc = normal && lo >= 0 && d < R2 && lex_less(d, label, bD, bS);
+        const bool bc = normal & hasB & ((d < bD) | (isT & (d == bD) & (d < R2) & (hi < bS)));
         bD = bc ? d : bD;
         bS = bc ? label : bS;
         bW = bc ? w[k] : bW;
-        const bool uc = normal && !orph && lex_less(dij[k], label, uD, uS);
+        const bool uc = normal & !orph & ((dij[k] < uD) | (isT & (dij[k] == uD) & (hi < uS)));
         uD = uc ? dij[k] : uD;
         uS = uc ? label : uS;
         uNN = uc ? m : uNN;
-        int idx = 0;
-        if (orph) idx = atomicAdd(&sh.norph[par], 1);
-        if (orph && idx < OCH) {
-          Orph o;
-          o.rap = rap[k];
-          o.phi = phi[k];
-          o.w = w[k];
-          o.label = label;
-          o.pad = 0;
-          sh.orph[par][idx] = o;
+        omask |= orph ? (1u << k) : 0u;
+      }
+      /* orphans (about one per step): one LDS atomic per wave and slot index */
+      if (__ballot(omask != 0)) { /* uniform */
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (k * T >= n_old) continue;
+          const bool orph = (omask >> k) & 1u;
+          const unsigned long long obal = __ballot(orph);
+          if (obal) { /* uniform */
+            int obase = 0;
+            if (lane == 0) obase = atomicAdd(&sh.norph[par], (int)__popcll(obal));
+            obase = __builtin_amdgcn_readfirstlane(obase);
+            const int idx = obase + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(obal >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)obal, 0u));
+            if (orph & (idx < OCH)) {
+              Orph o;
+              o.rap = rap[k];
+              o.phi = phi[k];
+              o.w = w[k];
+              o.label = (k * T + t == tail) ? hi : k * T + t;
+              o.pad = 0;
+              sh.orph[par][idx] = o;
+            }
+            pend |= (orph & (idx >= OCH)) ? (1u << k) : 0u;
+          }
         }
-        const bool pending = orph && idx >= OCH;
-        if (normal && s == tail) { /* hi != tail here: hand the moved jet to slot hi's owner */
-          Mail ml;
-          ml.rap = rap[k];
-          ml.phi = phi[k];
-          ml.w = w[k];
-          ml.nnd = nnd[k];
-          ml.dij = dij[k];
-          ml.nn = m;
-          ml.pend = pending ? 1 : 0;
-          sh.mail = ml;
+      }
+      if (hi != tail) { /* uniform: hand the moved jet to slot hi's owner */
+        const int kt = tail / T;
+        if ((tail % T) == t) {
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            if (k == kt) {
+              Mail ml;
+              ml.rap = rap[k];
+              ml.phi = phi[k];
+              ml.w = w[k];
+              ml.nnd = nnd[k];
+              ml.dij = dij[k];
+              ml.nn = nn[k];
+              ml.pend = (pend >> k) & 1u;
+              sh.mail = ml;
+            }
         }
-        pend |= (pending && s != tail) ? (1u << k) : 0u;
+        pend &= ((tail % T) == t) ? ~(1u << kt) : ~0u;
       }
       Red ru = wave_argmin(uD, uS, uNN, 0.0);
       Red rb = wave_argmin(bD, bS, 0, bW);
