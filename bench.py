@@ -144,10 +144,12 @@ def main():
             from oracle import oracle as orc
             cs = min(a.cpu_sample, ne)
             threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-            secs = orc.time_batch(p4[:off[cs]], off[:cs + 1], p, R, threads)
+            secs = orc.time_batch(p4[:off[cs]], off[:cs + 1], p, R, threads, "tiled")
+            secs_plain = orc.time_batch(p4[:off[cs]], off[:cs + 1], p, R, threads, "n2plain")
             cpu = {"value": cs / secs, "unit": "events/s", "cores": threads, "kind": "port",
-                   "sample": f"first {cs} events of the same batch, FastJet N2Plain restatement (oracle/), "
-                             f"{threads} threads, {secs:.2f} s wall"}
+                   "sample": f"first {cs} events of the same batch, oracle/ tiled N2Plain (FastJet N2Tiled-style) "
+                             f"on {threads} threads, {secs:.2f} s wall; the N2Plain restatement took "
+                             f"{secs_plain:.2f} s ({cs / secs_plain:.0f} events/s)"}
         except Exception as ex:
             cpu = {"value": None, "unit": "events/s", "cores": 0, "kind": "port", "sample": f"failed: {ex}"}
 

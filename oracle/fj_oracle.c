@@ -365,12 +365,286 @@ int fjo_cluster_pairwise(const double *p4, int n, int p, double R, int mm, int32
 }
 
 /* ---------------------------------------------------------------------------
- * CPU baseline timing: events spread over threads, n2plain, shared math.
+ * Tiled N2Plain: the same decisions as fjo_cluster_n2plain (every tie included),
+ * with nearest neighbours maintained through a (rapidity, phi) tiling whose tile
+ * side is at least R (phi wraps around), in the spirit of FastJet's N2Tiled and of
+ * the reference's grid engine (grid.cu:55-106 geometry, 337-549 neighbour
+ * repair).  Jets keep their N2Plain slot number as a *label*: the argmin breaks
+ * ties by label, the nearest-neighbour search breaks distance ties by label, and
+ * the compaction of N2Plain only relabels the last jet.  Any two jets closer than
+ * R sit in the same or adjacent tiles, so every candidate is found.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  double eta, phi, w, nnd;
+  int nn;    /* physical index of the nearest neighbour, or -1 */
+  int label; /* N2Plain slot */
+  int tile;
+  int prev, next; /* tile list */
+  int jet;        /* index into book_t */
+} tj_t;
+
+typedef struct {
+  int ny, nphi;
+  double y0, sy, sphi;
+} tgeom_t;
+
+static void tgeom_init(tgeom_t *g, const tj_t *J, int n, double R) {
+  const double side = R * (1.0 + 0x1p-20); /* margin against rounding at tile edges */
+  double lo = 0.0, hi = 0.0;
+  int any = 0;
+  for (int i = 0; i < n; ++i) {
+    double y = J[i].eta;
+    if (!(y > -1e4 && y < 1e4)) continue; /* zero-pt momenta at +-(1e5+|pz|) go to the edge tiles */
+    if (!any || y < lo) lo = y;
+    if (!any || y > hi) hi = y;
+    any = 1;
+  }
+  int ny = (int)((hi - lo) / side) + 1;
+  if (ny > 256) ny = 256;
+  g->ny = ny;
+  g->y0 = lo;
+  g->sy = (hi - lo) / ny > side ? (hi - lo) / ny : side;
+  int np = (int)(FJMI_TWOPI / side);
+  if (np < 1) np = 1;
+  g->nphi = np;
+  g->sphi = FJMI_TWOPI / np;
+}
+
+static int tgeom_tile(const tgeom_t *g, double y, double phi) {
+  double fy = (y - g->y0) / g->sy;
+  int ty = fy < 0.0 ? 0 : (fy >= (double)g->ny ? g->ny - 1 : (int)fy);
+  int tp = (int)(phi / g->sphi);
+  if (tp >= g->nphi) tp = g->nphi - 1;
+  if (tp < 0) tp = 0;
+  return ty * g->nphi + tp;
+}
+
+/* the distinct tiles of the 3x3 neighbourhood of `tile` (up to 9) */
+static int tgeom_neigh(const tgeom_t *g, int tile, int *out) {
+  int ty = tile / g->nphi, tp = tile % g->nphi, k = 0;
+  for (int dy = -1; dy <= 1; ++dy) {
+    int y = ty + dy;
+    if (y < 0 || y >= g->ny) continue;
+    for (int dp = -1; dp <= 1; ++dp) {
+      int q = ((tp + dp) % g->nphi + g->nphi) % g->nphi;
+      int t = y * g->nphi + q, dup = 0;
+      for (int m = 0; m < k; ++m) dup |= out[m] == t;
+      if (!dup) out[k++] = t;
+    }
+  }
+  return k;
+}
+
+static void tl_insert(tj_t *J, int *head, int p) {
+  J[p].prev = -1;
+  J[p].next = head[J[p].tile];
+  if (J[p].next >= 0) J[J[p].next].prev = p;
+  head[J[p].tile] = p;
+}
+
+static void tl_remove(tj_t *J, int *head, int p) {
+  if (J[p].prev >= 0)
+    J[J[p].prev].next = J[p].next;
+  else
+    head[J[p].tile] = J[p].next;
+  if (J[p].next >= 0) J[J[p].next].prev = J[p].prev;
+}
+
+static inline double tj_dij(const tj_t *J, int p) {
+  double kt2 = J[p].w;
+  if (J[p].nn >= 0 && J[J[p].nn].w < kt2) kt2 = J[J[p].nn].w;
+  return J[p].nnd * kt2;
+}
+
+/* lexicographic (distance, label) nearest neighbour of p among the 3x3 tiles */
+static void tj_rescan(tj_t *J, const int *head, const tgeom_t *g, int p, double R2) {
+  int nb[9];
+  int k = tgeom_neigh(g, J[p].tile, nb);
+  double best = R2;
+  int bn = -1, bl = 0;
+  for (int m = 0; m < k; ++m)
+    for (int q = head[nb[m]]; q >= 0; q = J[q].next) {
+      if (q == p) continue;
+      double d = fjmi_dr2(J[p].eta, J[p].phi, J[q].eta, J[q].phi);
+      if (d < best || (d == best && bn >= 0 && J[q].label < bl)) {
+        best = d;
+        bn = q;
+        bl = J[q].label;
+      }
+    }
+  J[p].nn = bn;
+  J[p].nnd = best;
+}
+
+int fjo_cluster_tiled(const double *p4, int n, int p, double R, int mm, int32_t *hp1, int32_t *hp2, double *hdij,
+                      double *jets, int32_t *njets) {
+  if (n < 0 || !(R > 0.0) || p < -1 || p > 1) return -1;
+  *njets = 0;
+  if (n == 0) return 0;
+  const double R2 = R * R, invR2 = 1.0 / R2;
+  book_t bk;
+  if (book_init(&bk, p4, n, hp1, hp2, hdij, jets)) return -2;
+  tj_t *J = (tj_t *)malloc(sizeof(tj_t) * (size_t)n);
+  int *lab2 = (int *)malloc(sizeof(int) * (size_t)n);
+  double *dij = (double *)malloc(sizeof(double) * (size_t)n);
+  if (!J || !lab2 || !dij) {
+    free(J);
+    free(lab2);
+    free(dij);
+    book_free(&bk);
+    return -2;
+  }
+  for (int i = 0; i < n; ++i) {
+    set_jet(bk.p4 + 4 * (size_t)i, p, mm, &J[i].eta, &J[i].phi, &J[i].w);
+    J[i].jet = i;
+    J[i].label = i;
+    lab2[i] = i;
+  }
+  tgeom_t g;
+  tgeom_init(&g, J, n, R);
+  const int ntiles = g.ny * g.nphi;
+  int *head = (int *)malloc(sizeof(int) * (size_t)ntiles);
+  if (!head) {
+    free(J);
+    free(lab2);
+    free(dij);
+    book_free(&bk);
+    return -2;
+  }
+  for (int t = 0; t < ntiles; ++t) head[t] = -1;
+  for (int i = n - 1; i >= 0; --i) {
+    J[i].tile = tgeom_tile(&g, J[i].eta, J[i].phi);
+    tl_insert(J, head, i);
+  }
+  for (int i = 0; i < n; ++i) tj_rescan(J, head, &g, i, R2);
+  for (int i = 0; i < n; ++i) dij[i] = tj_dij(J, i);
+
+  int nb[9], nb2[9];
+  while (n > 0) {
+    /* argmin in label order: strict < keeps the lowest label */
+    int A = lab2[0];
+    double dmin = dij[A];
+    for (int l = 1; l < n; ++l) {
+      int q = lab2[l];
+      if (dij[q] < dmin) {
+        dmin = dij[q];
+        A = q;
+      }
+    }
+    int B = J[A].nn;
+    double dd = dmin * invR2;
+    int phi_, plo; /* physical slots of the higher and lower label */
+    if (B >= 0) {
+      phi_ = J[A].label > J[B].label ? A : B;
+      plo = phi_ == A ? B : A;
+    } else {
+      phi_ = A;
+      plo = -1;
+    }
+    const int hl = J[phi_].label;
+    const int tail = n - 1;
+    const int pt = lab2[tail];
+    const int oldtile_hi = J[phi_].tile;
+    const int oldtile_lo = plo >= 0 ? J[plo].tile : -1;
+    if (plo >= 0) {
+      int k = book_merge(&bk, J[phi_].jet, J[plo].jet, dd);
+      tl_remove(J, head, plo);
+      set_jet(bk.p4 + 4 * (size_t)k, p, mm, &J[plo].eta, &J[plo].phi, &J[plo].w);
+      J[plo].jet = k;
+      J[plo].nn = -1;
+      J[plo].nnd = R2;
+      J[plo].tile = tgeom_tile(&g, J[plo].eta, J[plo].phi);
+      tl_insert(J, head, plo);
+    } else {
+      book_beam(&bk, J[phi_].jet, dd);
+    }
+    tl_remove(J, head, phi_);
+    J[phi_].tile = -1;
+    J[phi_].label = -1;
+    if (pt != phi_) { /* N2Plain compaction: the last jet takes the removed slot */
+      J[pt].label = hl;
+      lab2[hl] = pt;
+    }
+    n--;
+    if (n == 0) break;
+    /* orphans: jets whose NN was the removed or the merged jet; they lie in the
+     * 3x3 tiles of the two old positions */
+    int norph_t = tgeom_neigh(&g, oldtile_hi, nb);
+    if (oldtile_lo >= 0) {
+      int k2 = tgeom_neigh(&g, oldtile_lo, nb2);
+      for (int m = 0; m < k2; ++m) {
+        int dup = 0;
+        for (int u = 0; u < norph_t; ++u) dup |= nb[u] == nb2[m];
+        if (!dup) nb[norph_t++] = nb2[m];
+      }
+    }
+    /* nb may now hold up to 18 tiles */
+    int orph[64], no = 0, overflow = 0;
+    for (int m = 0; m < norph_t; ++m)
+      for (int q = head[nb[m]]; q >= 0; q = J[q].next)
+        if (q != plo && (J[q].nn == phi_ || (plo >= 0 && J[q].nn == plo))) {
+          if (no < 64)
+            orph[no++] = q;
+          else
+            overflow = 1;
+        }
+    if (overflow) { /* pathological: rescan every jet of those tiles that lost its NN */
+      for (int m = 0; m < norph_t; ++m)
+        for (int q = head[nb[m]]; q >= 0; q = J[q].next)
+          if (q != plo && (J[q].nn == phi_ || (plo >= 0 && J[q].nn == plo))) J[q].nn = -2;
+      for (int m = 0; m < norph_t; ++m)
+        for (int q = head[nb[m]]; q >= 0; q = J[q].next)
+          if (J[q].nn == -2) {
+            tj_rescan(J, head, &g, q, R2);
+            dij[q] = tj_dij(J, q);
+          }
+    } else {
+      for (int o = 0; o < no; ++o) {
+        tj_rescan(J, head, &g, orph[o], R2);
+        dij[orph[o]] = tj_dij(J, orph[o]);
+      }
+    }
+    if (plo >= 0) {
+      /* the merged jet: update the jets around it and find its own NN */
+      int k = tgeom_neigh(&g, J[plo].tile, nb);
+      double best = R2;
+      int bn = -1, bl = 0;
+      for (int m = 0; m < k; ++m)
+        for (int q = head[nb[m]]; q >= 0; q = J[q].next) {
+          if (q == plo) continue;
+          double d = fjmi_dr2(J[q].eta, J[q].phi, J[plo].eta, J[plo].phi);
+          if (d < J[q].nnd) { /* orphans were rescanned against the new jet already */
+            J[q].nnd = d;
+            J[q].nn = plo;
+            dij[q] = tj_dij(J, q);
+          }
+          if (d < best || (d == best && bn >= 0 && J[q].label < bl)) {
+            best = d;
+            bn = q;
+            bl = J[q].label;
+          }
+        }
+      J[plo].nn = bn;
+      J[plo].nnd = best;
+      dij[plo] = tj_dij(J, plo);
+    }
+  }
+  *njets = bk.nout;
+  free(J);
+  free(lab2);
+  free(dij);
+  free(head);
+  book_free(&bk);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * CPU baseline timing: events spread over threads, n2plain or tiled, shared math.
  * ------------------------------------------------------------------------- */
 typedef struct {
   const double *p4;
   const int64_t *off;
-  int n_events, p, nthreads, tid;
+  int n_events, p, nthreads, tid, method;
   double R;
   int err;
 } tb_arg_t;
@@ -389,9 +663,11 @@ static void *tb_worker(void *vp) {
   if (!h1 || !h2 || !hd || !jt) a->err = 1;
   for (int e = a->tid; e < a->n_events && !a->err; e += a->nthreads) {
     int32_t nj;
-    if (fjo_cluster_n2plain(a->p4 + 4 * a->off[e], (int)(a->off[e + 1] - a->off[e]), a->p, a->R, FJO_MATH_SHARED, h1,
-                            h2, hd, jt, &nj))
-      a->err = 1;
+    const double *q = a->p4 + 4 * a->off[e];
+    const int n = (int)(a->off[e + 1] - a->off[e]);
+    int rc = a->method ? fjo_cluster_tiled(q, n, a->p, a->R, FJO_MATH_SHARED, h1, h2, hd, jt, &nj)
+                       : fjo_cluster_n2plain(q, n, a->p, a->R, FJO_MATH_SHARED, h1, h2, hd, jt, &nj);
+    if (rc) a->err = 1;
   }
   free(h1);
   free(h2);
@@ -400,7 +676,8 @@ static void *tb_worker(void *vp) {
   return NULL;
 }
 
-double fjo_time_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R, int nthreads) {
+double fjo_time_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R, int nthreads,
+                      int method) {
   if (nthreads < 1 || nthreads > 1024) return -1.0;
   pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
   tb_arg_t *args = (tb_arg_t *)malloc(sizeof(tb_arg_t) * (size_t)nthreads);
@@ -408,7 +685,7 @@ double fjo_time_batch(const double *p4, const int64_t *offsets, int n_events, in
   struct timespec t0, t1;
   clock_gettime(CLOCK_MONOTONIC, &t0);
   for (int t = 0; t < nthreads; ++t) {
-    args[t] = (tb_arg_t){p4, offsets, n_events, p, nthreads, t, R, 0};
+    args[t] = (tb_arg_t){p4, offsets, n_events, p, nthreads, t, method, R, 0};
     pthread_create(&th[t], NULL, tb_worker, &args[t]);
   }
   int err = 0;

@@ -53,14 +53,21 @@ int fjo_cluster_brute(const double *p4, int n, int p, double R, int math_mode, i
 int fjo_cluster_pairwise(const double *p4, int n, int p, double R, int math_mode, int32_t *hist_p1,
                          int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets);
 
+/* N2Plain decisions with (rapidity, phi) tiling of side >= R for the nearest-
+ * neighbour bookkeeping (N2Plain slot numbers kept as labels for every tie). */
+int fjo_cluster_tiled(const double *p4, int n, int p, double R, int math_mode, int32_t *hist_p1, int32_t *hist_p2,
+                      double *hist_dij, double *jets, int32_t *njets);
+
 /* Kinematics exported for tests. */
 double fjo_log(double x);
 double fjo_atan2(double y, double x);
 void fjo_set_jet(const double *p4, int p, int math_mode, double *rap, double *phi, double *w);
 
 /* CPU baseline: clusters n_events events (p4 packed, offsets[n_events+1] in
- * particles) with n2plain on nthreads threads; returns wall seconds, or -1. */
-double fjo_time_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R, int nthreads);
+ * particles) on nthreads threads with n2plain (method 0) or tiled (method 1);
+ * returns wall seconds, or -1. */
+double fjo_time_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R, int nthreads,
+                      int method);
 
 #ifdef __cplusplus
 }

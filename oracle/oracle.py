@@ -23,7 +23,7 @@ def lib():
         if not os.path.exists(_PATH):
             raise RuntimeError(f"{_PATH} missing; run `make -C oracle`")
         L = ctypes.CDLL(_PATH)
-        for name in ("fjo_cluster_n2plain", "fjo_cluster_brute", "fjo_cluster_pairwise"):
+        for name in ("fjo_cluster_n2plain", "fjo_cluster_brute", "fjo_cluster_pairwise", "fjo_cluster_tiled"):
             f = getattr(L, name)
             f.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
             f.restype = ctypes.c_int
@@ -33,7 +33,7 @@ def lib():
         L.fjo_atan2.restype = ctypes.c_double
         L.fjo_set_jet.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
         L.fjo_set_jet.restype = None
-        L.fjo_time_batch.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.fjo_time_batch.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.fjo_time_batch.restype = ctypes.c_double
         _lib = L
     return _lib
@@ -67,11 +67,17 @@ def pairwise(p4, p, R, math_mode=MATH_SHARED):
     return _run("fjo_cluster_pairwise", p4, p, R, math_mode)
 
 
-def time_batch(p4, offsets, p, R, nthreads):
+def tiled(p4, p, R, math_mode=MATH_SHARED):
+    """N2Plain decisions with tiled nearest-neighbour bookkeeping."""
+    return _run("fjo_cluster_tiled", p4, p, R, math_mode)
+
+
+def time_batch(p4, offsets, p, R, nthreads, method="tiled"):
+    """Wall seconds to cluster the events on `nthreads` host threads."""
     p4 = np.ascontiguousarray(p4, dtype=np.float64)
     offsets = np.ascontiguousarray(offsets, dtype=np.int64)
     return lib().fjo_time_batch(p4.ctypes.data_as(_vp), offsets.ctypes.data_as(_vp), len(offsets) - 1, int(p),
-                                float(R), int(nthreads))
+                                float(R), int(nthreads), 1 if method == "tiled" else 0)
 
 
 def set_jet(p4row, p, math_mode=MATH_SHARED):

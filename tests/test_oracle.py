@@ -168,3 +168,36 @@ def test_glibc_math_gives_same_merges(fname):
                 assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
                 assert np.allclose(a[2], b[2], rtol=1e-12, atol=0)
                 assert np.array_equal(a[3], b[3])
+
+
+@settings(max_examples=60, deadline=None)
+@given(n=st.integers(1, 120), seed=st.integers(0, 2 ** 31 - 1), p=st.sampled_from(ALGOS),
+       R=st.sampled_from([0.1, 0.4, 0.7, 1.0, 2.5, 7.0]), spread=st.sampled_from([0.2, 1.0, 3.0]))
+def test_tiled_equals_n2plain(n, seed, p, R, spread):
+    """The tiled bookkeeping gives the N2Plain decisions bit for bit, including
+    R > 2pi/3 (fewer than three phi tiles) and very dense or sparse events."""
+    ev = random_event(np.random.default_rng(seed), n, spread)
+    assert same(orc.n2plain(ev, p, R), orc.tiled(ev, p, R))
+
+
+@pytest.mark.parametrize("p", ALGOS)
+def test_tiled_equals_n2plain_on_ties(p):
+    """Exact duplicates and a regular lattice: every kind of exact tie."""
+    base = np.array([[3.0, 1.0, 0.5, 4.0], [0.5, 2.0, -1.0, 3.0], [-1.0, -1.0, 0.2, 2.0]])
+    dup = np.repeat(base, 30, axis=0)
+    g = []
+    for i in range(14):
+        for j in range(14):
+            eta, phi = -1.5 + 0.25 * i, 0.25 * j
+            pt = 1.0 + 0.125 * ((i * 7 + j * 3) % 5)
+            g.append([pt * math.cos(phi), pt * math.sin(phi), pt * math.sinh(eta), pt * math.cosh(eta)])
+    for ev, R in ((dup, 0.7), (np.array(g), 0.3), (np.array(g), 0.25), (np.array(g), 0.5)):
+        assert same(orc.n2plain(ev, p, R), orc.tiled(ev, p, R))
+
+
+def test_tiled_equals_n2plain_on_reference_files():
+    for fname in ("354_fastjet.dat", "808_cms.dat", "multi-event.dat"):
+        for ev in orc.read_events(os.path.join(GOLDEN, fname)):
+            for p in ALGOS:
+                for R in (0.4, 1.0, 3.0):
+                    assert same(orc.n2plain(ev, p, R), orc.tiled(ev, p, R)), (fname, len(ev), p, R)
