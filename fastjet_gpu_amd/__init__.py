@@ -31,8 +31,8 @@ class FjmiError(RuntimeError):
 
 
 class Bucket(ctypes.Structure):
-    _fields_ = [("max_n", ctypes.c_int32), ("threads", ctypes.c_int32), ("slots", ctypes.c_int32),
-                ("p4_global", ctypes.c_int32)]
+    _fields_ = [("max_n", ctypes.c_int32), ("engine", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("slots", ctypes.c_int32), ("p4_global", ctypes.c_int32)]
 
 
 class BucketStats(ctypes.Structure):
@@ -132,14 +132,14 @@ class Engine:
         n = self.lib.fjmi_kernel_configs(None, 0)
         arr = (Bucket * n)()
         self.lib.fjmi_kernel_configs(ctypes.cast(arr, _vp), n)
-        return [(b.max_n, b.threads, b.slots, b.p4_global) for b in arr]
+        return [(b.max_n, b.engine, b.threads, b.slots, b.p4_global) for b in arr]
 
     def buckets(self):
         arr = (Bucket * 64)()
         n = ctypes.c_int32()
         _check(self.lib, self.lib.fjmi_get_buckets(self.ctx, ctypes.cast(arr, _vp), 64, ctypes.byref(n)),
                "fjmi_get_buckets")
-        return [(arr[k].max_n, arr[k].threads, arr[k].slots, arr[k].p4_global) for k in range(n.value)]
+        return [(arr[k].max_n, arr[k].engine, arr[k].threads, arr[k].slots, arr[k].p4_global) for k in range(n.value)]
 
     def set_buckets(self, rows):
         arr = (Bucket * len(rows))(*[Bucket(*r) for r in rows])
@@ -218,7 +218,7 @@ class Batch:
         out = []
         for k in range(n.value):
             b = arr[k]
-            out.append(dict(max_n=b.bucket.max_n, threads=b.bucket.threads, slots=b.bucket.slots,
+            out.append(dict(max_n=b.bucket.max_n, engine=b.bucket.engine, threads=b.bucket.threads, slots=b.bucket.slots,
                             p4_global=b.bucket.p4_global, n_events=b.n_events, grid=b.grid,
                             particles=b.particles, sum_n2=b.sum_n2, ms=b.ms))
         return out

@@ -52,11 +52,12 @@ def build(jobs=4, cli=True, oracle=True):
         os.makedirs(d, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         f_dense = ex.submit(hip_obj, "dense.hip")
+        f_tiled = ex.submit(hip_obj, "tiled.hip")
         f_rt = ex.submit(hip_obj, "runtime.hip")
         f_gen = ex.submit(run, ["gcc", *CC_FLAGS, "-shared", "-o", os.path.join(LIB, "libfjmi_gen.so"),
                                 os.path.join(CSRC, "fjmi_gen.c"), "-lm"])
         f_or = ex.submit(run, ["make", "-s", "-C", os.path.join(ROOT, "oracle")]) if oracle else None
-        objs = [f_dense.result(), f_rt.result()]
+        objs = [f_dense.result(), f_tiled.result(), f_rt.result()]
         f_gen.result()
         if f_or:
             f_or.result()

@@ -5,6 +5,7 @@
 #ifndef FJMI_INTERNAL_H
 #define FJMI_INTERNAL_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 namespace fjmi {
@@ -46,7 +47,51 @@ struct LaunchArgs {
   int stage_chunk; /* number of staging entries in LDS when four-momenta live in global memory */
 };
 
-enum { STATUS_CAPACITY = 1 };
+enum { STATUS_CAPACITY = 1, STATUS_INTERNAL = 2 };
+
+/* Global per-particle scratch of the tiled engine (indexed by particle like the
+ * input), plus the tile-segment pool: 3 entries per particle and pool_per_event
+ * per event (event e's pool starts at 3*offsets[e] + e*pool_per_event). */
+struct TiledScratch {
+  double *rp;   /* 2 per particle: rapidity, phi */
+  double *wn;   /* 2 per particle: weight, nearest-neighbour distance */
+  double *dij;  /* 1 per particle */
+  int *meta;    /* 4 per particle: nn, label, tile, segment position */
+  double *p4;   /* 4 per particle */
+  int *hist;    /* 1 per particle */
+  int *lab2;    /* 1 per particle: label -> particle */
+  int *pool;
+  int pool_per_event;
+};
+
+/* Bytes of tiled scratch for `particles` particles in `events` events. */
+inline size_t tiled_scratch_bytes(int64_t particles, int64_t events, int pool_per_event) {
+  return (size_t)particles * (16 + 16 + 8 + 16 + 32 + 4 + 4 + 12) + (size_t)events * pool_per_event * 4 + 1024;
+}
+
+/* Carves a TiledScratch out of one allocation of tiled_scratch_bytes(). */
+inline TiledScratch tiled_scratch_carve(void *mem, int64_t particles, int pool_per_event) {
+  char *c = static_cast<char *>(mem);
+  TiledScratch t;
+  const size_t P = (size_t)particles;
+  t.p4 = reinterpret_cast<double *>(c);
+  c += 32 * P;
+  t.rp = reinterpret_cast<double *>(c);
+  c += 16 * P;
+  t.wn = reinterpret_cast<double *>(c);
+  c += 16 * P;
+  t.meta = reinterpret_cast<int *>(c);
+  c += 16 * P;
+  t.dij = reinterpret_cast<double *>(c);
+  c += 8 * P;
+  t.hist = reinterpret_cast<int *>(c);
+  c += 4 * P;
+  t.lab2 = reinterpret_cast<int *>(c);
+  c += 4 * P;
+  t.pool = reinterpret_cast<int *>(c);
+  t.pool_per_event = pool_per_event;
+  return t;
+}
 
 }  // namespace fjmi
 

@@ -21,6 +21,12 @@ extern const int kNumDenseKernels;
 extern const void *const kEmptyKernel; /* __global__ void(int) */
 extern const void *const kZeroKernel;  /* __global__ void(unsigned *, int) */
 
+extern const void *const kTiledKernel; /* __global__ void(LaunchArgs, TiledScratch), 64 threads */
+extern const int kTiledMaxTiles;
+
+/* Pool entries per event of the tiled engine beyond 3 per particle (>= 2 per tile). */
+constexpr int kTiledPoolPerEvent = 2048;
+
 /* LDS staging entries used by the p4g kernels for the initial nearest-neighbour pass. */
 constexpr int kStageChunk = 2048;
 

@@ -108,11 +108,15 @@ struct fjmi_context {
   /* one-event path */
   DevBuf ctrl;
   uint32_t head_base = 0;
-  DevBuf hist1, hist2, hdij, jets, njets, p4s, coords, p4in;
+  DevBuf hist1, hist2, hdij, jets, njets, p4s, coords, p4in, tscratch;
   int max_lds = 0;
 };
 
 struct BucketRun {
+  bool tiled;
+  fjmi::TiledScratch ts;
+  int threads;
+  const void *fn;
   const KernelDesc *kd;
   int grid;
   size_t lds;
@@ -126,7 +130,7 @@ struct BucketRun {
 struct fjmi_batch {
   fjmi_context *ctx = nullptr;
   int n_events = 0;
-  DevBuf queue, heads, p4s, coords;
+  DevBuf queue, heads, p4s, coords, tscratch;
   std::vector<BucketRun> runs;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -147,6 +151,11 @@ int check_table(const fjmi_bucket *rows, int n) {
   for (int i = 0; i < n; ++i) {
     const fjmi_bucket &b = rows[i];
     if (i > 0 && b.max_n <= rows[i - 1].max_n) return fail(FJMI_EINVAL, "bucket table not sorted by max_n");
+    if (b.engine == FJMI_ENGINE_TILED) {
+      if (b.threads != 64) return fail(FJMI_EINVAL, "tiled rows run 64 threads");
+      continue;
+    }
+    if (b.engine != FJMI_ENGINE_DENSE) return fail(FJMI_EINVAL, "bucket engine must be dense or tiled");
     const KernelDesc *kd = find_kernel(b.threads, b.slots, b.p4_global);
     if (!kd)
       return fail(FJMI_EINVAL, "no compiled dense kernel with threads=%d slots=%d p4_global=%d", b.threads, b.slots,
@@ -157,11 +166,15 @@ int check_table(const fjmi_bucket *rows, int n) {
   return FJMI_OK;
 }
 
-const fjmi_bucket *bucket_for(const fjmi_context *ctx, int n) {
+/* The row serving an event of n particles: the first row with max_n >= n whose
+ * engine matches `engine` (any engine for FJMI_ENGINE_AUTO). */
+const fjmi_bucket *bucket_for(const fjmi_context *ctx, int n, int engine) {
   for (const fjmi_bucket &b : ctx->buckets)
-    if (n <= b.max_n) return &b;
+    if (n <= b.max_n && (engine == FJMI_ENGINE_AUTO || b.engine == engine)) return &b;
   return nullptr;
 }
+
+const fjmi_bucket kTiledRow = {0x7fffffff, FJMI_ENGINE_TILED, 64, 0, 0};
 
 LaunchArgs base_args() {
   LaunchArgs a;
@@ -180,15 +193,20 @@ int launch_dense(const KernelDesc &kd, int grid, size_t lds, const LaunchArgs &a
 }
 
 /* One event on the device: one kernel launch with one workgroup. */
-int cluster_one_device(fjmi_context *ctx, const double *d_in, int in_stride, int in_off, int n, int p, double R,
-                       int32_t *h1, int32_t *h2, double *hd, double *jets, int32_t *nj, unsigned char *pj,
+int cluster_one_device(fjmi_context *ctx, int engine, const double *d_in, int in_stride, int in_off, int n, int p,
+                       double R, int32_t *h1, int32_t *h2, double *hd, double *jets, int32_t *nj, unsigned char *pj,
                        hipStream_t stream) {
-  const fjmi_bucket *b = bucket_for(ctx, n);
-  if (!b) return fail(FJMI_ECAPACITY, "event with %d particles exceeds the largest bucket", n);
-  const KernelDesc *kd = find_kernel(b->threads, b->slots, b->p4_global);
+  const fjmi_bucket *b = engine == FJMI_ENGINE_TILED ? &kTiledRow : bucket_for(ctx, n, engine);
+  if (!b) return fail(FJMI_ECAPACITY, "event with %d particles exceeds the largest bucket of engine %d", n, engine);
   int rc;
   if ((rc = ctx->ctrl.ensure(sizeof(Ctrl)))) return rc;
-  if (kd->p4g) {
+  const bool tiled = b->engine == FJMI_ENGINE_TILED;
+  const KernelDesc *kd = tiled ? nullptr : find_kernel(b->threads, b->slots, b->p4_global);
+  fjmi::TiledScratch ts{};
+  if (tiled) {
+    if ((rc = ctx->tscratch.ensure(fjmi::tiled_scratch_bytes(n, 1, fjmi::kTiledPoolPerEvent)))) return rc;
+    ts = fjmi::tiled_scratch_carve(ctx->tscratch.ptr, n, fjmi::kTiledPoolPerEvent);
+  } else if (kd->p4g) {
     if ((rc = ctx->p4s.ensure(sizeof(double) * 4 * (size_t)n))) return rc;
     if ((rc = ctx->coords.ensure(sizeof(double) * 4 * (size_t)n))) return rc;
   }
@@ -214,8 +232,19 @@ int cluster_one_device(fjmi_context *ctx, const double *d_in, int in_stride, int
   a.p4_scratch = ctx->p4s.as<double>();
   a.coord_scratch = ctx->coords.as<double>();
   a.status = &c->status;
-  if ((rc = launch_dense(*kd, 1, dyn_lds_bytes(*kd), a, stream))) return rc;
+  if (tiled) {
+    void *params[] = {&a, &ts};
+    HIPCHK(hipLaunchKernel(fjmi::kTiledKernel, dim3(1), dim3(64), params, 0, stream));
+  } else if ((rc = launch_dense(*kd, 1, dyn_lds_bytes(*kd), a, stream))) {
+    return rc;
+  }
   ctx->head_base += 2; /* one successful and one failing queue pop */
+  return FJMI_OK;
+}
+
+int check_engine(int engine) {
+  if (engine != FJMI_ENGINE_AUTO && engine != FJMI_ENGINE_DENSE && engine != FJMI_ENGINE_TILED)
+    return fail(FJMI_EUNSUPPORTED, "engine %d not available for this entry point", engine);
   return FJMI_OK;
 }
 
@@ -238,11 +267,13 @@ int fjmi_kernel_configs(fjmi_bucket *out, int cap) {
   for (int i = 0; i < fjmi::kNumDenseKernels && i < cap; ++i) {
     const KernelDesc &d = fjmi::kDenseKernels[i];
     out[i].max_n = d.T * d.K;
+    out[i].engine = FJMI_ENGINE_DENSE;
     out[i].threads = d.T;
     out[i].slots = d.K;
     out[i].p4_global = d.p4g ? 1 : 0;
   }
-  return fjmi::kNumDenseKernels;
+  if (fjmi::kNumDenseKernels < cap) out[fjmi::kNumDenseKernels] = kTiledRow;
+  return fjmi::kNumDenseKernels + 1;
 }
 
 int fjmi_create(int device, fjmi_context **out) {
@@ -300,7 +331,7 @@ int fjmi_destroy(fjmi_context *ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
   for (DevBuf *b : {&ctx->ctrl, &ctx->hist1, &ctx->hist2, &ctx->hdij, &ctx->jets, &ctx->njets, &ctx->p4s,
-                    &ctx->coords, &ctx->p4in})
+                    &ctx->coords, &ctx->p4in, &ctx->tscratch})
     b->release();
   delete ctx;
   return FJMI_OK;
@@ -329,8 +360,7 @@ int fjmi_cluster_event_device(fjmi_context *ctx, const double *d_p4, int n, int 
   if (!ctx) return fail(FJMI_EINVAL, "null context");
   int rc = check_args(n, p, R);
   if (rc) return rc;
-  if (engine != FJMI_ENGINE_AUTO && engine != FJMI_ENGINE_DENSE)
-    return fail(FJMI_EUNSUPPORTED, "engine %d not available for this entry point", engine);
+  if ((rc = check_engine(engine))) return rc;
   if (n > 0 && (!d_p4 || !d_h1 || !d_h2 || !d_hd || !d_jets || !d_nj)) return fail(FJMI_EINVAL, "null buffer");
   std::lock_guard<std::mutex> lk(ctx->mtx);
   if ((rc = set_device(ctx))) return rc;
@@ -338,7 +368,8 @@ int fjmi_cluster_event_device(fjmi_context *ctx, const double *d_p4, int n, int 
     HIPCHK(hipMemsetAsync(d_nj, 0, sizeof(int32_t), (hipStream_t)stream));
     return FJMI_OK;
   }
-  return cluster_one_device(ctx, d_p4, 4, 0, n, p, R, d_h1, d_h2, d_hd, d_jets, d_nj, nullptr, (hipStream_t)stream);
+  return cluster_one_device(ctx, engine, d_p4, 4, 0, n, p, R, d_h1, d_h2, d_hd, d_jets, d_nj, nullptr,
+                            (hipStream_t)stream);
 }
 
 int fjmi_cluster_pseudojets(fjmi_context *ctx, void *d_particles, int n, int p, double R, int engine, void *stream) {
@@ -361,13 +392,13 @@ int fjmi_cluster_pseudojets(fjmi_context *ctx, void *d_particles, int n, int p, 
     }
     return FJMI_OK;
   }
-  if (engine != FJMI_ENGINE_AUTO && engine != FJMI_ENGINE_DENSE)
-    return fail(FJMI_EUNSUPPORTED, "engine %d not available", engine);
+  if ((rc = check_engine(engine))) return rc;
   const size_t nn = (size_t)n;
   if ((rc = ctx->hist1.ensure(4 * nn)) || (rc = ctx->hist2.ensure(4 * nn)) || (rc = ctx->hdij.ensure(8 * nn)) ||
       (rc = ctx->jets.ensure(32 * nn)) || (rc = ctx->njets.ensure(4)))
     return rc;
-  return cluster_one_device(ctx, static_cast<const double *>(d_particles), 5, 1, n, p, R, ctx->hist1.as<int32_t>(),
+  return cluster_one_device(ctx, engine, static_cast<const double *>(d_particles), 5, 1, n, p, R,
+                            ctx->hist1.as<int32_t>(),
                             ctx->hist2.as<int32_t>(), ctx->hdij.as<double>(), ctx->jets.as<double>(),
                             ctx->njets.as<int32_t>(), static_cast<unsigned char *>(d_particles), s);
 }
@@ -383,8 +414,7 @@ int fjmi_cluster_event(fjmi_context *ctx, const double *p4, int n, int p, double
     return FJMI_OK;
   }
   if (!p4 || !h1 || !h2 || !hd || !jets) return fail(FJMI_EINVAL, "null buffer");
-  if (engine != FJMI_ENGINE_AUTO && engine != FJMI_ENGINE_DENSE)
-    return fail(FJMI_EUNSUPPORTED, "engine %d not available", engine);
+  if ((rc = check_engine(engine))) return rc;
   std::lock_guard<std::mutex> lk(ctx->mtx);
   if ((rc = set_device(ctx))) return rc;
   const size_t nn = (size_t)n;
@@ -394,7 +424,7 @@ int fjmi_cluster_event(fjmi_context *ctx, const double *p4, int n, int p, double
   HIPCHK(hipMemcpy(ctx->p4in.ptr, p4, 32 * nn, hipMemcpyHostToDevice));
   Ctrl *c = ctx->ctrl.as<Ctrl>();
   HIPCHK(hipMemsetAsync(&c->status, 0, sizeof(uint32_t), nullptr));
-  if ((rc = cluster_one_device(ctx, ctx->p4in.as<double>(), 4, 0, n, p, R, ctx->hist1.as<int32_t>(),
+  if ((rc = cluster_one_device(ctx, engine, ctx->p4in.as<double>(), 4, 0, n, p, R, ctx->hist1.as<int32_t>(),
                                ctx->hist2.as<int32_t>(), ctx->hdij.as<double>(), ctx->jets.as<double>(),
                                ctx->njets.as<int32_t>(), nullptr, nullptr)))
     return rc;
@@ -426,6 +456,7 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
   const int nb = (int)ctx->buckets.size();
   std::vector<std::vector<int32_t>> lists(nb);
   int64_t max_p4g_total = 0;
+  bool any_tiled = false;
   for (int e = 0; e < n_events; ++e) {
     const int64_t n = h_offsets[e + 1] - h_offsets[e];
     if (n < 0 || h_offsets[e] < 0) return fail(FJMI_EINVAL, "offsets not monotonic at event %d", e);
@@ -433,7 +464,10 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
     while (b < nb && n > ctx->buckets[b].max_n) ++b;
     if (b == nb) return fail(FJMI_ECAPACITY, "event %d has %lld particles, above the largest bucket", e, (long long)n);
     lists[b].push_back(e);
-    if (ctx->buckets[b].p4_global) max_p4g_total = h_offsets[n_events];
+    if (ctx->buckets[b].engine == FJMI_ENGINE_TILED)
+      any_tiled = true;
+    else if (ctx->buckets[b].p4_global)
+      max_p4g_total = h_offsets[n_events];
   }
   auto *bt = new fjmi_batch();
   bt->ctx = ctx;
@@ -458,6 +492,11 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
     if ((rc = bt->p4s.ensure(32 * (size_t)max_p4g_total))) return cleanup(rc);
     if ((rc = bt->coords.ensure(32 * (size_t)max_p4g_total))) return cleanup(rc);
   }
+  if (any_tiled) {
+    const int64_t P = h_offsets[n_events];
+    if ((rc = bt->tscratch.ensure(fjmi::tiled_scratch_bytes(P, n_events, fjmi::kTiledPoolPerEvent))))
+      return cleanup(rc);
+  }
   if (!flat.empty()) {
     hipError_t e = hipMemcpy(bt->queue.ptr, flat.data(), sizeof(int32_t) * flat.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return cleanup(fail(FJMI_EHIP, "queue upload: %s", hipGetErrorString(e)));
@@ -468,10 +507,15 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
     if (lists[b].empty()) continue;
     const fjmi_bucket &bk = ctx->buckets[b];
     BucketRun r;
-    r.kd = find_kernel(bk.threads, bk.slots, bk.p4_global);
-    r.lds = dyn_lds_bytes(*r.kd);
+    r.tiled = bk.engine == FJMI_ENGINE_TILED;
+    r.kd = r.tiled ? nullptr : find_kernel(bk.threads, bk.slots, bk.p4_global);
+    r.fn = r.tiled ? fjmi::kTiledKernel : r.kd->fn;
+    r.threads = r.tiled ? 64 : r.kd->T;
+    r.lds = r.tiled ? 0 : dyn_lds_bytes(*r.kd);
+    r.ts = r.tiled ? fjmi::tiled_scratch_carve(bt->tscratch.ptr, h_offsets[n_events], fjmi::kTiledPoolPerEvent)
+                   : fjmi::TiledScratch{};
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, r.kd->fn, r.kd->T, r.lds);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, r.fn, r.threads, r.lds);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     r.n_events = (int)lists[b].size();
     r.max_n = bk.max_n;
@@ -523,8 +567,8 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
     for (size_t i = 0; e == hipSuccess && i < bt->runs.size(); ++i) {
       e = hipStreamWaitEvent(ss[i], bt->tev[0], 0);
       BucketRun &r = bt->runs[i];
-      void *params[] = {&r.args};
-      if (e == hipSuccess) e = hipLaunchKernel(r.kd->fn, dim3(r.grid), dim3(r.kd->T), params, r.lds, ss[i]);
+      void *params[] = {&r.args, &r.ts};
+      if (e == hipSuccess) e = hipLaunchKernel(r.fn, dim3(r.grid), dim3(r.threads), params, r.lds, ss[i]);
       if (e == hipSuccess) e = hipEventRecord(bt->tev[i + 1], ss[i]);
       if (e == hipSuccess) e = hipStreamWaitEvent(s0, bt->tev[i + 1], 0);
     }
@@ -565,9 +609,10 @@ int fjmi_batch_stats(fjmi_batch *b, fjmi_bucket_stats *out, int cap, int *n) {
     const BucketRun &r = b->runs[i];
     fjmi_bucket_stats &o = out[i];
     o.bucket.max_n = r.max_n;
-    o.bucket.threads = r.kd->T;
-    o.bucket.slots = r.kd->K;
-    o.bucket.p4_global = r.kd->p4g ? 1 : 0;
+    o.bucket.engine = r.tiled ? FJMI_ENGINE_TILED : FJMI_ENGINE_DENSE;
+    o.bucket.threads = r.threads;
+    o.bucket.slots = r.tiled ? 0 : r.kd->K;
+    o.bucket.p4_global = r.tiled ? 0 : (r.kd->p4g ? 1 : 0);
     o.n_events = r.n_events;
     o.grid = r.grid;
     o.particles = r.particles;
@@ -591,6 +636,7 @@ int fjmi_batch_destroy(fjmi_batch *b) {
   b->heads.release();
   b->p4s.release();
   b->coords.release();
+  b->tscratch.release();
   delete b;
   return FJMI_OK;
 }

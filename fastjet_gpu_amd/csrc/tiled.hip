@@ -1,0 +1,766 @@
+/*
+ * tiled.hip — the (rapidity, phi)-tiled clustering engine for gfx950.
+ *
+ * Replaces the reference's grid engine (grid.cu): jets are binned into tiles of
+ * side >= R in rapidity and phi (phi wraps around), so every pair closer than R
+ * sits in the same or adjacent tiles (grid.cu:55-106), and after each step only
+ * the 3x3 neighbourhoods of the jets involved are revisited (grid.cu:403-549).
+ * Unlike the reference the rapidity range comes from the event itself (edge
+ * tiles absorb anything beyond it), the tile lists have no fixed capacity, and
+ * the decisions are exactly FastJet's N2Plain ones, ties included:
+ *   - every jet carries its N2Plain slot number as a *label* (the slot it would
+ *     occupy in N2Plain's compacted array); the physical storage never moves;
+ *   - the argmin is the lexicographic (diJ, label) minimum, nearest-neighbour
+ *     ties go to the lowest label, and N2Plain's compaction is a relabelling of
+ *     the last jet (label n-1 -> label of the removed slot);
+ *   - nearest neighbours are physical indices, so the relabelling needs no
+ *     pointer fix-up.
+ *
+ * One wave (64 lanes) clusters one event at a time; persistent workgroups pull
+ * events from the queue of LaunchArgs.  Per-jet records live in global memory
+ * (structure of arrays, indexed by particle); per tile, the lexicographic
+ * (diJ, label) minimum and the member segment descriptor live in LDS.
+ *
+ * Per step (n live jets):
+ *   1. argmin over the tile minima (LDS scan + DPP wave reduction);
+ *   2. the merge (E-scheme, kinematics of the new jet) and the structural
+ *      updates (tile membership, relabelling) by lane 0;
+ *   3. one gather of the 3x3 tiles around the two old positions and the new one:
+ *      orphans (jets whose nearest neighbour was removed or merged), jets now
+ *      closer to the new jet, and the new jet's own nearest neighbour;
+ *   4. a 3x3 rescan per orphan;
+ *   5. the minima of the gathered tiles recomputed with LDS atomic min
+ *      (diJ bits, then label, then the winner's index).
+ */
+#include <hip/hip_runtime.h>
+
+#include "fjmi_internal.h"
+#include "fjmi_kernels.h"
+#include "fjmi_math.h"
+
+namespace fjmi {
+
+namespace tiled {
+
+constexpr int W = 64;       /* one wave per event */
+constexpr int MAXT = 512;   /* tiles per event */
+constexpr int MAXG = 32;    /* tiles gathered per step (3 x 3x3 + 1 <= 28) */
+constexpr int MAXO = 128;   /* orphans listed per step (more: rescan in rounds) */
+constexpr int BIG = 0x7fffffff;
+
+struct Shared {
+  unsigned long long tmin_d[MAXT]; /* diJ bits of the tile minimum (diJ >= 0, so bits order like values) */
+  int tmin_l[MAXT];                /* its label */
+  int tmin_p[MAXT];                /* its physical index */
+  int seg_start[MAXT], seg_count[MAXT], seg_cap[MAXT];
+  int g_tiles[MAXG];
+  int g_pref[MAXG + 1];
+  int orph[MAXO];
+  int norph;
+  int gcount;
+  int stamp[MAXT]; /* last gather that listed the tile (dedup) */
+  int stampv;
+  int pool_top;
+  int rebuild;
+  int ev;
+  int ny, nphi;
+  double y0, sy, sphi;
+};
+
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ double uni_d(double x) {
+  return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(x)),
+                          __builtin_amdgcn_readfirstlane(__double2loint(x)));
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l), __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_d(double v) {
+  int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, RM, 0xF, false);
+  int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, RM, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, RM, 0xF, false);
+}
+__device__ __forceinline__ double wave_min_d(double v) {
+  v = __builtin_fmin(v, dpp_d<0xB1, 0xF>(v));
+  v = __builtin_fmin(v, dpp_d<0x4E, 0xF>(v));
+  v = __builtin_fmin(v, dpp_d<0x141, 0xF>(v));
+  v = __builtin_fmin(v, dpp_d<0x140, 0xF>(v));
+  v = __builtin_fmin(v, dpp_d<0x142, 0xA>(v));
+  v = __builtin_fmin(v, dpp_d<0x143, 0xC>(v));
+  return readlane_d(v, 63);
+}
+__device__ __forceinline__ int wave_min_i(int v) {
+  v = min(v, dpp_i<0xB1, 0xF>(v));
+  v = min(v, dpp_i<0x4E, 0xF>(v));
+  v = min(v, dpp_i<0x141, 0xF>(v));
+  v = min(v, dpp_i<0x140, 0xF>(v));
+  v = min(v, dpp_i<0x142, 0xA>(v));
+  v = min(v, dpp_i<0x143, 0xC>(v));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+  v = max(v, dpp_i<0xB1, 0xF>(v));
+  v = max(v, dpp_i<0x4E, 0xF>(v));
+  v = max(v, dpp_i<0x141, 0xF>(v));
+  v = max(v, dpp_i<0x140, 0xF>(v));
+  v = max(v, dpp_i<0x142, 0xA>(v));
+  v = max(v, dpp_i<0x143, 0xC>(v));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+/* Lexicographic (d, lab) minimum over the wave; returns the winner's lane or -1. */
+__device__ __forceinline__ int wave_argmin(double d, int lab, double *dmin, int *lmin) {
+  const double m = wave_min_d(d);
+  unsigned long long bal = __ballot(d == m);
+  int ls;
+  if (__popcll(bal) == 1) {
+    const int ln = __ffsll(bal) - 1;
+    ls = __builtin_amdgcn_readlane(lab, ln);
+    *dmin = m;
+    *lmin = ls;
+    return ls == BIG ? -1 : ln;
+  }
+  const int key = (d == m) ? lab : BIG;
+  ls = wave_min_i(key);
+  *dmin = m;
+  *lmin = ls;
+  if (ls == BIG) return -1;
+  bal = __ballot(key == ls);
+  return __ffsll(bal) - 1;
+}
+
+/* Inclusive prefix sum over the wave (Hillis-Steele through DPP shifts is not
+ * available across rows on every control, so lanes go through LDS-free
+ * shuffles). */
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+struct Geo {
+  int ny, nphi;
+  double y0, sy, sphi;
+  __device__ int tile(double y, double phi) const {
+    const double fy = (y - y0) / sy;
+    const int ty = fy < 0.0 ? 0 : (fy >= (double)ny ? ny - 1 : (int)fy);
+    int tp = (int)(phi / sphi);
+    tp = tp >= nphi ? nphi - 1 : (tp < 0 ? 0 : tp);
+    return ty * nphi + tp;
+  }
+  /* distinct tiles of the 3x3 neighbourhood of t, written to out (returns count) */
+  __device__ int neigh(int t, int *out) const {
+    const int ty = t / nphi, tp = t % nphi;
+    int k = 0;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = ty + dy;
+      if (y < 0 || y >= ny) continue;
+      for (int dp = -1; dp <= 1; ++dp) {
+        const int q = (tp + dp + nphi) % nphi;
+        const int u = y * nphi + q;
+        bool dup = false;
+        for (int m = 0; m < k; ++m) dup |= out[m] == u;
+        if (!dup) out[k++] = u;
+      }
+    }
+    return k;
+  }
+};
+
+/* The per-event views of the global scratch. */
+struct Ev {
+  double2 *rp;  /* (rapidity, phi) */
+  double2 *wn;  /* (weight, NN distance) */
+  double *dij;  /* diJ */
+  int4 *meta;   /* (nn physical or -1, label, tile, position in the tile segment) */
+  double4 *p4;
+  int *hist;
+  int *lab2;    /* label -> physical */
+  int *pool;    /* tile segments */
+  int pool_size;
+};
+
+__device__ __noinline__ double4 kin_coords_t(double px, double py, double pz, double E, int p) {
+  const double kt2 = px * px + py * py;
+  return make_double4(fjmi_rap(pz, E, kt2), fjmi_phi(px, py, kt2), fjmi_weight(kt2, p), 0.0);
+}
+
+__device__ __forceinline__ double diJ_of(double nnd, double w, int nn, double wnn) {
+  return nnd * ((nn >= 0 && wnn < w) ? wnn : w);
+}
+
+/* Rebuilds every tile segment from the live jets (labels 0..n-1): counting
+ * sort into the pool with 50% slack.  Used at the start of an event and when a
+ * relocation finds the pool exhausted. */
+__device__ void rebuild_segments(Shared &sh, const Ev &ev, int n, int ntiles, int lane) {
+  for (int t = lane; t < ntiles; t += W) sh.seg_count[t] = 0;
+  __syncthreads();
+  for (int l = lane; l < n; l += W) {
+    const int p = ev.lab2[l];
+    atomicAdd(&sh.seg_count[ev.meta[p].z], 1);
+  }
+  __syncthreads();
+  int carry = 0;
+  for (int t0 = 0; t0 < ntiles; t0 += W) {
+    const int t = t0 + lane;
+    const int c = t < ntiles ? sh.seg_count[t] : 0;
+    const int cap = c + (c >> 1) + 2;
+    const int incl = wave_incl_scan(cap, lane);
+    if (t < ntiles) {
+      sh.seg_start[t] = carry + incl - cap;
+      sh.seg_cap[t] = cap;
+      sh.seg_count[t] = 0;
+    }
+    carry += __shfl(incl, 63, 64);
+  }
+  if (lane == 0) {
+    sh.pool_top = carry;
+    sh.rebuild = 0;
+  }
+  __syncthreads();
+  for (int l = lane; l < n; l += W) {
+    const int p = ev.lab2[l];
+    const int t = ev.meta[p].z;
+    const int pos = sh.seg_start[t] + atomicAdd(&sh.seg_count[t], 1);
+    ev.pool[pos] = p;
+    ev.meta[p].w = pos;
+  }
+  __threadfence_block();
+  __syncthreads();
+}
+
+__device__ __forceinline__ int gathered(const Shared &sh, const Ev &ev, int ng, int j);
+
+/* Recomputes the (diJ, label) minimum of tiles listed in sh.g_tiles[0..ng) from
+ * all their members. */
+__device__ void refresh_minima(Shared &sh, const Ev &ev, int ng, int lane) {
+  if (lane < ng) {
+    const int t = sh.g_tiles[lane];
+    sh.tmin_d[t] = ~0ull;
+    sh.tmin_l[t] = BIG;
+    sh.tmin_p[t] = -1;
+  }
+  __syncthreads();
+  const int total = sh.g_pref[ng];
+  /* three passes over all members: the diJ minimum, then the lowest label among
+   * members at that minimum, then its index (each pass must see the final result
+   * of the previous one across every chunk of 64 members) */
+  for (int phase = 0; phase < 3; ++phase) {
+    for (int j0 = 0; j0 < total; j0 += W) {
+      const int p = gathered(sh, ev, ng, j0 + lane);
+      if (p >= 0) {
+        const int t = ev.meta[p].z;
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(ev.dij[p]);
+        if (phase == 0) {
+          atomicMin(&sh.tmin_d[t], bits);
+        } else if (sh.tmin_d[t] == bits) {
+          const int lab = ev.meta[p].y;
+          if (phase == 1)
+            atomicMin(&sh.tmin_l[t], lab);
+          else if (sh.tmin_l[t] == lab)
+            sh.tmin_p[t] = p;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+/* Sets sh.g_tiles to the distinct union of the 3x3 neighbourhoods of the given
+ * tiles (centres < 0 are skipped) plus `extra` (if >= 0), and sh.g_pref to the
+ * prefix sums of their member counts.  Lane i < 27 proposes neighbour i%9 of
+ * centre i/9 (c0, c1, c2); duplicates are dropped through a per-tile stamp. */
+__device__ int build_gather(Shared &sh, const Geo &geo, int c0, int c1, int c2, int extra, int lane) {
+  if (lane == 0) sh.stampv += 1;
+  __syncthreads();
+  const int stampv = uni(sh.stampv);
+  int cand = -1;
+  if (lane < 27) {
+    const int c = lane < 9 ? c0 : (lane < 18 ? c1 : c2);
+    if (c >= 0) {
+      const int ty = c / geo.nphi, tp = c % geo.nphi;
+      const int y = ty + (lane % 9) / 3 - 1;
+      if (y >= 0 && y < geo.ny) cand = y * geo.nphi + (tp + (lane % 9) % 3 - 1 + geo.nphi) % geo.nphi;
+    }
+  } else if (lane == 27) {
+    cand = extra;
+  }
+  bool keep = false;
+  if (cand >= 0) keep = atomicExch(&sh.stamp[cand], stampv) != stampv;
+  const unsigned long long kb = __ballot(keep);
+  const int idx = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(kb >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)kb, 0u));
+  const int cnt = keep ? sh.seg_count[cand] : 0;
+  const int incl = wave_incl_scan(cnt, lane);
+  if (keep) {
+    sh.g_tiles[idx] = cand;
+    sh.g_pref[idx + 1] = incl;
+  }
+  if (lane == 0) sh.g_pref[0] = 0;
+  __syncthreads();
+  return (int)__popcll(kb);
+}
+
+/* Member j of the gathered tiles: physical index (or -1 past the end). */
+__device__ __forceinline__ int gathered(const Shared &sh, const Ev &ev, int ng, int j) {
+  if (j >= sh.g_pref[ng]) return -1;
+  int lo = 0, hi = ng - 1; /* largest g with g_pref[g] <= j */
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sh.g_pref[mid] <= j)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int t = sh.g_tiles[lo];
+  return ev.pool[sh.seg_start[t] + (j - sh.g_pref[lo])];
+}
+
+/* Removes physical p from tile t's segment (swap with the segment's last). */
+__device__ void seg_remove(Shared &sh, const Ev &ev, int t, int p) {
+  const int pos = ev.meta[p].w;
+  const int last = sh.seg_start[t] + sh.seg_count[t] - 1;
+  if (pos != last) {
+    const int q = ev.pool[last];
+    ev.pool[pos] = q;
+    ev.meta[q].w = pos;
+  }
+  sh.seg_count[t] -= 1;
+}
+
+/* Appends physical p to tile t's segment; relocates the segment to the end of
+ * the pool with twice the capacity when full, or flags a rebuild when the pool
+ * is exhausted (p is then recorded only in its metadata). */
+__device__ void seg_insert(Shared &sh, const Ev &ev, int t, int p) {
+  if (sh.seg_count[t] == sh.seg_cap[t]) {
+    const int ncap = 2 * sh.seg_cap[t] + 2;
+    if (sh.pool_top + ncap > ev.pool_size) {
+      sh.rebuild = 1;
+      return;
+    }
+    const int ns = sh.pool_top;
+    for (int i = 0; i < sh.seg_count[t]; ++i) {
+      const int q = ev.pool[sh.seg_start[t] + i];
+      ev.pool[ns + i] = q;
+      ev.meta[q].w = ns + i;
+    }
+    sh.seg_start[t] = ns;
+    sh.seg_cap[t] = ncap;
+    sh.pool_top += ncap;
+  }
+  const int pos = sh.seg_start[t] + sh.seg_count[t];
+  ev.pool[pos] = p;
+  ev.meta[p].w = pos;
+  sh.seg_count[t] += 1;
+}
+
+/* Lexicographic (distance, label) nearest neighbour of physical o among the
+ * gathered tiles (all lanes). Returns the winner's physical index or -1. */
+__device__ int nn_scan(const Shared &sh, const Ev &ev, int ng, int o, double orap, double ophi, double R2, int lane,
+                       double *best_d, double *best_w) {
+  const int total = sh.g_pref[ng];
+  double bd = R2, bw = 0.0;
+  int bl = BIG, bp = -1;
+  for (int j0 = 0; j0 < total; j0 += W) {
+    const int q = gathered(sh, ev, ng, j0 + lane);
+    if (q >= 0 && q != o) {
+      const double2 rp = ev.rp[q];
+      const double d = fjmi_dr2(orap, ophi, rp.x, rp.y);
+      const int lab = ev.meta[q].y;
+      if (d < R2 && (d < bd || (d == bd && lab < bl))) {
+        bd = d;
+        bl = lab;
+        bp = q;
+        bw = ev.wn[q].x;
+      }
+    }
+  }
+  double md;
+  int ml;
+  const int ln = wave_argmin(bd, bl, &md, &ml);
+  *best_d = md;
+  if (ln < 0) {
+    *best_w = 0.0;
+    return -1;
+  }
+  *best_w = readlane_d(bw, ln);
+  return __builtin_amdgcn_readlane(bp, ln);
+}
+
+}  // namespace tiled
+
+using namespace tiled;
+
+__global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts) {
+  __shared__ Shared sh;
+  const int lane = threadIdx.x;
+  for (int t = lane; t < MAXT; t += W) sh.stamp[t] = 0;
+  if (lane == 0) sh.stampv = 0;
+  for (;;) {
+    __syncthreads();
+    if (lane == 0) sh.ev = (int)(atomicAdd(a.head, 1u) - a.head_base);
+    __syncthreads();
+    const int qi = uni(sh.ev);
+    if (qi >= a.n_queue) break;
+    const int e = a.queue ? uni(a.queue[qi]) : qi;
+    const int64_t base = a.offsets ? a.offsets[e] : 0;
+    const int N = a.offsets ? uni((int)(a.offsets[e + 1] - base)) : a.n_single;
+    if (N <= 0) {
+      if (lane == 0) a.njets[e] = 0;
+      continue;
+    }
+    const int p = a.p_arr ? a.p_arr[e] : a.p_uni;
+    const double R = a.R_arr ? a.R_arr[e] : a.R_uni;
+    const double R2 = R * R;
+    const double invR2 = 1.0 / R2;
+    Ev ev;
+    ev.rp = reinterpret_cast<double2 *>(ts.rp) + base;
+    ev.wn = reinterpret_cast<double2 *>(ts.wn) + base;
+    ev.dij = ts.dij + base;
+    ev.meta = reinterpret_cast<int4 *>(ts.meta) + base;
+    ev.p4 = reinterpret_cast<double4 *>(ts.p4) + base;
+    ev.hist = ts.hist + base;
+    ev.lab2 = ts.lab2 + base;
+    ev.pool = ts.pool + (3 * base + (int64_t)e * ts.pool_per_event);
+    ev.pool_size = 3 * N + ts.pool_per_event;
+
+    /* ---- prologue: kinematics, rapidity range ---- */
+    double ylo = 1e300, yhi = -1e300;
+    for (int s = lane; s < N; s += W) {
+      const double *q = a.in + (size_t)(base + s) * a.in_stride + a.in_off;
+      const double4 q4 = make_double4(q[0], q[1], q[2], q[3]);
+      const double4 c = kin_coords_t(q4.x, q4.y, q4.z, q4.w, p);
+      ev.rp[s] = make_double2(c.x, c.y);
+      ev.wn[s] = make_double2(c.z, R2);
+      ev.p4[s] = q4;
+      ev.hist[s] = s;
+      ev.lab2[s] = s;
+      if (c.x > -1e4 && c.x < 1e4) {
+        ylo = c.x < ylo ? c.x : ylo;
+        yhi = c.x > yhi ? c.x : yhi;
+      }
+    }
+    {
+      const double mn = wave_min_d(ylo);
+      const double mx = -wave_min_d(-yhi);
+      if (lane == 0) {
+        const double side = R * (1.0 + 0x1p-20);
+        double lo = mn, hi = mx;
+        if (!(lo <= hi)) {
+          lo = 0.0;
+          hi = 0.0;
+        }
+        int np = (int)(FJMI_TWOPI / side);
+        np = np < 1 ? 1 : (np > MAXT ? MAXT : np);
+        int nyc = (int)((hi - lo) / side) + 1;
+        const int nymax = MAXT / np;
+        nyc = nyc > nymax ? nymax : (nyc < 1 ? 1 : nyc);
+        sh.ny = nyc;
+        sh.nphi = np;
+        sh.y0 = lo;
+        sh.sy = (hi - lo) / nyc > side ? (hi - lo) / nyc : side;
+        sh.sphi = FJMI_TWOPI / np;
+      }
+    }
+    __syncthreads();
+    Geo geo;
+    geo.ny = uni(sh.ny);
+    geo.nphi = uni(sh.nphi);
+    geo.y0 = uni_d(sh.y0);
+    geo.sy = uni_d(sh.sy);
+    geo.sphi = uni_d(sh.sphi);
+    const int ntiles = geo.ny * geo.nphi;
+    for (int s = lane; s < N; s += W) {
+      const double2 rp = ev.rp[s];
+      ev.meta[s] = make_int4(-1, s, geo.tile(rp.x, rp.y), 0);
+    }
+    __threadfence_block();
+    __syncthreads();
+    rebuild_segments(sh, ev, N, ntiles, lane);
+
+    /* ---- initial nearest neighbours: each lane scans the 3x3 tiles of its jets ---- */
+    for (int s = lane; s < N; s += W) {
+      const double2 rp = ev.rp[s];
+      const double ws = ev.wn[s].x;
+      int nb[9];
+      const int m = geo.neigh(ev.meta[s].z, nb);
+      double bd = R2, bw = 0.0;
+      int bl = BIG, bp = -1;
+      for (int i = 0; i < m; ++i) {
+        const int t = nb[i];
+        const int st = sh.seg_start[t], ct = sh.seg_count[t];
+        for (int u = 0; u < ct; ++u) {
+          const int q = ev.pool[st + u];
+          if (q == s) continue;
+          const double2 rq = ev.rp[q];
+          const double d = fjmi_dr2(rp.x, rp.y, rq.x, rq.y);
+          if (d < bd || (d == bd && d < R2 && q < bl)) { /* labels == physical indices here */
+            bd = d;
+            bl = q;
+            bp = q;
+          }
+        }
+      }
+      if (bp >= 0) bw = ev.wn[bp].x;
+      ev.meta[s].x = bp;
+      ev.wn[s] = make_double2(ws, bd);
+      ev.dij[s] = diJ_of(bd, ws, bp, bw);
+    }
+    __threadfence_block();
+    __syncthreads();
+    /* tile minima of every tile */
+    for (int t0 = 0; t0 < ntiles; t0 += MAXG) {
+      const int cnt = min(MAXG, ntiles - t0);
+      if (lane == 0) {
+        int acc = 0;
+        sh.g_pref[0] = 0;
+        for (int u = 0; u < cnt; ++u) {
+          sh.g_tiles[u] = t0 + u;
+          acc += sh.seg_count[t0 + u];
+          sh.g_pref[u + 1] = acc;
+        }
+      }
+      __syncthreads();
+      refresh_minima(sh, ev, cnt, lane);
+    }
+
+    /* ---- the N recombination steps ---- */
+    int nj = 0;
+    for (int step = 0; step < N; ++step) {
+      const int n_old = N - step;
+      /* 1. argmin over the tile minima */
+      double bd = __builtin_inf();
+      int bl = BIG, bt = -1;
+      for (int t = lane; t < ntiles; t += W) {
+        const double d = __longlong_as_double((long long)sh.tmin_d[t]);
+        const int l = sh.tmin_l[t];
+        if (l != BIG && (d < bd || (d == bd && l < bl))) {
+          bd = d;
+          bl = l;
+          bt = t;
+        }
+      }
+      double dmin;
+      int lA;
+      const int wl = wave_argmin(bd, bl, &dmin, &lA);
+      const int tA = wl < 0 ? -1 : uni(__builtin_amdgcn_readlane(bt, wl));
+      const int A = tA < 0 ? -1 : uni(sh.tmin_p[tA]);
+      if (A < 0) { /* no live jet found: inconsistent state, stop this event */
+        if (lane == 0) atomicOr(a.status, 2u);
+        break;
+      }
+      /* 2. the merge */
+      const int4 mA = ev.meta[A];
+      const int B = uni(mA.x);
+      int phys_hi, phys_lo, lab_hi;
+      int4 mB = make_int4(-1, -1, -1, -1);
+      if (B >= 0) {
+        mB = ev.meta[B];
+        const bool a_hi = mA.y > mB.y;
+        phys_hi = a_hi ? A : B;
+        phys_lo = a_hi ? B : A;
+        lab_hi = uni(a_hi ? mA.y : mB.y);
+      } else {
+        phys_hi = A;
+        phys_lo = -1;
+        lab_hi = uni(mA.y);
+      }
+      const int tail = n_old - 1;
+      const int pt = uni(ev.lab2[tail]);
+      const int tile_hi = uni(phys_hi == A ? mA.z : mB.z);
+      const int tile_lo_old = phys_lo < 0 ? -1 : uni(phys_lo == A ? mA.z : mB.z);
+      const int tile_pt = uni(ev.meta[pt].z);
+      double nrap = 0.0, nphi = 0.0, nw = 0.0;
+      int tile_new = -1;
+      const double4 qhi = ev.p4[phys_hi];
+      const int hhi = ev.hist[phys_hi];
+      if (phys_lo >= 0) {
+        const double4 qlo = ev.p4[phys_lo];
+        const int hlo = ev.hist[phys_lo];
+        const double4 qn = make_double4(qhi.x + qlo.x, qhi.y + qlo.y, qhi.z + qlo.z, qhi.w + qlo.w);
+        const double4 c = kin_coords_t(qn.x, qn.y, qn.z, qn.w, p);
+        nrap = c.x;
+        nphi = c.y;
+        nw = c.z;
+        tile_new = uni(geo.tile(nrap, nphi));
+        if (lane == 0) {
+          a.hist_p1[base + step] = min(hhi, hlo);
+          a.hist_p2[base + step] = max(hhi, hlo);
+          a.hist_dij[base + step] = dmin * invR2;
+          ev.p4[phys_lo] = qn;
+          ev.hist[phys_lo] = N + step;
+          ev.rp[phys_lo] = make_double2(nrap, nphi);
+          ev.wn[phys_lo] = make_double2(nw, R2);
+          ev.meta[phys_lo].x = -1;
+          if (tile_new != tile_lo_old) {
+            seg_remove(sh, ev, tile_lo_old, phys_lo);
+            ev.meta[phys_lo].z = tile_new;
+            seg_insert(sh, ev, tile_new, phys_lo);
+          }
+        }
+      } else if (lane == 0) {
+        a.hist_p1[base + step] = hhi;
+        a.hist_p2[base + step] = -1;
+        a.hist_dij[base + step] = dmin * invR2;
+        double *jo = a.jets + 4 * (base + nj);
+        jo[0] = qhi.x;
+        jo[1] = qhi.y;
+        jo[2] = qhi.z;
+        jo[3] = qhi.w;
+      }
+      if (lane == 0) {
+        if (a.pj_out) {
+          const double4 qt = ev.p4[pt];
+          unsigned char *rec = a.pj_out + 40 * (size_t)(base + tail);
+          reinterpret_cast<int *>(rec)[0] = tail;
+          reinterpret_cast<int *>(rec)[1] = phys_lo < 0 ? 1 : 0;
+          double *dp = reinterpret_cast<double *>(rec + 8);
+          const double4 o = phys_lo < 0 ? qhi : qt;
+          dp[0] = o.x;
+          dp[1] = o.y;
+          dp[2] = o.z;
+          dp[3] = o.w;
+        }
+        seg_remove(sh, ev, tile_hi, phys_hi);
+        ev.meta[phys_hi] = make_int4(-1, -1, -1, -1);
+        if (pt != phys_hi) { /* N2Plain compaction: the last jet takes the removed slot's label */
+          ev.meta[pt].y = lab_hi;
+          ev.lab2[lab_hi] = pt;
+        }
+      }
+      if (phys_lo < 0) ++nj;
+      __threadfence_block();
+      __syncthreads();
+      const int n = n_old - 1;
+      if (n == 0) break;
+      if (sh.rebuild) { /* pool exhausted by a relocation: lay all segments out again */
+        rebuild_segments(sh, ev, n, ntiles, lane);
+      }
+
+      /* 3. gather the 3x3 tiles around the old and new positions and the moved jet's tile */
+      const int ng = build_gather(sh, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane);
+      const int total = sh.g_pref[ng];
+      if (lane == 0) sh.norph = 0;
+      __syncthreads();
+      double cbd = R2, cbw = 0.0;
+      int cbl = BIG, cbp = -1;
+      for (int j0 = 0; j0 < total; j0 += W) {
+        const int q = gathered(sh, ev, ng, j0 + lane);
+        bool orph = false;
+        if (q >= 0 && q != phys_lo) {
+          int4 mq = ev.meta[q];
+          orph = (mq.x == phys_hi) || (phys_lo >= 0 && mq.x == phys_lo);
+          if (phys_lo >= 0) {
+            const double2 rq = ev.rp[q];
+            const double2 wq = ev.wn[q];
+            const double d = fjmi_dr2(rq.x, rq.y, nrap, nphi);
+            if (!orph && d < wq.y) {
+              ev.meta[q].x = phys_lo;
+              ev.wn[q] = make_double2(wq.x, d);
+              ev.dij[q] = d * (nw < wq.x ? nw : wq.x);
+            }
+            if (d < R2 && (d < cbd || (d == cbd && mq.y < cbl))) {
+              cbd = d;
+              cbl = mq.y;
+              cbp = q;
+              cbw = wq.x;
+            }
+          }
+        }
+        const unsigned long long ob = __ballot(orph);
+        if (ob) {
+          int obase = 0;
+          if (lane == 0) obase = atomicAdd(&sh.norph, (int)__popcll(ob));
+          obase = uni(obase);
+          const int idx = obase + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(ob >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((unsigned)ob, 0u));
+          if (orph && idx < MAXO) sh.orph[idx] = q;
+        }
+      }
+      if (phys_lo >= 0) { /* the new jet's own nearest neighbour */
+        double md;
+        int ml;
+        const int ln = wave_argmin(cbd, cbl, &md, &ml);
+        const int nnB = ln < 0 ? -1 : __builtin_amdgcn_readlane(cbp, ln);
+        const double wB = ln < 0 ? 0.0 : readlane_d(cbw, ln);
+        if (lane == 0) {
+          ev.meta[phys_lo].x = nnB;
+          ev.wn[phys_lo] = make_double2(nw, md);
+          ev.dij[phys_lo] = diJ_of(md, nw, nnB, wB);
+        }
+      }
+      __threadfence_block();
+      __syncthreads();
+
+      /* 4. orphan rescans over their own 3x3 tiles */
+      const int norph = uni(sh.norph);
+      if (norph > 0) {
+        const int nlist = min(norph, MAXO);
+        for (int i = 0; i < nlist; ++i) {
+          const int o = uni(sh.orph[i]);
+          const int4 mo = ev.meta[o];
+          const double2 ro = ev.rp[o];
+          const double wo = ev.wn[o].x;
+          const int ngo = build_gather(sh, geo, mo.z, -1, -1, -1, lane);
+          double od, owb;
+          const int nno = nn_scan(sh, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
+          if (lane == 0) {
+            ev.meta[o].x = nno;
+            ev.wn[o] = make_double2(wo, od);
+            ev.dij[o] = diJ_of(od, wo, nno, owb);
+          }
+          __threadfence_block();
+          __syncthreads();
+        }
+        if (norph > MAXO) { /* more orphans than the list holds: rescan the rest from the gather */
+          const int ng2 = build_gather(sh, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane);
+          const int tot2 = sh.g_pref[ng2];
+          for (int j0 = 0; j0 < tot2; j0 += W) {
+            const int q = gathered(sh, ev, ng2, j0 + lane);
+            const bool orph = q >= 0 && q != phys_lo &&
+                              (ev.meta[q].x == phys_hi || (phys_lo >= 0 && ev.meta[q].x == phys_lo));
+            unsigned long long ob = __ballot(orph);
+            while (ob) {
+              const int ln = __ffsll(ob) - 1;
+              ob &= ob - 1;
+              const int o = __builtin_amdgcn_readlane(q, ln);
+              const int4 mo = ev.meta[o];
+              const double2 ro = ev.rp[o];
+              const double wo = ev.wn[o].x;
+              const int ngo = build_gather(sh, geo, mo.z, -1, -1, -1, lane);
+              double od, owb;
+              const int nno = nn_scan(sh, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
+              if (lane == 0) {
+                ev.meta[o].x = nno;
+                ev.wn[o] = make_double2(wo, od);
+                ev.dij[o] = diJ_of(od, wo, nno, owb);
+              }
+              __threadfence_block();
+              __syncthreads();
+              (void)build_gather(sh, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane);
+            }
+          }
+        }
+      }
+
+      /* 5. tile minima of every gathered tile */
+      const int ng3 = build_gather(sh, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane);
+      refresh_minima(sh, ev, ng3, lane);
+    }
+    if (lane == 0) a.njets[e] = nj;
+  }
+}
+
+const void *const kTiledKernel = reinterpret_cast<const void *>(&tiled_kernel);
+const int kTiledMaxTiles = MAXT;
+
+}  // namespace fjmi

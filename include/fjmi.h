@@ -43,19 +43,23 @@ enum fjmi_status {
 };
 
 enum fjmi_engine {
-  FJMI_ENGINE_AUTO = 0,  /* dense for every multiplicity the tuning table covers */
+  FJMI_ENGINE_AUTO = 0,  /* the engine of the event's multiplicity bucket */
   FJMI_ENGINE_DENSE = 1, /* replaces tri_matrix.cu / n_array.cu */
   FJMI_ENGINE_TILED = 2, /* replaces grid.cu */
   FJMI_ENGINE_EMPTY = 3  /* replaces tri_matrix_empty.cu: per-step launch floor, no result */
 };
 
 /* One row of the multiplicity-bucket table: events with N <= max_n (and above
- * the previous row's max_n) run on the dense kernel with `threads` threads per
- * workgroup, `slots` slots per thread (capacity threads*slots >= max_n), and
- * four-momenta in LDS (p4_global = 0) or global memory (p4_global = 1).
+ * the previous row's max_n) run on `engine`:
+ *   FJMI_ENGINE_DENSE: `threads` threads per workgroup, `slots` slots per thread
+ *     (capacity threads*slots >= max_n), four-momenta in LDS (p4_global = 0) or
+ *     global memory (p4_global = 1);
+ *   FJMI_ENGINE_TILED: one 64-lane wave per event (threads = 64, slots and
+ *     p4_global unused).
  * Replaces the reference's occupancy-derived launch shapes (launch.h:11-60). */
 typedef struct {
   int32_t max_n;
+  int32_t engine;
   int32_t threads;
   int32_t slots;
   int32_t p4_global;
@@ -74,7 +78,8 @@ int fjmi_create(int device, fjmi_context **out);
 int fjmi_destroy(fjmi_context *ctx);
 
 /* Kernel configurations compiled into the library: writes up to `cap` rows
- * (max_n = capacity of the configuration) and returns how many exist. */
+ * (max_n = capacity of the configuration; the tiled engine reports INT32_MAX)
+ * and returns how many exist. */
 int fjmi_kernel_configs(fjmi_bucket *out, int cap);
 
 /* Replace / read the multiplicity-bucket table (rows sorted by max_n). */

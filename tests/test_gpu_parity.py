@@ -31,39 +31,55 @@ def first_diff(a, b):
     return None
 
 
-def check_event(engine, p4, p, R, label):
-    print(f"[gpu parity] {label} N={len(p4)} p={p} R={R}", flush=True)
-    want = orc.n2plain(p4, p, R)
-    got = engine.cluster_event(p4, p, R)
+ENGINES = [(1, "dense"), (2, "tiled")]
+
+
+def check_event(engine, p4, p, R, label, eng=0):
+    print(f"[gpu parity] {label} N={len(p4)} p={p} R={R} engine={eng}", flush=True)
+    want = orc.n2plain(p4, p, R) if len(p4) <= 6000 else orc.tiled(p4, p, R)
+    got = engine.cluster_event(p4, p, R, engine=eng)
     assert same(want, got), f"{label}: first differing step {first_diff(want, got)}"
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("eng,ename", ENGINES)
 @pytest.mark.parametrize("fname", ["354_fastjet.dat", "808_cms.dat"])
 @pytest.mark.parametrize("p,alg", ALGOS)
 @pytest.mark.parametrize("R", [0.4, 1.0])
-def test_reference_files(engine, fname, p, alg, R):
+def test_reference_files(engine, fname, p, alg, R, eng, ename):
     ev = orc.read_events(os.path.join(GOLDEN, fname))[0]
-    check_event(engine, ev, p, R, f"{fname}/{alg}")
+    check_event(engine, ev, p, R, f"{fname}/{alg}/{ename}", eng)
 
 
 @pytest.mark.gpu
-def test_multi_event_file(engine):
+@pytest.mark.parametrize("eng,ename", ENGINES)
+def test_multi_event_file(engine, eng, ename):
     events = orc.read_events(os.path.join(GOLDEN, "multi-event.dat"))
     assert [len(e) for e in events] == [354, 713, 676, 552, 520, 294, 582, 515, 417, 837]
     for k, ev in enumerate(events):
         for p, alg in ALGOS:
-            check_event(engine, ev, p, 0.4, f"multi-event[{k}]/{alg}")
+            check_event(engine, ev, p, 0.4, f"multi-event[{k}]/{alg}/{ename}", eng)
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("eng,ename", ENGINES)
 @pytest.mark.parametrize("workload", [1, 2])
 @pytest.mark.parametrize("p,alg", ALGOS)
-def test_generated_single(engine, workload, p, alg):
+def test_generated_single(engine, workload, p, alg, eng, ename):
     import fastjet_gpu_amd as fj
     ev = fj.gen_event(workload, 0)
-    for R in (0.4, 1.0):
-        check_event(engine, ev, p, R, f"workload{workload}/{alg}")
+    for R in (0.4, 1.0, 3.0):
+        check_event(engine, ev, p, R, f"workload{workload}/{alg}/{ename}", eng)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("p,alg", [(1, "kt"), (-1, "antikt")])
+def test_high_multiplicity_tiled(engine, p, alg):
+    """Workload 4 (N ~ U[5000, 20000]) on the tiled engine, R = 0.4."""
+    import fastjet_gpu_amd as fj
+    for e in (0, 2):
+        ev = fj.gen_event(4, e)
+        check_event(engine, ev, p, 0.4, f"workload4[{e}]/{alg}/tiled", 2)
 
 
 @pytest.mark.gpu
@@ -72,31 +88,32 @@ def test_every_bucket_size(engine):
     import fastjet_gpu_amd as fj
     rng = np.random.default_rng(7)
     big = fj.gen_event(4, 3)
-    for (max_n, threads, slots, g) in engine.buckets():
+    for (max_n, eng, threads, slots, g) in engine.buckets():
         for n in sorted({max(1, max_n - 1), max_n, max_n + 1}):
-            if n > len(big) or n > engine.buckets()[-1][0]:
+            if n > len(big):
                 continue
             sel = big[rng.permutation(len(big))[:n]]
-            check_event(engine, sel, -1, 0.4, f"bucket<= {max_n} ({threads}x{slots},g{g})")
+            check_event(engine, sel, -1, 0.4, f"bucket<= {max_n} (engine {eng} {threads}x{slots},g{g})")
 
 
 @pytest.mark.gpu
-def test_edge_cases(engine):
+@pytest.mark.parametrize("eng,ename", ENGINES)
+def test_edge_cases(engine, eng, ename):
     one = np.array([[1.0, 2.0, 3.0, 10.0]])
-    check_event(engine, one, 1, 0.4, "single particle")
+    check_event(engine, one, 1, 0.4, "single particle", eng)
     two = np.array([[1.0, 0.0, 0.0, 1.0], [1.0, 0.01, 0.0, 1.0001]])
     for p, alg in ALGOS:
-        check_event(engine, two, p, 0.4, f"two particles/{alg}")
+        check_event(engine, two, p, 0.4, f"two particles/{alg}", eng)
     # exact duplicates: many exact distance ties and orphans
     base = np.array([[3.0, 1.0, 0.5, 4.0], [0.5, 2.0, -1.0, 3.0], [-1.0, -1.0, 0.2, 2.0]])
     dup = np.repeat(base, 40, axis=0)
     for p, alg in ALGOS:
-        check_event(engine, dup, p, 0.7, f"duplicates/{alg}")
+        check_event(engine, dup, p, 0.7, f"duplicates/{alg}", eng)
     # zero-pt and beam-like momenta (rapidity +-(1e5+|pz|))
     zp = np.array([[0.0, 0.0, 5.0, 5.0], [0.0, 0.0, -7.0, 7.0], [1.0, 1.0, 0.0, 2.0], [0.0, 0.0, 1.0, 2.0],
                    [1e-3, 0.0, 0.0, 1e-3], [0.0, 0.0, 0.0, 0.0]])
     for p, alg in ALGOS:
-        check_event(engine, zp, p, 0.4, f"zero pt/{alg}")
+        check_event(engine, zp, p, 0.4, f"zero pt/{alg}", eng)
     # a regular lattice: many exact geometric ties
     g = []
     for i in range(12):
@@ -105,13 +122,36 @@ def test_edge_cases(engine):
             pt = 1.0 + 0.125 * ((i * 7 + j * 3) % 5)
             g.append([pt * np.cos(phi), pt * np.sin(phi), pt * np.sinh(eta), pt * np.cosh(eta)])
     for p, alg in ALGOS:
-        check_event(engine, np.array(g), p, 0.3, f"lattice/{alg}")
+        check_event(engine, np.array(g), p, 0.3, f"lattice/{alg}", eng)
 
 
 @pytest.mark.gpu
-def test_empty_event(engine):
-    h1, h2, hd, jets = engine.cluster_event(np.zeros((0, 4)), -1, 0.4)
+@pytest.mark.parametrize("eng", [1, 2])
+def test_empty_event(engine, eng):
+    h1, h2, hd, jets = engine.cluster_event(np.zeros((0, 4)), -1, 0.4, engine=eng)
     assert len(h1) == 0 and len(jets) == 0
+
+
+@pytest.mark.gpu
+def test_batch_all_tiled(engine):
+    """A batch with every event on the tiled engine (bucket table with one tiled row)."""
+    import fastjet_gpu_amd as fj
+    saved = engine.buckets()
+    try:
+        engine.set_buckets([(1000000000, 2, 64, 0, 0)])
+        p4, off = fj.gen_workload(3, 64, first=100)
+        ne = len(off) - 1
+        ps = np.array([(-1, 0, 1)[e % 3] for e in range(ne)], np.int32)
+        Rs = np.array([(0.4, 0.8, 1.0, 0.6)[e % 4] for e in range(ne)], np.float64)
+        print(f"[gpu parity] tiled batch workload3 x{ne}", flush=True)
+        h1, h2, hd, jets, nj = engine.cluster_batch(p4, off, ps, Rs)
+        for e in range(ne):
+            a, b = off[e], off[e + 1]
+            want = orc.n2plain(p4[a:b], int(ps[e]), float(Rs[e]))
+            got = (h1[a:b], h2[a:b], hd[a:b], jets[a:a + nj[e]])
+            assert same(want, got), f"tiled batch event {e} (N={b - a}): {first_diff(want, got)}"
+    finally:
+        engine.set_buckets(saved)
 
 
 @pytest.mark.gpu
