@@ -48,13 +48,27 @@ constexpr int MAXG = 32;    /* tiles gathered per step (3 x 3x3 + 1 <= 28) */
 constexpr int MAXO = 128;   /* orphans listed per step (more: rescan in rounds) */
 constexpr int BIG = 0x7fffffff;
 
+/* (diJ, label) minimum of a tile: diJ as its bit pattern (diJ >= 0, so the bits
+ * order like the values), the label, and the jet's physical index. */
+struct TMin {
+  unsigned long long d;
+  int l;
+  int p;
+};
+
+/* A list of gathered tiles and the prefix sums of their member counts. */
+struct Gather {
+  int tiles[MAXG];
+  int pref[MAXG + 1];
+};
+
 struct Shared {
-  unsigned long long tmin_d[MAXT]; /* diJ bits of the tile minimum (diJ >= 0, so bits order like values) */
-  int tmin_l[MAXT];                /* its label */
-  int tmin_p[MAXT];                /* its physical index */
+  TMin tmin[MAXT];
+  unsigned long long mt_d[W]; /* member table of a single-chunk gather: diJ bits, */
+  int mt_l[W];                /* labels */
+  int mt_p[W];                /* and physical indices, by gather index */
   int seg_start[MAXT], seg_count[MAXT], seg_cap[MAXT];
-  int g_tiles[MAXG];
-  int g_pref[MAXG + 1];
+  Gather g[2]; /* [0]: the step's gather, [1]: orphan rescans */
   int orph[MAXO];
   int norph;
   int gcount;
@@ -136,15 +150,16 @@ __device__ __forceinline__ int wave_argmin(double d, int lab, double *dmin, int 
   return __ffsll(bal) - 1;
 }
 
-/* Inclusive prefix sum over the wave (Hillis-Steele through DPP shifts is not
- * available across rows on every control, so lanes go through LDS-free
- * shuffles). */
-__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int u = __shfl_up(v, o, 64);
-    if (lane >= o) v += u;
-  }
+/* Inclusive prefix sum over the wave: row_shr 1,2,4,8 (zero beyond the row
+ * start) gives the scan within each row of 16; row_bcast15 (rows 1,3) and
+ * row_bcast31 (rows 2,3) carry the row totals. */
+__device__ __forceinline__ int wave_incl_scan(int v, int) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);
   return v;
 }
 
@@ -190,7 +205,7 @@ struct Ev {
   int pool_size;
 };
 
-__device__ __noinline__ double4 kin_coords_t(double px, double py, double pz, double E, int p) {
+__device__ __forceinline__ double4 kin_coords_t(double px, double py, double pz, double E, int p) {
   const double kt2 = px * px + py * py;
   return make_double4(fjmi_rap(pz, E, kt2), fjmi_phi(px, py, kt2), fjmi_weight(kt2, p), 0.0);
 }
@@ -221,7 +236,7 @@ __device__ void rebuild_segments(Shared &sh, const Ev &ev, int n, int ntiles, in
       sh.seg_cap[t] = cap;
       sh.seg_count[t] = 0;
     }
-    carry += __shfl(incl, 63, 64);
+    carry += __builtin_amdgcn_readlane(incl, 63);
   }
   if (lane == 0) {
     sh.pool_top = carry;
@@ -239,36 +254,37 @@ __device__ void rebuild_segments(Shared &sh, const Ev &ev, int n, int ntiles, in
   __syncthreads();
 }
 
-__device__ __forceinline__ int gathered(const Shared &sh, const Ev &ev, int ng, int j);
+__device__ __forceinline__ int gathered(const Shared &sh, const Gather &G, const Ev &ev, int ng, int j,
+                                        int *tile = nullptr);
 
-/* Recomputes the (diJ, label) minimum of tiles listed in sh.g_tiles[0..ng) from
+/* Recomputes the (diJ, label) minimum of tiles listed in G.tiles[0..ng) from
  * all their members. */
-__device__ void refresh_minima(Shared &sh, const Ev &ev, int ng, int lane) {
+__device__ void refresh_minima(Shared &sh, const Gather &G, const Ev &ev, int ng, int lane) {
   if (lane < ng) {
-    const int t = sh.g_tiles[lane];
-    sh.tmin_d[t] = ~0ull;
-    sh.tmin_l[t] = BIG;
-    sh.tmin_p[t] = -1;
+    const int t = G.tiles[lane];
+    sh.tmin[t].d = ~0ull;
+    sh.tmin[t].l = BIG;
+    sh.tmin[t].p = -1;
   }
   __syncthreads();
-  const int total = sh.g_pref[ng];
+  const int total = G.pref[ng];
   /* three passes over all members: the diJ minimum, then the lowest label among
    * members at that minimum, then its index (each pass must see the final result
    * of the previous one across every chunk of 64 members) */
   for (int phase = 0; phase < 3; ++phase) {
     for (int j0 = 0; j0 < total; j0 += W) {
-      const int p = gathered(sh, ev, ng, j0 + lane);
+      const int p = gathered(sh, G, ev, ng, j0 + lane);
       if (p >= 0) {
         const int t = ev.meta[p].z;
         const unsigned long long bits = (unsigned long long)__double_as_longlong(ev.dij[p]);
         if (phase == 0) {
-          atomicMin(&sh.tmin_d[t], bits);
-        } else if (sh.tmin_d[t] == bits) {
+          atomicMin(&sh.tmin[t].d, bits);
+        } else if (sh.tmin[t].d == bits) {
           const int lab = ev.meta[p].y;
           if (phase == 1)
-            atomicMin(&sh.tmin_l[t], lab);
-          else if (sh.tmin_l[t] == lab)
-            sh.tmin_p[t] = p;
+            atomicMin(&sh.tmin[t].l, lab);
+          else if (sh.tmin[t].l == lab)
+            sh.tmin[t].p = p;
         }
       }
     }
@@ -276,11 +292,42 @@ __device__ void refresh_minima(Shared &sh, const Ev &ev, int ng, int lane) {
   }
 }
 
-/* Sets sh.g_tiles to the distinct union of the 3x3 neighbourhoods of the given
- * tiles (centres < 0 are skipped) plus `extra` (if >= 0), and sh.g_pref to the
+/* Tile minima of the gathered tiles from the member table (gathers of at most
+ * 64 members): lane j < total wrote member j; lane g < ng scans tile g's range. */
+__device__ void refresh_from_table(Shared &sh, const Gather &G, int ng, int total, int lane, int myq, int mylab,
+                                   unsigned long long mybits) {
+  if (lane < total) {
+    sh.mt_d[lane] = mybits;
+    sh.mt_l[lane] = mylab;
+    sh.mt_p[lane] = myq;
+  }
+  __syncthreads();
+  if (lane < ng) {
+    const int t = G.tiles[lane];
+    const int j1 = G.pref[lane + 1];
+    TMin best;
+    best.d = ~0ull;
+    best.l = BIG;
+    best.p = -1;
+    for (int j = G.pref[lane]; j < j1; ++j) {
+      const unsigned long long d = sh.mt_d[j];
+      const int l = sh.mt_l[j];
+      if (d < best.d || (d == best.d && l < best.l)) {
+        best.d = d;
+        best.l = l;
+        best.p = sh.mt_p[j];
+      }
+    }
+    sh.tmin[t] = best;
+  }
+  __syncthreads();
+}
+
+/* Sets G.tiles to the distinct union of the 3x3 neighbourhoods of the given
+ * tiles (centres < 0 are skipped) plus `extra` (if >= 0), and G.pref to the
  * prefix sums of their member counts.  Lane i < 27 proposes neighbour i%9 of
  * centre i/9 (c0, c1, c2); duplicates are dropped through a per-tile stamp. */
-__device__ int build_gather(Shared &sh, const Geo &geo, int c0, int c1, int c2, int extra, int lane) {
+__device__ int build_gather(Shared &sh, Gather &G, const Geo &geo, int c0, int c1, int c2, int extra, int lane) {
   if (lane == 0) sh.stampv += 1;
   __syncthreads();
   const int stampv = uni(sh.stampv);
@@ -302,27 +349,28 @@ __device__ int build_gather(Shared &sh, const Geo &geo, int c0, int c1, int c2, 
   const int cnt = keep ? sh.seg_count[cand] : 0;
   const int incl = wave_incl_scan(cnt, lane);
   if (keep) {
-    sh.g_tiles[idx] = cand;
-    sh.g_pref[idx + 1] = incl;
+    G.tiles[idx] = cand;
+    G.pref[idx + 1] = incl;
   }
-  if (lane == 0) sh.g_pref[0] = 0;
+  if (lane == 0) G.pref[0] = 0;
   __syncthreads();
   return (int)__popcll(kb);
 }
 
 /* Member j of the gathered tiles: physical index (or -1 past the end). */
-__device__ __forceinline__ int gathered(const Shared &sh, const Ev &ev, int ng, int j) {
-  if (j >= sh.g_pref[ng]) return -1;
+__device__ __forceinline__ int gathered(const Shared &sh, const Gather &G, const Ev &ev, int ng, int j, int *tile) {
+  if (j >= G.pref[ng]) return -1;
   int lo = 0, hi = ng - 1; /* largest g with g_pref[g] <= j */
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (sh.g_pref[mid] <= j)
+    if (G.pref[mid] <= j)
       lo = mid;
     else
       hi = mid - 1;
   }
-  const int t = sh.g_tiles[lo];
-  return ev.pool[sh.seg_start[t] + (j - sh.g_pref[lo])];
+  const int t = G.tiles[lo];
+  if (tile) *tile = t;
+  return ev.pool[sh.seg_start[t] + (j - G.pref[lo])];
 }
 
 /* Removes physical p from tile t's segment (swap with the segment's last). */
@@ -365,13 +413,14 @@ __device__ void seg_insert(Shared &sh, const Ev &ev, int t, int p) {
 
 /* Lexicographic (distance, label) nearest neighbour of physical o among the
  * gathered tiles (all lanes). Returns the winner's physical index or -1. */
-__device__ int nn_scan(const Shared &sh, const Ev &ev, int ng, int o, double orap, double ophi, double R2, int lane,
+__device__ int nn_scan(const Shared &sh, const Gather &G, const Ev &ev, int ng, int o, double orap, double ophi, double R2,
+                       int lane,
                        double *best_d, double *best_w) {
-  const int total = sh.g_pref[ng];
+  const int total = G.pref[ng];
   double bd = R2, bw = 0.0;
   int bl = BIG, bp = -1;
   for (int j0 = 0; j0 < total; j0 += W) {
-    const int q = gathered(sh, ev, ng, j0 + lane);
+    const int q = gathered(sh, G, ev, ng, j0 + lane);
     if (q >= 0 && q != o) {
       const double2 rp = ev.rp[q];
       const double d = fjmi_dr2(orap, ophi, rp.x, rp.y);
@@ -518,19 +567,21 @@ __global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts
     __threadfence_block();
     __syncthreads();
     /* tile minima of every tile */
+    Gather &G = sh.g[0];
+    Gather &GO = sh.g[1];
     for (int t0 = 0; t0 < ntiles; t0 += MAXG) {
       const int cnt = min(MAXG, ntiles - t0);
       if (lane == 0) {
         int acc = 0;
-        sh.g_pref[0] = 0;
+        G.pref[0] = 0;
         for (int u = 0; u < cnt; ++u) {
-          sh.g_tiles[u] = t0 + u;
+          G.tiles[u] = t0 + u;
           acc += sh.seg_count[t0 + u];
-          sh.g_pref[u + 1] = acc;
+          G.pref[u + 1] = acc;
         }
       }
       __syncthreads();
-      refresh_minima(sh, ev, cnt, lane);
+      refresh_minima(sh, G, ev, cnt, lane);
     }
 
     /* ---- the N recombination steps ---- */
@@ -540,20 +591,23 @@ __global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts
       /* 1. argmin over the tile minima */
       double bd = __builtin_inf();
       int bl = BIG, bt = -1;
-      for (int t = lane; t < ntiles; t += W) {
-        const double d = __longlong_as_double((long long)sh.tmin_d[t]);
-        const int l = sh.tmin_l[t];
-        if (l != BIG && (d < bd || (d == bd && l < bl))) {
-          bd = d;
-          bl = l;
-          bt = t;
+#pragma unroll
+      for (int k = 0; k < MAXT / W; ++k) {
+        const int t = k * W + lane;
+        if (k * W < ntiles) { /* uniform */
+          const TMin m = sh.tmin[t < ntiles ? t : 0];
+          const double d = __longlong_as_double((long long)m.d);
+          const bool c = (t < ntiles) & (m.l != BIG) & ((d < bd) | ((d == bd) & (m.l < bl)));
+          bd = c ? d : bd;
+          bl = c ? m.l : bl;
+          bt = c ? t : bt;
         }
       }
       double dmin;
       int lA;
       const int wl = wave_argmin(bd, bl, &dmin, &lA);
       const int tA = wl < 0 ? -1 : uni(__builtin_amdgcn_readlane(bt, wl));
-      const int A = tA < 0 ? -1 : uni(sh.tmin_p[tA]);
+      const int A = tA < 0 ? -1 : uni(sh.tmin[tA].p);
       if (A < 0) { /* no live jet found: inconsistent state, stop this event */
         if (lane == 0) atomicOr(a.status, 2u);
         break;
@@ -579,6 +633,17 @@ __global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts
       const int tile_hi = uni(phys_hi == A ? mA.z : mB.z);
       const int tile_lo_old = phys_lo < 0 ? -1 : uni(phys_lo == A ? mA.z : mB.z);
       const int tile_pt = uni(ev.meta[pt].z);
+      /* the removed jet leaves its tile and the last label moves (N2Plain
+       * compaction) before the new jet's kinematics, so their memory traffic
+       * overlaps the arithmetic */
+      if (lane == 0) {
+        seg_remove(sh, ev, tile_hi, phys_hi);
+        ev.meta[phys_hi] = make_int4(-1, -1, -1, -1);
+        if (pt != phys_hi) {
+          ev.meta[pt].y = lab_hi;
+          ev.lab2[lab_hi] = pt;
+        }
+      }
       double nrap = 0.0, nphi = 0.0, nw = 0.0;
       int tile_new = -1;
       const double4 qhi = ev.p4[phys_hi];
@@ -630,12 +695,6 @@ __global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts
           dp[2] = o.z;
           dp[3] = o.w;
         }
-        seg_remove(sh, ev, tile_hi, phys_hi);
-        ev.meta[phys_hi] = make_int4(-1, -1, -1, -1);
-        if (pt != phys_hi) { /* N2Plain compaction: the last jet takes the removed slot's label */
-          ev.meta[pt].y = lab_hi;
-          ev.lab2[lab_hi] = pt;
-        }
       }
       if (phys_lo < 0) ++nj;
       __threadfence_block();
@@ -647,26 +706,35 @@ __global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts
       }
 
       /* 3. gather the 3x3 tiles around the old and new positions and the moved jet's tile */
-      const int ng = build_gather(sh, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane);
-      const int total = sh.g_pref[ng];
+      const int ng = build_gather(sh, G, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane);
+      const int total = G.pref[ng];
       if (lane == 0) sh.norph = 0;
       __syncthreads();
       double cbd = R2, cbw = 0.0;
       int cbl = BIG, cbp = -1;
+      /* a gather of at most 64 members keeps each member's (index, label, diJ)
+       * in its lane for the tile-minimum refresh */
+      const bool single = total <= W;
+      int myq = -1, mylab = BIG;
+      unsigned long long mybits = ~0ull;
       for (int j0 = 0; j0 < total; j0 += W) {
-        const int q = gathered(sh, ev, ng, j0 + lane);
+        const int q = gathered(sh, G, ev, ng, j0 + lane);
         bool orph = false;
         if (q >= 0 && q != phys_lo) {
           int4 mq = ev.meta[q];
+          double dq = ev.dij[q];
           orph = (mq.x == phys_hi) || (phys_lo >= 0 && mq.x == phys_lo);
+          myq = q;
+          mylab = mq.y;
           if (phys_lo >= 0) {
             const double2 rq = ev.rp[q];
             const double2 wq = ev.wn[q];
             const double d = fjmi_dr2(rq.x, rq.y, nrap, nphi);
             if (!orph && d < wq.y) {
+              dq = d * (nw < wq.x ? nw : wq.x);
               ev.meta[q].x = phys_lo;
               ev.wn[q] = make_double2(wq.x, d);
-              ev.dij[q] = d * (nw < wq.x ? nw : wq.x);
+              ev.dij[q] = dq;
             }
             if (d < R2 && (d < cbd || (d == cbd && mq.y < cbl))) {
               cbd = d;
@@ -675,6 +743,10 @@ __global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts
               cbw = wq.x;
             }
           }
+          mybits = (unsigned long long)__double_as_longlong(dq);
+        } else if (q >= 0) { /* the new jet itself: filled in below */
+          myq = q;
+          mylab = ev.meta[q].y;
         }
         const unsigned long long ob = __ballot(orph);
         if (ob) {
@@ -692,11 +764,13 @@ __global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts
         const int ln = wave_argmin(cbd, cbl, &md, &ml);
         const int nnB = ln < 0 ? -1 : __builtin_amdgcn_readlane(cbp, ln);
         const double wB = ln < 0 ? 0.0 : readlane_d(cbw, ln);
+        const double dB = diJ_of(md, nw, nnB, wB);
         if (lane == 0) {
           ev.meta[phys_lo].x = nnB;
           ev.wn[phys_lo] = make_double2(nw, md);
-          ev.dij[phys_lo] = diJ_of(md, nw, nnB, wB);
+          ev.dij[phys_lo] = dB;
         }
+        if (myq == phys_lo) mybits = (unsigned long long)__double_as_longlong(dB);
       }
       __threadfence_block();
       __syncthreads();
@@ -710,22 +784,24 @@ __global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts
           const int4 mo = ev.meta[o];
           const double2 ro = ev.rp[o];
           const double wo = ev.wn[o].x;
-          const int ngo = build_gather(sh, geo, mo.z, -1, -1, -1, lane);
+          const int ngo = build_gather(sh, GO, geo, mo.z, -1, -1, -1, lane);
           double od, owb;
-          const int nno = nn_scan(sh, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
+          const int nno = nn_scan(sh, GO, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
+          const double dO = diJ_of(od, wo, nno, owb);
           if (lane == 0) {
             ev.meta[o].x = nno;
             ev.wn[o] = make_double2(wo, od);
-            ev.dij[o] = diJ_of(od, wo, nno, owb);
+            ev.dij[o] = dO;
           }
+          if (myq == o) mybits = (unsigned long long)__double_as_longlong(dO);
           __threadfence_block();
           __syncthreads();
         }
         if (norph > MAXO) { /* more orphans than the list holds: rescan the rest from the gather */
-          const int ng2 = build_gather(sh, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane);
-          const int tot2 = sh.g_pref[ng2];
+          const int ng2 = ng;
+          const int tot2 = G.pref[ng2];
           for (int j0 = 0; j0 < tot2; j0 += W) {
-            const int q = gathered(sh, ev, ng2, j0 + lane);
+            const int q = gathered(sh, G, ev, ng2, j0 + lane);
             const bool orph = q >= 0 && q != phys_lo &&
                               (ev.meta[q].x == phys_hi || (phys_lo >= 0 && ev.meta[q].x == phys_lo));
             unsigned long long ob = __ballot(orph);
@@ -736,9 +812,9 @@ __global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts
               const int4 mo = ev.meta[o];
               const double2 ro = ev.rp[o];
               const double wo = ev.wn[o].x;
-              const int ngo = build_gather(sh, geo, mo.z, -1, -1, -1, lane);
+              const int ngo = build_gather(sh, GO, geo, mo.z, -1, -1, -1, lane);
               double od, owb;
-              const int nno = nn_scan(sh, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
+              const int nno = nn_scan(sh, GO, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
               if (lane == 0) {
                 ev.meta[o].x = nno;
                 ev.wn[o] = make_double2(wo, od);
@@ -746,15 +822,16 @@ __global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts
               }
               __threadfence_block();
               __syncthreads();
-              (void)build_gather(sh, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane);
             }
           }
         }
       }
 
       /* 5. tile minima of every gathered tile */
-      const int ng3 = build_gather(sh, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane);
-      refresh_minima(sh, ev, ng3, lane);
+      if (single && norph <= MAXO)
+        refresh_from_table(sh, G, ng, total, lane, myq, mylab, mybits);
+      else
+        refresh_minima(sh, G, ev, ng, lane);
     }
     if (lane == 0) a.njets[e] = nj;
   }
