@@ -206,7 +206,10 @@ __global__ __launch_bounds__(T, OCC) void dense_kernel(LaunchArgs a) {
     if (t == 0) sh.ev = (int)(atomicAdd(a.head, 1u) - a.head_base);
     __syncthreads();
     const int qi = uni(sh.ev);
-    if (qi >= a.n_queue) break;
+    if (qi >= a.n_queue) {
+      if (a.t_end && threadIdx.x == 0) atomicMax(a.t_end, (unsigned long long)wall_clock64());
+      break;
+    }
     const int e = a.queue ? uni(a.queue[qi]) : qi;
     const int64_t base = a.offsets ? a.offsets[e] : 0;
     const int N = a.offsets ? uni((int)(a.offsets[e + 1] - base)) : a.n_single;
@@ -663,8 +666,12 @@ const KernelDesc kDenseKernels[] = {
 const int kNumDenseKernels = sizeof(kDenseKernels) / sizeof(kDenseKernels[0]);
 const void *const kEmptyKernel = reinterpret_cast<const void *>(&empty_kernel);
 
-__global__ void zero_u32_kernel(unsigned int *p, int n) {
+/* Start of a batch graph: zeroes the queue heads and status word, and the
+ * per-bucket end times; t[0] receives the start time. */
+__global__ void zero_u32_kernel(unsigned int *p, int n, unsigned long long *t, int nt) {
   for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < n; i += blockDim.x * gridDim.x) p[i] = 0u;
+  for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < nt; i += blockDim.x * gridDim.x)
+    t[i] = i == 0 ? (unsigned long long)wall_clock64() : 0ull;
 }
 const void *const kZeroKernel = reinterpret_cast<const void *>(&zero_u32_kernel);
 

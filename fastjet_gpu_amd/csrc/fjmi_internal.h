@@ -42,8 +42,11 @@ struct LaunchArgs {
   double *p4_scratch;
   /* global scratch for (y, phi, w) when the event does not fit the LDS stage: 4 doubles per particle */
   double *coord_scratch;
-  /* error word: bit 0 = event larger than the kernel's capacity */
+  /* error word: bit 0 = event larger than the kernel's capacity, bit 1 = internal */
   unsigned int *status;
+  /* optional: each workgroup leaving the queue raises this to its wall-clock
+   * time (the constant-rate counter of wall_clock64(), 100 MHz on gfx950) */
+  unsigned long long *t_end;
   int stage_chunk; /* number of staging entries in LDS when four-momenta live in global memory */
 };
 

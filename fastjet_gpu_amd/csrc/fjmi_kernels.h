@@ -19,7 +19,7 @@ struct KernelDesc {
 extern const KernelDesc kDenseKernels[];
 extern const int kNumDenseKernels;
 extern const void *const kEmptyKernel; /* __global__ void(int) */
-extern const void *const kZeroKernel;  /* __global__ void(unsigned *, int) */
+extern const void *const kZeroKernel;  /* __global__ void(unsigned *, int, unsigned long long *, int) */
 
 extern const void *const kTiledKernel; /* __global__ void(LaunchArgs, TiledScratch), 64 threads */
 extern const int kTiledMaxTiles;

@@ -130,7 +130,7 @@ struct BucketRun {
 struct fjmi_batch {
   fjmi_context *ctx = nullptr;
   int n_events = 0;
-  DevBuf queue, heads, p4s, coords, tscratch;
+  DevBuf queue, heads, p4s, coords, tscratch, times;
   std::vector<BucketRun> runs;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -488,6 +488,7 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
   }
   if ((rc = bt->queue.ensure(sizeof(int32_t) * std::max<size_t>(flat.size(), 1)))) return cleanup(rc);
   if ((rc = bt->heads.ensure(sizeof(uint32_t) * (nb + 1)))) return cleanup(rc);
+  if ((rc = bt->times.ensure(sizeof(unsigned long long) * (nb + 1)))) return cleanup(rc);
   if (max_p4g_total > 0) {
     if ((rc = bt->p4s.ensure(32 * (size_t)max_p4g_total))) return cleanup(rc);
     if ((rc = bt->coords.ensure(32 * (size_t)max_p4g_total))) return cleanup(rc);
@@ -546,6 +547,7 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
     a.p4_scratch = bt->p4s.as<double>();
     a.coord_scratch = bt->coords.as<double>();
     a.status = bt->d_status;
+    a.t_end = bt->times.as<unsigned long long>() + 1 + bt->runs.size();
     r.args = a;
     bt->runs.push_back(r);
   }
@@ -561,7 +563,9 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
   if (e == hipSuccess) {
     unsigned *hp = heads;
     int cnt = nb + 1;
-    void *zp[] = {&hp, &cnt};
+    unsigned long long *tp = bt->times.as<unsigned long long>();
+    int tcnt = (int)bt->runs.size() + 1;
+    void *zp[] = {&hp, &cnt, &tp, &tcnt};
     e = hipLaunchKernel(fjmi::kZeroKernel, dim3(1), dim3(64), zp, 0, s0);
     if (e == hipSuccess) e = hipEventRecord(bt->tev[0], s0);
     for (size_t i = 0; e == hipSuccess && i < bt->runs.size(); ++i) {
@@ -617,10 +621,13 @@ int fjmi_batch_stats(fjmi_batch *b, fjmi_bucket_stats *out, int cap, int *n) {
     o.grid = r.grid;
     o.particles = r.particles;
     o.sum_n2 = r.sum_n2;
-    float ms = -1.0f;
-    if (hipEventElapsedTime(&ms, b->tev[0], b->tev[i + 1]) != hipSuccess) ms = -1.0f;
-    o.ms = ms;
+    o.ms = -1.0f;
   }
+  /* completion times measured by the kernels themselves (100 MHz wall clock) */
+  std::vector<unsigned long long> t(b->runs.size() + 1, 0);
+  HIPCHK(hipMemcpy(t.data(), b->times.ptr, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost));
+  for (int i = 0; i < *n && i < cap; ++i)
+    if (t[i + 1] >= t[0]) out[i].ms = (float)((double)(t[i + 1] - t[0]) / 100e3);
   return FJMI_OK;
 }
 
@@ -637,6 +644,7 @@ int fjmi_batch_destroy(fjmi_batch *b) {
   b->p4s.release();
   b->coords.release();
   b->tscratch.release();
+  b->times.release();
   delete b;
   return FJMI_OK;
 }

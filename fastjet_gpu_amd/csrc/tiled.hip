@@ -459,7 +459,10 @@ __global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts
     if (lane == 0) sh.ev = (int)(atomicAdd(a.head, 1u) - a.head_base);
     __syncthreads();
     const int qi = uni(sh.ev);
-    if (qi >= a.n_queue) break;
+    if (qi >= a.n_queue) {
+      if (a.t_end && threadIdx.x == 0) atomicMax(a.t_end, (unsigned long long)wall_clock64());
+      break;
+    }
     const int e = a.queue ? uni(a.queue[qi]) : qi;
     const int64_t base = a.offsets ? a.offsets[e] : 0;
     const int N = a.offsets ? uni((int)(a.offsets[e + 1] - base)) : a.n_single;
