@@ -2,6 +2,8 @@
 
   lib/libfjmi.so      HIP engine + C ABI (include/fjmi.h), gfx950 code objects
   lib/libfjmi_gen.so  seeded event generator (include/fjmi_gen.h), host C
+  lib/libfjmi_cluster.so  the drop-in cluster() of include/cluster.h (engine chosen
+                      per event by the bucket table), for programs written against it
   bin/tri_matrix, bin/n_array, bin/grid, bin/tri_matrix_empty
                       the drop-in CLI linked with one engine each
   oracle/_build/libfj_oracle.so   CPU oracle (test infrastructure, via oracle/Makefile)
@@ -33,7 +35,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result", f"-I{INC}", f"-I{CSRC}",
 ]
 CC_FLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", f"-I{INC}"]
-CXX_FLAGS = ["-O2", "-std=c++17", "-ffp-contract=off", "-Wall", f"-I{INC}"]
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+CXX_FLAGS = ["-O2", "-std=c++17", "-ffp-contract=off", "-Wall", f"-I{INC}", f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
 
 
 def run(cmd):
@@ -72,13 +75,20 @@ def build(jobs=4, cli=True, oracle=True):
 def build_cli():
     main_o = os.path.join(OBJ, "cli_main.o")
     run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "cli_main.cpp"), "-o", main_o])
-    for name, engine in (("tri_matrix", "FJMI_ENGINE_DENSE"), ("n_array", "FJMI_ENGINE_DENSE"),
+    # tri_matrix: the bucket table's choice (dense for small events, tiled above);
+    # n_array: dense only; grid: tiled only; tri_matrix_empty: per-step launch floor
+    for name, engine in (("tri_matrix", "FJMI_ENGINE_AUTO"), ("n_array", "FJMI_ENGINE_DENSE"),
                          ("grid", "FJMI_ENGINE_TILED"), ("tri_matrix_empty", "FJMI_ENGINE_EMPTY")):
         eo = os.path.join(OBJ, f"engine_{name}.o")
-        run(["g++", *CXX_FLAGS, f"-DFJMI_LINKED_ENGINE={engine}", f'-DFJMI_ENGINE_NAME="{name}"', "-c",
+        run(["g++", *CXX_FLAGS, f"-DFJMI_LINKED_ENGINE={engine}", "-c",
              os.path.join(CSRC, "cluster_dropin.cpp"), "-o", eo])
         run(["g++", "-o", os.path.join(BIN, name), main_o, eo, f"-L{LIB}", "-lfjmi",
-             f"-Wl,-rpath,$ORIGIN/../lib", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
+             "-Wl,-rpath,$ORIGIN/../lib", f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"])
+    so = os.path.join(LIB, "libfjmi_cluster.so")
+    run(["g++", *CXX_FLAGS, "-fPIC", "-shared", "-DFJMI_LINKED_ENGINE=FJMI_ENGINE_AUTO", "-o", so + ".tmp",
+         os.path.join(CSRC, "cluster_dropin.cpp"), f"-L{LIB}", "-lfjmi", "-Wl,-rpath,$ORIGIN",
+         f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"])
+    os.replace(so + ".tmp", so)
 
 
 def main():

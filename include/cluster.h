@@ -10,8 +10,9 @@
  *   Errors print "file, line: call; name: message" to stderr and abort(), like the
  *   reference's cudaCheck (cudaCheck.h:11-47).
  * Algorithm's value is the generalised-kt exponent p (cluster.h:4 of the reference).
- * Which engine runs is chosen at link time (dense: tri_matrix / n_array binaries;
- * tiled: grid; per-step launch floor: tri_matrix_empty), as in the reference.
+ * Which engine runs is chosen at link time, as in the reference: the bucket table's
+ * choice per event (tri_matrix binary, lib/libfjmi_cluster.so), dense (n_array),
+ * tiled (grid), or the per-step launch floor with no result (tri_matrix_empty).
  */
 #ifndef cluster_h
 #define cluster_h

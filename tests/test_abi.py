@@ -1,0 +1,87 @@
+"""The C-ABI libraries load without a GPU and export every function their headers
+declare (include/fjmi.h, include/fjmi_gen.h, include/cluster.h); the PseudoJet
+record keeps the reference's 40-byte layout (PseudoJet.h:4-11 of the reference).
+No compute call is made here."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from tests.conftest import ROOT
+
+INC = os.path.join(ROOT, "include")
+LIB = os.path.join(ROOT, "fastjet_gpu_amd", "lib")
+
+
+def declared(header):
+    text = open(os.path.join(INC, header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(fjmi_\w+)\s*\(", text, flags=re.M)))
+
+
+def load(name):
+    path = os.path.join(LIB, name)
+    if not os.path.exists(path):
+        pytest.skip(f"{path} not built")
+    return ctypes.CDLL(path)
+
+
+def test_header_parse_finds_every_entry_point():
+    names = declared("fjmi.h")
+    for must in ("fjmi_create", "fjmi_cluster_event_device", "fjmi_cluster_pseudojets", "fjmi_batch_create",
+                 "fjmi_batch_launch", "fjmi_batch_stats", "fjmi_cluster_batch"):
+        assert must in names
+    assert len(names) == 16
+
+
+@pytest.mark.parametrize("header,lib", [("fjmi.h", "libfjmi.so"), ("fjmi_gen.h", "libfjmi_gen.so")])
+def test_exports(header, lib):
+    so = load(lib)
+    missing = [n for n in declared(header) if not hasattr(so, n)]
+    assert not missing, f"{lib} lacks {missing}"
+
+
+def test_dropin_cluster_symbol():
+    """cluster(PseudoJet*, int, Algorithm, double) with C++ linkage, as cluster.h:6 declares it."""
+    path = os.path.join(LIB, "libfjmi_cluster.so")
+    if not os.path.exists(path):
+        pytest.skip("libfjmi_cluster.so not built")
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    assert "_Z7clusterP9PseudoJeti9Algorithmd" in out
+
+
+def test_calls_without_gpu_are_errors_not_crashes():
+    so = load("libfjmi.so")
+    so.fjmi_version.restype = ctypes.c_int
+    assert so.fjmi_version() >= 100
+    so.fjmi_last_error.restype = ctypes.c_char_p
+    so.fjmi_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    ctx = ctypes.c_void_p()
+    rc = so.fjmi_create(-1, ctypes.byref(ctx))
+    assert rc in (-1, -2) and so.fjmi_last_error()
+
+
+def test_pseudojet_layout():
+    """Compile a tiny C++ probe against include/PseudoJet.h: 40 bytes, fields at the reference's offsets."""
+    src = r'''
+#include <cstddef>
+#include <cstdio>
+#include "PseudoJet.h"
+#include "cluster.h"
+int main() {
+  std::printf("%zu %zu %zu %zu %zu %zu %zu %d %d %d\n", sizeof(PseudoJet), offsetof(PseudoJet, index),
+              offsetof(PseudoJet, isJet), offsetof(PseudoJet, px), offsetof(PseudoJet, py),
+              offsetof(PseudoJet, pz), offsetof(PseudoJet, E), (int)Algorithm::Kt,
+              (int)Algorithm::CambridgeAachen, (int)Algorithm::AntiKt);
+}
+'''
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "p.cpp")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "p")
+        subprocess.run(["g++", "-std=c++17", f"-I{INC}", c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    assert out == ["40", "0", "4", "8", "16", "24", "32", "1", "0", "-1"]
