@@ -1,0 +1,209 @@
+"""Autotuner of the multiplicity-bucket table (the launch shape per event size).
+
+For each multiplicity interval (lo, hi] it times every candidate kernel
+configuration on a saturating batch of events of that interval alone, on the
+current GPU, and keeps the fastest:
+
+  dense engine  every compiled (threads, slots, four-momenta in LDS/global)
+                configuration whose capacity threads*slots is at least hi
+                (and at most 16*hi, larger ones only waste LDS and lanes);
+  tiled engine  tile side R, 1.25 R and 1.5 R, with each compiled kernel
+                variant (tile tables of 512, 256 or 128 tiles, and so 8, 12 or
+                16 events per CU).
+
+Adjacent intervals with the same winner are merged; the last row is always a
+tiled row covering every larger event.  The reference picks its launch shapes
+per call from the occupancy calculator instead (launch.h:11-60, grid.cu:597-633).
+
+Events of an interval come from the seeded generator: events of the 4096-event
+N ~ U[50, 2000] workload whose N falls in the interval, or, outside [50, 2000],
+random subsets of the right size of lognormal-multiplicity events.
+
+  python -m fastjet_gpu_amd.autotune [--events 4096] [--json out.json] [--write]
+
+--write replaces csrc/tune_table.inc (the library's default table; rebuild with
+python -m fastjet_gpu_amd.build afterwards).  Needs a GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_EDGES = [32, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 4096, 20000]
+TILE_QUARTERS = [4, 5, 6]
+LAST_ROW = 1000000000
+
+
+def _sizes(g, workload, count):
+    return np.array([g.fjmi_gen_event_size(workload, e) for e in range(count)], np.int64)
+
+
+class EventSource:
+    """Seeded events of a given multiplicity interval (host arrays)."""
+
+    def __init__(self, fj, pool=16384):
+        self.fj = fj
+        g = fj.load_lib("libfjmi_gen.so")
+        self.s3 = _sizes(g, 3, pool)
+        self.s5 = _sizes(g, 5, 4096)
+        self.cache = {}
+
+    def _event(self, workload, e):
+        key = (workload, e)
+        if key not in self.cache:
+            self.cache[key] = self.fj.gen_event(workload, e)
+        return self.cache[key]
+
+    def interval(self, lo, hi, count, distinct=512, seed=1):
+        """`count` events (drawn with repetition from `distinct` ones) with lo < N <= hi."""
+        rng = np.random.default_rng(seed + 7919 * hi)
+        uniq = []
+        inside = np.nonzero((self.s3 > lo) & (self.s3 <= hi))[0]
+        if lo >= 49 and hi <= 2000 and len(inside) >= 16:
+            for e in inside[:distinct]:
+                uniq.append(self._event(3, int(e)))
+        else:
+            for _ in range(distinct):
+                n = int(rng.integers(lo + 1, hi + 1))
+                src = np.nonzero(self.s5 >= n)[0]
+                ev = self._event(5, int(rng.choice(src))) if len(src) else self._event(4, int(rng.integers(0, 64)))
+                n = min(n, len(ev))
+                uniq.append(ev[np.sort(rng.choice(len(ev), n, replace=False))])
+        pick = [uniq[k % len(uniq)] for k in range(count)]
+        offsets = np.zeros(count + 1, np.int64)
+        np.cumsum([len(e) for e in pick], out=offsets[1:])
+        return np.concatenate(pick), offsets
+
+
+def candidates(configs, lo, hi):
+    out = []
+    for (cap, eng, T, K, G) in configs:
+        if eng != 1 or cap < hi or cap > max(16 * hi, 64) or hi > 8192:
+            continue
+        out.append((T, K, G))
+    rows = [(1, T, K, G) for (T, K, G) in sorted(set(out))]
+    variants = sorted({G for (cap, eng, T, K, G) in configs if eng == 2})
+    rows += [(2, 64, q, v) for v in variants for q in TILE_QUARTERS]
+    return rows
+
+
+def time_config(fj, torch, eng, cfg, hi, p4, off, d, p, R, reps):
+    engine_id, T, K, G = cfg
+    if engine_id == 1:
+        eng.set_buckets([(hi, 1, T, K, G), (LAST_ROW, 2, 64, 0, 0)])
+    else:
+        eng.set_buckets([(LAST_ROW, 2, 64, K, G)])
+    b = eng.batch_create(off, d["p4"].data_ptr(), d["off"].data_ptr(), None, None, p, R, d["h1"].data_ptr(),
+                         d["h2"].data_ptr(), d["hd"].data_ptr(), d["j"].data_ptr(), d["nj"].data_ptr())
+    try:
+        b.launch()
+        b.sync()
+        best = float("inf")
+        for _ in range(reps):
+            t = time.perf_counter()
+            b.launch()
+            b.sync()
+            best = min(best, time.perf_counter() - t)
+        return (len(off) - 1) / best
+    finally:
+        b.close()
+
+
+def merge(choices):
+    """[(hi, cfg)] -> bucket rows, merging neighbours with the same cfg; last row tiled, open-ended."""
+    rows = []
+    for hi, (e, T, K, G) in choices:
+        row = [hi, e, T, K, G]
+        if rows and rows[-1][1:] == row[1:]:
+            rows[-1][0] = hi
+        else:
+            rows.append(row)
+    if rows[-1][1] == 2:
+        rows[-1][0] = LAST_ROW
+    else:
+        tiled = [c for _, c in choices if c[0] == 2]
+        q, v = (tiled[-1][2], tiled[-1][3]) if tiled else (0, 0)
+        rows.append([LAST_ROW, 2, 64, q, v])
+    return [tuple(r) for r in rows]
+
+
+def write_inc(rows, note):
+    path = os.path.join(PKG, "csrc", "tune_table.inc")
+    lines = ["/* Multiplicity buckets: {max_n, engine, threads, slots, four-momenta",
+             " * in global memory} (engine 1 = dense; engine 2 = tiled, slots = tile side",
+             " * in quarters of R with 0 = R, and the last field the kernel variant);",
+             " * rows sorted by max_n.",
+             f" * Written by fastjet_gpu_amd/autotune.py: {note} */"]
+    lines += ["{%d, %d, %d, %d, %d}," % r for r in rows]
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return path
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--events", type=int, default=4096, help="events per interval and candidate")
+    ap.add_argument("--edges", default=",".join(map(str, DEFAULT_EDGES)))
+    ap.add_argument("--p", type=int, default=-1)
+    ap.add_argument("--R", type=float, default=0.4)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--json", default=None)
+    ap.add_argument("--write", action="store_true")
+    a = ap.parse_args(argv)
+
+    sys.path.insert(0, os.path.dirname(PKG))
+    import torch
+    import fastjet_gpu_amd as fj
+
+    eng = fj.Engine(0)
+    saved = eng.buckets()
+    configs = eng.kernel_configs()
+    src = EventSource(fj)
+    edges = [int(x) for x in a.edges.split(",")]
+    dev = torch.device("cuda:0")
+    results, choices = [], []
+    lo = 0
+    try:
+        for hi in edges:
+            count = a.events if hi <= 2048 else max(512, min(a.events, int(8e6 // ((lo + hi) / 2))))
+            p4, off = src.interval(lo, hi, count)
+            tot = len(p4)
+            d = dict(p4=torch.from_numpy(p4).to(dev), off=torch.from_numpy(off).to(dev),
+                     h1=torch.empty(tot, dtype=torch.int32, device=dev),
+                     h2=torch.empty(tot, dtype=torch.int32, device=dev),
+                     hd=torch.empty(tot, dtype=torch.float64, device=dev),
+                     j=torch.empty(4 * tot, dtype=torch.float64, device=dev),
+                     nj=torch.empty(count, dtype=torch.int32, device=dev))
+            best = None
+            for cfg in candidates(configs, lo, hi):
+                rate = time_config(fj, torch, eng, cfg, hi, p4, off, d, a.p, a.R, a.reps)
+                results.append(dict(lo=lo, hi=hi, engine=cfg[0], threads=cfg[1], slots=cfg[2], p4_global=cfg[3],
+                                    events_per_s=rate, mean_n=float(np.mean(np.diff(off)))))
+                if best is None or rate > best[0]:
+                    best = (rate, cfg)
+            print(f"({lo:6d},{hi:6d}]  best {best[1]}  {best[0]:.0f} ev/s", flush=True)
+            choices.append((hi, best[1]))
+            lo = hi
+            del d
+    finally:
+        eng.set_buckets(saved)
+    rows = merge(choices)
+    dev_name = torch.cuda.get_device_name(0)
+    out = dict(device=dev_name, p=a.p, R=a.R, events=a.events, edges=edges, measurements=results,
+               table=[list(r) for r in rows])
+    print(json.dumps(dict(table=out["table"])), flush=True)
+    if a.json:
+        with open(a.json, "w") as fh:
+            json.dump(out, fh, indent=1)
+    if a.write:
+        note = f"{a.events} events per interval, anti-kt R={a.R}" if a.p == -1 else f"{a.events} events, p={a.p} R={a.R}"
+        print("wrote", write_inc(rows, f"{note}, {dev_name}"))
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -59,7 +59,7 @@ void banner() {
   HIP_CHECK(hipSetDevice(0));
   hipDeviceProp_t prop;
   HIP_CHECK(hipGetDeviceProperties(&prop, 0));
-  std::cout << "Running on HIP device " << prop.name << std::endl;
+  std::cout << "Running on HIP device " << (prop.name[0] ? prop.name : prop.gcnArchName) << std::endl;
   std::cout << "  - maximum shared memory per block: " << prop.sharedMemPerBlock / 1024 << " kB" << std::endl;
 }
 

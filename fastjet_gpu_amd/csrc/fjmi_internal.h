@@ -48,6 +48,7 @@ struct LaunchArgs {
    * time (the constant-rate counter of wall_clock64(), 100 MHz on gfx950) */
   unsigned long long *t_end;
   int stage_chunk; /* number of staging entries in LDS when four-momenta live in global memory */
+  int tile_q;      /* tiled engine: tile side = max(tile_q, 4)/4 * R (rounded up) */
 };
 
 enum { STATUS_CAPACITY = 1, STATUS_INTERNAL = 2 };

@@ -21,8 +21,11 @@ extern const int kNumDenseKernels;
 extern const void *const kEmptyKernel; /* __global__ void(int) */
 extern const void *const kZeroKernel;  /* __global__ void(unsigned *, int, unsigned long long *, int) */
 
-extern const void *const kTiledKernel; /* __global__ void(LaunchArgs, TiledScratch), 64 threads */
-extern const int kTiledMaxTiles;
+/* tiled engine, __global__ void(LaunchArgs, TiledScratch), 64 threads, by tile
+ * table size (index = a tiled bucket row's p4_global field) */
+constexpr int kNumTiledKernels = 4;
+extern const void *const kTiledKernels[kNumTiledKernels];
+extern const int kTiledMaxTiles[kNumTiledKernels];
 
 /* Pool entries per event of the tiled engine beyond 3 per particle (>= 2 per tile). */
 constexpr int kTiledPoolPerEvent = 2048;

@@ -123,6 +123,8 @@ struct BucketRun {
   LaunchArgs args;
   int n_events;
   int max_n;
+  int tile_q;
+  int tile_table;
   int64_t particles;
   double sum_n2;
 };
@@ -153,6 +155,10 @@ int check_table(const fjmi_bucket *rows, int n) {
     if (i > 0 && b.max_n <= rows[i - 1].max_n) return fail(FJMI_EINVAL, "bucket table not sorted by max_n");
     if (b.engine == FJMI_ENGINE_TILED) {
       if (b.threads != 64) return fail(FJMI_EINVAL, "tiled rows run 64 threads");
+      if (b.slots != 0 && (b.slots < 4 || b.slots > 16))
+        return fail(FJMI_EINVAL, "tiled rows: slots is the tile side in quarters of R, 4..16 (0 = 4)");
+      if (b.p4_global < 0 || b.p4_global >= fjmi::kNumTiledKernels)
+        return fail(FJMI_EINVAL, "tiled rows: p4_global selects the tile table size, 0..%d", fjmi::kNumTiledKernels - 1);
       continue;
     }
     if (b.engine != FJMI_ENGINE_DENSE) return fail(FJMI_EINVAL, "bucket engine must be dense or tiled");
@@ -176,6 +182,23 @@ const fjmi_bucket *bucket_for(const fjmi_context *ctx, int n, int engine) {
 
 const fjmi_bucket kTiledRow = {0x7fffffff, FJMI_ENGINE_TILED, 64, 0, 0};
 
+/* A dense row for an event no dense table row covers (engine forced to dense):
+ * the compiled kernel of capacity >= n with the fewest slots per thread (the
+ * shortest per-step scan), then the smallest capacity, then LDS four-momenta. */
+bool dense_row_for(int n, fjmi_bucket *out) {
+  const KernelDesc *best = nullptr;
+  for (int i = 0; i < fjmi::kNumDenseKernels; ++i) {
+    const KernelDesc &d = fjmi::kDenseKernels[i];
+    if ((int64_t)d.T * d.K < n) continue;
+    if (!best || d.K < best->K || (d.K == best->K && d.T < best->T) ||
+        (d.K == best->K && d.T == best->T && !d.p4g && best->p4g))
+      best = &d;
+  }
+  if (!best) return false;
+  *out = fjmi_bucket{best->T * best->K, FJMI_ENGINE_DENSE, best->T, best->K, best->p4g ? 1 : 0};
+  return true;
+}
+
 LaunchArgs base_args() {
   LaunchArgs a;
   std::memset(&a, 0, sizeof a);
@@ -197,6 +220,8 @@ int cluster_one_device(fjmi_context *ctx, int engine, const double *d_in, int in
                        double R, int32_t *h1, int32_t *h2, double *hd, double *jets, int32_t *nj, unsigned char *pj,
                        hipStream_t stream) {
   const fjmi_bucket *b = engine == FJMI_ENGINE_TILED ? &kTiledRow : bucket_for(ctx, n, engine);
+  fjmi_bucket forced;
+  if (!b && engine == FJMI_ENGINE_DENSE && dense_row_for(n, &forced)) b = &forced;
   if (!b) return fail(FJMI_ECAPACITY, "event with %d particles exceeds the largest bucket of engine %d", n, engine);
   int rc;
   if ((rc = ctx->ctrl.ensure(sizeof(Ctrl)))) return rc;
@@ -232,9 +257,10 @@ int cluster_one_device(fjmi_context *ctx, int engine, const double *d_in, int in
   a.p4_scratch = ctx->p4s.as<double>();
   a.coord_scratch = ctx->coords.as<double>();
   a.status = &c->status;
+  a.tile_q = tiled ? b->slots : 0;
   if (tiled) {
     void *params[] = {&a, &ts};
-    HIPCHK(hipLaunchKernel(fjmi::kTiledKernel, dim3(1), dim3(64), params, 0, stream));
+    HIPCHK(hipLaunchKernel(fjmi::kTiledKernels[b->p4_global], dim3(1), dim3(64), params, 0, stream));
   } else if ((rc = launch_dense(*kd, 1, dyn_lds_bytes(*kd), a, stream))) {
     return rc;
   }
@@ -272,8 +298,13 @@ int fjmi_kernel_configs(fjmi_bucket *out, int cap) {
     out[i].slots = d.K;
     out[i].p4_global = d.p4g ? 1 : 0;
   }
-  if (fjmi::kNumDenseKernels < cap) out[fjmi::kNumDenseKernels] = kTiledRow;
-  return fjmi::kNumDenseKernels + 1;
+  for (int k = 0; k < fjmi::kNumTiledKernels; ++k) {
+    if (fjmi::kNumDenseKernels + k >= cap) break;
+    fjmi_bucket r = kTiledRow;
+    r.p4_global = k;
+    out[fjmi::kNumDenseKernels + k] = r;
+  }
+  return fjmi::kNumDenseKernels + fjmi::kNumTiledKernels;
 }
 
 int fjmi_create(int device, fjmi_context **out) {
@@ -510,7 +541,7 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
     BucketRun r;
     r.tiled = bk.engine == FJMI_ENGINE_TILED;
     r.kd = r.tiled ? nullptr : find_kernel(bk.threads, bk.slots, bk.p4_global);
-    r.fn = r.tiled ? fjmi::kTiledKernel : r.kd->fn;
+    r.fn = r.tiled ? fjmi::kTiledKernels[bk.p4_global] : r.kd->fn;
     r.threads = r.tiled ? 64 : r.kd->T;
     r.lds = r.tiled ? 0 : dyn_lds_bytes(*r.kd);
     r.ts = r.tiled ? fjmi::tiled_scratch_carve(bt->tscratch.ptr, h_offsets[n_events], fjmi::kTiledPoolPerEvent)
@@ -520,6 +551,8 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     r.n_events = (int)lists[b].size();
     r.max_n = bk.max_n;
+    r.tile_q = r.tiled ? bk.slots : 0;
+    r.tile_table = r.tiled ? bk.p4_global : 0;
     r.particles = 0;
     r.sum_n2 = 0.0;
     for (int32_t ev : lists[b]) {
@@ -548,6 +581,7 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
     a.coord_scratch = bt->coords.as<double>();
     a.status = bt->d_status;
     a.t_end = bt->times.as<unsigned long long>() + 1 + bt->runs.size();
+    a.tile_q = r.tile_q;
     r.args = a;
     bt->runs.push_back(r);
   }
@@ -615,8 +649,8 @@ int fjmi_batch_stats(fjmi_batch *b, fjmi_bucket_stats *out, int cap, int *n) {
     o.bucket.max_n = r.max_n;
     o.bucket.engine = r.tiled ? FJMI_ENGINE_TILED : FJMI_ENGINE_DENSE;
     o.bucket.threads = r.threads;
-    o.bucket.slots = r.tiled ? 0 : r.kd->K;
-    o.bucket.p4_global = r.tiled ? 0 : (r.kd->p4g ? 1 : 0);
+    o.bucket.slots = r.tiled ? r.tile_q : r.kd->K;
+    o.bucket.p4_global = r.tiled ? r.tile_table : (r.kd->p4g ? 1 : 0);
     o.n_events = r.n_events;
     o.grid = r.grid;
     o.particles = r.particles;

@@ -43,7 +43,6 @@ namespace fjmi {
 namespace tiled {
 
 constexpr int W = 64;       /* one wave per event */
-constexpr int MAXT = 512;   /* tiles per event */
 constexpr int MAXG = 32;    /* tiles gathered per step (3 x 3x3 + 1 <= 28) */
 constexpr int MAXO = 128;   /* orphans listed per step (more: rescan in rounds) */
 constexpr int BIG = 0x7fffffff;
@@ -62,6 +61,9 @@ struct Gather {
   int pref[MAXG + 1];
 };
 
+/* Per-workgroup state of the event in flight; MAXT = tiles per event (the
+ * tile table size: fewer tiles need less LDS, so more workgroups fit a CU). */
+template <int MAXT>
 struct Shared {
   TMin tmin[MAXT];
   unsigned long long mt_d[W]; /* member table of a single-chunk gather: diJ bits, */
@@ -217,7 +219,8 @@ __device__ __forceinline__ double diJ_of(double nnd, double w, int nn, double wn
 /* Rebuilds every tile segment from the live jets (labels 0..n-1): counting
  * sort into the pool with 50% slack.  Used at the start of an event and when a
  * relocation finds the pool exhausted. */
-__device__ void rebuild_segments(Shared &sh, const Ev &ev, int n, int ntiles, int lane) {
+template <class SH>
+__device__ void rebuild_segments(SH &sh, const Ev &ev, int n, int ntiles, int lane) {
   for (int t = lane; t < ntiles; t += W) sh.seg_count[t] = 0;
   __syncthreads();
   for (int l = lane; l < n; l += W) {
@@ -254,12 +257,14 @@ __device__ void rebuild_segments(Shared &sh, const Ev &ev, int n, int ntiles, in
   __syncthreads();
 }
 
-__device__ __forceinline__ int gathered(const Shared &sh, const Gather &G, const Ev &ev, int ng, int j,
+template <class SH>
+__device__ __forceinline__ int gathered(const SH &sh, const Gather &G, const Ev &ev, int ng, int j,
                                         int *tile = nullptr);
 
 /* Recomputes the (diJ, label) minimum of tiles listed in G.tiles[0..ng) from
  * all their members. */
-__device__ void refresh_minima(Shared &sh, const Gather &G, const Ev &ev, int ng, int lane) {
+template <class SH>
+__device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, int lane) {
   if (lane < ng) {
     const int t = G.tiles[lane];
     sh.tmin[t].d = ~0ull;
@@ -294,7 +299,8 @@ __device__ void refresh_minima(Shared &sh, const Gather &G, const Ev &ev, int ng
 
 /* Tile minima of the gathered tiles from the member table (gathers of at most
  * 64 members): lane j < total wrote member j; lane g < ng scans tile g's range. */
-__device__ void refresh_from_table(Shared &sh, const Gather &G, int ng, int total, int lane, int myq, int mylab,
+template <class SH>
+__device__ void refresh_from_table(SH &sh, const Gather &G, int ng, int total, int lane, int myq, int mylab,
                                    unsigned long long mybits) {
   if (lane < total) {
     sh.mt_d[lane] = mybits;
@@ -327,7 +333,8 @@ __device__ void refresh_from_table(Shared &sh, const Gather &G, int ng, int tota
  * tiles (centres < 0 are skipped) plus `extra` (if >= 0), and G.pref to the
  * prefix sums of their member counts.  Lane i < 27 proposes neighbour i%9 of
  * centre i/9 (c0, c1, c2); duplicates are dropped through a per-tile stamp. */
-__device__ int build_gather(Shared &sh, Gather &G, const Geo &geo, int c0, int c1, int c2, int extra, int lane) {
+template <class SH>
+__device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, int c2, int extra, int lane) {
   if (lane == 0) sh.stampv += 1;
   __syncthreads();
   const int stampv = uni(sh.stampv);
@@ -358,7 +365,8 @@ __device__ int build_gather(Shared &sh, Gather &G, const Geo &geo, int c0, int c
 }
 
 /* Member j of the gathered tiles: physical index (or -1 past the end). */
-__device__ __forceinline__ int gathered(const Shared &sh, const Gather &G, const Ev &ev, int ng, int j, int *tile) {
+template <class SH>
+__device__ __forceinline__ int gathered(const SH &sh, const Gather &G, const Ev &ev, int ng, int j, int *tile) {
   if (j >= G.pref[ng]) return -1;
   int lo = 0, hi = ng - 1; /* largest g with g_pref[g] <= j */
   while (lo < hi) {
@@ -374,7 +382,8 @@ __device__ __forceinline__ int gathered(const Shared &sh, const Gather &G, const
 }
 
 /* Removes physical p from tile t's segment (swap with the segment's last). */
-__device__ void seg_remove(Shared &sh, const Ev &ev, int t, int p) {
+template <class SH>
+__device__ void seg_remove(SH &sh, const Ev &ev, int t, int p) {
   const int pos = ev.meta[p].w;
   const int last = sh.seg_start[t] + sh.seg_count[t] - 1;
   if (pos != last) {
@@ -388,7 +397,8 @@ __device__ void seg_remove(Shared &sh, const Ev &ev, int t, int p) {
 /* Appends physical p to tile t's segment; relocates the segment to the end of
  * the pool with twice the capacity when full, or flags a rebuild when the pool
  * is exhausted (p is then recorded only in its metadata). */
-__device__ void seg_insert(Shared &sh, const Ev &ev, int t, int p) {
+template <class SH>
+__device__ void seg_insert(SH &sh, const Ev &ev, int t, int p) {
   if (sh.seg_count[t] == sh.seg_cap[t]) {
     const int ncap = 2 * sh.seg_cap[t] + 2;
     if (sh.pool_top + ncap > ev.pool_size) {
@@ -413,7 +423,8 @@ __device__ void seg_insert(Shared &sh, const Ev &ev, int t, int p) {
 
 /* Lexicographic (distance, label) nearest neighbour of physical o among the
  * gathered tiles (all lanes). Returns the winner's physical index or -1. */
-__device__ int nn_scan(const Shared &sh, const Gather &G, const Ev &ev, int ng, int o, double orap, double ophi, double R2,
+template <class SH>
+__device__ int nn_scan(const SH &sh, const Gather &G, const Ev &ev, int ng, int o, double orap, double ophi, double R2,
                        int lane,
                        double *best_d, double *best_w) {
   const int total = G.pref[ng];
@@ -449,8 +460,11 @@ __device__ int nn_scan(const Shared &sh, const Gather &G, const Ev &ev, int ng, 
 
 using namespace tiled;
 
-__global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts) {
-  __shared__ Shared sh;
+/* MAXT tiles per event; at least WPE waves per SIMD (the register budget:
+ * 512 / WPE VGPRs per lane). */
+template <int MAXT, int WPE>
+__global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScratch ts) {
+  __shared__ Shared<MAXT> sh;
   const int lane = threadIdx.x;
   for (int t = lane; t < MAXT; t += W) sh.stamp[t] = 0;
   if (lane == 0) sh.stampv = 0;
@@ -505,7 +519,8 @@ __global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts
       const double mn = wave_min_d(ylo);
       const double mx = -wave_min_d(-yhi);
       if (lane == 0) {
-        const double side = R * (1.0 + 0x1p-20);
+        /* side >= R: the 3x3 neighbourhood of a tile holds every jet within R of its members */
+        const double side = R * ((a.tile_q > 4 ? a.tile_q : 4) * 0.25) * (1.0 + 0x1p-20);
         double lo = mn, hi = mx;
         if (!(lo <= hi)) {
           lo = 0.0;
@@ -840,7 +855,11 @@ __global__ __launch_bounds__(64) void tiled_kernel(LaunchArgs a, TiledScratch ts
   }
 }
 
-const void *const kTiledKernel = reinterpret_cast<const void *>(&tiled_kernel);
-const int kTiledMaxTiles = MAXT;
+/* 512 tiles: LDS holds 8 workgroups per CU; 256 tiles: registers hold 3 waves
+ * per SIMD, or 4 with a few spills; 128 tiles: 4 waves per SIMD. */
+const void *const kTiledKernels[kNumTiledKernels] = {
+    reinterpret_cast<const void *>(&tiled_kernel<512, 2>), reinterpret_cast<const void *>(&tiled_kernel<256, 3>),
+    reinterpret_cast<const void *>(&tiled_kernel<256, 4>), reinterpret_cast<const void *>(&tiled_kernel<128, 4>)};
+const int kTiledMaxTiles[kNumTiledKernels] = {512, 256, 256, 128};
 
 }  // namespace fjmi

@@ -54,8 +54,13 @@ enum fjmi_engine {
  *   FJMI_ENGINE_DENSE: `threads` threads per workgroup, `slots` slots per thread
  *     (capacity threads*slots >= max_n), four-momenta in LDS (p4_global = 0) or
  *     global memory (p4_global = 1);
- *   FJMI_ENGINE_TILED: one 64-lane wave per event (threads = 64, slots and
- *     p4_global unused).
+ *   FJMI_ENGINE_TILED: one 64-lane wave per event (threads = 64); `slots` is the
+ *     tile side in quarters of R, 4..16 (0 means 4: side R); `p4_global` selects
+ *     the kernel variant: 0 = 512 tiles (8 events per CU), 1 = 256 tiles
+ *     (12 per CU), 2 = 256 tiles (16 per CU, a few register spills), 3 = 128
+ *     tiles (16 per CU).  A smaller tile table takes less LDS, so more events
+ *     run per CU; an event whose tiles would not fit gets taller tiles in
+ *     rapidity.
  * Replaces the reference's occupancy-derived launch shapes (launch.h:11-60). */
 typedef struct {
   int32_t max_n;
@@ -78,8 +83,8 @@ int fjmi_create(int device, fjmi_context **out);
 int fjmi_destroy(fjmi_context *ctx);
 
 /* Kernel configurations compiled into the library: writes up to `cap` rows
- * (max_n = capacity of the configuration; the tiled engine reports INT32_MAX)
- * and returns how many exist. */
+ * (max_n = capacity of the configuration; the tiled engine reports INT32_MAX,
+ * one row per tile table size) and returns how many exist. */
 int fjmi_kernel_configs(fjmi_bucket *out, int cap);
 
 /* Replace / read the multiplicity-bucket table (rows sorted by max_n). */

@@ -83,6 +83,30 @@ def test_high_multiplicity_tiled(engine, p, alg):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("var", [2, 3])
+def test_high_multiplicity_small_tile_table(engine, var):
+    """Workload-4 events through a batch on the 256- and 128-tile kernels (tiles
+    taller in rapidity than R), kt and anti-kt R = 0.4."""
+    import fastjet_gpu_amd as fj
+    evs = [fj.gen_event(4, e) for e in (0, 5)]
+    off = np.array([0, len(evs[0]), len(evs[0]) + len(evs[1])], np.int64)
+    p4 = np.concatenate(evs)
+    saved = engine.buckets()
+    try:
+        engine.set_buckets([(1000000000, 2, 64, 0, var)])
+        for p in (1, -1):
+            print(f"[gpu parity] workload4 events 0,5 N={off[1]},{off[2] - off[1]} p={p} variant {var}", flush=True)
+            h1, h2, hd, jets, nj = engine.cluster_batch(p4, off, p, 0.4)
+            for e in range(2):
+                a, b = off[e], off[e + 1]
+                want = orc.tiled(p4[a:b], p, 0.4)
+                got = (h1[a:b], h2[a:b], hd[a:b], jets[a:a + nj[e]])
+                assert same(want, got), f"event {e}: {first_diff(want, got)}"
+    finally:
+        engine.set_buckets(saved)
+
+
+@pytest.mark.gpu
 def test_every_bucket_size(engine):
     """One event per multiplicity bucket boundary (and just above it)."""
     import fastjet_gpu_amd as fj
@@ -133,25 +157,56 @@ def test_empty_event(engine, eng):
 
 
 @pytest.mark.gpu
-def test_batch_all_tiled(engine):
-    """A batch with every event on the tiled engine (bucket table with one tiled row)."""
+@pytest.mark.parametrize("tq,var", [(0, 0), (6, 1), (16, 2), (5, 3), (4, 3)])
+def test_batch_all_tiled(engine, tq, var):
+    """A batch with every event on the tiled engine (bucket table with one tiled row):
+    tile side R (tq 0 or 4), 1.25 R, 1.5 R and 4 R, with each kernel variant (tile
+    tables of 512, 256 and 128 tiles): the history depends on neither."""
     import fastjet_gpu_amd as fj
     saved = engine.buckets()
     try:
-        engine.set_buckets([(1000000000, 2, 64, 0, 0)])
-        p4, off = fj.gen_workload(3, 64, first=100)
+        engine.set_buckets([(1000000000, 2, 64, tq, var)])
+        assert engine.buckets() == [(1000000000, 2, 64, tq, var)]
+        p4, off = fj.gen_workload(3, 64, first=100 + tq)
         ne = len(off) - 1
         ps = np.array([(-1, 0, 1)[e % 3] for e in range(ne)], np.int32)
         Rs = np.array([(0.4, 0.8, 1.0, 0.6)[e % 4] for e in range(ne)], np.float64)
-        print(f"[gpu parity] tiled batch workload3 x{ne}", flush=True)
+        print(f"[gpu parity] tiled batch workload3 x{ne} tile side {max(tq, 4) / 4} R variant {var}", flush=True)
         h1, h2, hd, jets, nj = engine.cluster_batch(p4, off, ps, Rs)
         for e in range(ne):
             a, b = off[e], off[e + 1]
             want = orc.n2plain(p4[a:b], int(ps[e]), float(Rs[e]))
             got = (h1[a:b], h2[a:b], hd[a:b], jets[a:a + nj[e]])
             assert same(want, got), f"tiled batch event {e} (N={b - a}): {first_diff(want, got)}"
+        a, b = off[0], off[1]
+        alone = engine.cluster_event(p4[a:b], int(ps[0]), float(Rs[0]))  # AUTO: the tiled row, same tile side
+        assert same(alone, (h1[a:b], h2[a:b], hd[a:b], jets[a:a + nj[0]]))
     finally:
         engine.set_buckets(saved)
+
+
+@pytest.mark.gpu
+def test_bucket_table_validation(engine):
+    import fastjet_gpu_amd as fj
+    saved = engine.buckets()
+    bad = [[(1000000000, 2, 64, 3, 0)], [(1000000000, 2, 64, 17, 0)], [(1000000000, 2, 128, 0, 0)],
+           [(1000000000, 2, 64, 0, 4)], [(1000000000, 2, 64, 0, -1)],
+           [(300, 1, 64, 4, 0)], [(256, 1, 64, 4, 0), (128, 2, 64, 0, 0)], [(64, 1, 64, 3, 0)], []]
+    for rows in bad:
+        with pytest.raises(fj.FjmiError):
+            engine.set_buckets(rows)
+        assert engine.buckets() == saved
+
+
+@pytest.mark.gpu
+def test_forced_dense_beyond_table(engine):
+    """engine=dense for an event larger than every dense table row runs the smallest
+    compiled dense kernel that holds it."""
+    import fastjet_gpu_amd as fj
+    ev = fj.gen_event(4, 1)[:3000]
+    check_event(engine, ev, -1, 0.4, "forced dense N=3000", 1)
+    with pytest.raises(fj.FjmiError):
+        engine.cluster_event(fj.gen_event(4, 2)[:9000], -1, 0.4, engine=1)
 
 
 @pytest.mark.gpu
