@@ -50,10 +50,14 @@ def main():
     ap.add_argument("--cpu-repeat", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="before the timed region, fill and copy a 256 MiB buffer: reference kernels of known HBM "
+                         "traffic for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU only (events are independent; multi-GPU sharding is "
-              "described in DESIGN.md and not run). Use --gpus 1.", file=sys.stderr)
+        print("bench.py: measures one GPU only (events are independent, so several GPUs would each run "
+              "this same batch on their own share of the events; that is not run here). Use --gpus 1.",
+              file=sys.stderr)
         return 2
 
     import torch
@@ -75,6 +79,14 @@ def main():
     d_hd = torch.zeros(tot, dtype=torch.float64, device=dev)
     d_j = torch.zeros(tot * 4, dtype=torch.float64, device=dev)
     d_nj = torch.zeros(ne, dtype=torch.int32, device=dev)
+    if a.calibrate_copy:
+        src = torch.empty(64 << 20, dtype=torch.float32, device=dev)
+        src.fill_(1.0)  # writes 256 MiB
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # reads 256 MiB, writes 256 MiB
+        torch.cuda.synchronize()
+        del src, dst
     stream = torch.cuda.current_stream().cuda_stream
     batch = eng.batch_create(off, d_p4.data_ptr(), d_off.data_ptr(), None, None, p, R, d_h1.data_ptr(),
                              d_h2.data_ptr(), d_hd.data_ptr(), d_j.data_ptr(), d_nj.data_ptr())

@@ -4,7 +4,7 @@
  *   void cluster(PseudoJet *particles, int size, Algorithm algo, double r);
  *
  * Same contract as the reference's cluster() (cluster.h:6, implemented per engine
- * at tri_matrix.cu:307, n_array.cu:229, grid.cu:580, tri_matrix_empty.cu:257):
+ * at tri_matrix.cu:286, n_array.cu:229, grid.cu:580, tri_matrix_empty.cu:257):
  * `particles` is a device pointer, the clustering is done in place, records with
  * isJet hold the jets, and errors abort after printing "file, line:" and the
  * failing call (the reference's cudaCheck, cudaCheck.h:11-47).

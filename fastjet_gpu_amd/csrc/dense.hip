@@ -7,21 +7,21 @@
  * persistent grid over a multiplicity bucket).
  *
  * Semantics: FastJet's N2Plain (the method the reference approximates with its
- * all-pairs triangle, tri_matrix.cu:307-357, and its single-block recompute loop,
+ * all-pairs triangle, tri_matrix.cu:138-159 and 286-336, and its single-block recompute loop,
  * n_array.cu:129-200), restated so that every decision, including exact ties, is
  * the same as the CPU oracle's:
  *   - jets live in slots 0..n-1; each has its nearest neighbour (NN) within R in
  *     (y, phi) (lowest slot on equal distance), NN distance and
  *     diJ = NN_dist * min(w_i, w_NN) (R^2 w_i without a NN);
  *   - each step takes the lowest slot with the smallest diJ (the reference's
- *     reduction_min/reduction_blocks argmin, tri_matrix.cu:182-257, with an
+ *     reduction_min/reduction_blocks argmin, tri_matrix.cu:161-257, with an
  *     index tie-break instead of CUB's tree order);
- *   - a pair merges (E-scheme, tri_matrix.cu:269-272) into the lower slot and the
+ *   - a pair merges (E-scheme, tri_matrix.cu:247-250) into the lower slot and the
  *     last slot moves into the higher one; a beam step removes the jet and the
- *     last slot moves into its place (tri_matrix.cu:262-276);
+ *     last slot moves into its place (tri_matrix.cu:240-255);
  *   - after a step only the merged jet's row is evaluated against every jet, and
  *     jets that had the removed or merged jet as NN rescan (incremental update,
- *     instead of the reference's full-triangle rescan, tri_matrix.cu:337-349).
+ *     instead of the reference's full-triangle rescan, tri_matrix.cu:161-202).
  *
  * Layout (MI355X-first):
  *   - slot s is owned by thread s % T and lives in that thread's registers at

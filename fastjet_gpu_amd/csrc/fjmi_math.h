@@ -21,8 +21,8 @@
  *
  * The kinematic conventions follow FastJet's PseudoJet (rapidity with the
  * effective-mass clamp, phi in [0, 2pi), MaxRap = 1e5 for zero-pt momenta) as
- * restated by the reference: tri_matrix.cu:77-121 (constants and _set_jet),
- * grid.cu:110-166 (weights per algorithm), tri_matrix.cu:123-130 (distance).
+ * restated by the reference: tri_matrix.cu:56-100 (constants and _set_jet),
+ * grid.cu:110-166 (weights per algorithm), tri_matrix.cu:102-113 (distance).
  */
 #ifndef FJMI_MATH_H
 #define FJMI_MATH_H
@@ -35,7 +35,7 @@
 #define FJMI_HD static inline
 #endif
 
-/* Constants (FastJet conventions; tri_matrix.cu:77-79). */
+/* Constants (FastJet conventions; tri_matrix.cu:56-58). */
 #define FJMI_PI 0x1.921fb54442d18p+1
 #define FJMI_TWOPI 0x1.921fb54442d18p+2
 #define FJMI_MAXRAP 1e5
@@ -232,7 +232,7 @@ FJMI_HD double fjmi_atan2(double y, double x) {
   return yneg ? -th2.hi : th2.hi;
 }
 
-/* ---- FastJet PseudoJet kinematics (tri_matrix.cu:82-121, grid.cu:112-166) ---- */
+/* ---- FastJet PseudoJet kinematics (tri_matrix.cu:61-100, grid.cu:112-166) ---- */
 
 /* Azimuth in [0, 2pi); 0 for zero transverse momentum. */
 FJMI_HD double fjmi_phi(double px, double py, double kt2) {
@@ -257,14 +257,14 @@ FJMI_HD double fjmi_rap(double pz, double E, double kt2) {
   return rap;
 }
 
-/* Generalised-kt weight kt^(2p) for p in {1, 0, -1} (cluster.h:4; tri_matrix.cu:61-74, :84). */
+/* Generalised-kt weight kt^(2p) for p in {1, 0, -1} (cluster.h:4; tri_matrix.cu:40-53, :63). */
 FJMI_HD double fjmi_weight(double kt2, int p) {
   if (p > 0) return kt2;
   if (p == 0) return 1.0;
   return kt2 > 1e-300 ? 1.0 / kt2 : 1e300;
 }
 
-/* Squared (rapidity, phi) distance with the phi wrap at pi (tri_matrix.cu:123-130). */
+/* Squared (rapidity, phi) distance with the phi wrap at pi (tri_matrix.cu:102-109). */
 FJMI_HD double fjmi_dr2(double rap1, double phi1, double rap2, double phi2) {
   double dphi = __builtin_fabs(phi1 - phi2);
   if (dphi > FJMI_PI) dphi = FJMI_TWOPI - dphi;

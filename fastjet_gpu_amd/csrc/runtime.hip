@@ -5,7 +5,7 @@
  * bucket table, kernel attributes), the one-event-per-launch path, the batched
  * path (bucketed persistent queues captured into one hipGraph) and the
  * host-memory conveniences used by tests and tools.  Replaces the reference's
- * per-call allocation and launch sequences (tri_matrix.cu:307-357,
+ * per-call allocation and launch sequences (tri_matrix.cu:286-336,
  * n_array.cu:229-243, grid.cu:580-640) and its launch-shape helpers
  * (launch.h:11-60).
  */

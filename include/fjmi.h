@@ -78,7 +78,7 @@ int fjmi_version(void);
 
 /* Context on HIP device `device`: device buffers, kernels, tuning table.
  * Replaces the per-call cudaMalloc/cudaFree of every engine
- * (tri_matrix.cu:308-320, :353-356; grid.cu:586-593, :636-639). */
+ * (tri_matrix.cu:288-299, :331-335; grid.cu:586-593, :636-639). */
 int fjmi_create(int device, fjmi_context **out);
 int fjmi_destroy(fjmi_context *ctx);
 
@@ -93,7 +93,7 @@ int fjmi_get_buckets(fjmi_context *ctx, fjmi_bucket *rows, int cap, int *n_rows)
 
 /* Single event, host memory in and out, synchronous.  Replaces one call of the
  * reference's cluster() with the CLI's H2D/D2H around it (main.cc:291-298,
- * tri_matrix.cu:307-357).  hist_* and jets hold n and 4n entries. */
+ * tri_matrix.cu:286-336).  hist_* and jets hold n and 4n entries. */
 int fjmi_cluster_event(fjmi_context *ctx, const double *p4, int n, int p, double R, int engine, int32_t *hist_p1,
                        int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets);
 
@@ -105,7 +105,7 @@ int fjmi_cluster_event_device(fjmi_context *ctx, const double *d_p4, int n, int 
 
 /* Drop-in: in-place clustering of `n` PseudoJet records (40 bytes each) in device
  * memory, the contract of the reference's cluster() (cluster.h:6; output layout
- * of tri_matrix.cu:293-305): afterwards records with isJet hold the jets.
+ * of tri_matrix.cu:240-245, 272-284): afterwards records with isJet hold the jets.
  * Asynchronous on `stream`. */
 int fjmi_cluster_pseudojets(fjmi_context *ctx, void *d_particles, int n, int p, double R, int engine, void *stream);
 

@@ -19,7 +19,7 @@
 double fjo_log(double x) { return fjmi_log(x); }
 double fjo_atan2(double y, double x) { return fjmi_atan2(y, x); }
 
-/* PseudoJet rap/phi and algorithm weight (tri_matrix.cu:82-121, grid.cu:153-166). */
+/* PseudoJet rap/phi and algorithm weight (tri_matrix.cu:61-100, grid.cu:153-166). */
 static void set_jet(const double *q, int p, int math_mode, double *rap, double *phi, double *w) {
   const double px = q[0], py = q[1], pz = q[2], E = q[3];
   const double kt2 = px * px + py * py;
@@ -87,7 +87,7 @@ static void book_free(book_t *b) {
   free(b->hist);
 }
 
-/* E-scheme recombination (tri_matrix.cu:269-272): returns the new jet index. */
+/* E-scheme recombination (tri_matrix.cu:247-250): returns the new jet index. */
 static int book_merge(book_t *b, int ja, int jb, double dij) {
   int k = b->njets_store++;
   for (int c = 0; c < 4; ++c) b->p4[4 * k + c] = b->p4[4 * ja + c] + b->p4[4 * jb + c];

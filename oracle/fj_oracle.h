@@ -10,9 +10,9 @@
  *  - FastJet 3.3.2's N2Plain strategy (ClusterSequence::_simple_N2_cluster, the
  *    published nearest-neighbour method of Cacciari & Salam, hep-ph/0512210),
  *    which is the algorithm the reference engines approximate on the GPU
- *    (README.md:1-4; tri_matrix.cu:307-357 all-pairs form; n_array.cu:129-200).
- *    Kinematics and distances follow tri_matrix.cu:82-134 and grid.cu:110-201.
- *    Recombination / compaction follow tri_matrix.cu:262-276 (beam: the jet is
+ *    (README.md:1-4; tri_matrix.cu:286-336 all-pairs form; n_array.cu:129-200).
+ *    Kinematics and distances follow tri_matrix.cu:56-113 and grid.cu:110-201.
+ *    Recombination / compaction follow tri_matrix.cu:240-255 (beam: the jet is
  *    removed and the last slot moves into its place; pair: the merged jet takes
  *    the lower slot and the last slot moves into the higher one).
  *  - The FastJet history numbering: particles are 0..N-1, the jet made at step s

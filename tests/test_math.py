@@ -6,7 +6,7 @@ kernels and the oracle use.
 - their agreement rate with glibc (which FastJet uses) is measured and bounded;
 - special values follow C99 Annex F;
 - rapidity/phi conventions of FastJet's PseudoJet as restated by the reference
-  (tri_matrix.cu:82-121): closed-form cases, symmetries, zero-pt handling.
+  (tri_matrix.cu:61-100): closed-form cases, symmetries, zero-pt handling.
 """
 import math
 import random
