@@ -13,8 +13,8 @@ import os
 
 import numpy as np
 
-__all__ = ["Engine", "FjmiError", "KT", "CAMBRIDGE_AACHEN", "ANTIKT", "ENGINE_AUTO", "ENGINE_DENSE",
-           "ENGINE_TILED", "ENGINE_EMPTY", "gen_event", "gen_workload", "lib_path", "load_lib"]
+__all__ = ["Engine", "Server", "FjmiError", "KT", "CAMBRIDGE_AACHEN", "ANTIKT", "ENGINE_AUTO", "ENGINE_DENSE",
+           "ENGINE_TILED", "ENGINE_EMPTY", "gen_event", "gen_serving_attrs", "gen_workload", "lib_path", "load_lib"]
 
 KT, CAMBRIDGE_AACHEN, ANTIKT = 1, 0, -1
 ENGINE_AUTO, ENGINE_DENSE, ENGINE_TILED, ENGINE_EMPTY = 0, 1, 2, 3
@@ -28,6 +28,9 @@ _vp = ctypes.c_void_p
 
 class FjmiError(RuntimeError):
     pass
+
+
+FJMI_EAGAIN = -6
 
 
 class Bucket(ctypes.Structure):
@@ -69,6 +72,25 @@ def load_lib(name="libfjmi.so"):
     return lib
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device_budget", ctypes.c_int64), ("host_budget", ctypes.c_int64), ("max_events", ctypes.c_int32),
+                ("variant", ctypes.c_int32), ("tile_q", ctypes.c_int32), ("idle_us", ctypes.c_int32)]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("handle", ctypes.c_int32), ("n", ctypes.c_int32),
+                ("njets", ctypes.c_int32), ("pad", ctypes.c_int32), ("hist_p1", _c_int_p), ("hist_p2", _c_int_p),
+                ("hist_dij", ctypes.POINTER(ctypes.c_double)), ("jets", ctypes.POINTER(ctypes.c_double))]
+
+
+class ServerInfo(ctypes.Structure):
+    _fields_ = [("particle_capacity", ctypes.c_int64), ("event_capacity", ctypes.c_int32), ("grid", ctypes.c_int32),
+                ("device_bytes", ctypes.c_int64), ("host_bytes", ctypes.c_int64), ("submitted", ctypes.c_int64),
+                ("completed", ctypes.c_int64), ("in_flight", ctypes.c_int64),
+                ("particles_in_flight", ctypes.c_int64), ("peak_particles_in_flight", ctypes.c_int64),
+                ("launches", ctypes.c_int64)]
+
+
 def _declare(lib):
     d = ctypes.c_double
     i = ctypes.c_int
@@ -90,6 +112,12 @@ def _declare(lib):
         "fjmi_batch_destroy": ([_vp], i),
         "fjmi_batch_stats": ([_vp, _vp, i, _c_int_p], i),
         "fjmi_cluster_batch": ([_vp, i, _vp, _vp, _vp, _vp, i, d, _vp, _vp, _vp, _vp, _vp], i),
+        "fjmi_server_create": ([_vp, _vp, ctypes.POINTER(_vp)], i),
+        "fjmi_server_submit": ([_vp, _vp, i, i, d, ctypes.c_uint64], i),
+        "fjmi_server_poll": ([_vp, _vp, i, _c_int_p], i),
+        "fjmi_server_release": ([_vp, ctypes.c_int32], i),
+        "fjmi_server_info_get": ([_vp, _vp], i),
+        "fjmi_server_destroy": ([_vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -189,6 +217,10 @@ class Engine:
         _check(self.lib, self.lib.fjmi_cluster_pseudojets(self.ctx, d_particles, n, int(p), float(R), engine,
                                                           stream or None), "fjmi_cluster_pseudojets")
 
+    def server(self, **cfg):
+        """A continuous-batching Server on this device (see Server)."""
+        return Server(self, **cfg)
+
     def batch_create(self, h_offsets, d_p4, d_offsets, d_p, d_R, p_uni, R_uni, d_h1, d_h2, d_hd, d_jets, d_nj):
         h_offsets = np.ascontiguousarray(h_offsets, dtype=np.int64)
         b = _vp()
@@ -235,6 +267,65 @@ class Batch:
             pass
 
 
+class Server:
+    """Continuous-batching server on one device (fjmi_server_*, include/fjmi.h).
+
+    submit() admits an event or returns False when the memory budget or the
+    event slots are full; poll() returns the completed events (results copied
+    into numpy arrays, then released)."""
+
+    def __init__(self, eng, device_budget=0, host_budget=0, max_events=0, variant=0, tile_q=0, idle_us=0):
+        self.eng = eng
+        cfg = ServerConfig(device_budget, host_budget, max_events, variant, tile_q, idle_us)
+        h = _vp()
+        _check(eng.lib, eng.lib.fjmi_server_create(eng.ctx, ctypes.byref(cfg), ctypes.byref(h)), "fjmi_server_create")
+        self.h = h
+        self._res = (ServerResult * 1024)()
+
+    def submit(self, p4, p, R, tag):
+        p4 = np.ascontiguousarray(p4, dtype=np.float64).reshape(-1, 4)
+        rc = self.eng.lib.fjmi_server_submit(self.h, _ptr(p4), len(p4), int(p), float(R), int(tag))
+        if rc == FJMI_EAGAIN:
+            return False
+        _check(self.eng.lib, rc, "fjmi_server_submit")
+        return True
+
+    def poll(self):
+        """Completed events as dicts: tag, n, njets, hist_p1, hist_p2, hist_dij, jets (numpy copies)."""
+        out = []
+        while True:
+            k = ctypes.c_int32()
+            _check(self.eng.lib, self.eng.lib.fjmi_server_poll(self.h, ctypes.cast(self._res, _vp), len(self._res),
+                                                               ctypes.byref(k)), "fjmi_server_poll")
+            for r in self._res[:k.value]:
+                n, nj = r.n, r.njets
+                out.append(dict(tag=r.tag, n=n, njets=nj,
+                                hist_p1=np.ctypeslib.as_array(r.hist_p1, (n,)).copy() if n else np.zeros(0, np.int32),
+                                hist_p2=np.ctypeslib.as_array(r.hist_p2, (n,)).copy() if n else np.zeros(0, np.int32),
+                                hist_dij=np.ctypeslib.as_array(r.hist_dij, (n,)).copy() if n else np.zeros(0),
+                                jets=np.ctypeslib.as_array(r.jets, (nj, 4)).copy() if nj > 0 else np.zeros((0, 4))))
+                _check(self.eng.lib, self.eng.lib.fjmi_server_release(self.h, r.handle), "fjmi_server_release")
+            if k.value < len(self._res):
+                return out
+
+    def info(self):
+        i = ServerInfo()
+        _check(self.eng.lib, self.eng.lib.fjmi_server_info_get(self.h, ctypes.byref(i)), "fjmi_server_info_get")
+        return {f: getattr(i, f) for f, _ in ServerInfo._fields_}
+
+    def close(self):
+        if self.h:
+            rc = self.eng.lib.fjmi_server_destroy(self.h)
+            self.h = None
+            _check(self.eng.lib, rc, "fjmi_server_destroy")
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # -- generator --
 def gen_event(workload, event):
     """(N, 4) float64 array of event `event` of workload 1..5 (see csrc/fjmi_gen.c)."""
@@ -247,6 +338,17 @@ def gen_event(workload, event):
     if rc != n:
         raise FjmiError(f"generator failed ({rc})")
     return out[:n]
+
+
+def gen_serving_attrs(event):
+    """(p, R, gap) of serving request `event`: algorithm and radius drawn from
+    {anti-kt 0.4, anti-kt 0.8, C/A 0.8, kt 0.4}, and a unit-mean exponential gap."""
+    g = load_lib("libfjmi_gen.so")
+    p = ctypes.c_int32()
+    R = ctypes.c_double()
+    gap = ctypes.c_double()
+    g.fjmi_gen_serving_attrs(event, ctypes.byref(p), ctypes.byref(R), ctypes.byref(gap))
+    return p.value, R.value, gap.value
 
 
 def gen_workload(workload, n_events, first=0):

@@ -12,8 +12,17 @@ current GPU, and keeps the fastest:
                 16 events per CU).
 
 Adjacent intervals with the same winner are merged; the last row is always a
-tiled row covering every larger event.  The reference picks its launch shapes
-per call from the occupancy calculator instead (launch.h:11-60, grid.cu:597-633).
+tiled row covering every larger event.
+
+A second, whole-batch stage then times complete tables on a mixed batch (by
+default the 4096-event N ~ U[50, 2000] workload): the merged per-interval table
+and simpler ones (the per-interval dense winners up to a cut, then one tiled
+row), and keeps the fastest.  Per-interval throughput does not decide alone:
+in a mixed batch the kernels of all buckets share the GPU, the batch ends with
+its largest events, and every distinct kernel competes for instruction cache.
+
+The reference picks its launch shapes per call from the occupancy calculator
+instead (launch.h:11-60, grid.cu:597-633).
 
 Events of an interval come from the seeded generator: events of the 4096-event
 N ~ U[50, 2000] workload whose N falls in the interval, or, outside [50, 2000],
@@ -135,6 +144,50 @@ def merge(choices):
     return [tuple(r) for r in rows]
 
 
+def simple_tables(choices):
+    """Dense per-interval winners up to a cut, then the tiled winner of the largest interval."""
+    tiled = [c for _, c in choices if c[0] == 2]
+    last = tiled[-1] if tiled else (2, 64, 0, 0)
+    out = []
+    for cut in (0, 128, 256, 512):
+        dense = [(hi, c) for hi, c in choices if hi <= cut and c[0] == 1]
+        if cut and (not dense or dense[-1][0] != cut):
+            continue
+        rows = [r for r in merge(dense + [(LAST_ROW, last)])]
+        out.append((f"dense<= {cut} + tiled", rows))
+        if dense:
+            one = (cut, dense[-1][1])
+            out.append((f"one dense row <= {cut} + tiled", merge([one, (LAST_ROW, last)])))
+    return out
+
+
+def time_table(eng, rows, p4, off, d, p, R, reps):
+    eng.set_buckets(rows)
+    b = eng.batch_create(off, d["p4"].data_ptr(), d["off"].data_ptr(), None, None, p, R, d["h1"].data_ptr(),
+                         d["h2"].data_ptr(), d["hd"].data_ptr(), d["j"].data_ptr(), d["nj"].data_ptr())
+    try:
+        b.launch()
+        b.sync()
+        best = float("inf")
+        for _ in range(max(reps, 5)):
+            t = time.perf_counter()
+            b.launch()
+            b.sync()
+            best = min(best, time.perf_counter() - t)
+        return (len(off) - 1) / best
+    finally:
+        b.close()
+
+
+def device_buffers(torch, dev, p4, off):
+    tot = len(p4)
+    return dict(p4=torch.from_numpy(p4).to(dev), off=torch.from_numpy(off).to(dev),
+                h1=torch.empty(tot, dtype=torch.int32, device=dev), h2=torch.empty(tot, dtype=torch.int32, device=dev),
+                hd=torch.empty(tot, dtype=torch.float64, device=dev),
+                j=torch.empty(4 * tot, dtype=torch.float64, device=dev),
+                nj=torch.empty(len(off) - 1, dtype=torch.int32, device=dev))
+
+
 def write_inc(rows, note):
     path = os.path.join(PKG, "csrc", "tune_table.inc")
     lines = ["/* Multiplicity buckets: {max_n, engine, threads, slots, four-momenta",
@@ -155,6 +208,8 @@ def main(argv=None):
     ap.add_argument("--p", type=int, default=-1)
     ap.add_argument("--R", type=float, default=0.4)
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--batch-workload", type=int, default=3, help="generator workload of the whole-batch stage")
+    ap.add_argument("--batch-events", type=int, default=4096)
     ap.add_argument("--json", default=None)
     ap.add_argument("--write", action="store_true")
     a = ap.parse_args(argv)
@@ -175,13 +230,7 @@ def main(argv=None):
         for hi in edges:
             count = a.events if hi <= 2048 else max(512, min(a.events, int(8e6 // ((lo + hi) / 2))))
             p4, off = src.interval(lo, hi, count)
-            tot = len(p4)
-            d = dict(p4=torch.from_numpy(p4).to(dev), off=torch.from_numpy(off).to(dev),
-                     h1=torch.empty(tot, dtype=torch.int32, device=dev),
-                     h2=torch.empty(tot, dtype=torch.int32, device=dev),
-                     hd=torch.empty(tot, dtype=torch.float64, device=dev),
-                     j=torch.empty(4 * tot, dtype=torch.float64, device=dev),
-                     nj=torch.empty(count, dtype=torch.int32, device=dev))
+            d = device_buffers(torch, dev, p4, off)
             best = None
             for cfg in candidates(configs, lo, hi):
                 rate = time_config(fj, torch, eng, cfg, hi, p4, off, d, a.p, a.R, a.reps)
@@ -193,19 +242,33 @@ def main(argv=None):
             choices.append((hi, best[1]))
             lo = hi
             del d
+        # whole-batch stage
+        p4, off = fj.gen_workload(a.batch_workload, a.batch_events)
+        d = device_buffers(torch, dev, p4, off)
+        tables = [("per-interval winners", merge(choices))] + simple_tables(choices)
+        tables.append(("library default", saved))
+        whole = []
+        for name, rows in tables:
+            rate = time_table(eng, rows, p4, off, d, a.p, a.R, a.reps)
+            whole.append(dict(name=name, table=[list(r) for r in rows], events_per_s=rate))
+            print(f"batch of {len(off) - 1} (workload {a.batch_workload}): {name:32s} {rate:10.0f} ev/s  {rows}",
+                  flush=True)
+        del d
     finally:
         eng.set_buckets(saved)
-    rows = merge(choices)
+    best_whole = max(whole, key=lambda w: w["events_per_s"])
+    rows = [tuple(r) for r in best_whole["table"]]
     props = torch.cuda.get_device_properties(0)
     dev_name = f"{props.name} ({getattr(props, 'gcnArchName', '?')})"
     out = dict(device=dev_name, p=a.p, R=a.R, events=a.events, edges=edges, measurements=results,
-               table=[list(r) for r in rows])
+               whole_batch=whole, chosen=best_whole["name"], table=[list(r) for r in rows])
     print(json.dumps(dict(table=out["table"])), flush=True)
     if a.json:
         with open(a.json, "w") as fh:
             json.dump(out, fh, indent=1)
     if a.write:
-        note = f"{a.events} events per interval, anti-kt R={a.R}" if a.p == -1 else f"{a.events} events, p={a.p} R={a.R}"
+        note = (f"whole-batch stage on {a.batch_events} events of workload {a.batch_workload}, p={a.p} R={a.R}: "
+                f"{best_whole['name']}")
         print("wrote", write_inc(rows, f"{note}, {dev_name}"))
     eng.close()
 

@@ -6,6 +6,7 @@
                       per event by the bucket table), for programs written against it
   bin/tri_matrix, bin/n_array, bin/grid, bin/tri_matrix_empty
                       the drop-in CLI linked with one engine each
+  bin/fjmi_serve      open-loop Poisson load generator for the serving path
   oracle/_build/libfj_oracle.so   CPU oracle (test infrastructure, via oracle/Makefile)
 
 Usage: python -m fastjet_gpu_amd.build [--jobs N] [--no-cli] [--no-oracle]
@@ -57,10 +58,11 @@ def build(jobs=4, cli=True, oracle=True):
         f_dense = ex.submit(hip_obj, "dense.hip")
         f_tiled = ex.submit(hip_obj, "tiled.hip")
         f_rt = ex.submit(hip_obj, "runtime.hip")
+        f_sv = ex.submit(hip_obj, "serve.hip")
         f_gen = ex.submit(run, ["gcc", *CC_FLAGS, "-shared", "-o", os.path.join(LIB, "libfjmi_gen.so"),
                                 os.path.join(CSRC, "fjmi_gen.c"), "-lm"])
         f_or = ex.submit(run, ["make", "-s", "-C", os.path.join(ROOT, "oracle")]) if oracle else None
-        objs = [f_dense.result(), f_tiled.result(), f_rt.result()]
+        objs = [f_dense.result(), f_tiled.result(), f_rt.result(), f_sv.result()]
         f_gen.result()
         if f_or:
             f_or.result()
@@ -84,6 +86,10 @@ def build_cli():
              os.path.join(CSRC, "cluster_dropin.cpp"), "-o", eo])
         run(["g++", "-o", os.path.join(BIN, name), main_o, eo, f"-L{LIB}", "-lfjmi",
              "-Wl,-rpath,$ORIGIN/../lib", f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"])
+    serve_o = os.path.join(OBJ, "serve_main.o")
+    run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "serve_main.cpp"), "-o", serve_o])
+    run(["g++", "-o", os.path.join(BIN, "fjmi_serve"), serve_o, f"-L{LIB}", "-lfjmi", "-lfjmi_gen",
+         "-Wl,-rpath,$ORIGIN/../lib", f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"])
     so = os.path.join(LIB, "libfjmi_cluster.so")
     run(["g++", *CXX_FLAGS, "-fPIC", "-shared", "-DFJMI_LINKED_ENGINE=FJMI_ENGINE_AUTO", "-o", so + ".tmp",
          os.path.join(CSRC, "cluster_dropin.cpp"), f"-L{LIB}", "-lfjmi", "-Wl,-rpath,$ORIGIN",

@@ -12,6 +12,40 @@ namespace fjmi {
 
 /* Arguments of one clustering launch: a queue of events (indices into the
  * packed particle arrays) that persistent workgroups pull from. */
+/* Serving: one admitted event, published by the host in fine-grained host memory. */
+struct ServeDesc {
+  int64_t base;     /* first particle slot in the arenas */
+  int64_t pool_off; /* first tile-segment pool entry (3n + pool_per_event entries) */
+  int32_t n;
+  int32_t p;
+  double R;
+  int32_t slot; /* event slot: result njets index, completion token - 1 */
+  int32_t pad;
+};
+
+/* Serving: words the host writes and the kernels read (fine-grained host memory). */
+struct ServeHost {
+  unsigned long long tail; /* descriptors published (host: release store) */
+  unsigned int stop;       /* the host asks the kernels to leave */
+  unsigned int pad;
+};
+
+/* Serving: what a serving launch of the tiled kernel needs beside LaunchArgs. */
+struct ServeArgs {
+  ServeHost *host;        /* fine-grained host memory */
+  const ServeDesc *ring;  /* fine-grained host memory, ring_cap entries */
+  int32_t *comp;          /* fine-grained host memory, ring_cap entries: slot + 1 per completion, in order */
+  int32_t *res_nj;        /* fine-grained host memory, per slot (-1: the event failed) */
+  int32_t *res_p1, *res_p2;
+  double *res_dij, *res_jets; /* fine-grained host memory, indexed like the particles */
+  /* device memory: [0] descriptors claimed, [1] mirror of host->tail, [2] poll
+   * lock (one wave at a time reads the host words), [3] mirror of host->stop,
+   * [4] completions pushed, [5] wall clock of the last claim */
+  unsigned long long *dev;
+  unsigned int ring_cap;
+  unsigned long long idle_ticks; /* wall_clock64 ticks without any claim after which waves leave */
+};
+
 struct LaunchArgs {
   /* input particles: event e occupies particles [offsets[e], offsets[e+1]);
    * particle i's (px,py,pz,E) are in[i*in_stride + in_off + 0..3] */
@@ -49,18 +83,26 @@ struct LaunchArgs {
   unsigned long long *t_end;
   int stage_chunk; /* number of staging entries in LDS when four-momenta live in global memory */
   int tile_q;      /* tiled engine: tile side = max(tile_q, 4)/4 * R (rounded up) */
+  ServeArgs serve; /* serving launches of the tiled engine only */
 };
 
 enum { STATUS_CAPACITY = 1, STATUS_INTERNAL = 2 };
+
+/* The tiled engine's per-jet record: one 64-byte block (half a cache line), so
+ * that visiting a jet in a gather or a rescan costs one line, not four. */
+struct alignas(64) JetRec {
+  double2 rp;  /* rapidity, phi */
+  double2 wn;  /* weight, nearest-neighbour distance */
+  int4 meta;   /* nearest neighbour (particle index or -1), label, tile, position in the tile segment */
+  double dij;  /* diJ = NN distance * min(w, w_NN), or R^2 w without a NN */
+  double pad;
+};
 
 /* Global per-particle scratch of the tiled engine (indexed by particle like the
  * input), plus the tile-segment pool: 3 entries per particle and pool_per_event
  * per event (event e's pool starts at 3*offsets[e] + e*pool_per_event). */
 struct TiledScratch {
-  double *rp;   /* 2 per particle: rapidity, phi */
-  double *wn;   /* 2 per particle: weight, nearest-neighbour distance */
-  double *dij;  /* 1 per particle */
-  int *meta;    /* 4 per particle: nn, label, tile, segment position */
+  JetRec *rec;  /* 1 per particle */
   double *p4;   /* 4 per particle */
   int *hist;    /* 1 per particle */
   int *lab2;    /* 1 per particle: label -> particle */
@@ -68,26 +110,25 @@ struct TiledScratch {
   int pool_per_event;
 };
 
+/* Bytes of tiled scratch per particle: record, four-momentum, history index,
+ * label map and 3 pool entries. */
+constexpr size_t kTiledBytesPerParticle = sizeof(JetRec) + 32 + 4 + 4 + 12;
+
 /* Bytes of tiled scratch for `particles` particles in `events` events. */
 inline size_t tiled_scratch_bytes(int64_t particles, int64_t events, int pool_per_event) {
-  return (size_t)particles * (16 + 16 + 8 + 16 + 32 + 4 + 4 + 12) + (size_t)events * pool_per_event * 4 + 1024;
+  return (size_t)particles * kTiledBytesPerParticle + (size_t)events * pool_per_event * 4 + 1024;
 }
 
-/* Carves a TiledScratch out of one allocation of tiled_scratch_bytes(). */
+/* Carves a TiledScratch out of one allocation of tiled_scratch_bytes() (the
+ * records first, so they keep the allocation's 256-byte alignment). */
 inline TiledScratch tiled_scratch_carve(void *mem, int64_t particles, int pool_per_event) {
   char *c = static_cast<char *>(mem);
   TiledScratch t;
   const size_t P = (size_t)particles;
+  t.rec = reinterpret_cast<JetRec *>(c);
+  c += sizeof(JetRec) * P;
   t.p4 = reinterpret_cast<double *>(c);
   c += 32 * P;
-  t.rp = reinterpret_cast<double *>(c);
-  c += 16 * P;
-  t.wn = reinterpret_cast<double *>(c);
-  c += 16 * P;
-  t.meta = reinterpret_cast<int *>(c);
-  c += 16 * P;
-  t.dij = reinterpret_cast<double *>(c);
-  c += 8 * P;
   t.hist = reinterpret_cast<int *>(c);
   c += 4 * P;
   t.lab2 = reinterpret_cast<int *>(c);

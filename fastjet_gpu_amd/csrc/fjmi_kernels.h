@@ -25,6 +25,7 @@ extern const void *const kZeroKernel;  /* __global__ void(unsigned *, int, unsig
  * table size (index = a tiled bucket row's p4_global field) */
 constexpr int kNumTiledKernels = 4;
 extern const void *const kTiledKernels[kNumTiledKernels];
+extern const void *const kTiledServeKernels[kNumTiledKernels]; /* same, serving mode (LaunchArgs::serve) */
 extern const int kTiledMaxTiles[kNumTiledKernels];
 
 /* Pool entries per event of the tiled engine beyond 3 per particle (>= 2 per tile). */

@@ -1,0 +1,74 @@
+/*
+ * fjmi_runtime.h — host-side pieces shared by the library's translation units
+ * (runtime.hip: contexts, single-event and batched paths; serve.hip: the
+ * serving loop): the error reporting behind fjmi_last_error(), a growable
+ * device buffer, and the per-device context.
+ */
+#ifndef FJMI_RUNTIME_H
+#define FJMI_RUNTIME_H
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+#include "../../include/fjmi.h"
+
+namespace fjmi {
+namespace rt {
+
+/* Records a printf-style message for fjmi_last_error() (per thread) and returns code. */
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define HIPCHK(call)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess)                                                                              \
+      return ::fjmi::rt::fail(FJMI_EHIP, "%s:%d: %s: %s: %s", __FILE__, __LINE__, #call,              \
+                              hipGetErrorName(e_), hipGetErrorString(e_));                             \
+  } while (0)
+
+/* Growable device buffer. */
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need) {
+    if (need <= bytes) return FJMI_OK;
+    if (ptr) HIPCHK(hipFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+    size_t b = std::max(need, (size_t)256);
+    HIPCHK(hipMalloc(&ptr, b));
+    bytes = b;
+    return FJMI_OK;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(ptr);
+  }
+};
+
+}  // namespace rt
+}  // namespace fjmi
+
+/* Per-device library context. */
+struct fjmi_context {
+  int device = 0;
+  int n_cu = 0;
+  std::vector<fjmi_bucket> buckets;
+  std::mutex mtx;
+  /* one-event path */
+  fjmi::rt::DevBuf ctrl;
+  uint32_t head_base = 0;
+  fjmi::rt::DevBuf hist1, hist2, hdij, jets, njets, p4s, coords, p4in, tscratch;
+  int max_lds = 0;
+};
+
+#endif

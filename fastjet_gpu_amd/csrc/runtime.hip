@@ -22,15 +22,14 @@
 #include "../../include/fjmi.h"
 #include "fjmi_internal.h"
 #include "fjmi_kernels.h"
+#include "fjmi_runtime.h"
 
 using fjmi::KernelDesc;
 using fjmi::LaunchArgs;
 
-namespace {
-
 thread_local std::string g_err;
 
-int fail(int code, const char *fmt, ...) {
+int fjmi::rt::fail(int code, const char *fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -40,38 +39,10 @@ int fail(int code, const char *fmt, ...) {
   return code;
 }
 
-#define HIPCHK(call)                                                                                   \
-  do {                                                                                                 \
-    hipError_t e_ = (call);                                                                            \
-    if (e_ != hipSuccess)                                                                              \
-      return fail(FJMI_EHIP, "%s:%d: %s: %s: %s", __FILE__, __LINE__, #call, hipGetErrorName(e_),      \
-                  hipGetErrorString(e_));                                                              \
-  } while (0)
+namespace {
 
-/* Growable device buffer. */
-struct DevBuf {
-  void *ptr = nullptr;
-  size_t bytes = 0;
-  int ensure(size_t need) {
-    if (need <= bytes) return FJMI_OK;
-    if (ptr) HIPCHK(hipFree(ptr));
-    ptr = nullptr;
-    bytes = 0;
-    size_t b = std::max(need, (size_t)256);
-    HIPCHK(hipMalloc(&ptr, b));
-    bytes = b;
-    return FJMI_OK;
-  }
-  void release() {
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    bytes = 0;
-  }
-  template <class T>
-  T *as() const {
-    return static_cast<T *>(ptr);
-  }
-};
+using fjmi::rt::DevBuf;
+using fjmi::rt::fail;
 
 const fjmi_bucket kDefaultBuckets[] = {
 #include "tune_table.inc"
@@ -100,17 +71,6 @@ struct Ctrl {
 
 }  // namespace
 
-struct fjmi_context {
-  int device = 0;
-  int n_cu = 0;
-  std::vector<fjmi_bucket> buckets;
-  std::mutex mtx;
-  /* one-event path */
-  DevBuf ctrl;
-  uint32_t head_base = 0;
-  DevBuf hist1, hist2, hdij, jets, njets, p4s, coords, p4in, tscratch;
-  int max_lds = 0;
-};
 
 struct BucketRun {
   bool tiled;

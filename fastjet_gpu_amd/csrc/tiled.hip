@@ -79,6 +79,7 @@ struct Shared {
   int pool_top;
   int rebuild;
   int ev;
+  ServeDesc sd; /* serving: the claimed event's descriptor */
   int ny, nphi;
   double y0, sy, sphi;
 };
@@ -196,10 +197,7 @@ struct Geo {
 
 /* The per-event views of the global scratch. */
 struct Ev {
-  double2 *rp;  /* (rapidity, phi) */
-  double2 *wn;  /* (weight, NN distance) */
-  double *dij;  /* diJ */
-  int4 *meta;   /* (nn physical or -1, label, tile, position in the tile segment) */
+  JetRec *rec;  /* per-jet record: everything a gather or a rescan reads */
   double4 *p4;
   int *hist;
   int *lab2;    /* label -> physical */
@@ -225,7 +223,7 @@ __device__ void rebuild_segments(SH &sh, const Ev &ev, int n, int ntiles, int la
   __syncthreads();
   for (int l = lane; l < n; l += W) {
     const int p = ev.lab2[l];
-    atomicAdd(&sh.seg_count[ev.meta[p].z], 1);
+    atomicAdd(&sh.seg_count[ev.rec[p].meta.z], 1);
   }
   __syncthreads();
   int carry = 0;
@@ -248,10 +246,10 @@ __device__ void rebuild_segments(SH &sh, const Ev &ev, int n, int ntiles, int la
   __syncthreads();
   for (int l = lane; l < n; l += W) {
     const int p = ev.lab2[l];
-    const int t = ev.meta[p].z;
+    const int t = ev.rec[p].meta.z;
     const int pos = sh.seg_start[t] + atomicAdd(&sh.seg_count[t], 1);
     ev.pool[pos] = p;
-    ev.meta[p].w = pos;
+    ev.rec[p].meta.w = pos;
   }
   __threadfence_block();
   __syncthreads();
@@ -280,12 +278,12 @@ __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, in
     for (int j0 = 0; j0 < total; j0 += W) {
       const int p = gathered(sh, G, ev, ng, j0 + lane);
       if (p >= 0) {
-        const int t = ev.meta[p].z;
-        const unsigned long long bits = (unsigned long long)__double_as_longlong(ev.dij[p]);
+        const int t = ev.rec[p].meta.z;
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(ev.rec[p].dij);
         if (phase == 0) {
           atomicMin(&sh.tmin[t].d, bits);
         } else if (sh.tmin[t].d == bits) {
-          const int lab = ev.meta[p].y;
+          const int lab = ev.rec[p].meta.y;
           if (phase == 1)
             atomicMin(&sh.tmin[t].l, lab);
           else if (sh.tmin[t].l == lab)
@@ -384,12 +382,12 @@ __device__ __forceinline__ int gathered(const SH &sh, const Gather &G, const Ev 
 /* Removes physical p from tile t's segment (swap with the segment's last). */
 template <class SH>
 __device__ void seg_remove(SH &sh, const Ev &ev, int t, int p) {
-  const int pos = ev.meta[p].w;
+  const int pos = ev.rec[p].meta.w;
   const int last = sh.seg_start[t] + sh.seg_count[t] - 1;
   if (pos != last) {
     const int q = ev.pool[last];
     ev.pool[pos] = q;
-    ev.meta[q].w = pos;
+    ev.rec[q].meta.w = pos;
   }
   sh.seg_count[t] -= 1;
 }
@@ -409,7 +407,7 @@ __device__ void seg_insert(SH &sh, const Ev &ev, int t, int p) {
     for (int i = 0; i < sh.seg_count[t]; ++i) {
       const int q = ev.pool[sh.seg_start[t] + i];
       ev.pool[ns + i] = q;
-      ev.meta[q].w = ns + i;
+      ev.rec[q].meta.w = ns + i;
     }
     sh.seg_start[t] = ns;
     sh.seg_cap[t] = ncap;
@@ -417,7 +415,7 @@ __device__ void seg_insert(SH &sh, const Ev &ev, int t, int p) {
   }
   const int pos = sh.seg_start[t] + sh.seg_count[t];
   ev.pool[pos] = p;
-  ev.meta[p].w = pos;
+  ev.rec[p].meta.w = pos;
   sh.seg_count[t] += 1;
 }
 
@@ -433,14 +431,14 @@ __device__ int nn_scan(const SH &sh, const Gather &G, const Ev &ev, int ng, int 
   for (int j0 = 0; j0 < total; j0 += W) {
     const int q = gathered(sh, G, ev, ng, j0 + lane);
     if (q >= 0 && q != o) {
-      const double2 rp = ev.rp[q];
+      const double2 rp = ev.rec[q].rp;
       const double d = fjmi_dr2(orap, ophi, rp.x, rp.y);
-      const int lab = ev.meta[q].y;
+      const int lab = ev.rec[q].meta.y;
       if (d < R2 && (d < bd || (d == bd && lab < bl))) {
         bd = d;
         bl = lab;
         bp = q;
-        bw = ev.wn[q].x;
+        bw = ev.rec[q].wn.x;
       }
     }
   }
@@ -460,9 +458,75 @@ __device__ int nn_scan(const SH &sh, const Gather &G, const Ev &ev, int ng, int 
 
 using namespace tiled;
 
+/* Serving: claims the next published event descriptor (lane 0 only).  Waves
+ * claim through a device-memory head and a device-memory mirror of the host's
+ * published count; one wave at a time (the poll lock) reads the host words over
+ * PCIe and refreshes the mirrors.  Returns the descriptor index, or -1 when the
+ * host asked to stop or when no wave has claimed anything for idle_ticks: the
+ * idle clock is the whole launch's (the last claim by any wave), so waves leave
+ * together once the server is idle, never one by one under a trickle of work,
+ * and a launch always ends once work stops coming. */
+__device__ __forceinline__ long long serve_claim(const ServeArgs &s) {
+  unsigned long long *head = s.dev, *tail = s.dev + 1, *lock = s.dev + 2, *stop = s.dev + 3,
+                     *last_claim = s.dev + 5;
+  const unsigned long long t0 = wall_clock64();
+  int nap = 1;
+  for (;;) {
+    const unsigned long long h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long t = __hip_atomic_load(tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    if (h < t) {
+      if (atomicCAS(head, h, h + 1) == h) {
+        __hip_atomic_fetch_max(last_claim, (unsigned long long)wall_clock64(), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        return (long long)h;
+      }
+      continue;
+    }
+    if (__hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return -1;
+    if (atomicCAS(lock, 0ull, 1ull) == 0ull) {
+      const unsigned long long ht = __hip_atomic_load(&s.host->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+      const unsigned int st = __hip_atomic_load(&s.host->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_fetch_max(tail, ht, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (st) __hip_atomic_store(stop, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(lock, 0ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (ht > h || st) continue;
+    }
+    const unsigned long long lc = __hip_atomic_load(last_claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (wall_clock64() - (lc > t0 ? lc : t0) > s.idle_ticks) return -1;
+    for (int k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(64);
+    nap = nap < 8 ? 2 * nap : 8;
+  }
+}
+
+/* Serving: copies an event's history and jets from the device arenas to the
+ * host result arrays with wide coalesced stores, then announces the slot on the
+ * completion ring (after a system-scope fence, so the host sees the results
+ * first).  njets = -1 marks an event whose clustering failed. */
+__device__ __forceinline__ void serve_finish(const LaunchArgs &a, int slot, int64_t base, int N, int nj, int lane) {
+  const ServeArgs &s = a.serve;
+  __threadfence_block();
+  __syncthreads();
+  for (int i = lane; i < N; i += W) {
+    s.res_p1[base + i] = a.hist_p1[base + i];
+    s.res_p2[base + i] = a.hist_p2[base + i];
+    s.res_dij[base + i] = a.hist_dij[base + i];
+  }
+  const int nd = nj > 0 ? 4 * nj : 0;
+  for (int i = lane; i < nd; i += W) s.res_jets[4 * base + i] = a.jets[4 * base + i];
+  if (lane == 0) s.res_nj[slot] = nj;
+  __threadfence_system();
+  __syncthreads();
+  if (lane == 0) {
+    const unsigned long long c = atomicAdd(s.dev + 4, 1ull);
+    __hip_atomic_store(s.comp + (unsigned)(c % s.ring_cap), slot + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 /* MAXT tiles per event; at least WPE waves per SIMD (the register budget:
- * 512 / WPE VGPRs per lane). */
-template <int MAXT, int WPE>
+ * 512 / WPE VGPRs per lane).  SERVE: events come from the host-published
+ * descriptor ring of LaunchArgs::serve instead of a fixed queue, and each
+ * event's results are copied to host memory and announced on completion. */
+template <int MAXT, int WPE, bool SERVE>
 __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScratch ts) {
   __shared__ Shared<MAXT> sh;
   const int lane = threadIdx.x;
@@ -470,33 +534,63 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
   if (lane == 0) sh.stampv = 0;
   for (;;) {
     __syncthreads();
-    if (lane == 0) sh.ev = (int)(atomicAdd(a.head, 1u) - a.head_base);
+    if (lane == 0) {
+      if constexpr (SERVE) {
+        const long long h = serve_claim(a.serve);
+        sh.ev = h < 0 ? -1 : 0;
+        if (h >= 0) {
+          const ServeDesc *d = a.serve.ring + (unsigned long long)h % a.serve.ring_cap;
+          sh.sd.base = d->base;
+          sh.sd.pool_off = d->pool_off;
+          sh.sd.n = d->n;
+          sh.sd.p = d->p;
+          sh.sd.R = d->R;
+          sh.sd.slot = d->slot;
+        }
+      } else {
+        sh.ev = (int)(atomicAdd(a.head, 1u) - a.head_base);
+      }
+    }
     __syncthreads();
     const int qi = uni(sh.ev);
-    if (qi >= a.n_queue) {
+    if (SERVE ? qi < 0 : qi >= a.n_queue) {
       if (a.t_end && threadIdx.x == 0) atomicMax(a.t_end, (unsigned long long)wall_clock64());
       break;
     }
-    const int e = a.queue ? uni(a.queue[qi]) : qi;
-    const int64_t base = a.offsets ? a.offsets[e] : 0;
-    const int N = a.offsets ? uni((int)(a.offsets[e + 1] - base)) : a.n_single;
+    int e, N, p;
+    int64_t base, pool_off;
+    double R;
+    if constexpr (SERVE) {
+      e = uni(sh.sd.slot);
+      N = uni(sh.sd.n);
+      p = uni(sh.sd.p);
+      R = uni_d(sh.sd.R);
+      base = ((int64_t)(unsigned)uni((int)(sh.sd.base >> 32)) << 32) | (unsigned)uni((int)sh.sd.base);
+      pool_off = ((int64_t)(unsigned)uni((int)(sh.sd.pool_off >> 32)) << 32) | (unsigned)uni((int)sh.sd.pool_off);
+    } else {
+      e = a.queue ? uni(a.queue[qi]) : qi;
+      base = a.offsets ? a.offsets[e] : 0;
+      N = a.offsets ? uni((int)(a.offsets[e + 1] - base)) : a.n_single;
+      p = a.p_arr ? a.p_arr[e] : a.p_uni;
+      R = a.R_arr ? a.R_arr[e] : a.R_uni;
+      pool_off = 3 * base + (int64_t)e * ts.pool_per_event;
+    }
     if (N <= 0) {
-      if (lane == 0) a.njets[e] = 0;
+      if constexpr (SERVE) {
+        serve_finish(a, e, base, 0, 0, lane);
+      } else if (lane == 0) {
+        a.njets[e] = 0;
+      }
       continue;
     }
-    const int p = a.p_arr ? a.p_arr[e] : a.p_uni;
-    const double R = a.R_arr ? a.R_arr[e] : a.R_uni;
     const double R2 = R * R;
     const double invR2 = 1.0 / R2;
     Ev ev;
-    ev.rp = reinterpret_cast<double2 *>(ts.rp) + base;
-    ev.wn = reinterpret_cast<double2 *>(ts.wn) + base;
-    ev.dij = ts.dij + base;
-    ev.meta = reinterpret_cast<int4 *>(ts.meta) + base;
+    ev.rec = ts.rec + base;
     ev.p4 = reinterpret_cast<double4 *>(ts.p4) + base;
     ev.hist = ts.hist + base;
     ev.lab2 = ts.lab2 + base;
-    ev.pool = ts.pool + (3 * base + (int64_t)e * ts.pool_per_event);
+    ev.pool = ts.pool + pool_off;
     ev.pool_size = 3 * N + ts.pool_per_event;
 
     /* ---- prologue: kinematics, rapidity range ---- */
@@ -505,8 +599,8 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       const double *q = a.in + (size_t)(base + s) * a.in_stride + a.in_off;
       const double4 q4 = make_double4(q[0], q[1], q[2], q[3]);
       const double4 c = kin_coords_t(q4.x, q4.y, q4.z, q4.w, p);
-      ev.rp[s] = make_double2(c.x, c.y);
-      ev.wn[s] = make_double2(c.z, R2);
+      ev.rec[s].rp = make_double2(c.x, c.y);
+      ev.rec[s].wn = make_double2(c.z, R2);
       ev.p4[s] = q4;
       ev.hist[s] = s;
       ev.lab2[s] = s;
@@ -547,8 +641,8 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
     geo.sphi = uni_d(sh.sphi);
     const int ntiles = geo.ny * geo.nphi;
     for (int s = lane; s < N; s += W) {
-      const double2 rp = ev.rp[s];
-      ev.meta[s] = make_int4(-1, s, geo.tile(rp.x, rp.y), 0);
+      const double2 rp = ev.rec[s].rp;
+      ev.rec[s].meta = make_int4(-1, s, geo.tile(rp.x, rp.y), 0);
     }
     __threadfence_block();
     __syncthreads();
@@ -556,10 +650,10 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
 
     /* ---- initial nearest neighbours: each lane scans the 3x3 tiles of its jets ---- */
     for (int s = lane; s < N; s += W) {
-      const double2 rp = ev.rp[s];
-      const double ws = ev.wn[s].x;
+      const double2 rp = ev.rec[s].rp;
+      const double ws = ev.rec[s].wn.x;
       int nb[9];
-      const int m = geo.neigh(ev.meta[s].z, nb);
+      const int m = geo.neigh(ev.rec[s].meta.z, nb);
       double bd = R2, bw = 0.0;
       int bl = BIG, bp = -1;
       for (int i = 0; i < m; ++i) {
@@ -568,7 +662,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         for (int u = 0; u < ct; ++u) {
           const int q = ev.pool[st + u];
           if (q == s) continue;
-          const double2 rq = ev.rp[q];
+          const double2 rq = ev.rec[q].rp;
           const double d = fjmi_dr2(rp.x, rp.y, rq.x, rq.y);
           if (d < bd || (d == bd && d < R2 && q < bl)) { /* labels == physical indices here */
             bd = d;
@@ -577,10 +671,10 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           }
         }
       }
-      if (bp >= 0) bw = ev.wn[bp].x;
-      ev.meta[s].x = bp;
-      ev.wn[s] = make_double2(ws, bd);
-      ev.dij[s] = diJ_of(bd, ws, bp, bw);
+      if (bp >= 0) bw = ev.rec[bp].wn.x;
+      ev.rec[s].meta.x = bp;
+      ev.rec[s].wn = make_double2(ws, bd);
+      ev.rec[s].dij = diJ_of(bd, ws, bp, bw);
     }
     __threadfence_block();
     __syncthreads();
@@ -604,6 +698,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
 
     /* ---- the N recombination steps ---- */
     int nj = 0;
+    bool bad = false;
     for (int step = 0; step < N; ++step) {
       const int n_old = N - step;
       /* 1. argmin over the tile minima */
@@ -628,15 +723,16 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       const int A = tA < 0 ? -1 : uni(sh.tmin[tA].p);
       if (A < 0) { /* no live jet found: inconsistent state, stop this event */
         if (lane == 0) atomicOr(a.status, 2u);
+        bad = true;
         break;
       }
       /* 2. the merge */
-      const int4 mA = ev.meta[A];
+      const int4 mA = ev.rec[A].meta;
       const int B = uni(mA.x);
       int phys_hi, phys_lo, lab_hi;
       int4 mB = make_int4(-1, -1, -1, -1);
       if (B >= 0) {
-        mB = ev.meta[B];
+        mB = ev.rec[B].meta;
         const bool a_hi = mA.y > mB.y;
         phys_hi = a_hi ? A : B;
         phys_lo = a_hi ? B : A;
@@ -650,15 +746,15 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       const int pt = uni(ev.lab2[tail]);
       const int tile_hi = uni(phys_hi == A ? mA.z : mB.z);
       const int tile_lo_old = phys_lo < 0 ? -1 : uni(phys_lo == A ? mA.z : mB.z);
-      const int tile_pt = uni(ev.meta[pt].z);
+      const int tile_pt = uni(ev.rec[pt].meta.z);
       /* the removed jet leaves its tile and the last label moves (N2Plain
        * compaction) before the new jet's kinematics, so their memory traffic
        * overlaps the arithmetic */
       if (lane == 0) {
         seg_remove(sh, ev, tile_hi, phys_hi);
-        ev.meta[phys_hi] = make_int4(-1, -1, -1, -1);
+        ev.rec[phys_hi].meta = make_int4(-1, -1, -1, -1);
         if (pt != phys_hi) {
-          ev.meta[pt].y = lab_hi;
+          ev.rec[pt].meta.y = lab_hi;
           ev.lab2[lab_hi] = pt;
         }
       }
@@ -681,12 +777,12 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           a.hist_dij[base + step] = dmin * invR2;
           ev.p4[phys_lo] = qn;
           ev.hist[phys_lo] = N + step;
-          ev.rp[phys_lo] = make_double2(nrap, nphi);
-          ev.wn[phys_lo] = make_double2(nw, R2);
-          ev.meta[phys_lo].x = -1;
+          ev.rec[phys_lo].rp = make_double2(nrap, nphi);
+          ev.rec[phys_lo].wn = make_double2(nw, R2);
+          ev.rec[phys_lo].meta.x = -1;
           if (tile_new != tile_lo_old) {
             seg_remove(sh, ev, tile_lo_old, phys_lo);
-            ev.meta[phys_lo].z = tile_new;
+            ev.rec[phys_lo].meta.z = tile_new;
             seg_insert(sh, ev, tile_new, phys_lo);
           }
         }
@@ -739,20 +835,20 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         const int q = gathered(sh, G, ev, ng, j0 + lane);
         bool orph = false;
         if (q >= 0 && q != phys_lo) {
-          int4 mq = ev.meta[q];
-          double dq = ev.dij[q];
+          int4 mq = ev.rec[q].meta;
+          double dq = ev.rec[q].dij;
           orph = (mq.x == phys_hi) || (phys_lo >= 0 && mq.x == phys_lo);
           myq = q;
           mylab = mq.y;
           if (phys_lo >= 0) {
-            const double2 rq = ev.rp[q];
-            const double2 wq = ev.wn[q];
+            const double2 rq = ev.rec[q].rp;
+            const double2 wq = ev.rec[q].wn;
             const double d = fjmi_dr2(rq.x, rq.y, nrap, nphi);
             if (!orph && d < wq.y) {
               dq = d * (nw < wq.x ? nw : wq.x);
-              ev.meta[q].x = phys_lo;
-              ev.wn[q] = make_double2(wq.x, d);
-              ev.dij[q] = dq;
+              ev.rec[q].meta.x = phys_lo;
+              ev.rec[q].wn = make_double2(wq.x, d);
+              ev.rec[q].dij = dq;
             }
             if (d < R2 && (d < cbd || (d == cbd && mq.y < cbl))) {
               cbd = d;
@@ -764,7 +860,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           mybits = (unsigned long long)__double_as_longlong(dq);
         } else if (q >= 0) { /* the new jet itself: filled in below */
           myq = q;
-          mylab = ev.meta[q].y;
+          mylab = ev.rec[q].meta.y;
         }
         const unsigned long long ob = __ballot(orph);
         if (ob) {
@@ -784,9 +880,9 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         const double wB = ln < 0 ? 0.0 : readlane_d(cbw, ln);
         const double dB = diJ_of(md, nw, nnB, wB);
         if (lane == 0) {
-          ev.meta[phys_lo].x = nnB;
-          ev.wn[phys_lo] = make_double2(nw, md);
-          ev.dij[phys_lo] = dB;
+          ev.rec[phys_lo].meta.x = nnB;
+          ev.rec[phys_lo].wn = make_double2(nw, md);
+          ev.rec[phys_lo].dij = dB;
         }
         if (myq == phys_lo) mybits = (unsigned long long)__double_as_longlong(dB);
       }
@@ -799,17 +895,17 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         const int nlist = min(norph, MAXO);
         for (int i = 0; i < nlist; ++i) {
           const int o = uni(sh.orph[i]);
-          const int4 mo = ev.meta[o];
-          const double2 ro = ev.rp[o];
-          const double wo = ev.wn[o].x;
+          const int4 mo = ev.rec[o].meta;
+          const double2 ro = ev.rec[o].rp;
+          const double wo = ev.rec[o].wn.x;
           const int ngo = build_gather(sh, GO, geo, mo.z, -1, -1, -1, lane);
           double od, owb;
           const int nno = nn_scan(sh, GO, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
           const double dO = diJ_of(od, wo, nno, owb);
           if (lane == 0) {
-            ev.meta[o].x = nno;
-            ev.wn[o] = make_double2(wo, od);
-            ev.dij[o] = dO;
+            ev.rec[o].meta.x = nno;
+            ev.rec[o].wn = make_double2(wo, od);
+            ev.rec[o].dij = dO;
           }
           if (myq == o) mybits = (unsigned long long)__double_as_longlong(dO);
           __threadfence_block();
@@ -821,22 +917,22 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           for (int j0 = 0; j0 < tot2; j0 += W) {
             const int q = gathered(sh, G, ev, ng2, j0 + lane);
             const bool orph = q >= 0 && q != phys_lo &&
-                              (ev.meta[q].x == phys_hi || (phys_lo >= 0 && ev.meta[q].x == phys_lo));
+                              (ev.rec[q].meta.x == phys_hi || (phys_lo >= 0 && ev.rec[q].meta.x == phys_lo));
             unsigned long long ob = __ballot(orph);
             while (ob) {
               const int ln = __ffsll(ob) - 1;
               ob &= ob - 1;
               const int o = __builtin_amdgcn_readlane(q, ln);
-              const int4 mo = ev.meta[o];
-              const double2 ro = ev.rp[o];
-              const double wo = ev.wn[o].x;
+              const int4 mo = ev.rec[o].meta;
+              const double2 ro = ev.rec[o].rp;
+              const double wo = ev.rec[o].wn.x;
               const int ngo = build_gather(sh, GO, geo, mo.z, -1, -1, -1, lane);
               double od, owb;
               const int nno = nn_scan(sh, GO, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
               if (lane == 0) {
-                ev.meta[o].x = nno;
-                ev.wn[o] = make_double2(wo, od);
-                ev.dij[o] = diJ_of(od, wo, nno, owb);
+                ev.rec[o].meta.x = nno;
+                ev.rec[o].wn = make_double2(wo, od);
+                ev.rec[o].dij = diJ_of(od, wo, nno, owb);
               }
               __threadfence_block();
               __syncthreads();
@@ -851,15 +947,23 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       else
         refresh_minima(sh, G, ev, ng, lane);
     }
-    if (lane == 0) a.njets[e] = nj;
+    if constexpr (SERVE) {
+      serve_finish(a, e, base, N, bad ? -1 : nj, lane);
+    } else if (lane == 0) {
+      a.njets[e] = nj;
+    }
   }
 }
 
 /* 512 tiles: LDS holds 8 workgroups per CU; 256 tiles: registers hold 3 waves
  * per SIMD, or 4 with a few spills; 128 tiles: 4 waves per SIMD. */
 const void *const kTiledKernels[kNumTiledKernels] = {
-    reinterpret_cast<const void *>(&tiled_kernel<512, 2>), reinterpret_cast<const void *>(&tiled_kernel<256, 3>),
-    reinterpret_cast<const void *>(&tiled_kernel<256, 4>), reinterpret_cast<const void *>(&tiled_kernel<128, 4>)};
+    reinterpret_cast<const void *>(&tiled_kernel<512, 2, false>), reinterpret_cast<const void *>(&tiled_kernel<256, 3, false>),
+    reinterpret_cast<const void *>(&tiled_kernel<256, 4, false>), reinterpret_cast<const void *>(&tiled_kernel<128, 4, false>)};
+/* serving launches: the same variants, events from the host-published ring */
+const void *const kTiledServeKernels[kNumTiledKernels] = {
+    reinterpret_cast<const void *>(&tiled_kernel<512, 2, true>), reinterpret_cast<const void *>(&tiled_kernel<256, 3, true>),
+    reinterpret_cast<const void *>(&tiled_kernel<256, 4, true>), reinterpret_cast<const void *>(&tiled_kernel<128, 4, true>)};
 const int kTiledMaxTiles[kNumTiledKernels] = {512, 256, 256, 128};
 
 }  // namespace fjmi

@@ -39,7 +39,8 @@ enum fjmi_status {
   FJMI_EHIP = -2,        /* a HIP runtime call failed */
   FJMI_ECAPACITY = -3,   /* an event is larger than any configured kernel */
   FJMI_ENOMEM = -4,      /* allocation failed / over the memory budget */
-  FJMI_EUNSUPPORTED = -5 /* engine not available */
+  FJMI_EUNSUPPORTED = -5, /* engine not available */
+  FJMI_EAGAIN = -6        /* serving: the memory budget or the event slots are full; poll, release, retry */
 };
 
 enum fjmi_engine {
@@ -144,6 +145,70 @@ int fjmi_batch_stats(fjmi_batch *b, fjmi_bucket_stats *out, int cap, int *n_buck
 int fjmi_cluster_batch(fjmi_context *ctx, int n_events, const int64_t *offsets, const double *p4, const int32_t *p,
                        const double *R, int p_uni, double R_uni, int32_t *hist_p1, int32_t *hist_p2, double *hist_dij,
                        double *jets, int32_t *njets);
+
+/* ---- Serving: continuous batching of arriving events ------------------------
+ * Not in the reference (its CLI clusters one event per call, main.cc:269-368).
+ * A server keeps a fixed device-memory arena for the state of admitted events
+ * (per particle: tiled scratch, history and jets; per event: a tile-segment
+ * pool) and pinned host memory for their inputs and results.  submit() copies
+ * an event into pinned memory and publishes it; persistent tiled-engine
+ * workgroups on the GPU claim published events as they free up, cluster them,
+ * copy the results to pinned memory and announce each completion, so events of
+ * any size flow through without batch boundaries.  An event is admitted only
+ * if its state fits the budget; otherwise submit() returns FJMI_EAGAIN and the
+ * caller keeps it queued.  The GPU workgroups leave together once no event has
+ * been claimed for idle_us, and are relaunched by the next submit() or poll().
+ * A server is driven by one host thread; distinct servers are independent. */
+typedef struct fjmi_server fjmi_server;
+
+typedef struct {
+  int64_t device_budget; /* bytes of device memory for event state (0: 16 GiB) */
+  int64_t host_budget;   /* bytes of pinned host memory for inputs and results (0: 8 GiB) */
+  int32_t max_events;    /* event slots, i.e. events admitted at once (0: 65536) */
+  int32_t variant;       /* tiled kernel variant, as the p4_global field of a tiled bucket row */
+  int32_t tile_q;        /* tile side in quarters of R (0: R) */
+  int32_t idle_us;       /* microseconds without any claimed event before the GPU workgroups leave (0: 20000) */
+} fjmi_server_config;
+
+typedef struct {
+  uint64_t tag;    /* as given to fjmi_server_submit */
+  int32_t handle;  /* pass to fjmi_server_release once the result has been read */
+  int32_t n;       /* particles */
+  int32_t njets;   /* jets, or -1 if the clustering failed */
+  int32_t pad;
+  /* the event's results in pinned host memory, valid until fjmi_server_release:
+   * n history steps and njets jets, laid out as in the other entry points */
+  const int32_t *hist_p1;
+  const int32_t *hist_p2;
+  const double *hist_dij;
+  const double *jets;
+} fjmi_server_result;
+
+typedef struct {
+  int64_t particle_capacity; /* particles the arenas hold at once */
+  int32_t event_capacity;
+  int32_t grid;              /* workgroups of a serving launch */
+  int64_t device_bytes;      /* device memory held by the server */
+  int64_t host_bytes;        /* pinned host memory held by the server */
+  int64_t submitted, completed;
+  int64_t in_flight;         /* admitted and not yet released */
+  int64_t particles_in_flight, peak_particles_in_flight;
+  int64_t launches;          /* serving kernel launches so far */
+} fjmi_server_info;
+
+/* cfg may be NULL (all defaults). */
+int fjmi_server_create(fjmi_context *ctx, const fjmi_server_config *cfg, fjmi_server **out);
+/* Admits one event (p4: 4n doubles of host memory, copied before returning):
+ * FJMI_OK, FJMI_EAGAIN (no room now), or an error. */
+int fjmi_server_submit(fjmi_server *s, const double *p4, int n, int p, double R, uint64_t tag);
+/* Completed events since the last poll, in completion order (non-blocking);
+ * writes up to cap results and their count to *n_out. */
+int fjmi_server_poll(fjmi_server *s, fjmi_server_result *out, int cap, int *n_out);
+/* Frees a completed event's memory and slot. */
+int fjmi_server_release(fjmi_server *s, int32_t handle);
+int fjmi_server_info_get(fjmi_server *s, fjmi_server_info *out);
+/* Stops the GPU workgroups (waiting for events already claimed) and frees everything. */
+int fjmi_server_destroy(fjmi_server *s);
 
 #ifdef __cplusplus
 }
