@@ -88,7 +88,7 @@ class ServerInfo(ctypes.Structure):
                 ("device_bytes", ctypes.c_int64), ("host_bytes", ctypes.c_int64), ("submitted", ctypes.c_int64),
                 ("completed", ctypes.c_int64), ("in_flight", ctypes.c_int64),
                 ("particles_in_flight", ctypes.c_int64), ("peak_particles_in_flight", ctypes.c_int64),
-                ("launches", ctypes.c_int64)]
+                ("launches", ctypes.c_int64), ("gpu_peak_busy", ctypes.c_int64)]
 
 
 def _declare(lib):

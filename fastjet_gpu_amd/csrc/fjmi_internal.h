@@ -30,6 +30,23 @@ struct ServeHost {
   unsigned int pad;
 };
 
+/* Serving: indices of the device-memory control words in ServeArgs::dev, 16
+ * words (128 bytes) apart so that idle waves polling one of them do not queue
+ * behind atomics on another. */
+enum : int {
+  kServeHead = 0,        /* descriptors claimed */
+  kServeTail = 16,       /* mirror of ServeHost::tail */
+  kServeLock = 32,       /* poll lock: one wave at a time reads the host words */
+  kServeStop = 48,       /* mirror of ServeHost::stop */
+  kServeCompleted = 64,  /* completions pushed */
+  kServeLastClaim = 80,  /* wall clock of the last claim */
+  kServeBusy = 96,       /* events being clustered now */
+  kServePeakBusy = 112,  /* the most ever clustered at once */
+  kServeLastPoll = 128,  /* wall clock of the last read of the host words */
+  kServeExitEpoch = 144, /* launch epoch whose waves are leaving */
+  kServeCtlWords = 160
+};
+
 /* Serving: what a serving launch of the tiled kernel needs beside LaunchArgs. */
 struct ServeArgs {
   ServeHost *host;        /* fine-grained host memory */
@@ -38,12 +55,11 @@ struct ServeArgs {
   int32_t *res_nj;        /* fine-grained host memory, per slot (-1: the event failed) */
   int32_t *res_p1, *res_p2;
   double *res_dij, *res_jets; /* fine-grained host memory, indexed like the particles */
-  /* device memory: [0] descriptors claimed, [1] mirror of host->tail, [2] poll
-   * lock (one wave at a time reads the host words), [3] mirror of host->stop,
-   * [4] completions pushed, [5] wall clock of the last claim */
+  /* device-memory control words, each in its own 128-byte line (indices kServe*) */
   unsigned long long *dev;
   unsigned int ring_cap;
   unsigned long long idle_ticks; /* wall_clock64 ticks without any claim after which waves leave */
+  unsigned long long epoch;      /* this launch's number (1, 2, ...) */
 };
 
 struct LaunchArgs {

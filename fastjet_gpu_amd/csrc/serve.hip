@@ -136,8 +136,9 @@ struct fjmi_server {
   uint64_t published = 0, comp_next = 0;
   int64_t in_flight = 0, particles_in_flight = 0, peak_particles = 0, completed = 0, launches = 0;
   /* launch */
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr, copy_stream = nullptr;
   hipEvent_t done_ev = nullptr;
+  unsigned long long *dctl = nullptr; /* the kernels' device control words */
   bool running = false;
   std::chrono::steady_clock::time_point last_check;
   const void *fn = nullptr;
@@ -157,6 +158,7 @@ void free_server(fjmi_server *s) {
   if (s->running && s->stream) (void)hipStreamSynchronize(s->stream);
   if (s->done_ev) (void)hipEventDestroy(s->done_ev);
   if (s->stream) (void)hipStreamDestroy(s->stream);
+  if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
   if (s->dev_block) (void)hipFree(s->dev_block);
   if (s->host_block) (void)hipHostFree(s->host_block);
   delete s;
@@ -174,6 +176,7 @@ int ensure_running(fjmi_server *s, bool force_check) {
     s->running = false;
   }
   if ((int64_t)(s->published - s->comp_next) <= 0) return FJMI_OK;
+  s->args.serve.epoch = (unsigned long long)s->launches + 1;
   void *params[] = {&s->args, &s->ts};
   HIPCHK(hipLaunchKernel(s->fn, dim3(s->grid), dim3(64), params, 0, s->stream));
   HIPCHK(hipEventRecord(s->done_ev, s->stream));
@@ -254,7 +257,8 @@ int fjmi_server_create(fjmi_context *ctx, const fjmi_server_config *cfg_in, fjmi
   char *hdb = static_cast<char *>(hd);
 
   /* device block: control words | status | history | jets | tiled scratch */
-  const size_t d_ctl = 0, d_status = 64, d_p1 = 256, d_p2 = d_p1 + ((4 * (size_t)P + 255) & ~(size_t)255),
+  const size_t d_ctl = 0, d_status = 8 * (size_t)fjmi::kServeCtlWords, d_p1 = (d_status + 64 + 255) & ~(size_t)255,
+               d_p2 = d_p1 + ((4 * (size_t)P + 255) & ~(size_t)255),
                d_dij = d_p2 + ((4 * (size_t)P + 255) & ~(size_t)255),
                d_jets = d_dij + ((8 * (size_t)P + 255) & ~(size_t)255),
                d_ts = d_jets + ((32 * (size_t)P + 255) & ~(size_t)255);
@@ -265,8 +269,9 @@ int fjmi_server_create(fjmi_context *ctx, const fjmi_server_config *cfg_in, fjmi
     return bail(fail(FJMI_ENOMEM, "device arena (%zu bytes): %s", s->dev_bytes, hipGetErrorString(e)));
   }
   char *db = static_cast<char *>(s->dev_block);
-  if (hipMemset(db, 0, 256) != hipSuccess) return bail(fail(FJMI_EHIP, "hipMemset failed"));
+  if (hipMemset(db, 0, d_p1) != hipSuccess) return bail(fail(FJMI_EHIP, "hipMemset failed"));
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&s->done_ev, hipEventDisableTiming) != hipSuccess)
     return bail(fail(FJMI_EHIP, "stream/event creation failed"));
 
@@ -298,6 +303,7 @@ int fjmi_server_create(fjmi_context *ctx, const fjmi_server_config *cfg_in, fjmi
   sv.res_dij = reinterpret_cast<double *>(hdb + o_dij);
   sv.res_jets = reinterpret_cast<double *>(hdb + o_jets);
   sv.dev = reinterpret_cast<unsigned long long *>(db + d_ctl);
+  s->dctl = sv.dev;
   sv.ring_cap = (unsigned)E;
   sv.idle_ticks = (unsigned long long)(cfg.idle_us > 0 ? cfg.idle_us : 20000) * 100ull; /* 100 MHz clock */
 
@@ -406,6 +412,12 @@ int fjmi_server_info_get(fjmi_server *s, fjmi_server_info *o) {
   o->particles_in_flight = s->particles_in_flight;
   o->peak_particles_in_flight = s->peak_particles;
   o->launches = s->launches;
+  /* read while the kernels may run: a copy on its own stream */
+  unsigned long long peak = 0;
+  HIPCHK(hipMemcpyAsync(&peak, s->dctl + fjmi::kServePeakBusy, sizeof peak, hipMemcpyDeviceToHost,
+                         s->copy_stream));
+  HIPCHK(hipStreamSynchronize(s->copy_stream));
+  o->gpu_peak_busy = (int64_t)peak;
   return FJMI_OK;
 }
 

@@ -254,13 +254,13 @@ int main(int argc, char **argv) {
         "\"admission_wait_ms\": {\"p50\": %.3f, \"p99\": %.3f}, \"max_host_queue\": %ld, \"refused_submits\": %ld, "
         "\"peak_particles_in_flight\": %lld, \"particle_capacity\": %lld, \"event_slots\": %d, "
         "\"device_bytes\": %lld, \"host_bytes\": %lld, \"launches\": %lld, \"grid\": %d, "
-        "\"mean_n_unique\": %.1f, \"unique_events\": %d, \"server_create_s\": %.3f}\n",
+        "\"mean_n_unique\": %.1f, \"unique_events\": %d, \"server_create_s\": %.3f, \"gpu_peak_busy\": %lld}\n",
         rate, n_req, (double)n_req / span, (double)n_req / reqs[n_req - 1].arrival, 1e3 * percentile(lat, 0.50),
         1e3 * percentile(lat, 0.90), 1e3 * percentile(lat, 0.99), 1e3 * percentile(lat, 0.999),
         1e3 * percentile(lat, 1.0), 1e3 * percentile(admit_wait, 0.50), 1e3 * percentile(admit_wait, 0.99), max_fifo,
         eagain, (long long)after.peak_particles_in_flight, (long long)after.particle_capacity, after.event_capacity,
         (long long)after.device_bytes, (long long)after.host_bytes, (long long)(after.launches - before.launches),
-        after.grid, mean_n, unique, create_s);
+        after.grid, mean_n, unique, create_s, (long long)after.gpu_peak_busy);
     std::fflush(stdout);
   }
   if (fjmi_server_destroy(srv)) die("fjmi_server_destroy");

@@ -166,6 +166,23 @@ __device__ __forceinline__ int wave_incl_scan(int v, int) {
   return v;
 }
 
+/* The workgroup is exactly one wavefront (64 threads), and a wavefront's
+ * memory operations, global and LDS, take effect in program order for all its
+ * lanes, so lanes hand data to each other through memory with no wait on the
+ * memory system: a wavefront-scope fence (a compiler barrier) replaces the
+ * workgroup barrier and fence, which would wait for every outstanding store to
+ * complete.  (FJMI_TILED_WG_FENCES=1 builds the workgroup-scope form.) */
+#ifndef FJMI_TILED_WG_FENCES
+#define FJMI_TILED_WG_FENCES 0
+#endif
+__device__ __forceinline__ void wave_sync() {
+#if FJMI_TILED_WG_FENCES
+  wave_sync();
+#else
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#endif
+}
+
 struct Geo {
   int ny, nphi;
   double y0, sy, sphi;
@@ -175,23 +192,6 @@ struct Geo {
     int tp = (int)(phi / sphi);
     tp = tp >= nphi ? nphi - 1 : (tp < 0 ? 0 : tp);
     return ty * nphi + tp;
-  }
-  /* distinct tiles of the 3x3 neighbourhood of t, written to out (returns count) */
-  __device__ int neigh(int t, int *out) const {
-    const int ty = t / nphi, tp = t % nphi;
-    int k = 0;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int y = ty + dy;
-      if (y < 0 || y >= ny) continue;
-      for (int dp = -1; dp <= 1; ++dp) {
-        const int q = (tp + dp + nphi) % nphi;
-        const int u = y * nphi + q;
-        bool dup = false;
-        for (int m = 0; m < k; ++m) dup |= out[m] == u;
-        if (!dup) out[k++] = u;
-      }
-    }
-    return k;
   }
 };
 
@@ -220,12 +220,12 @@ __device__ __forceinline__ double diJ_of(double nnd, double w, int nn, double wn
 template <class SH>
 __device__ void rebuild_segments(SH &sh, const Ev &ev, int n, int ntiles, int lane) {
   for (int t = lane; t < ntiles; t += W) sh.seg_count[t] = 0;
-  __syncthreads();
+  wave_sync();
   for (int l = lane; l < n; l += W) {
     const int p = ev.lab2[l];
     atomicAdd(&sh.seg_count[ev.rec[p].meta.z], 1);
   }
-  __syncthreads();
+  wave_sync();
   int carry = 0;
   for (int t0 = 0; t0 < ntiles; t0 += W) {
     const int t = t0 + lane;
@@ -243,7 +243,7 @@ __device__ void rebuild_segments(SH &sh, const Ev &ev, int n, int ntiles, int la
     sh.pool_top = carry;
     sh.rebuild = 0;
   }
-  __syncthreads();
+  wave_sync();
   for (int l = lane; l < n; l += W) {
     const int p = ev.lab2[l];
     const int t = ev.rec[p].meta.z;
@@ -251,8 +251,7 @@ __device__ void rebuild_segments(SH &sh, const Ev &ev, int n, int ntiles, int la
     ev.pool[pos] = p;
     ev.rec[p].meta.w = pos;
   }
-  __threadfence_block();
-  __syncthreads();
+  wave_sync();
 }
 
 template <class SH>
@@ -269,7 +268,7 @@ __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, in
     sh.tmin[t].l = BIG;
     sh.tmin[t].p = -1;
   }
-  __syncthreads();
+  wave_sync();
   const int total = G.pref[ng];
   /* three passes over all members: the diJ minimum, then the lowest label among
    * members at that minimum, then its index (each pass must see the final result
@@ -291,7 +290,7 @@ __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, in
         }
       }
     }
-    __syncthreads();
+    wave_sync();
   }
 }
 
@@ -305,7 +304,7 @@ __device__ void refresh_from_table(SH &sh, const Gather &G, int ng, int total, i
     sh.mt_l[lane] = mylab;
     sh.mt_p[lane] = myq;
   }
-  __syncthreads();
+  wave_sync();
   if (lane < ng) {
     const int t = G.tiles[lane];
     const int j1 = G.pref[lane + 1];
@@ -324,7 +323,7 @@ __device__ void refresh_from_table(SH &sh, const Gather &G, int ng, int total, i
     }
     sh.tmin[t] = best;
   }
-  __syncthreads();
+  wave_sync();
 }
 
 /* Sets G.tiles to the distinct union of the 3x3 neighbourhoods of the given
@@ -334,7 +333,7 @@ __device__ void refresh_from_table(SH &sh, const Gather &G, int ng, int total, i
 template <class SH>
 __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, int c2, int extra, int lane) {
   if (lane == 0) sh.stampv += 1;
-  __syncthreads();
+  wave_sync();
   const int stampv = uni(sh.stampv);
   int cand = -1;
   if (lane < 27) {
@@ -358,7 +357,7 @@ __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, i
     G.pref[idx + 1] = incl;
   }
   if (lane == 0) G.pref[0] = 0;
-  __syncthreads();
+  wave_sync();
   return (int)__popcll(kb);
 }
 
@@ -377,19 +376,6 @@ __device__ __forceinline__ int gathered(const SH &sh, const Gather &G, const Ev 
   const int t = G.tiles[lo];
   if (tile) *tile = t;
   return ev.pool[sh.seg_start[t] + (j - G.pref[lo])];
-}
-
-/* Removes physical p from tile t's segment (swap with the segment's last). */
-template <class SH>
-__device__ void seg_remove(SH &sh, const Ev &ev, int t, int p) {
-  const int pos = ev.rec[p].meta.w;
-  const int last = sh.seg_start[t] + sh.seg_count[t] - 1;
-  if (pos != last) {
-    const int q = ev.pool[last];
-    ev.pool[pos] = q;
-    ev.rec[q].meta.w = pos;
-  }
-  sh.seg_count[t] -= 1;
 }
 
 /* Appends physical p to tile t's segment; relocates the segment to the end of
@@ -460,41 +446,62 @@ using namespace tiled;
 
 /* Serving: claims the next published event descriptor (lane 0 only).  Waves
  * claim through a device-memory head and a device-memory mirror of the host's
- * published count; one wave at a time (the poll lock) reads the host words over
- * PCIe and refreshes the mirrors.  Returns the descriptor index, or -1 when the
- * host asked to stop or when no wave has claimed anything for idle_ticks: the
- * idle clock is the whole launch's (the last claim by any wave), so waves leave
- * together once the server is idle, never one by one under a trickle of work,
- * and a launch always ends once work stops coming. */
+ * published count.  At most one wave per poll interval reads the host words over
+ * PCIe (the poll lock, tested before it is taken) and refreshes the mirrors, and
+ * idle waves back off to ~55 us between looks, so the idle waves' traffic on the
+ * control words (each in its own 128-byte line) stays far below what would queue
+ * behind it the busy waves' memory accesses.  Returns the descriptor index, or
+ * -1 when the launch ends: the host asked to stop, or no wave has claimed
+ * anything for idle_ticks while no event is being clustered.  The first wave to
+ * see that latches the launch's epoch into the exit word, after which no wave of
+ * the launch claims again, so the launch ends as a whole and the host's next
+ * submit or poll relaunches it with every workgroup; a launch never shrinks to
+ * the few waves that happened to be busy. */
 __device__ __forceinline__ long long serve_claim(const ServeArgs &s) {
-  unsigned long long *head = s.dev, *tail = s.dev + 1, *lock = s.dev + 2, *stop = s.dev + 3,
-                     *last_claim = s.dev + 5;
+  unsigned long long *head = s.dev + kServeHead, *tail = s.dev + kServeTail, *lock = s.dev + kServeLock,
+                     *stop = s.dev + kServeStop, *last_claim = s.dev + kServeLastClaim,
+                     *last_poll = s.dev + kServeLastPoll, *exit_epoch = s.dev + kServeExitEpoch;
+  constexpr unsigned long long kPollTicks = 100; /* 1 us of the 100 MHz wall clock */
   const unsigned long long t0 = wall_clock64();
   int nap = 1;
   for (;;) {
+    if (__hip_atomic_load(exit_epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == s.epoch) return -1;
     const unsigned long long h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned long long t = __hip_atomic_load(tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     if (h < t) {
       if (atomicCAS(head, h, h + 1) == h) {
         __hip_atomic_fetch_max(last_claim, (unsigned long long)wall_clock64(), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long busy = atomicAdd(s.dev + kServeBusy, 1ull) + 1;
+        __hip_atomic_fetch_max(s.dev + kServePeakBusy, busy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return (long long)h;
       }
       continue;
     }
     if (__hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return -1;
-    if (atomicCAS(lock, 0ull, 1ull) == 0ull) {
+    const unsigned long long now = wall_clock64();
+    if (now - __hip_atomic_load(last_poll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > kPollTicks &&
+        __hip_atomic_load(lock, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0ull &&
+        atomicCAS(lock, 0ull, 1ull) == 0ull) {
+      __hip_atomic_store(last_poll, now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const unsigned long long ht = __hip_atomic_load(&s.host->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
       const unsigned int st = __hip_atomic_load(&s.host->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       __hip_atomic_fetch_max(tail, ht, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       if (st) __hip_atomic_store(stop, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(lock, 0ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      if (ht > h || st) continue;
+      if (ht > h || st) {
+        nap = 1;
+        continue;
+      }
     }
     const unsigned long long lc = __hip_atomic_load(last_claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (wall_clock64() - (lc > t0 ? lc : t0) > s.idle_ticks) return -1;
+    if (now - (lc > t0 ? lc : t0) > s.idle_ticks &&
+        __hip_atomic_load(s.dev + kServeBusy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0ull) {
+      __hip_atomic_store(exit_epoch, s.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return -1;
+    }
     for (int k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(64);
-    nap = nap < 8 ? 2 * nap : 8;
+    nap = nap < 32 ? 2 * nap : 32;
   }
 }
 
@@ -504,8 +511,7 @@ __device__ __forceinline__ long long serve_claim(const ServeArgs &s) {
  * first).  njets = -1 marks an event whose clustering failed. */
 __device__ __forceinline__ void serve_finish(const LaunchArgs &a, int slot, int64_t base, int N, int nj, int lane) {
   const ServeArgs &s = a.serve;
-  __threadfence_block();
-  __syncthreads();
+  wave_sync();
   for (int i = lane; i < N; i += W) {
     s.res_p1[base + i] = a.hist_p1[base + i];
     s.res_p2[base + i] = a.hist_p2[base + i];
@@ -515,9 +521,10 @@ __device__ __forceinline__ void serve_finish(const LaunchArgs &a, int slot, int6
   for (int i = lane; i < nd; i += W) s.res_jets[4 * base + i] = a.jets[4 * base + i];
   if (lane == 0) s.res_nj[slot] = nj;
   __threadfence_system();
-  __syncthreads();
+  wave_sync();
   if (lane == 0) {
-    const unsigned long long c = atomicAdd(s.dev + 4, 1ull);
+    atomicSub(s.dev + kServeBusy, 1ull);
+    const unsigned long long c = atomicAdd(s.dev + kServeCompleted, 1ull);
     __hip_atomic_store(s.comp + (unsigned)(c % s.ring_cap), slot + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
@@ -533,7 +540,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
   for (int t = lane; t < MAXT; t += W) sh.stamp[t] = 0;
   if (lane == 0) sh.stampv = 0;
   for (;;) {
-    __syncthreads();
+    wave_sync();
     if (lane == 0) {
       if constexpr (SERVE) {
         const long long h = serve_claim(a.serve);
@@ -551,7 +558,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         sh.ev = (int)(atomicAdd(a.head, 1u) - a.head_base);
       }
     }
-    __syncthreads();
+    wave_sync();
     const int qi = uni(sh.ev);
     if (SERVE ? qi < 0 : qi >= a.n_queue) {
       if (a.t_end && threadIdx.x == 0) atomicMax(a.t_end, (unsigned long long)wall_clock64());
@@ -632,7 +639,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         sh.sphi = FJMI_TWOPI / np;
       }
     }
-    __syncthreads();
+    wave_sync();
     Geo geo;
     geo.ny = uni(sh.ny);
     geo.nphi = uni(sh.nphi);
@@ -644,20 +651,23 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       const double2 rp = ev.rec[s].rp;
       ev.rec[s].meta = make_int4(-1, s, geo.tile(rp.x, rp.y), 0);
     }
-    __threadfence_block();
-    __syncthreads();
+    wave_sync();
     rebuild_segments(sh, ev, N, ntiles, lane);
 
     /* ---- initial nearest neighbours: each lane scans the 3x3 tiles of its jets ---- */
     for (int s = lane; s < N; s += W) {
       const double2 rp = ev.rec[s].rp;
       const double ws = ev.rec[s].wn.x;
-      int nb[9];
-      const int m = geo.neigh(ev.rec[s].meta.z, nb);
+      const int ts0 = ev.rec[s].meta.z;
+      const int ty0 = ts0 / geo.nphi, tp0 = ts0 % geo.nphi;
       double bd = R2, bw = 0.0;
       int bl = BIG, bp = -1;
-      for (int i = 0; i < m; ++i) {
-        const int t = nb[i];
+      /* the distinct tiles of the 3x3 neighbourhood (with fewer than 3 phi
+       * columns the wrapped columns coincide and are visited once) */
+      for (int i = 0; i < 9; ++i) {
+        const int y = ty0 + i / 3 - 1, dp = i % 3 - 1;
+        if (y < 0 || y >= geo.ny || (geo.nphi == 1 && dp != 0) || (geo.nphi == 2 && dp < 0)) continue;
+        const int t = y * geo.nphi + (tp0 + dp + geo.nphi) % geo.nphi;
         const int st = sh.seg_start[t], ct = sh.seg_count[t];
         for (int u = 0; u < ct; ++u) {
           const int q = ev.pool[st + u];
@@ -676,8 +686,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       ev.rec[s].wn = make_double2(ws, bd);
       ev.rec[s].dij = diJ_of(bd, ws, bp, bw);
     }
-    __threadfence_block();
-    __syncthreads();
+    wave_sync();
     /* tile minima of every tile */
     Gather &G = sh.g[0];
     Gather &GO = sh.g[1];
@@ -692,7 +701,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           G.pref[u + 1] = acc;
         }
       }
-      __syncthreads();
+      wave_sync();
       refresh_minima(sh, G, ev, cnt, lane);
     }
 
@@ -701,21 +710,39 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
     bool bad = false;
     for (int step = 0; step < N; ++step) {
       const int n_old = N - step;
-      /* 1. argmin over the tile minima */
-      double bd = __builtin_inf();
-      int bl = BIG, bt = -1;
+      /* the jet with the last label moves in this step (N2Plain compaction); its
+       * index is loaded first so the load overlaps the argmin */
+      const int tail = n_old - 1;
+      const int pt_raw = ev.lab2[tail];
+      /* 1. argmin over the tile minima: every lane loads all its MAXT/W entries
+       * at once (always in bounds; entries past ntiles are masked), then a
+       * pairwise tree of lexicographic (diJ, label) compares */
+      constexpr int KT = MAXT / W;
+      double kd[KT];
+      int kl[KT], kt[KT];
+      uint4 raw[KT];
 #pragma unroll
-      for (int k = 0; k < MAXT / W; ++k) {
-        const int t = k * W + lane;
-        if (k * W < ntiles) { /* uniform */
-          const TMin m = sh.tmin[t < ntiles ? t : 0];
-          const double d = __longlong_as_double((long long)m.d);
-          const bool c = (t < ntiles) & (m.l != BIG) & ((d < bd) | ((d == bd) & (m.l < bl)));
-          bd = c ? d : bd;
-          bl = c ? m.l : bl;
-          bt = c ? t : bt;
+      for (int k = 0; k < KT; ++k) raw[k] = *reinterpret_cast<const uint4 *>(&sh.tmin[k * W + lane]);
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        const int l = (int)raw[k].z;
+        const bool ok = (k * W + lane < ntiles) & (l != BIG);
+        kd[k] = ok ? __hiloint2double((int)raw[k].y, (int)raw[k].x) : __builtin_inf();
+        kl[k] = ok ? l : BIG;
+        kt[k] = k * W + lane;
+      }
+#pragma unroll
+      for (int w = 1; w < KT; w *= 2) {
+#pragma unroll
+        for (int k = 0; k + w < KT; k += 2 * w) {
+          const bool c = (kd[k + w] < kd[k]) | ((kd[k + w] == kd[k]) & (kl[k + w] < kl[k]));
+          kd[k] = c ? kd[k + w] : kd[k];
+          kl[k] = c ? kl[k + w] : kl[k];
+          kt[k] = c ? kt[k + w] : kt[k];
         }
       }
+      const double bd = kd[0];
+      const int bl = kl[0], bt = kt[0];
       double dmin;
       int lA;
       const int wl = wave_argmin(bd, bl, &dmin, &lA);
@@ -726,45 +753,55 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         bad = true;
         break;
       }
-      /* 2. the merge */
+      /* 2. the merge: the loads that depend only on A, then only on B, are
+       * issued together */
+      const int pt = uni(pt_raw);
+      const int tile_pt_raw = ev.rec[pt].meta.z;
       const int4 mA = ev.rec[A].meta;
+      const double4 qA = ev.p4[A];
+      const int hA = ev.hist[A];
       const int B = uni(mA.x);
-      int phys_hi, phys_lo, lab_hi;
       int4 mB = make_int4(-1, -1, -1, -1);
+      double4 qB = make_double4(0.0, 0.0, 0.0, 0.0);
+      int hB = -1;
       if (B >= 0) {
         mB = ev.rec[B].meta;
-        const bool a_hi = mA.y > mB.y;
-        phys_hi = a_hi ? A : B;
-        phys_lo = a_hi ? B : A;
-        lab_hi = uni(a_hi ? mA.y : mB.y);
-      } else {
-        phys_hi = A;
-        phys_lo = -1;
-        lab_hi = uni(mA.y);
+        qB = ev.p4[B];
+        hB = ev.hist[B];
       }
-      const int tail = n_old - 1;
-      const int pt = uni(ev.lab2[tail]);
-      const int tile_hi = uni(phys_hi == A ? mA.z : mB.z);
-      const int tile_lo_old = phys_lo < 0 ? -1 : uni(phys_lo == A ? mA.z : mB.z);
-      const int tile_pt = uni(ev.rec[pt].meta.z);
-      /* the removed jet leaves its tile and the last label moves (N2Plain
-       * compaction) before the new jet's kinematics, so their memory traffic
-       * overlaps the arithmetic */
-      if (lane == 0) {
-        seg_remove(sh, ev, tile_hi, phys_hi);
-        ev.rec[phys_hi].meta = make_int4(-1, -1, -1, -1);
-        if (pt != phys_hi) {
-          ev.rec[pt].meta.y = lab_hi;
-          ev.lab2[lab_hi] = pt;
-        }
+      const bool a_hi = B < 0 || uni(mA.y) > uni(mB.y);
+      const int phys_hi = a_hi ? A : B;
+      const int phys_lo = B < 0 ? -1 : (a_hi ? B : A);
+      const int lab_hi = uni(a_hi ? mA.y : mB.y);
+      const int tile_hi = uni(a_hi ? mA.z : mB.z);
+      const int tile_lo_old = phys_lo < 0 ? -1 : uni(a_hi ? mB.z : mA.z);
+      const int tile_pt = uni(tile_pt_raw);
+      /* Segment removals swap the removed jet with its segment's last entry.  The
+       * pool entries that may be needed are loaded now, by every lane, so that
+       * their latency overlaps the new jet's kinematics:
+       *   removal 1: phys_hi leaves tile_hi (last entry at last_hi);
+       *   removal 2: phys_lo leaves tile_lo_old if the merged jet changes tile;
+       *   when both are in one tile, removal 2 sees the segment as removal 1
+       *   left it (one shorter, phys_lo possibly moved into phys_hi's place). */
+      const int pos_hi = uni(a_hi ? mA.w : mB.w);
+      const int pos_lo = phys_lo < 0 ? -1 : uni(a_hi ? mB.w : mA.w);
+      const int cnt_hi = uni(sh.seg_count[tile_hi]);
+      const int last_hi = uni(sh.seg_start[tile_hi]) + cnt_hi - 1;
+      const int q_last_hi_raw = ev.pool[last_hi];
+      const bool same_tile = phys_lo >= 0 && tile_lo_old == tile_hi;
+      int last2 = -1, q_last2_raw = -1;
+      if (phys_lo >= 0) {
+        last2 = uni(sh.seg_start[tile_lo_old]) + uni(sh.seg_count[tile_lo_old]) - 1 - (same_tile ? 1 : 0);
+        q_last2_raw = ev.pool[last2];
       }
+
       double nrap = 0.0, nphi = 0.0, nw = 0.0;
       int tile_new = -1;
-      const double4 qhi = ev.p4[phys_hi];
-      const int hhi = ev.hist[phys_hi];
+      const double4 qhi = a_hi ? qA : qB;
+      const int hhi = a_hi ? hA : hB;
       if (phys_lo >= 0) {
-        const double4 qlo = ev.p4[phys_lo];
-        const int hlo = ev.hist[phys_lo];
+        const double4 qlo = a_hi ? qB : qA;
+        const int hlo = a_hi ? hB : hA;
         const double4 qn = make_double4(qhi.x + qlo.x, qhi.y + qlo.y, qhi.z + qlo.z, qhi.w + qlo.w);
         const double4 c = kin_coords_t(qn.x, qn.y, qn.z, qn.w, p);
         nrap = c.x;
@@ -780,11 +817,6 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           ev.rec[phys_lo].rp = make_double2(nrap, nphi);
           ev.rec[phys_lo].wn = make_double2(nw, R2);
           ev.rec[phys_lo].meta.x = -1;
-          if (tile_new != tile_lo_old) {
-            seg_remove(sh, ev, tile_lo_old, phys_lo);
-            ev.rec[phys_lo].meta.z = tile_new;
-            seg_insert(sh, ev, tile_new, phys_lo);
-          }
         }
       } else if (lane == 0) {
         a.hist_p1[base + step] = hhi;
@@ -795,6 +827,34 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         jo[1] = qhi.y;
         jo[2] = qhi.z;
         jo[3] = qhi.w;
+      }
+      /* removal 1 and the N2Plain compaction (the last label moves), then, if
+       * the merged jet changes tile, removal 2 (from the segment as removal 1
+       * left it) and its insertion */
+      const int q_last_hi = uni(q_last_hi_raw);
+      if (lane == 0) {
+        if (pos_hi != last_hi) {
+          ev.pool[pos_hi] = q_last_hi;
+          ev.rec[q_last_hi].meta.w = pos_hi;
+        }
+        sh.seg_count[tile_hi] = cnt_hi - 1;
+        ev.rec[phys_hi].meta = make_int4(-1, -1, -1, -1);
+        if (pt != phys_hi) {
+          ev.rec[pt].meta.y = lab_hi;
+          ev.lab2[lab_hi] = pt;
+        }
+        if (phys_lo >= 0 && tile_new != tile_lo_old) {
+          const bool moved = same_tile && pos_lo == last_hi && pos_hi != last_hi;
+          const int pos2 = moved ? pos_hi : pos_lo;
+          const int q_last2 = (same_tile && last2 == pos_hi) ? q_last_hi : q_last2_raw;
+          if (pos2 != last2) {
+            ev.pool[pos2] = q_last2;
+            ev.rec[q_last2].meta.w = pos2;
+          }
+          sh.seg_count[tile_lo_old] -= 1;
+          ev.rec[phys_lo].meta.z = tile_new;
+          seg_insert(sh, ev, tile_new, phys_lo);
+        }
       }
       if (lane == 0) {
         if (a.pj_out) {
@@ -811,8 +871,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         }
       }
       if (phys_lo < 0) ++nj;
-      __threadfence_block();
-      __syncthreads();
+      wave_sync();
       const int n = n_old - 1;
       if (n == 0) break;
       if (sh.rebuild) { /* pool exhausted by a relocation: lay all segments out again */
@@ -822,24 +881,35 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       /* 3. gather the 3x3 tiles around the old and new positions and the moved jet's tile */
       const int ng = build_gather(sh, G, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane);
       const int total = G.pref[ng];
+      const int main_stamp = uni(sh.stampv);
       if (lane == 0) sh.norph = 0;
-      __syncthreads();
+      wave_sync();
       double cbd = R2, cbw = 0.0;
       int cbl = BIG, cbp = -1;
       /* a gather of at most 64 members keeps each member's (index, label, diJ)
        * in its lane for the tile-minimum refresh */
       const bool single = total <= W;
-      int myq = -1, mylab = BIG;
+      int myq = -1, mylab = BIG, mytile = -1;
       unsigned long long mybits = ~0ull;
+      double2 myrp = make_double2(0.0, 0.0);
+      double myw = 0.0;
+      unsigned long long orph_lanes = 0; /* single-chunk gathers: lanes holding orphans */
       for (int j0 = 0; j0 < total; j0 += W) {
         const int q = gathered(sh, G, ev, ng, j0 + lane);
         bool orph = false;
+        if (q >= 0) { /* every member's record, kept in its lane for the orphan rescans */
+          const JetRec &r = ev.rec[q];
+          const int4 mq0 = r.meta;
+          myq = q;
+          mylab = mq0.y;
+          mytile = mq0.z;
+          myrp = r.rp;
+          myw = r.wn.x;
+        }
         if (q >= 0 && q != phys_lo) {
           int4 mq = ev.rec[q].meta;
           double dq = ev.rec[q].dij;
           orph = (mq.x == phys_hi) || (phys_lo >= 0 && mq.x == phys_lo);
-          myq = q;
-          mylab = mq.y;
           if (phys_lo >= 0) {
             const double2 rq = ev.rec[q].rp;
             const double2 wq = ev.rec[q].wn;
@@ -858,11 +928,9 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
             }
           }
           mybits = (unsigned long long)__double_as_longlong(dq);
-        } else if (q >= 0) { /* the new jet itself: filled in below */
-          myq = q;
-          mylab = ev.rec[q].meta.y;
         }
         const unsigned long long ob = __ballot(orph);
+        orph_lanes = ob;
         if (ob) {
           int obase = 0;
           if (lane == 0) obase = atomicAdd(&sh.norph, (int)__popcll(ob));
@@ -886,12 +954,56 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         }
         if (myq == phys_lo) mybits = (unsigned long long)__double_as_longlong(dB);
       }
-      __threadfence_block();
-      __syncthreads();
+      wave_sync();
 
-      /* 4. orphan rescans over their own 3x3 tiles */
+      /* 4. orphan rescans over their own 3x3 tiles.  With a single-chunk gather
+       * whose tiles hold the orphan's whole 3x3 neighbourhood, the candidates
+       * are already in the lanes' registers: one distance per lane and one
+       * wave argmin (gathered jets outside the neighbourhood are farther than
+       * R and drop out).  Otherwise the neighbourhood is gathered afresh. */
       const int norph = uni(sh.norph);
-      if (norph > 0) {
+      if (norph > 0 && single) {
+        unsigned long long pend = orph_lanes;
+        while (pend) {
+          const int lo = __ffsll(pend) - 1;
+          pend &= pend - 1;
+          const int o = __builtin_amdgcn_readlane(myq, lo);
+          const int otile = __builtin_amdgcn_readlane(mytile, lo);
+          const double oy = readlane_d(myrp.x, lo), ophi = readlane_d(myrp.y, lo), ow = readlane_d(myw, lo);
+          bool outside = false;
+          if (lane < 9) {
+            const int y = otile / geo.nphi + lane / 3 - 1;
+            if (y >= 0 && y < geo.ny) {
+              const int t = y * geo.nphi + (otile % geo.nphi + lane % 3 - 1 + geo.nphi) % geo.nphi;
+              outside = sh.stamp[t] != main_stamp;
+            }
+          }
+          double od, owb;
+          int nno;
+          if (__ballot(outside) == 0) {
+            const bool cand = myq >= 0 && myq != o;
+            const double d = cand ? fjmi_dr2(oy, ophi, myrp.x, myrp.y) : R2;
+            const bool in = cand && d < R2;
+            double md;
+            int ml;
+            const int ln = wave_argmin(in ? d : R2, in ? mylab : BIG, &md, &ml);
+            od = md;
+            nno = ln < 0 ? -1 : __builtin_amdgcn_readlane(myq, ln);
+            owb = ln < 0 ? 0.0 : readlane_d(myw, ln);
+          } else {
+            const int ngo = build_gather(sh, GO, geo, otile, -1, -1, -1, lane);
+            nno = nn_scan(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &od, &owb);
+          }
+          const double dO = diJ_of(od, ow, nno, owb);
+          if (lane == 0) {
+            ev.rec[o].meta.x = nno;
+            ev.rec[o].wn = make_double2(ow, od);
+            ev.rec[o].dij = dO;
+          }
+          if (lane == lo) mybits = (unsigned long long)__double_as_longlong(dO);
+        }
+        wave_sync();
+      } else if (norph > 0) {
         const int nlist = min(norph, MAXO);
         for (int i = 0; i < nlist; ++i) {
           const int o = uni(sh.orph[i]);
@@ -908,8 +1020,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
             ev.rec[o].dij = dO;
           }
           if (myq == o) mybits = (unsigned long long)__double_as_longlong(dO);
-          __threadfence_block();
-          __syncthreads();
+          wave_sync();
         }
         if (norph > MAXO) { /* more orphans than the list holds: rescan the rest from the gather */
           const int ng2 = ng;
@@ -934,8 +1045,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
                 ev.rec[o].wn = make_double2(wo, od);
                 ev.rec[o].dij = diJ_of(od, wo, nno, owb);
               }
-              __threadfence_block();
-              __syncthreads();
+              wave_sync();
             }
           }
         }

@@ -157,7 +157,8 @@ int fjmi_cluster_batch(fjmi_context *ctx, int n_events, const int64_t *offsets, 
  * any size flow through without batch boundaries.  An event is admitted only
  * if its state fits the budget; otherwise submit() returns FJMI_EAGAIN and the
  * caller keeps it queued.  The GPU workgroups leave together once no event has
- * been claimed for idle_us, and are relaunched by the next submit() or poll().
+ * been claimed for idle_us and none is being clustered, and are relaunched by
+ * the next submit() or poll().
  * A server is driven by one host thread; distinct servers are independent. */
 typedef struct fjmi_server fjmi_server;
 
@@ -194,6 +195,7 @@ typedef struct {
   int64_t in_flight;         /* admitted and not yet released */
   int64_t particles_in_flight, peak_particles_in_flight;
   int64_t launches;          /* serving kernel launches so far */
+  int64_t gpu_peak_busy;     /* the most events the GPU has clustered at once */
 } fjmi_server_info;
 
 /* cfg may be NULL (all defaults). */
