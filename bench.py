@@ -34,6 +34,19 @@ METRIC = "events/s batched vs per-launch at N~U[50,2000], anti-kt R=0.4; p99 lat
 FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix)
 
 
+def measured_traffic(kernel_tag):
+    """HBM bytes per launch of the dominant kernel on this workload, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", "tiled_kernel_traffic_workload3.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:") and kernel_tag in line:
+                return int(line.split()[1])
+    except OSError:
+        pass
+    return None
+
+
 def flops(sum_n2, sum_n):
     """Algorithmic FP64 operations of the dense nearest-neighbour method:
     8 per pair evaluation for N(N-1)/2 initial pairs plus N(N-1)/2 merged-row
@@ -157,6 +170,9 @@ def main():
     dom_flops = flops(dom["sum_n2"], dom["particles"])
     achieved = dom_flops / (dom["ms"] * 1e-3) / 1e12 if dom["ms"] > 0 else None
     all_flops = flops(float(np.sum(sizes.astype(np.float64) ** 2)), float(tot))
+    variants = {0: "tiled_kernel<512, 2, false>", 1: "tiled_kernel<256, 3, false>", 2: "tiled_kernel<256, 4, false>",
+                3: "tiled_kernel<128, 4, false>"}
+    traffic = measured_traffic(variants.get(dom["p4_global"], "?")) if dom["engine"] == 2 and ne == 4096 else None
 
     # ---- CPU baseline: the oracle on the host cores, bounded sample ----
     cpu = None
@@ -203,9 +219,9 @@ def main():
                        "batch_p50": float(np.percentile(step_ms, 50)),
                        "batch_p99": float(np.percentile(step_ms, 99))},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": None,
-                     "kernel": (f"dense_kernel<{dom['threads']},{dom['slots']},{bool(dom['p4_global'])}>"
-                                if dom["engine"] == 1 else "tiled_kernel")
+                     "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": traffic,
+                     "kernel": (f"dense_kernel<{dom['threads']}, {dom['slots']}, {str(bool(dom['p4_global'])).lower()}, 1>"
+                                if dom["engine"] == 1 else variants.get(dom["p4_global"], "tiled_kernel"))
                                + f" (bucket N<={min(dom['max_n'], int(sizes.max()))}, {dom['n_events']} events, "
                                f"{dom['ms']:.2f} ms)",
                      "fp64_ops_per_launch": dom_flops, "batch_fp64_ops": all_flops,

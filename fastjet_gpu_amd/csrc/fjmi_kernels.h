@@ -24,6 +24,9 @@ extern const void *const kZeroKernel;  /* __global__ void(unsigned *, int, unsig
 /* tiled engine, __global__ void(LaunchArgs, TiledScratch), 64 threads, by tile
  * table size (index = a tiled bucket row's p4_global field) */
 constexpr int kNumTiledKernels = 4;
+/* largest event of the tiled engine: labels and particle indices share one
+ * 32-bit word (16 bits each) in its tile minima */
+constexpr int kTiledMaxN = 65535;
 extern const void *const kTiledKernels[kNumTiledKernels];
 extern const void *const kTiledServeKernels[kNumTiledKernels]; /* same, serving mode (LaunchArgs::serve) */
 extern const int kTiledMaxTiles[kNumTiledKernels];

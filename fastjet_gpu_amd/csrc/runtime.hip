@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -183,6 +184,8 @@ int cluster_one_device(fjmi_context *ctx, int engine, const double *d_in, int in
   fjmi_bucket forced;
   if (!b && engine == FJMI_ENGINE_DENSE && dense_row_for(n, &forced)) b = &forced;
   if (!b) return fail(FJMI_ECAPACITY, "event with %d particles exceeds the largest bucket of engine %d", n, engine);
+  if (b->engine == FJMI_ENGINE_TILED && n > fjmi::kTiledMaxN)
+    return fail(FJMI_ECAPACITY, "event with %d particles: the tiled engine takes at most %d", n, fjmi::kTiledMaxN);
   int rc;
   if ((rc = ctx->ctrl.ensure(sizeof(Ctrl)))) return rc;
   const bool tiled = b->engine == FJMI_ENGINE_TILED;
@@ -454,6 +457,9 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
     int b = 0;
     while (b < nb && n > ctx->buckets[b].max_n) ++b;
     if (b == nb) return fail(FJMI_ECAPACITY, "event %d has %lld particles, above the largest bucket", e, (long long)n);
+    if (ctx->buckets[b].engine == FJMI_ENGINE_TILED && n > fjmi::kTiledMaxN)
+      return fail(FJMI_ECAPACITY, "event %d has %lld particles: the tiled engine takes at most %d", e, (long long)n,
+                  fjmi::kTiledMaxN);
     lists[b].push_back(e);
     if (ctx->buckets[b].engine == FJMI_ENGINE_TILED)
       any_tiled = true;
@@ -509,6 +515,12 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, r.fn, r.threads, r.lds);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    /* FJMI_MAX_WG_PER_CU caps the persistent workgroups per CU (fewer events in
+     * flight: a smaller working set for the caches) */
+    if (const char *cap = std::getenv("FJMI_MAX_WG_PER_CU")) {
+      const int c = std::atoi(cap);
+      if (c >= 1 && c < per_cu) per_cu = c;
+    }
     r.n_events = (int)lists[b].size();
     r.max_n = bk.max_n;
     r.tile_q = r.tiled ? bk.slots : 0;

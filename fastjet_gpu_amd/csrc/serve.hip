@@ -322,6 +322,8 @@ int fjmi_server_submit(fjmi_server *s, const double *p4, int n, int p, double R,
   if (p < -1 || p > 1) return fail(FJMI_EINVAL, "exponent p must be -1, 0 or 1");
   if (!(R > 0.0) || !(R < 1e150)) return fail(FJMI_EINVAL, "radius must be positive and finite");
   if (n > s->P) return fail(FJMI_ECAPACITY, "event of %d particles exceeds the server's %lld", n, (long long)s->P);
+  if (n > fjmi::kTiledMaxN)
+    return fail(FJMI_ECAPACITY, "event of %d particles: the tiled engine takes at most %d", n, fjmi::kTiledMaxN);
   if (s->free_slots.empty()) return fail(FJMI_EAGAIN, "all %u event slots in use", s->E);
   const int64_t base = s->parts.alloc(alloc_len(n));
   if (base < 0) return fail(FJMI_EAGAIN, "particle arena full");

@@ -47,13 +47,22 @@ constexpr int MAXG = 32;    /* tiles gathered per step (3 x 3x3 + 1 <= 28) */
 constexpr int MAXO = 128;   /* orphans listed per step (more: rescan in rounds) */
 constexpr int BIG = 0x7fffffff;
 
-/* (diJ, label) minimum of a tile: diJ as its bit pattern (diJ >= 0, so the bits
- * order like the values), the label, and the jet's physical index. */
+/* The (diJ, label) minimum of a tile: diJ as its bit pattern (diJ >= 0, so the
+ * bits order like the values), label << 16 | particle index of the minimum jet
+ * (events hold at most kTiledMaxN particles), and that jet's nearest neighbour,
+ * so that the merge loads both jets' records at once.  Empty tile: d = ~0,
+ * lp = LP_NONE, nn = -1. */
 struct TMin {
   unsigned long long d;
-  int l;
-  int p;
+  unsigned int lp;
+  int nn;
 };
+constexpr unsigned int LP_NONE = 0xFFFFFFFFu;
+/* lp as a signed key that orders like lp (LP_NONE maps to BIG) */
+__device__ __forceinline__ int lp_key(unsigned int lp) { return (int)(lp ^ 0x80000000u); }
+__device__ __forceinline__ unsigned int make_lp(int label, int phys) {
+  return ((unsigned int)label << 16) | (unsigned int)phys;
+}
 
 /* A list of gathered tiles and the prefix sums of their member counts. */
 struct Gather {
@@ -67,8 +76,8 @@ template <int MAXT>
 struct Shared {
   TMin tmin[MAXT];
   unsigned long long mt_d[W]; /* member table of a single-chunk gather: diJ bits, */
-  int mt_l[W];                /* labels */
-  int mt_p[W];                /* and physical indices, by gather index */
+  unsigned int mt_lp[W];      /* label << 16 | index */
+  int mt_nn[W];               /* and nearest neighbours, by gather index */
   int seg_start[MAXT], seg_count[MAXT], seg_cap[MAXT];
   Gather g[2]; /* [0]: the step's gather, [1]: orphan rescans */
   int orph[MAXO];
@@ -265,28 +274,30 @@ __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, in
   if (lane < ng) {
     const int t = G.tiles[lane];
     sh.tmin[t].d = ~0ull;
-    sh.tmin[t].l = BIG;
-    sh.tmin[t].p = -1;
+    sh.tmin[t].lp = LP_NONE;
+    sh.tmin[t].nn = -1;
   }
   wave_sync();
   const int total = G.pref[ng];
-  /* three passes over all members: the diJ minimum, then the lowest label among
-   * members at that minimum, then its index (each pass must see the final result
-   * of the previous one across every chunk of 64 members) */
+  /* three passes over all members: the diJ minimum, then the lowest label (and
+   * so index) among members at that minimum, then that jet's nearest neighbour
+   * (each pass must see the final result of the previous one across every
+   * chunk of 64 members) */
   for (int phase = 0; phase < 3; ++phase) {
     for (int j0 = 0; j0 < total; j0 += W) {
       const int p = gathered(sh, G, ev, ng, j0 + lane);
       if (p >= 0) {
-        const int t = ev.rec[p].meta.z;
+        const int4 m = ev.rec[p].meta;
+        const int t = m.z;
         const unsigned long long bits = (unsigned long long)__double_as_longlong(ev.rec[p].dij);
         if (phase == 0) {
           atomicMin(&sh.tmin[t].d, bits);
         } else if (sh.tmin[t].d == bits) {
-          const int lab = ev.rec[p].meta.y;
+          const unsigned int lp = make_lp(m.y, p);
           if (phase == 1)
-            atomicMin(&sh.tmin[t].l, lab);
-          else if (sh.tmin[t].l == lab)
-            sh.tmin[t].p = p;
+            atomicMin(&sh.tmin[t].lp, lp);
+          else if (sh.tmin[t].lp == lp)
+            sh.tmin[t].nn = m.x;
         }
       }
     }
@@ -298,11 +309,11 @@ __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, in
  * 64 members): lane j < total wrote member j; lane g < ng scans tile g's range. */
 template <class SH>
 __device__ void refresh_from_table(SH &sh, const Gather &G, int ng, int total, int lane, int myq, int mylab,
-                                   unsigned long long mybits) {
+                                   int mynn, unsigned long long mybits) {
   if (lane < total) {
     sh.mt_d[lane] = mybits;
-    sh.mt_l[lane] = mylab;
-    sh.mt_p[lane] = myq;
+    sh.mt_lp[lane] = make_lp(mylab, myq);
+    sh.mt_nn[lane] = mynn;
   }
   wave_sync();
   if (lane < ng) {
@@ -310,15 +321,15 @@ __device__ void refresh_from_table(SH &sh, const Gather &G, int ng, int total, i
     const int j1 = G.pref[lane + 1];
     TMin best;
     best.d = ~0ull;
-    best.l = BIG;
-    best.p = -1;
+    best.lp = LP_NONE;
+    best.nn = -1;
     for (int j = G.pref[lane]; j < j1; ++j) {
       const unsigned long long d = sh.mt_d[j];
-      const int l = sh.mt_l[j];
-      if (d < best.d || (d == best.d && l < best.l)) {
+      const unsigned int lp = sh.mt_lp[j];
+      if (d < best.d || (d == best.d && lp < best.lp)) {
         best.d = d;
-        best.l = l;
-        best.p = sh.mt_p[j];
+        best.lp = lp;
+        best.nn = sh.mt_nn[j];
       }
     }
     sh.tmin[t] = best;
@@ -582,6 +593,15 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       R = a.R_arr ? a.R_arr[e] : a.R_uni;
       pool_off = 3 * base + (int64_t)e * ts.pool_per_event;
     }
+    if (N > kTiledMaxN) { /* the host rejects such events; never clustered here */
+      if (lane == 0) atomicOr(a.status, (unsigned)STATUS_CAPACITY);
+      if constexpr (SERVE) {
+        serve_finish(a, e, base, 0, -1, lane);
+      } else if (lane == 0) {
+        a.njets[e] = 0;
+      }
+      continue;
+    }
     if (N <= 0) {
       if constexpr (SERVE) {
         serve_finish(a, e, base, 0, 0, lane);
@@ -719,17 +739,18 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
        * pairwise tree of lexicographic (diJ, label) compares */
       constexpr int KT = MAXT / W;
       double kd[KT];
-      int kl[KT], kt[KT];
+      int kl[KT], kt[KT], kn[KT];
       uint4 raw[KT];
 #pragma unroll
       for (int k = 0; k < KT; ++k) raw[k] = *reinterpret_cast<const uint4 *>(&sh.tmin[k * W + lane]);
 #pragma unroll
       for (int k = 0; k < KT; ++k) {
-        const int l = (int)raw[k].z;
+        const int l = lp_key(raw[k].z);
         const bool ok = (k * W + lane < ntiles) & (l != BIG);
         kd[k] = ok ? __hiloint2double((int)raw[k].y, (int)raw[k].x) : __builtin_inf();
         kl[k] = ok ? l : BIG;
         kt[k] = k * W + lane;
+        kn[k] = (int)raw[k].w;
       }
 #pragma unroll
       for (int w = 1; w < KT; w *= 2) {
@@ -739,15 +760,17 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           kd[k] = c ? kd[k + w] : kd[k];
           kl[k] = c ? kl[k + w] : kl[k];
           kt[k] = c ? kt[k + w] : kt[k];
+          kn[k] = c ? kn[k + w] : kn[k];
         }
       }
       const double bd = kd[0];
-      const int bl = kl[0], bt = kt[0];
+      const int bl = kl[0], bt = kt[0], bn = kn[0];
       double dmin;
       int lA;
       const int wl = wave_argmin(bd, bl, &dmin, &lA);
       const int tA = wl < 0 ? -1 : uni(__builtin_amdgcn_readlane(bt, wl));
-      const int A = tA < 0 ? -1 : uni(sh.tmin[tA].p);
+      const int A = tA < 0 ? -1 : (int)(((unsigned)lA ^ 0x80000000u) & 0xFFFFu);
+      const int B = tA < 0 ? -1 : uni(__builtin_amdgcn_readlane(bn, wl));
       if (A < 0) { /* no live jet found: inconsistent state, stop this event */
         if (lane == 0) atomicOr(a.status, 2u);
         bad = true;
@@ -760,7 +783,6 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       const int4 mA = ev.rec[A].meta;
       const double4 qA = ev.p4[A];
       const int hA = ev.hist[A];
-      const int B = uni(mA.x);
       int4 mB = make_int4(-1, -1, -1, -1);
       double4 qB = make_double4(0.0, 0.0, 0.0, 0.0);
       int hB = -1;
@@ -889,7 +911,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       /* a gather of at most 64 members keeps each member's (index, label, diJ)
        * in its lane for the tile-minimum refresh */
       const bool single = total <= W;
-      int myq = -1, mylab = BIG, mytile = -1;
+      int myq = -1, mylab = BIG, mytile = -1, mynn = -1;
       unsigned long long mybits = ~0ull;
       double2 myrp = make_double2(0.0, 0.0);
       double myw = 0.0;
@@ -903,6 +925,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           myq = q;
           mylab = mq0.y;
           mytile = mq0.z;
+          mynn = mq0.x;
           myrp = r.rp;
           myw = r.wn.x;
         }
@@ -919,6 +942,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
               ev.rec[q].meta.x = phys_lo;
               ev.rec[q].wn = make_double2(wq.x, d);
               ev.rec[q].dij = dq;
+              mynn = phys_lo;
             }
             if (d < R2 && (d < cbd || (d == cbd && mq.y < cbl))) {
               cbd = d;
@@ -952,7 +976,10 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           ev.rec[phys_lo].wn = make_double2(nw, md);
           ev.rec[phys_lo].dij = dB;
         }
-        if (myq == phys_lo) mybits = (unsigned long long)__double_as_longlong(dB);
+        if (myq == phys_lo) {
+          mybits = (unsigned long long)__double_as_longlong(dB);
+          mynn = nnB;
+        }
       }
       wave_sync();
 
@@ -1000,7 +1027,10 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
             ev.rec[o].wn = make_double2(ow, od);
             ev.rec[o].dij = dO;
           }
-          if (lane == lo) mybits = (unsigned long long)__double_as_longlong(dO);
+          if (lane == lo) {
+            mybits = (unsigned long long)__double_as_longlong(dO);
+            mynn = nno;
+          }
         }
         wave_sync();
       } else if (norph > 0) {
@@ -1019,7 +1049,10 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
             ev.rec[o].wn = make_double2(wo, od);
             ev.rec[o].dij = dO;
           }
-          if (myq == o) mybits = (unsigned long long)__double_as_longlong(dO);
+          if (myq == o) {
+            mybits = (unsigned long long)__double_as_longlong(dO);
+            mynn = nno;
+          }
           wave_sync();
         }
         if (norph > MAXO) { /* more orphans than the list holds: rescan the rest from the gather */
@@ -1053,7 +1086,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
 
       /* 5. tile minima of every gathered tile */
       if (single && norph <= MAXO)
-        refresh_from_table(sh, G, ng, total, lane, myq, mylab, mybits);
+        refresh_from_table(sh, G, ng, total, lane, myq, mylab, mynn, mybits);
       else
         refresh_minima(sh, G, ev, ng, lane);
     }

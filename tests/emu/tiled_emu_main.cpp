@@ -1,0 +1,123 @@
+// Runs the tiled kernel's source (fastjet_gpu_amd/csrc/tiled.hip) on the CPU as
+// one emulated wavefront and checks every result bit-exactly against the oracle
+// (N2Plain, or its tiled form above 3000 particles).  Test infrastructure only.
+//   tiled_emu WORKLOAD E0 E1 [VARIANT P R TILE_Q NMAX]   generated events
+//   tiled_emu file PATH VARIANT P R                       raw little-endian px,py,pz,E doubles
+#include "tiled.hip"
+
+#include <vector>
+
+#include "fj_oracle.h"
+#include "fjmi_gen.h"
+
+#include <functional>
+namespace emu {
+void run_wave(const std::function<void()> &fn);
+}
+
+template <int MAXT, int WPE>
+static int run_event(const double *p4, int N, int p, double R, int tile_q, bool verbose) {
+  using namespace fjmi;
+  std::vector<JetRec> rec(N > 0 ? N : 1);
+  std::vector<double> p4s(4 * (size_t)(N > 0 ? N : 1));
+  std::vector<int> hist(N > 0 ? N : 1), lab2(N > 0 ? N : 1), pool(3 * (size_t)N + kTiledPoolPerEvent);
+  TiledScratch ts{rec.data(), p4s.data(), hist.data(), lab2.data(), pool.data(), kTiledPoolPerEvent};
+  std::vector<int32_t> h1(N > 0 ? N : 1, -7), h2(N > 0 ? N : 1, -7);
+  std::vector<double> hd(N > 0 ? N : 1, -7.0), jets(4 * (size_t)(N > 0 ? N : 1), 0.0);
+  int32_t nj = -1;
+  unsigned head = 0, status = 0;
+  int64_t offs[2] = {0, N};
+  LaunchArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.in = p4;
+  a.in_stride = 4;
+  a.in_off = 0;
+  a.offsets = offs;
+  a.n_single = N;
+  a.queue = nullptr;
+  a.n_queue = 1;
+  a.head = &head;
+  a.head_base = 0;
+  a.p_uni = p;
+  a.R_uni = R;
+  a.hist_p1 = h1.data();
+  a.hist_p2 = h2.data();
+  a.hist_dij = hd.data();
+  a.jets = jets.data();
+  a.njets = &nj;
+  a.status = &status;
+  a.stage_chunk = kStageChunk;
+  a.tile_q = tile_q;
+  emu::run_wave([&] { tiled_kernel<MAXT, WPE, false>(a, ts); });
+  std::vector<int32_t> w1(N > 0 ? N : 1), w2(N > 0 ? N : 1);
+  std::vector<double> wd(N > 0 ? N : 1), wj(4 * (size_t)(N > 0 ? N : 1));
+  int32_t wn = 0;
+  if (N <= 3000)
+    fjo_cluster_n2plain(p4, N, p, R, 0, w1.data(), w2.data(), wd.data(), wj.data(), &wn);
+  else
+    fjo_cluster_tiled(p4, N, p, R, 0, w1.data(), w2.data(), wd.data(), wj.data(), &wn);
+  int bad = status != 0 || nj != wn;
+  int first = -1;
+  for (int s = 0; s < N && !bad; ++s)
+    if (h1[s] != w1[s] || h2[s] != w2[s] || std::memcmp(&hd[s], &wd[s], 8) != 0) {
+      bad = 1;
+      first = s;
+    }
+  if (!bad && std::memcmp(jets.data(), wj.data(), 32 * (size_t)wn) != 0) bad = 1;
+  if (bad || verbose)
+    std::printf("N=%d p=%d R=%g status=%u nj=%d/%d first_diff_step=%d %s\n", N, p, R, status, nj, wn, first,
+                bad ? "MISMATCH" : "ok");
+  return bad;
+}
+
+static int run_file(const char *path, int variant, int p, double R) {
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return 2;
+  std::vector<double> p4;
+  double x;
+  while (std::fread(&x, 8, 1, f) == 1) p4.push_back(x);
+  std::fclose(f);
+  const int n = (int)(p4.size() / 4);
+  switch (variant) {
+    case 0: return run_event<512, 2>(p4.data(), n, p, R, 0, true);
+    case 1: return run_event<256, 3>(p4.data(), n, p, R, 0, true);
+    case 2: return run_event<256, 4>(p4.data(), n, p, R, 0, true);
+    default: return run_event<128, 4>(p4.data(), n, p, R, 0, true);
+  }
+}
+
+int main(int argc, char **argv) {
+  if (argc >= 2 && std::strcmp(argv[1], "file") == 0)
+    return run_file(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atof(argv[5]));
+  if (argc < 4) {
+    std::fprintf(stderr, "usage: emu workload e0 e1 [variant p R tile_q]\n");
+    return 2;
+  }
+  const int workload = std::atoi(argv[1]);
+  const long e0 = std::atol(argv[2]), e1 = std::atol(argv[3]);
+  const int variant = argc > 4 ? std::atoi(argv[4]) : 0;
+  const int p = argc > 5 ? std::atoi(argv[5]) : -1;
+  const double R = argc > 6 ? std::atof(argv[6]) : 0.4;
+  const int tq = argc > 7 ? std::atoi(argv[7]) : 0;
+  const int nmax = argc > 8 ? std::atoi(argv[8]) : 1 << 30;
+  int fails = 0;
+  for (long e = e0; e < e1; ++e) {
+    const int n = fjmi_gen_event_size(workload, e);
+    if (n > nmax) continue;
+    std::vector<double> p4(4 * (size_t)(n > 0 ? n : 1));
+    fjmi_gen_event(workload, e, p4.data(), n);
+    std::printf("[event %ld N=%d]\n", e, n);
+    std::fflush(stdout);
+    int b = 0;
+    switch (variant) {
+      case 0: b = run_event<512, 2>(p4.data(), n, p, R, tq, false); break;
+      case 1: b = run_event<256, 3>(p4.data(), n, p, R, tq, false); break;
+      case 2: b = run_event<256, 4>(p4.data(), n, p, R, tq, false); break;
+      default: b = run_event<128, 4>(p4.data(), n, p, R, tq, false); break;
+    }
+    fails += b;
+    std::fflush(stdout);
+  }
+  std::printf("done: %d mismatches\n", fails);
+  return fails ? 1 : 0;
+}
