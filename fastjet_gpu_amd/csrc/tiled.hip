@@ -47,22 +47,13 @@ constexpr int MAXG = 32;    /* tiles gathered per step (3 x 3x3 + 1 <= 28) */
 constexpr int MAXO = 128;   /* orphans listed per step (more: rescan in rounds) */
 constexpr int BIG = 0x7fffffff;
 
-/* The (diJ, label) minimum of a tile: diJ as its bit pattern (diJ >= 0, so the
- * bits order like the values), label << 16 | particle index of the minimum jet
- * (events hold at most kTiledMaxN particles), and that jet's nearest neighbour,
- * so that the merge loads both jets' records at once.  Empty tile: d = ~0,
- * lp = LP_NONE, nn = -1. */
+/* (diJ, label) minimum of a tile: diJ as its bit pattern (diJ >= 0, so the bits
+ * order like the values), the label, and the jet's physical index. */
 struct TMin {
   unsigned long long d;
-  unsigned int lp;
-  int nn;
+  int l;
+  int p;
 };
-constexpr unsigned int LP_NONE = 0xFFFFFFFFu;
-/* lp as a signed key that orders like lp (LP_NONE maps to BIG) */
-__device__ __forceinline__ int lp_key(unsigned int lp) { return (int)(lp ^ 0x80000000u); }
-__device__ __forceinline__ unsigned int make_lp(int label, int phys) {
-  return ((unsigned int)label << 16) | (unsigned int)phys;
-}
 
 /* A list of gathered tiles and the prefix sums of their member counts. */
 struct Gather {
@@ -76,8 +67,8 @@ template <int MAXT>
 struct Shared {
   TMin tmin[MAXT];
   unsigned long long mt_d[W]; /* member table of a single-chunk gather: diJ bits, */
-  unsigned int mt_lp[W];      /* label << 16 | index */
-  int mt_nn[W];               /* and nearest neighbours, by gather index */
+  int mt_l[W];                /* labels */
+  int mt_p[W];                /* and physical indices, by gather index */
   int seg_start[MAXT], seg_count[MAXT], seg_cap[MAXT];
   Gather g[2]; /* [0]: the step's gather, [1]: orphan rescans */
   int orph[MAXO];
@@ -274,30 +265,28 @@ __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, in
   if (lane < ng) {
     const int t = G.tiles[lane];
     sh.tmin[t].d = ~0ull;
-    sh.tmin[t].lp = LP_NONE;
-    sh.tmin[t].nn = -1;
+    sh.tmin[t].l = BIG;
+    sh.tmin[t].p = -1;
   }
   wave_sync();
   const int total = G.pref[ng];
-  /* three passes over all members: the diJ minimum, then the lowest label (and
-   * so index) among members at that minimum, then that jet's nearest neighbour
-   * (each pass must see the final result of the previous one across every
-   * chunk of 64 members) */
+  /* three passes over all members: the diJ minimum, then the lowest label among
+   * members at that minimum, then its index (each pass must see the final result
+   * of the previous one across every chunk of 64 members) */
   for (int phase = 0; phase < 3; ++phase) {
     for (int j0 = 0; j0 < total; j0 += W) {
       const int p = gathered(sh, G, ev, ng, j0 + lane);
       if (p >= 0) {
-        const int4 m = ev.rec[p].meta;
-        const int t = m.z;
+        const int t = ev.rec[p].meta.z;
         const unsigned long long bits = (unsigned long long)__double_as_longlong(ev.rec[p].dij);
         if (phase == 0) {
           atomicMin(&sh.tmin[t].d, bits);
         } else if (sh.tmin[t].d == bits) {
-          const unsigned int lp = make_lp(m.y, p);
+          const int lab = ev.rec[p].meta.y;
           if (phase == 1)
-            atomicMin(&sh.tmin[t].lp, lp);
-          else if (sh.tmin[t].lp == lp)
-            sh.tmin[t].nn = m.x;
+            atomicMin(&sh.tmin[t].l, lab);
+          else if (sh.tmin[t].l == lab)
+            sh.tmin[t].p = p;
         }
       }
     }
@@ -309,11 +298,11 @@ __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, in
  * 64 members): lane j < total wrote member j; lane g < ng scans tile g's range. */
 template <class SH>
 __device__ void refresh_from_table(SH &sh, const Gather &G, int ng, int total, int lane, int myq, int mylab,
-                                   int mynn, unsigned long long mybits) {
+                                   unsigned long long mybits) {
   if (lane < total) {
     sh.mt_d[lane] = mybits;
-    sh.mt_lp[lane] = make_lp(mylab, myq);
-    sh.mt_nn[lane] = mynn;
+    sh.mt_l[lane] = mylab;
+    sh.mt_p[lane] = myq;
   }
   wave_sync();
   if (lane < ng) {
@@ -321,15 +310,15 @@ __device__ void refresh_from_table(SH &sh, const Gather &G, int ng, int total, i
     const int j1 = G.pref[lane + 1];
     TMin best;
     best.d = ~0ull;
-    best.lp = LP_NONE;
-    best.nn = -1;
+    best.l = BIG;
+    best.p = -1;
     for (int j = G.pref[lane]; j < j1; ++j) {
       const unsigned long long d = sh.mt_d[j];
-      const unsigned int lp = sh.mt_lp[j];
-      if (d < best.d || (d == best.d && lp < best.lp)) {
+      const int l = sh.mt_l[j];
+      if (d < best.d || (d == best.d && l < best.l)) {
         best.d = d;
-        best.lp = lp;
-        best.nn = sh.mt_nn[j];
+        best.l = l;
+        best.p = sh.mt_p[j];
       }
     }
     sh.tmin[t] = best;
@@ -620,17 +609,19 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
     ev.pool = ts.pool + pool_off;
     ev.pool_size = 3 * N + ts.pool_per_event;
 
-    /* ---- prologue: kinematics, rapidity range ---- */
+    /* ---- prologue.  Jets are laid out in tile order: particle s (label s)
+     * gets the physical index i = (members of lower tiles) + (its rank in
+     * its tile), so that the members of a tile, and of a row of adjacent
+     * tiles, are contiguous records.  Labels, not physical indices, decide
+     * ties, so the layout does not change any result. ----
+     * 1. kinematics into a staging copy indexed by s (the p4 array, which is
+     *    written in tile order only in step 4), and the rapidity range */
+    double4 *stage = ev.p4;
     double ylo = 1e300, yhi = -1e300;
     for (int s = lane; s < N; s += W) {
       const double *q = a.in + (size_t)(base + s) * a.in_stride + a.in_off;
-      const double4 q4 = make_double4(q[0], q[1], q[2], q[3]);
-      const double4 c = kin_coords_t(q4.x, q4.y, q4.z, q4.w, p);
-      ev.rec[s].rp = make_double2(c.x, c.y);
-      ev.rec[s].wn = make_double2(c.z, R2);
-      ev.p4[s] = q4;
-      ev.hist[s] = s;
-      ev.lab2[s] = s;
+      const double4 c = kin_coords_t(q[0], q[1], q[2], q[3], p);
+      stage[s] = make_double4(c.x, c.y, c.z, 0.0);
       if (c.x > -1e4 && c.x < 1e4) {
         ylo = c.x < ylo ? c.x : ylo;
         yhi = c.x > yhi ? c.x : yhi;
@@ -667,63 +658,130 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
     geo.sy = uni_d(sh.sy);
     geo.sphi = uni_d(sh.sphi);
     const int ntiles = geo.ny * geo.nphi;
+    for (int t = lane; t < ntiles; t += W) sh.seg_count[t] = 0;
+    wave_sync();
+    /* 2. each particle's tile and its rank among the tile's members (the
+     *    order of equal-tile particles is immaterial: ties go by label) */
     for (int s = lane; s < N; s += W) {
-      const double2 rp = ev.rec[s].rp;
-      ev.rec[s].meta = make_int4(-1, s, geo.tile(rp.x, rp.y), 0);
+      const double4 c = stage[s];
+      const int t = geo.tile(c.x, c.y);
+      const int rank = atomicAdd(&sh.seg_count[t], 1);
+      stage[s].w = __hiloint2double(t, rank);
     }
     wave_sync();
-    rebuild_segments(sh, ev, N, ntiles, lane);
-
-    /* ---- initial nearest neighbours: each lane scans the 3x3 tiles of its jets ---- */
+    /* 3. segments with 50% slack in the pool, and each tile's first physical
+     *    index (kept in tmin[t].p until the tile minima are set) */
+    {
+      int carry = 0, pcarry = 0;
+      for (int t0 = 0; t0 < ntiles; t0 += W) {
+        const int t = t0 + lane;
+        const int c = t < ntiles ? sh.seg_count[t] : 0;
+        const int cap = c + (c >> 1) + 2;
+        const int incl = wave_incl_scan(cap, lane);
+        const int pincl = wave_incl_scan(c, lane);
+        if (t < ntiles) {
+          sh.seg_start[t] = carry + incl - cap;
+          sh.seg_cap[t] = cap;
+          sh.tmin[t].p = pcarry + pincl - c;
+        }
+        carry += __builtin_amdgcn_readlane(incl, 63);
+        pcarry += __builtin_amdgcn_readlane(pincl, 63);
+      }
+      if (lane == 0) {
+        sh.pool_top = carry;
+        sh.rebuild = 0;
+      }
+    }
+    wave_sync();
+    /* 4. the records in tile order */
     for (int s = lane; s < N; s += W) {
-      const double2 rp = ev.rec[s].rp;
-      const double ws = ev.rec[s].wn.x;
-      const int ts0 = ev.rec[s].meta.z;
-      const int ty0 = ts0 / geo.nphi, tp0 = ts0 % geo.nphi;
-      double bd = R2, bw = 0.0;
+      const double4 c = stage[s];
+      const int t = __double2hiint(c.w), rank = __double2loint(c.w);
+      const int i = sh.tmin[t].p + rank;
+      const int pos = sh.seg_start[t] + rank;
+      ev.rec[i].rp = make_double2(c.x, c.y);
+      ev.rec[i].wn = make_double2(c.z, R2);
+      ev.rec[i].meta = make_int4(-1, s, t, pos);
+      ev.pool[pos] = i;
+      ev.lab2[s] = i;
+      ev.hist[i] = s;
+    }
+    wave_sync(); /* the staging copy is read in full before p4 is overwritten */
+    for (int s = lane; s < N; s += W) {
+      const double *q = a.in + (size_t)(base + s) * a.in_stride + a.in_off;
+      ev.p4[ev.lab2[s]] = make_double4(q[0], q[1], q[2], q[3]);
+    }
+
+    /* 5. initial nearest neighbours, one jet per lane in tile order.  The 3x3
+     *    neighbourhood of a tile is, per tile row, one contiguous range of
+     *    physical indices (two when the row wraps around in phi, the whole
+     *    row when it has at most 3 columns). */
+    for (int i = lane; i < N; i += W) {
+      const double2 rp = ev.rec[i].rp;
+      const double wi = ev.rec[i].wn.x;
+      const int ti = ev.rec[i].meta.z;
+      const int ty0 = ti / geo.nphi, tp0 = ti - ty0 * geo.nphi;
+      double bd = R2;
       int bl = BIG, bp = -1;
-      /* the distinct tiles of the 3x3 neighbourhood (with fewer than 3 phi
-       * columns the wrapped columns coincide and are visited once) */
-      for (int i = 0; i < 9; ++i) {
-        const int y = ty0 + i / 3 - 1, dp = i % 3 - 1;
-        if (y < 0 || y >= geo.ny || (geo.nphi == 1 && dp != 0) || (geo.nphi == 2 && dp < 0)) continue;
-        const int t = y * geo.nphi + (tp0 + dp + geo.nphi) % geo.nphi;
-        const int st = sh.seg_start[t], ct = sh.seg_count[t];
-        for (int u = 0; u < ct; ++u) {
-          const int q = ev.pool[st + u];
-          if (q == s) continue;
-          const double2 rq = ev.rec[q].rp;
+      for (int r = 0; r < 6; ++r) {
+        const int y = ty0 + r / 2 - 1;
+        if (y < 0 || y >= geo.ny) continue;
+        const int row = y * geo.nphi;
+        int c0, c1; /* columns [c0, c1] */
+        if (geo.nphi <= 3) {
+          if (r & 1) continue;
+          c0 = 0;
+          c1 = geo.nphi - 1;
+        } else if (tp0 == 0) {
+          c0 = (r & 1) ? geo.nphi - 1 : 0;
+          c1 = (r & 1) ? geo.nphi - 1 : 1;
+        } else if (tp0 == geo.nphi - 1) {
+          c0 = (r & 1) ? 0 : tp0 - 1;
+          c1 = (r & 1) ? 0 : tp0;
+        } else {
+          if (r & 1) continue;
+          c0 = tp0 - 1;
+          c1 = tp0 + 1;
+        }
+        const int j0 = sh.tmin[row + c0].p, j1 = sh.tmin[row + c1].p + sh.seg_count[row + c1];
+        for (int j = j0; j < j1; ++j) {
+          const double2 rq = ev.rec[j].rp;
+          const int lq = ev.rec[j].meta.y;
           const double d = fjmi_dr2(rp.x, rp.y, rq.x, rq.y);
-          if (d < bd || (d == bd && d < R2 && q < bl)) { /* labels == physical indices here */
+          if (j != i && (d < bd || (d == bd && d < R2 && lq < bl))) {
             bd = d;
-            bl = q;
-            bp = q;
+            bl = lq;
+            bp = j;
           }
         }
       }
-      if (bp >= 0) bw = ev.rec[bp].wn.x;
-      ev.rec[s].meta.x = bp;
-      ev.rec[s].wn = make_double2(ws, bd);
-      ev.rec[s].dij = diJ_of(bd, ws, bp, bw);
+      const double bw = bp >= 0 ? ev.rec[bp].wn.x : 0.0;
+      ev.rec[i].meta.x = bp;
+      ev.rec[i].wn = make_double2(wi, bd);
+      ev.rec[i].dij = diJ_of(bd, wi, bp, bw);
     }
     wave_sync();
-    /* tile minima of every tile */
-    Gather &G = sh.g[0];
-    Gather &GO = sh.g[1];
-    for (int t0 = 0; t0 < ntiles; t0 += MAXG) {
-      const int cnt = min(MAXG, ntiles - t0);
-      if (lane == 0) {
-        int acc = 0;
-        G.pref[0] = 0;
-        for (int u = 0; u < cnt; ++u) {
-          G.tiles[u] = t0 + u;
-          acc += sh.seg_count[t0 + u];
-          G.pref[u + 1] = acc;
+    /* 6. tile minima, one tile per lane over its contiguous members */
+    for (int t = lane; t < ntiles; t += W) {
+      const int j0 = sh.tmin[t].p, j1 = j0 + sh.seg_count[t];
+      unsigned long long bd = ~0ull;
+      int bl = BIG, bp = -1;
+      for (int j = j0; j < j1; ++j) {
+        const unsigned long long d = (unsigned long long)__double_as_longlong(ev.rec[j].dij);
+        const int lq = ev.rec[j].meta.y;
+        if (d < bd || (d == bd && lq < bl)) {
+          bd = d;
+          bl = lq;
+          bp = j;
         }
       }
-      wave_sync();
-      refresh_minima(sh, G, ev, cnt, lane);
+      sh.tmin[t].d = bd;
+      sh.tmin[t].l = bl;
+      sh.tmin[t].p = bp;
     }
+    wave_sync();
+    Gather &G = sh.g[0];
+    Gather &GO = sh.g[1];
 
     /* ---- the N recombination steps ---- */
     int nj = 0;
@@ -739,18 +797,17 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
        * pairwise tree of lexicographic (diJ, label) compares */
       constexpr int KT = MAXT / W;
       double kd[KT];
-      int kl[KT], kt[KT], kn[KT];
+      int kl[KT], kt[KT];
       uint4 raw[KT];
 #pragma unroll
       for (int k = 0; k < KT; ++k) raw[k] = *reinterpret_cast<const uint4 *>(&sh.tmin[k * W + lane]);
 #pragma unroll
       for (int k = 0; k < KT; ++k) {
-        const int l = lp_key(raw[k].z);
+        const int l = (int)raw[k].z;
         const bool ok = (k * W + lane < ntiles) & (l != BIG);
         kd[k] = ok ? __hiloint2double((int)raw[k].y, (int)raw[k].x) : __builtin_inf();
         kl[k] = ok ? l : BIG;
         kt[k] = k * W + lane;
-        kn[k] = (int)raw[k].w;
       }
 #pragma unroll
       for (int w = 1; w < KT; w *= 2) {
@@ -760,17 +817,15 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           kd[k] = c ? kd[k + w] : kd[k];
           kl[k] = c ? kl[k + w] : kl[k];
           kt[k] = c ? kt[k + w] : kt[k];
-          kn[k] = c ? kn[k + w] : kn[k];
         }
       }
       const double bd = kd[0];
-      const int bl = kl[0], bt = kt[0], bn = kn[0];
+      const int bl = kl[0], bt = kt[0];
       double dmin;
       int lA;
       const int wl = wave_argmin(bd, bl, &dmin, &lA);
       const int tA = wl < 0 ? -1 : uni(__builtin_amdgcn_readlane(bt, wl));
-      const int A = tA < 0 ? -1 : (int)(((unsigned)lA ^ 0x80000000u) & 0xFFFFu);
-      const int B = tA < 0 ? -1 : uni(__builtin_amdgcn_readlane(bn, wl));
+      const int A = tA < 0 ? -1 : uni(sh.tmin[tA].p);
       if (A < 0) { /* no live jet found: inconsistent state, stop this event */
         if (lane == 0) atomicOr(a.status, 2u);
         bad = true;
@@ -783,6 +838,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       const int4 mA = ev.rec[A].meta;
       const double4 qA = ev.p4[A];
       const int hA = ev.hist[A];
+      const int B = uni(mA.x);
       int4 mB = make_int4(-1, -1, -1, -1);
       double4 qB = make_double4(0.0, 0.0, 0.0, 0.0);
       int hB = -1;
@@ -911,7 +967,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       /* a gather of at most 64 members keeps each member's (index, label, diJ)
        * in its lane for the tile-minimum refresh */
       const bool single = total <= W;
-      int myq = -1, mylab = BIG, mytile = -1, mynn = -1;
+      int myq = -1, mylab = BIG, mytile = -1;
       unsigned long long mybits = ~0ull;
       double2 myrp = make_double2(0.0, 0.0);
       double myw = 0.0;
@@ -925,7 +981,6 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           myq = q;
           mylab = mq0.y;
           mytile = mq0.z;
-          mynn = mq0.x;
           myrp = r.rp;
           myw = r.wn.x;
         }
@@ -942,7 +997,6 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
               ev.rec[q].meta.x = phys_lo;
               ev.rec[q].wn = make_double2(wq.x, d);
               ev.rec[q].dij = dq;
-              mynn = phys_lo;
             }
             if (d < R2 && (d < cbd || (d == cbd && mq.y < cbl))) {
               cbd = d;
@@ -976,10 +1030,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           ev.rec[phys_lo].wn = make_double2(nw, md);
           ev.rec[phys_lo].dij = dB;
         }
-        if (myq == phys_lo) {
-          mybits = (unsigned long long)__double_as_longlong(dB);
-          mynn = nnB;
-        }
+        if (myq == phys_lo) mybits = (unsigned long long)__double_as_longlong(dB);
       }
       wave_sync();
 
@@ -1027,10 +1078,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
             ev.rec[o].wn = make_double2(ow, od);
             ev.rec[o].dij = dO;
           }
-          if (lane == lo) {
-            mybits = (unsigned long long)__double_as_longlong(dO);
-            mynn = nno;
-          }
+          if (lane == lo) mybits = (unsigned long long)__double_as_longlong(dO);
         }
         wave_sync();
       } else if (norph > 0) {
@@ -1049,10 +1097,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
             ev.rec[o].wn = make_double2(wo, od);
             ev.rec[o].dij = dO;
           }
-          if (myq == o) {
-            mybits = (unsigned long long)__double_as_longlong(dO);
-            mynn = nno;
-          }
+          if (myq == o) mybits = (unsigned long long)__double_as_longlong(dO);
           wave_sync();
         }
         if (norph > MAXO) { /* more orphans than the list holds: rescan the rest from the gather */
@@ -1086,7 +1131,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
 
       /* 5. tile minima of every gathered tile */
       if (single && norph <= MAXO)
-        refresh_from_table(sh, G, ng, total, lane, myq, mylab, mynn, mybits);
+        refresh_from_table(sh, G, ng, total, lane, myq, mylab, mybits);
       else
         refresh_minima(sh, G, ev, ng, lane);
     }
