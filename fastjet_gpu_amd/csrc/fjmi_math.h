@@ -134,25 +134,16 @@ FJMI_HD fjmi_dd fjmi_dd_make(double hi, double lo) {
  * subtraction), |r| < 2^-7.4;  log x = e ln2 - log(INV) + log1p(r), with
  * -log(INV) from the table and log1p(r) = r - r^2/2 + r^3 P(r): r and r^2/2 in
  * double-double, the degree-8 tail in double.  No division. */
-FJMI_HD double fjmi_log(double x) {
-  if (!(x > 0.0)) {
-    if (x == 0.0) return -__builtin_inf();
-    return __builtin_nan(""); /* negative or NaN */
-  }
-  if (x == __builtin_inf()) return x;
-  int e = 0;
-  uint64_t u = fjmi_d2u(x);
-  if (u < 0x0010000000000000ULL) { /* subnormal */
-    x *= 0x1p54;
-    u = fjmi_d2u(x);
-    e = -54;
-  }
-  e += (int)(u >> 52) - 1023;
+/* log x for positive normal finite x = 2^e0 * (the argument): the table and
+ * polynomial path, written without branches so that a caller's independent
+ * work interleaves with it. */
+FJMI_HD double fjmi_log_core(double x, int e0) {
+  const uint64_t u = fjmi_d2u(x);
+  int e = e0 + (int)(u >> 52) - 1023;
   double m = fjmi_u2d((u & 0x000FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL);
-  if (m > 0x1.6a09e667f3bcdp+0) {
-    m *= 0.5;
-    e += 1;
-  }
+  const int big = m > 0x1.6a09e667f3bcdp+0;
+  m = big ? m * 0.5 : m;
+  e = big ? e + 1 : e;
   const int j = (int)(m * 128.0 + 0.5); /* 91..181 */
   const double *tab = fjmi_log_tab[j - FJMI_LOG_J0];
   const fjmi_dd pr = fjmi_two_prod(m, tab[0]);
@@ -172,9 +163,21 @@ FJMI_HD double fjmi_log(double x) {
   const double tail = (r2.hi * rh) * P;
   fjmi_dd L = fjmi_dd_add(r, fjmi_dd_make(-0.5 * r2.hi, -0.5 * r2.lo));
   L = fjmi_dd_add_d(L, tail);
-  fjmi_dd acc = fjmi_dd_make(tab[1], tab[2]);
-  if (e != 0) acc = fjmi_dd_add(fjmi_dd_make((double)e * FJMI_LN2_HI, (double)e * FJMI_LN2_LO), acc);
+  const fjmi_dd acc0 = fjmi_dd_make(tab[1], tab[2]);
+  const fjmi_dd acc1 = fjmi_dd_add(fjmi_dd_make((double)e * FJMI_LN2_HI, (double)e * FJMI_LN2_LO), acc0);
+  /* selects per component: a select of whole structs may be lowered through memory */
+  const fjmi_dd acc = fjmi_dd_make(e != 0 ? acc1.hi : acc0.hi, e != 0 ? acc1.lo : acc0.lo);
   return fjmi_dd_add(acc, L).hi;
+}
+
+FJMI_HD double fjmi_log(double x) {
+  if (!(x > 0.0)) {
+    if (x == 0.0) return -__builtin_inf();
+    return __builtin_nan(""); /* negative or NaN */
+  }
+  if (x == __builtin_inf()) return x;
+  if (fjmi_d2u(x) < 0x0010000000000000ULL) return fjmi_log_core(x * 0x1p54, -54); /* subnormal */
+  return fjmi_log_core(x, 0);
 }
 
 /* ---- atan2 ----
@@ -183,6 +186,8 @@ FJMI_HD double fjmi_log(double x) {
  * |u| <= 2^-7, atan(u) = u + u^3 Q(u^2) with the degree-5 Q in double; then
  * octant/quadrant reconstruction in double-double.  IEEE special cases follow
  * C99 Annex F. */
+FJMI_HD double fjmi_atan2_core(double y, double x);
+
 FJMI_HD double fjmi_atan2(double y, double x) {
   if (x != x || y != y) return x + y;
   const int xneg = (int)(fjmi_d2u(x) >> 63);
@@ -204,6 +209,15 @@ FJMI_HD double fjmi_atan2(double y, double x) {
       r = 0x1.921fb54442d18p+0;
     return yneg ? -r : r;
   }
+  return fjmi_atan2_core(y, x);
+}
+
+/* atan2(y, x) for finite nonzero x and y: the table and polynomial path,
+ * without branches (see fjmi_log_core). */
+FJMI_HD double fjmi_atan2_core(double y, double x) {
+  const int xneg = (int)(fjmi_d2u(x) >> 63);
+  const int yneg = (int)(fjmi_d2u(y) >> 63);
+  const double ax = __builtin_fabs(x), ay = __builtin_fabs(y);
   const int swap = ay > ax;
   const double num = swap ? ax : ay;
   const double den = swap ? ay : ax;
@@ -226,10 +240,14 @@ FJMI_HD double fjmi_atan2(double y, double x) {
   Q = __builtin_fma(w, Q, 1.0 / 5.0);
   Q = __builtin_fma(w, Q, -1.0 / 3.0);
   const fjmi_dd at = fjmi_dd_add_d(uu, (uu.hi * w) * Q);
-  fjmi_dd th2 = j ? fjmi_dd_add(fjmi_dd_make(fjmi_atan_tab[j][0], fjmi_atan_tab[j][1]), at) : at;
-  if (swap) th2 = fjmi_dd_add(fjmi_dd_make(0x1.921fb54442d18p+0, 0x1.1a62633145c07p-54), fjmi_dd_neg(th2));
-  if (xneg) th2 = fjmi_dd_add(fjmi_dd_make(0x1.921fb54442d18p+1, 0x1.1a62633145c07p-53), fjmi_dd_neg(th2));
-  return yneg ? -th2.hi : th2.hi;
+  const fjmi_dd th1 = fjmi_dd_add(fjmi_dd_make(fjmi_atan_tab[j][0], fjmi_atan_tab[j][1]), at);
+  /* selects per component: a select of whole structs may be lowered through memory */
+  fjmi_dd th2 = fjmi_dd_make(j ? th1.hi : at.hi, j ? th1.lo : at.lo);
+  const fjmi_dd th3 = fjmi_dd_add(fjmi_dd_make(0x1.921fb54442d18p+0, 0x1.1a62633145c07p-54), fjmi_dd_neg(th2));
+  th2 = fjmi_dd_make(swap ? th3.hi : th2.hi, swap ? th3.lo : th2.lo);
+  const fjmi_dd th4 = fjmi_dd_add(fjmi_dd_make(0x1.921fb54442d18p+1, 0x1.1a62633145c07p-53), fjmi_dd_neg(th2));
+  const double hi = xneg ? th4.hi : th2.hi;
+  return yneg ? -hi : hi;
 }
 
 /* ---- FastJet PseudoJet kinematics (tri_matrix.cu:61-100, grid.cu:112-166) ---- */

@@ -205,9 +205,37 @@ struct Ev {
   int pool_size;
 };
 
+/* (rapidity, azimuth, weight) of a momentum, bit-identical to fjmi_rap,
+ * fjmi_phi and fjmi_weight.  In the common case (finite nonzero px and py, a
+ * normal positive log argument) both go through the branch-free cores in one
+ * straight-line block, so that the two dependent chains interleave. */
 __device__ __forceinline__ double4 kin_coords_t(double px, double py, double pz, double E, int p) {
   const double kt2 = px * px + py * py;
-  return make_double4(fjmi_rap(pz, E, kt2), fjmi_phi(px, py, kt2), fjmi_weight(kt2, p), 0.0);
+  const double w = fjmi_weight(kt2, p);
+  const double apz = __builtin_fabs(pz);
+  const double m2 = (E + pz) * (E - pz) - kt2;
+  const double em2 = m2 > 0.0 ? m2 : 0.0;
+  const double epz = E + apz;
+  const double arg = (kt2 + em2) / (epz * epz);
+  const unsigned long long ua = (unsigned long long)__double_as_longlong(arg);
+  const double ax = __builtin_fabs(px), ay = __builtin_fabs(py);
+  const bool fast = kt2 != 0.0 && ua >= 0x0010000000000000ull && ua < 0x7FF0000000000000ull &&
+                    ax != 0.0 && ay != 0.0 && ax < __builtin_inf() && ay < __builtin_inf();
+  if (fast) {
+    double rap = 0.5 * fjmi_log_core(arg, 0);
+    rap = pz > 0 ? -rap : rap;
+    double phi = fjmi_atan2_core(py, px);
+    phi = phi < 0.0 ? phi + FJMI_TWOPI : phi;
+    phi = phi >= FJMI_TWOPI ? phi - FJMI_TWOPI : phi;
+    return make_double4(rap, phi, w, 0.0);
+  }
+  return make_double4(fjmi_rap(pz, E, kt2), fjmi_phi(px, py, kt2), w, 0.0);
+}
+
+/* particle s of the event starting at base, from the caller's input rows */
+__device__ __forceinline__ double4 load_in(const LaunchArgs &a, int64_t base, int s) {
+  const double *q = a.in + (size_t)(base + s) * a.in_stride + a.in_off;
+  return make_double4(q[0], q[1], q[2], q[3]);
 }
 
 __device__ __forceinline__ double diJ_of(double nnd, double w, int nn, double wnn) {
@@ -617,10 +645,18 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
      * 1. kinematics into a staging copy indexed by s (the p4 array, which is
      *    written in tile order only in step 4), and the rapidity range */
     double4 *stage = ev.p4;
+    /* loads in flight per lane in the prologue's passes (PB particles) and
+     * scans (NB candidates); fewer where more waves share the registers */
+    constexpr int PB = WPE <= 2 ? 4 : (WPE == 3 ? 2 : 1);
+    constexpr int NB = WPE <= 2 ? 8 : (WPE == 3 ? 4 : 2);
     double ylo = 1e300, yhi = -1e300;
+    /* the next particle's momentum is loaded before this one's kinematics */
+    double4 qn = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (lane < N) qn = load_in(a, base, lane);
     for (int s = lane; s < N; s += W) {
-      const double *q = a.in + (size_t)(base + s) * a.in_stride + a.in_off;
-      const double4 c = kin_coords_t(q[0], q[1], q[2], q[3], p);
+      const double4 q4 = qn;
+      if (s + W < N) qn = load_in(a, base, s + W);
+      const double4 c = kin_coords_t(q4.x, q4.y, q4.z, q4.w, p);
       stage[s] = make_double4(c.x, c.y, c.z, 0.0);
       if (c.x > -1e4 && c.x < 1e4) {
         ylo = c.x < ylo ? c.x : ylo;
@@ -662,11 +698,18 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
     wave_sync();
     /* 2. each particle's tile and its rank among the tile's members (the
      *    order of equal-tile particles is immaterial: ties go by label) */
-    for (int s = lane; s < N; s += W) {
-      const double4 c = stage[s];
-      const int t = geo.tile(c.x, c.y);
-      const int rank = atomicAdd(&sh.seg_count[t], 1);
-      stage[s].w = __hiloint2double(t, rank);
+    for (int s0 = lane; s0 < N; s0 += PB * W) { /* PB particles' loads at once */
+      double2 c[PB];
+#pragma unroll
+      for (int k = 0; k < PB; ++k)
+        if (s0 + k * W < N) c[k] = make_double2(stage[s0 + k * W].x, stage[s0 + k * W].y);
+#pragma unroll
+      for (int k = 0; k < PB; ++k)
+        if (s0 + k * W < N) {
+          const int t = geo.tile(c[k].x, c[k].y);
+          const int rank = atomicAdd(&sh.seg_count[t], 1);
+          stage[s0 + k * W].w = __hiloint2double(t, rank);
+        }
     }
     wave_sync();
     /* 3. segments with 50% slack in the pool, and each tile's first physical
@@ -694,23 +737,28 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
     }
     wave_sync();
     /* 4. the records in tile order */
-    for (int s = lane; s < N; s += W) {
-      const double4 c = stage[s];
-      const int t = __double2hiint(c.w), rank = __double2loint(c.w);
-      const int i = sh.tmin[t].p + rank;
-      const int pos = sh.seg_start[t] + rank;
-      ev.rec[i].rp = make_double2(c.x, c.y);
-      ev.rec[i].wn = make_double2(c.z, R2);
-      ev.rec[i].meta = make_int4(-1, s, t, pos);
-      ev.pool[pos] = i;
-      ev.lab2[s] = i;
-      ev.hist[i] = s;
+    for (int s0 = lane; s0 < N; s0 += PB * W) {
+      double4 c[PB];
+#pragma unroll
+      for (int k = 0; k < PB; ++k)
+        if (s0 + k * W < N) c[k] = stage[s0 + k * W];
+#pragma unroll
+      for (int k = 0; k < PB; ++k)
+        if (s0 + k * W < N) {
+          const int s = s0 + k * W;
+          const int t = __double2hiint(c[k].w), rank = __double2loint(c[k].w);
+          const int i = sh.tmin[t].p + rank;
+          const int pos = sh.seg_start[t] + rank;
+          ev.rec[i].rp = make_double2(c[k].x, c[k].y);
+          ev.rec[i].wn = make_double2(c[k].z, R2);
+          ev.rec[i].meta = make_int4(-1, s, t, pos);
+          ev.pool[pos] = i;
+          ev.lab2[s] = i;
+          ev.hist[i] = s;
+        }
     }
     wave_sync(); /* the staging copy is read in full before p4 is overwritten */
-    for (int s = lane; s < N; s += W) {
-      const double *q = a.in + (size_t)(base + s) * a.in_stride + a.in_off;
-      ev.p4[ev.lab2[s]] = make_double4(q[0], q[1], q[2], q[3]);
-    }
+    for (int s = lane; s < N; s += W) ev.p4[ev.lab2[s]] = load_in(a, base, s);
 
     /* 5. initial nearest neighbours, one jet per lane in tile order.  The 3x3
      *    neighbourhood of a tile is, per tile row, one contiguous range of
@@ -744,14 +792,24 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           c1 = tp0 + 1;
         }
         const int j0 = sh.tmin[row + c0].p, j1 = sh.tmin[row + c1].p + sh.seg_count[row + c1];
-        for (int j = j0; j < j1; ++j) {
-          const double2 rq = ev.rec[j].rp;
-          const int lq = ev.rec[j].meta.y;
-          const double d = fjmi_dr2(rp.x, rp.y, rq.x, rq.y);
-          if (j != i && (d < bd || (d == bd && d < R2 && lq < bl))) {
-            bd = d;
-            bl = lq;
-            bp = j;
+        /* candidates in blocks of NB, all of a block's loads issued together
+         * (indices past the range repeat its last member and are masked) */
+        for (int j = j0; j < j1; j += NB) {
+          double2 rq[NB];
+          int lq[NB];
+#pragma unroll
+          for (int k = 0; k < NB; ++k) {
+            const int jk = min(j + k, j1 - 1);
+            rq[k] = ev.rec[jk].rp;
+            lq[k] = ev.rec[jk].meta.y;
+          }
+#pragma unroll
+          for (int k = 0; k < NB; ++k) {
+            const double d = fjmi_dr2(rp.x, rp.y, rq[k].x, rq[k].y);
+            const bool take = (j + k < j1) & (j + k != i) & ((d < bd) | ((d == bd) & (d < R2) & (lq[k] < bl)));
+            bd = take ? d : bd;
+            bl = take ? lq[k] : bl;
+            bp = take ? j + k : bp;
           }
         }
       }
@@ -766,13 +824,21 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       const int j0 = sh.tmin[t].p, j1 = j0 + sh.seg_count[t];
       unsigned long long bd = ~0ull;
       int bl = BIG, bp = -1;
-      for (int j = j0; j < j1; ++j) {
-        const unsigned long long d = (unsigned long long)__double_as_longlong(ev.rec[j].dij);
-        const int lq = ev.rec[j].meta.y;
-        if (d < bd || (d == bd && lq < bl)) {
-          bd = d;
-          bl = lq;
-          bp = j;
+      for (int j = j0; j < j1; j += NB / 2) {
+        unsigned long long dq[NB / 2];
+        int lq[NB];
+#pragma unroll
+        for (int k = 0; k < NB / 2; ++k) {
+          const int jk = min(j + k, j1 - 1);
+          dq[k] = (unsigned long long)__double_as_longlong(ev.rec[jk].dij);
+          lq[k] = ev.rec[jk].meta.y;
+        }
+#pragma unroll
+        for (int k = 0; k < NB / 2; ++k) {
+          const bool take = (j + k < j1) & ((dq[k] < bd) | ((dq[k] == bd) & (lq[k] < bl)));
+          bd = take ? dq[k] : bd;
+          bl = take ? lq[k] : bl;
+          bp = take ? j + k : bp;
         }
       }
       sh.tmin[t].d = bd;

@@ -10,6 +10,8 @@
 #include "fj_oracle.h"
 #include "fjmi_gen.h"
 
+#include <cmath>
+#include <cstdint>
 #include <functional>
 namespace emu {
 void run_wave(const std::function<void()> &fn);
@@ -86,7 +88,50 @@ static int run_file(const char *path, int variant, int p, double R) {
   }
 }
 
+/* The kernel's kinematics (its straight-line common path and the general
+ * one) against the shared scalar functions, bit for bit, on edge cases and on
+ * random momenta spread over many decades. */
+static int run_kin(long n, unsigned seed) {
+  const double edge[] = {0.0, -0.0, 1e-320, -1e-320, 1e-200, -1e-200, 1e-160, 1e-30, 0.5, 1.0, -1.0, 3.0, 1e30,
+                         1e160, -1e160, 1e300, -1e300, __builtin_inf(), -__builtin_inf(), __builtin_nan("")};
+  const int ne = (int)(sizeof edge / sizeof edge[0]);
+  uint64_t x = 0x9E3779B97F4A7C15ull ^ seed;
+  auto rnd = [&]() {
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    return x;
+  };
+  auto pick = [&]() -> double {
+    const uint64_t r = rnd();
+    if ((r & 7) == 0) return edge[(r >> 3) % ne];
+    const double mant = (double)((r >> 11) & 0xFFFFFFFFFFFFFull) * 0x1p-52 + 1.0; /* [1, 2) */
+    const int ex = (int)((r >> 3) % 160) - 80;
+    return ((r >> 63) ? -1.0 : 1.0) * std::ldexp(mant, ex);
+  };
+  long bad = 0;
+  for (long i = 0; i < n; ++i) {
+    double px = pick(), py = pick(), pz = pick(), E = pick();
+    if (rnd() % 4 == 0) E = std::fabs(pz); /* E == |pz| */
+    if (rnd() % 4 == 0) E = std::sqrt(px * px + py * py + pz * pz); /* massless */
+    for (int p = -1; p <= 1; ++p) {
+      const double4 c = fjmi::tiled::kin_coords_t(px, py, pz, E, p);
+      const double kt2 = px * px + py * py;
+      const double r = fjmi_rap(pz, E, kt2), f = fjmi_phi(px, py, kt2), w = fjmi_weight(kt2, p);
+      if (std::memcmp(&c.x, &r, 8) != 0 || std::memcmp(&c.y, &f, 8) != 0 || std::memcmp(&c.z, &w, 8) != 0) {
+        if (bad < 5)
+          std::printf("kinematics differ: p4=(%a %a %a %a) p=%d: (%a %a %a) vs (%a %a %a)\n", px, py, pz, E, p, c.x, c.y,
+                      c.z, r, f, w);
+        ++bad;
+      }
+    }
+  }
+  std::printf("kinematics: %ld of %ld differ\n", bad, 3 * n);
+  return bad ? 1 : 0;
+}
+
 int main(int argc, char **argv) {
+  if (argc >= 2 && std::strcmp(argv[1], "kin") == 0) return run_kin(std::atol(argv[2]), (unsigned)std::atol(argv[3]));
   if (argc >= 2 && std::strcmp(argv[1], "file") == 0)
     return run_file(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atof(argv[5]));
   if (argc < 4) {

@@ -72,3 +72,13 @@ def test_edge_events(emu_bin, tmp_path, name, variant, p, R):
     _edge_events()[name].astype("<f8").tofile(path)
     out = run(emu_bin, "file", path, variant, p, R)
     assert " ok" in out and "MISMATCH" not in out
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_kernel_kinematics_bit_identical(emu_bin, seed):
+    """The kernel's straight-line common kinematics path and its fall-back give
+    the same bits as the shared scalar fjmi_rap/fjmi_phi/fjmi_weight, on edge
+    cases (zeros, subnormals, infinities, NaN, E == |pz|, underflowing kt^2)
+    and on random momenta over 160 binary decades."""
+    out = run(emu_bin, "kin", 100000, seed)
+    assert "kinematics: 0 of" in out
