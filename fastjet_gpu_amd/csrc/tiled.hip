@@ -47,8 +47,21 @@ constexpr int MAXG = 32;    /* tiles gathered per step (3 x 3x3 + 1 <= 28) */
 constexpr int MAXO = 128;   /* orphans listed per step (more: rescan in rounds) */
 constexpr int BIG = 0x7fffffff;
 
+/* A tile minimum's tie-break key: the minimum jet's label in the high half and
+ * its nearest neighbour's index (0xFFFF: none) in the low half, offset so that
+ * signed order is label order (labels are distinct; events hold at most
+ * kTiledMaxN particles).  The merge reads the partner straight from the key. */
+__device__ __forceinline__ int tkey(int label, int nn) {
+  return (int)((((unsigned)label << 16) | ((unsigned)nn & 0xFFFFu)) ^ 0x80000000u);
+}
+__device__ __forceinline__ int tkey_nn(int key) {
+  const int v = key & 0xFFFF;
+  return v == 0xFFFF ? -1 : v;
+}
+
 /* (diJ, label) minimum of a tile: diJ as its bit pattern (diJ >= 0, so the bits
- * order like the values), the label, and the jet's physical index. */
+ * order like the values), the key tkey(label, nearest neighbour), and the jet's
+ * physical index.  Empty tile: d = ~0, l = BIG, p = -1. */
 struct TMin {
   unsigned long long d;
   int l;
@@ -67,7 +80,7 @@ template <int MAXT>
 struct Shared {
   TMin tmin[MAXT];
   unsigned long long mt_d[W]; /* member table of a single-chunk gather: diJ bits, */
-  int mt_l[W];                /* labels */
+  int mt_l[W];                /* keys tkey(label, nearest neighbour) */
   int mt_p[W];                /* and physical indices, by gather index */
   int seg_start[MAXT], seg_count[MAXT], seg_cap[MAXT];
   Gather g[2]; /* [0]: the step's gather, [1]: orphan rescans */
@@ -305,15 +318,16 @@ __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, in
     for (int j0 = 0; j0 < total; j0 += W) {
       const int p = gathered(sh, G, ev, ng, j0 + lane);
       if (p >= 0) {
-        const int t = ev.rec[p].meta.z;
+        const int4 m = ev.rec[p].meta;
+        const int t = m.z;
         const unsigned long long bits = (unsigned long long)__double_as_longlong(ev.rec[p].dij);
         if (phase == 0) {
           atomicMin(&sh.tmin[t].d, bits);
         } else if (sh.tmin[t].d == bits) {
-          const int lab = ev.rec[p].meta.y;
+          const int key = tkey(m.y, m.x);
           if (phase == 1)
-            atomicMin(&sh.tmin[t].l, lab);
-          else if (sh.tmin[t].l == lab)
+            atomicMin(&sh.tmin[t].l, key);
+          else if (sh.tmin[t].l == key)
             sh.tmin[t].p = p;
         }
       }
@@ -325,11 +339,11 @@ __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, in
 /* Tile minima of the gathered tiles from the member table (gathers of at most
  * 64 members): lane j < total wrote member j; lane g < ng scans tile g's range. */
 template <class SH>
-__device__ void refresh_from_table(SH &sh, const Gather &G, int ng, int total, int lane, int myq, int mylab,
+__device__ void refresh_from_table(SH &sh, const Gather &G, int ng, int total, int lane, int myq, int mykey,
                                    unsigned long long mybits) {
   if (lane < total) {
     sh.mt_d[lane] = mybits;
-    sh.mt_l[lane] = mylab;
+    sh.mt_l[lane] = mykey;
     sh.mt_p[lane] = myq;
   }
   wave_sync();
@@ -831,7 +845,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         for (int k = 0; k < NB / 2; ++k) {
           const int jk = min(j + k, j1 - 1);
           dq[k] = (unsigned long long)__double_as_longlong(ev.rec[jk].dij);
-          lq[k] = ev.rec[jk].meta.y;
+          lq[k] = tkey(ev.rec[jk].meta.y, ev.rec[jk].meta.x);
         }
 #pragma unroll
         for (int k = 0; k < NB / 2; ++k) {
@@ -892,19 +906,19 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       const int wl = wave_argmin(bd, bl, &dmin, &lA);
       const int tA = wl < 0 ? -1 : uni(__builtin_amdgcn_readlane(bt, wl));
       const int A = tA < 0 ? -1 : uni(sh.tmin[tA].p);
+      const int B = tA < 0 ? -1 : uni(tkey_nn(lA));
       if (A < 0) { /* no live jet found: inconsistent state, stop this event */
         if (lane == 0) atomicOr(a.status, 2u);
         bad = true;
         break;
       }
-      /* 2. the merge: the loads that depend only on A, then only on B, are
-       * issued together */
+      /* 2. the merge: the partner B comes with the tile minimum, so both jets'
+       * records load at once */
       const int pt = uni(pt_raw);
       const int tile_pt_raw = ev.rec[pt].meta.z;
       const int4 mA = ev.rec[A].meta;
       const double4 qA = ev.p4[A];
       const int hA = ev.hist[A];
-      const int B = uni(mA.x);
       int4 mB = make_int4(-1, -1, -1, -1);
       double4 qB = make_double4(0.0, 0.0, 0.0, 0.0);
       int hB = -1;
@@ -1033,7 +1047,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       /* a gather of at most 64 members keeps each member's (index, label, diJ)
        * in its lane for the tile-minimum refresh */
       const bool single = total <= W;
-      int myq = -1, mylab = BIG, mytile = -1;
+      int myq = -1, mylab = BIG, mytile = -1, mynn = -1;
       unsigned long long mybits = ~0ull;
       double2 myrp = make_double2(0.0, 0.0);
       double myw = 0.0;
@@ -1047,6 +1061,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           myq = q;
           mylab = mq0.y;
           mytile = mq0.z;
+          mynn = mq0.x;
           myrp = r.rp;
           myw = r.wn.x;
         }
@@ -1063,6 +1078,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
               ev.rec[q].meta.x = phys_lo;
               ev.rec[q].wn = make_double2(wq.x, d);
               ev.rec[q].dij = dq;
+              mynn = phys_lo;
             }
             if (d < R2 && (d < cbd || (d == cbd && mq.y < cbl))) {
               cbd = d;
@@ -1096,7 +1112,10 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           ev.rec[phys_lo].wn = make_double2(nw, md);
           ev.rec[phys_lo].dij = dB;
         }
-        if (myq == phys_lo) mybits = (unsigned long long)__double_as_longlong(dB);
+        if (myq == phys_lo) {
+          mybits = (unsigned long long)__double_as_longlong(dB);
+          mynn = nnB;
+        }
       }
       wave_sync();
 
@@ -1144,7 +1163,10 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
             ev.rec[o].wn = make_double2(ow, od);
             ev.rec[o].dij = dO;
           }
-          if (lane == lo) mybits = (unsigned long long)__double_as_longlong(dO);
+          if (lane == lo) {
+            mybits = (unsigned long long)__double_as_longlong(dO);
+            mynn = nno;
+          }
         }
         wave_sync();
       } else if (norph > 0) {
@@ -1163,7 +1185,10 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
             ev.rec[o].wn = make_double2(wo, od);
             ev.rec[o].dij = dO;
           }
-          if (myq == o) mybits = (unsigned long long)__double_as_longlong(dO);
+          if (myq == o) {
+            mybits = (unsigned long long)__double_as_longlong(dO);
+            mynn = nno;
+          }
           wave_sync();
         }
         if (norph > MAXO) { /* more orphans than the list holds: rescan the rest from the gather */
@@ -1197,7 +1222,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
 
       /* 5. tile minima of every gathered tile */
       if (single && norph <= MAXO)
-        refresh_from_table(sh, G, ng, total, lane, myq, mylab, mybits);
+        refresh_from_table(sh, G, ng, total, lane, myq, tkey(mylab, mynn), mybits);
       else
         refresh_minima(sh, G, ev, ng, lane);
     }
