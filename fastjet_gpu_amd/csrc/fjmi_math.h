@@ -41,7 +41,11 @@
 #define FJMI_MAXRAP 1e5
 
 #if defined(__HIP_DEVICE_COMPILE__)
-#define FJMI_TABLE static __constant__
+/* constexpr, not __constant__: a __constant__ variable stays visible to the
+ * host runtime, so device code reaches it through the GOT (one more dependent
+ * scalar load per table lookup); a constexpr table is a private read-only
+ * global addressed relative to the program counter */
+#define FJMI_TABLE static constexpr
 #else
 #define FJMI_TABLE static
 #endif

@@ -72,6 +72,7 @@ struct TMin {
 struct Gather {
   int tiles[MAXG];
   int pref[MAXG + 1];
+  int gpos[W]; /* pool position of member j < W (further members: search pref) */
 };
 
 /* Per-workgroup state of the event in flight; MAXT = tiles per event (the
@@ -88,7 +89,6 @@ struct Shared {
   int norph;
   int gcount;
   int stamp[MAXT]; /* last gather that listed the tile (dedup) */
-  int stampv;
   int pool_top;
   int rebuild;
   int ev;
@@ -371,12 +371,14 @@ __device__ void refresh_from_table(SH &sh, const Gather &G, int ng, int total, i
 /* Sets G.tiles to the distinct union of the 3x3 neighbourhoods of the given
  * tiles (centres < 0 are skipped) plus `extra` (if >= 0), and G.pref to the
  * prefix sums of their member counts.  Lane i < 27 proposes neighbour i%9 of
- * centre i/9 (c0, c1, c2); duplicates are dropped through a per-tile stamp. */
+ * centre i/9 (c0, c1, c2); duplicates are dropped through a per-tile stamp:
+ * stampv, the wave's gather counter (kept in a register), is advanced and
+ * written into every listed tile's stamp. */
 template <class SH>
-__device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, int c2, int extra, int lane) {
-  if (lane == 0) sh.stampv += 1;
-  wave_sync();
-  const int stampv = uni(sh.stampv);
+__device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, int c2, int extra, int lane,
+                            int &stampv) {
+  stampv += 1;
+  wave_sync(); /* the callers' LDS writes (segment counts) come first */
   int cand = -1;
   if (lane < 27) {
     const int c = lane < 9 ? c0 : (lane < 18 ? c1 : c2);
@@ -393,10 +395,12 @@ __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, i
   const unsigned long long kb = __ballot(keep);
   const int idx = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(kb >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)kb, 0u));
   const int cnt = keep ? sh.seg_count[cand] : 0;
+  const int st = keep ? sh.seg_start[cand] : 0;
   const int incl = wave_incl_scan(cnt, lane);
   if (keep) {
     G.tiles[idx] = cand;
     G.pref[idx + 1] = incl;
+    for (int u = 0, j = incl - cnt; u < cnt && j < W; ++u, ++j) G.gpos[j] = st + u;
   }
   if (lane == 0) G.pref[0] = 0;
   wave_sync();
@@ -407,6 +411,7 @@ __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, i
 template <class SH>
 __device__ __forceinline__ int gathered(const SH &sh, const Gather &G, const Ev &ev, int ng, int j, int *tile) {
   if (j >= G.pref[ng]) return -1;
+  if (!tile && j < W) return ev.pool[G.gpos[j]];
   int lo = 0, hi = ng - 1; /* largest g with g_pref[g] <= j */
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -580,7 +585,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
   __shared__ Shared<MAXT> sh;
   const int lane = threadIdx.x;
   for (int t = lane; t < MAXT; t += W) sh.stamp[t] = 0;
-  if (lane == 0) sh.stampv = 0;
+  int stampv = 0; /* gather counter; the stamps start below it */
   for (;;) {
     wave_sync();
     if (lane == 0) {
@@ -912,10 +917,10 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         bad = true;
         break;
       }
-      /* 2. the merge: the partner B comes with the tile minimum, so both jets'
-       * records load at once */
-      const int pt = uni(pt_raw);
-      const int tile_pt_raw = ev.rec[pt].meta.z;
+      /* 2. the merge.  The partner B comes with the tile minimum, so both
+       * jets' records load at once.  Every value loaded for the merge is taken
+       * into registers before the merge's first store: a wait for a load also
+       * waits for every store issued before it. */
       const int4 mA = ev.rec[A].meta;
       const double4 qA = ev.p4[A];
       const int hA = ev.hist[A];
@@ -927,13 +932,16 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         qB = ev.p4[B];
         hB = ev.hist[B];
       }
+      const int pt = uni(pt_raw);
+      const int tile_pt_raw = ev.rec[pt].meta.z;
+      double4 qt_raw = make_double4(0.0, 0.0, 0.0, 0.0);
+      if (a.pj_out) qt_raw = ev.p4[pt];
       const bool a_hi = B < 0 || uni(mA.y) > uni(mB.y);
       const int phys_hi = a_hi ? A : B;
       const int phys_lo = B < 0 ? -1 : (a_hi ? B : A);
       const int lab_hi = uni(a_hi ? mA.y : mB.y);
       const int tile_hi = uni(a_hi ? mA.z : mB.z);
       const int tile_lo_old = phys_lo < 0 ? -1 : uni(a_hi ? mB.z : mA.z);
-      const int tile_pt = uni(tile_pt_raw);
       /* Segment removals swap the removed jet with its segment's last entry.  The
        * pool entries that may be needed are loaded now, by every lane, so that
        * their latency overlaps the new jet's kinematics:
@@ -957,16 +965,27 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       int tile_new = -1;
       const double4 qhi = a_hi ? qA : qB;
       const int hhi = a_hi ? hA : hB;
+      const double4 qlo = a_hi ? qB : qA;
+      const int hlo = a_hi ? hB : hA;
+      const double4 qn = make_double4(qhi.x + qlo.x, qhi.y + qlo.y, qhi.z + qlo.z, qhi.w + qlo.w);
       if (phys_lo >= 0) {
-        const double4 qlo = a_hi ? qB : qA;
-        const int hlo = a_hi ? hB : hA;
-        const double4 qn = make_double4(qhi.x + qlo.x, qhi.y + qlo.y, qhi.z + qlo.z, qhi.w + qlo.w);
         const double4 c = kin_coords_t(qn.x, qn.y, qn.z, qn.w, p);
         nrap = c.x;
         nphi = c.y;
         nw = c.z;
         tile_new = uni(geo.tile(nrap, nphi));
-        if (lane == 0) {
+      }
+      const int tile_pt = uni(tile_pt_raw);
+      const int q_last_hi = uni(q_last_hi_raw);
+      const int q_last2_v = uni(q_last2_raw);
+      const double4 qt = qt_raw;
+
+      /* the stores: the history entry and the new jet (or the beam jet), then
+       * removal 1 and the N2Plain compaction (the last label moves), then, if
+       * the merged jet changes tile, removal 2 (from the segment as removal 1
+       * left it) and its insertion */
+      if (lane == 0) {
+        if (phys_lo >= 0) {
           a.hist_p1[base + step] = min(hhi, hlo);
           a.hist_p2[base + step] = max(hhi, hlo);
           a.hist_dij[base + step] = dmin * invR2;
@@ -975,22 +994,16 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           ev.rec[phys_lo].rp = make_double2(nrap, nphi);
           ev.rec[phys_lo].wn = make_double2(nw, R2);
           ev.rec[phys_lo].meta.x = -1;
+        } else {
+          a.hist_p1[base + step] = hhi;
+          a.hist_p2[base + step] = -1;
+          a.hist_dij[base + step] = dmin * invR2;
+          double *jo = a.jets + 4 * (base + nj);
+          jo[0] = qhi.x;
+          jo[1] = qhi.y;
+          jo[2] = qhi.z;
+          jo[3] = qhi.w;
         }
-      } else if (lane == 0) {
-        a.hist_p1[base + step] = hhi;
-        a.hist_p2[base + step] = -1;
-        a.hist_dij[base + step] = dmin * invR2;
-        double *jo = a.jets + 4 * (base + nj);
-        jo[0] = qhi.x;
-        jo[1] = qhi.y;
-        jo[2] = qhi.z;
-        jo[3] = qhi.w;
-      }
-      /* removal 1 and the N2Plain compaction (the last label moves), then, if
-       * the merged jet changes tile, removal 2 (from the segment as removal 1
-       * left it) and its insertion */
-      const int q_last_hi = uni(q_last_hi_raw);
-      if (lane == 0) {
         if (pos_hi != last_hi) {
           ev.pool[pos_hi] = q_last_hi;
           ev.rec[q_last_hi].meta.w = pos_hi;
@@ -1004,7 +1017,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         if (phys_lo >= 0 && tile_new != tile_lo_old) {
           const bool moved = same_tile && pos_lo == last_hi && pos_hi != last_hi;
           const int pos2 = moved ? pos_hi : pos_lo;
-          const int q_last2 = (same_tile && last2 == pos_hi) ? q_last_hi : q_last2_raw;
+          const int q_last2 = (same_tile && last2 == pos_hi) ? q_last_hi : q_last2_v;
           if (pos2 != last2) {
             ev.pool[pos2] = q_last2;
             ev.rec[q_last2].meta.w = pos2;
@@ -1013,10 +1026,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           ev.rec[phys_lo].meta.z = tile_new;
           seg_insert(sh, ev, tile_new, phys_lo);
         }
-      }
-      if (lane == 0) {
         if (a.pj_out) {
-          const double4 qt = ev.p4[pt];
           unsigned char *rec = a.pj_out + 40 * (size_t)(base + tail);
           reinterpret_cast<int *>(rec)[0] = tail;
           reinterpret_cast<int *>(rec)[1] = phys_lo < 0 ? 1 : 0;
@@ -1037,9 +1047,9 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       }
 
       /* 3. gather the 3x3 tiles around the old and new positions and the moved jet's tile */
-      const int ng = build_gather(sh, G, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane);
+      const int ng = build_gather(sh, G, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane, stampv);
       const int total = G.pref[ng];
-      const int main_stamp = uni(sh.stampv);
+      const int main_stamp = stampv;
       if (lane == 0) sh.norph = 0;
       wave_sync();
       double cbd = R2, cbw = 0.0;
@@ -1154,7 +1164,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
             nno = ln < 0 ? -1 : __builtin_amdgcn_readlane(myq, ln);
             owb = ln < 0 ? 0.0 : readlane_d(myw, ln);
           } else {
-            const int ngo = build_gather(sh, GO, geo, otile, -1, -1, -1, lane);
+            const int ngo = build_gather(sh, GO, geo, otile, -1, -1, -1, lane, stampv);
             nno = nn_scan(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &od, &owb);
           }
           const double dO = diJ_of(od, ow, nno, owb);
@@ -1176,7 +1186,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           const int4 mo = ev.rec[o].meta;
           const double2 ro = ev.rec[o].rp;
           const double wo = ev.rec[o].wn.x;
-          const int ngo = build_gather(sh, GO, geo, mo.z, -1, -1, -1, lane);
+          const int ngo = build_gather(sh, GO, geo, mo.z, -1, -1, -1, lane, stampv);
           double od, owb;
           const int nno = nn_scan(sh, GO, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
           const double dO = diJ_of(od, wo, nno, owb);
@@ -1206,7 +1216,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
               const int4 mo = ev.rec[o].meta;
               const double2 ro = ev.rec[o].rp;
               const double wo = ev.rec[o].wn.x;
-              const int ngo = build_gather(sh, GO, geo, mo.z, -1, -1, -1, lane);
+              const int ngo = build_gather(sh, GO, geo, mo.z, -1, -1, -1, lane, stampv);
               double od, owb;
               const int nno = nn_scan(sh, GO, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
               if (lane == 0) {

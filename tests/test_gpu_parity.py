@@ -108,16 +108,24 @@ def test_high_multiplicity_small_tile_table(engine, var):
 
 @pytest.mark.gpu
 def test_every_bucket_size(engine):
-    """One event per multiplicity bucket boundary (and just above it)."""
+    """One event per multiplicity bucket boundary (and just above it), under the
+    library's table and under a mixed dense/tiled table."""
     import fastjet_gpu_amd as fj
     rng = np.random.default_rng(7)
     big = fj.gen_event(4, 3)
-    for (max_n, eng, threads, slots, g) in engine.buckets():
-        for n in sorted({max(1, max_n - 1), max_n, max_n + 1}):
-            if n > len(big):
-                continue
-            sel = big[rng.permutation(len(big))[:n]]
-            check_event(engine, sel, -1, 0.4, f"bucket<= {max_n} (engine {eng} {threads}x{slots},g{g})")
+    saved = engine.buckets()
+    mixed = [(64, 1, 64, 1, 0), (256, 1, 64, 4, 0), (1024, 2, 64, 5, 3), (1000000000, 2, 64, 4, 0)]
+    try:
+        for table in (saved, mixed):
+            engine.set_buckets(table)
+            for (max_n, eng, threads, slots, g) in table:
+                for n in sorted({max(1, max_n - 1), max_n, max_n + 1}):
+                    if n > len(big):
+                        continue
+                    sel = big[rng.permutation(len(big))[:n]]
+                    check_event(engine, sel, -1, 0.4, f"bucket<= {max_n} (engine {eng} {threads}x{slots},g{g})")
+    finally:
+        engine.set_buckets(saved)
 
 
 @pytest.mark.gpu
