@@ -17,20 +17,23 @@
  *     pointer fix-up.
  *
  * One wave (64 lanes) clusters one event at a time; persistent workgroups pull
- * events from the queue of LaunchArgs.  Per-jet records live in global memory
- * (structure of arrays, indexed by particle); per tile, the lexicographic
- * (diJ, label) minimum and the member segment descriptor live in LDS.
+ * events from the queue of LaunchArgs.  Per-jet records live in global memory,
+ * laid out in tile order when the event starts (a tile's members, and a row of
+ * adjacent tiles, are contiguous); per tile, the lexicographic (diJ, label)
+ * minimum and the member segment descriptor live in LDS.
  *
  * Per step (n live jets):
- *   1. argmin over the tile minima (LDS scan + DPP wave reduction);
+ *   1. argmin over the tile minima (LDS loads, a per-lane tree, a DPP wave
+ *      reduction); the winner's tie-break key also names its partner;
  *   2. the merge (E-scheme, kinematics of the new jet) and the structural
  *      updates (tile membership, relabelling) by lane 0;
  *   3. one gather of the 3x3 tiles around the two old positions and the new one:
  *      orphans (jets whose nearest neighbour was removed or merged), jets now
  *      closer to the new jet, and the new jet's own nearest neighbour;
  *   4. a 3x3 rescan per orphan;
- *   5. the minima of the gathered tiles recomputed with LDS atomic min
- *      (diJ bits, then label, then the winner's index).
+ *   5. the minima of the gathered tiles recomputed from a member table in LDS
+ *      (or, for large gathers, with LDS atomic minima: diJ bits, then key, then
+ *      the winner's index).
  */
 #include <hip/hip_runtime.h>
 
