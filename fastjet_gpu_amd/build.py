@@ -77,7 +77,7 @@ def build(jobs=4, cli=True, oracle=True):
 def build_cli():
     main_o = os.path.join(OBJ, "cli_main.o")
     run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "cli_main.cpp"), "-o", main_o])
-    # tri_matrix: the bucket table's choice (dense for small events, tiled above);
+    # tri_matrix: the bucket table's choice per event size;
     # n_array: dense only; grid: tiled only; tri_matrix_empty: per-step launch floor
     for name, engine in (("tri_matrix", "FJMI_ENGINE_AUTO"), ("n_array", "FJMI_ENGINE_DENSE"),
                          ("grid", "FJMI_ENGINE_TILED"), ("tri_matrix_empty", "FJMI_ENGINE_EMPTY")):
