@@ -115,24 +115,30 @@ struct alignas(64) JetRec {
 };
 
 /* Global per-particle scratch of the tiled engine (indexed by particle like the
- * input), plus the tile-segment pool: 3 entries per particle and pool_per_event
- * per event (event e's pool starts at 3*offsets[e] + e*pool_per_event). */
+ * input), plus the tile-segment pool: kTiledPoolPerParticle entries per particle
+ * and pool_per_event per event (event e's pool starts at 3*offsets[e] +
+ * e*pool_per_event).  The label map and the pool hold particle indices of one
+ * event, below kTiledMaxN = 65535, in 16 bits. */
 struct TiledScratch {
-  JetRec *rec;  /* 1 per particle */
-  double *p4;   /* 4 per particle */
-  int *hist;    /* 1 per particle */
-  int *lab2;    /* 1 per particle: label -> particle */
-  int *pool;
+  JetRec *rec;           /* 1 per particle */
+  double *p4;            /* 4 per particle */
+  int *hist;             /* 1 per particle */
+  unsigned short *lab2;  /* 1 per particle: label -> particle */
+  unsigned short *pool;  /* tile segments */
   int pool_per_event;
 };
 
+constexpr size_t kTiledPoolPerParticle = 3;
+constexpr size_t kTiledPoolEntryBytes = sizeof(unsigned short);
+
 /* Bytes of tiled scratch per particle: record, four-momentum, history index,
- * label map and 3 pool entries. */
-constexpr size_t kTiledBytesPerParticle = sizeof(JetRec) + 32 + 4 + 4 + 12;
+ * label map and its pool entries. */
+constexpr size_t kTiledBytesPerParticle =
+    sizeof(JetRec) + 32 + 4 + sizeof(unsigned short) + kTiledPoolPerParticle * kTiledPoolEntryBytes;
 
 /* Bytes of tiled scratch for `particles` particles in `events` events. */
 inline size_t tiled_scratch_bytes(int64_t particles, int64_t events, int pool_per_event) {
-  return (size_t)particles * kTiledBytesPerParticle + (size_t)events * pool_per_event * 4 + 1024;
+  return (size_t)particles * kTiledBytesPerParticle + (size_t)events * pool_per_event * kTiledPoolEntryBytes + 1024;
 }
 
 /* Carves a TiledScratch out of one allocation of tiled_scratch_bytes() (the
@@ -147,9 +153,9 @@ inline TiledScratch tiled_scratch_carve(void *mem, int64_t particles, int pool_p
   c += 32 * P;
   t.hist = reinterpret_cast<int *>(c);
   c += 4 * P;
-  t.lab2 = reinterpret_cast<int *>(c);
-  c += 4 * P;
-  t.pool = reinterpret_cast<int *>(c);
+  t.lab2 = reinterpret_cast<unsigned short *>(c);
+  c += sizeof(unsigned short) * P;
+  t.pool = reinterpret_cast<unsigned short *>(c);
   t.pool_per_event = pool_per_event;
   return t;
 }

@@ -92,9 +92,10 @@ class RangeAlloc {
   std::set<std::pair<int64_t, int64_t>> by_size_;
 };
 
-constexpr int64_t kDeviceBytesPerParticle =
-    (int64_t)fjmi::kTiledBytesPerParticle - 12 + 16 + 32; /* tiled scratch less its pool + history + jets */
-constexpr int64_t kPoolPerParticle = 3;                    /* tile-segment pool entries */
+constexpr int64_t kPoolPerParticle = (int64_t)fjmi::kTiledPoolPerParticle; /* tile-segment pool entries */
+constexpr int64_t kPoolEntryBytes = (int64_t)fjmi::kTiledPoolEntryBytes;
+constexpr int64_t kDeviceBytesPerParticle = /* tiled scratch less its pool + history + jets */
+    (int64_t)fjmi::kTiledBytesPerParticle - kPoolPerParticle * kPoolEntryBytes + 16 + 32;
 constexpr int64_t kHostBytesPerParticle = 32 + 16 + 32;    /* input + result history + result jets */
 constexpr int64_t kHostBytesPerSlot = sizeof(fjmi::ServeDesc) + 4 + 4;
 
@@ -204,9 +205,9 @@ int fjmi_server_create(fjmi_context *ctx, const fjmi_server_config *cfg_in, fjmi
     return fail(FJMI_EINVAL, "tile_q is the tile side in quarters of R, 4..16 (0 = 4)");
   if (E > (1 << 30)) return fail(FJMI_EINVAL, "max_events too large");
   const int64_t ppe = fjmi::kTiledPoolPerEvent;
-  const int64_t fixed_dev = E * ppe * 4 + 4096;
+  const int64_t fixed_dev = E * ppe * kPoolEntryBytes + 4096;
   const int64_t fixed_host = E * kHostBytesPerSlot + 4096;
-  const int64_t P_dev = (dev_budget - fixed_dev) / (kDeviceBytesPerParticle + 4 * kPoolPerParticle);
+  const int64_t P_dev = (dev_budget - fixed_dev) / (kDeviceBytesPerParticle + kPoolEntryBytes * kPoolPerParticle);
   const int64_t P_host = (host_budget - fixed_host) / kHostBytesPerParticle;
   const int64_t P = std::min(P_dev, P_host);
   if (P < 64) return fail(FJMI_EINVAL, "budgets too small for %lld event slots", (long long)E);

@@ -90,7 +90,6 @@ struct Shared {
   Gather g[2]; /* [0]: the step's gather, [1]: orphan rescans */
   int orph[MAXO];
   int norph;
-  int gcount;
   int stamp[MAXT]; /* last gather that listed the tile (dedup) */
   int pool_top;
   int rebuild;
@@ -216,8 +215,8 @@ struct Ev {
   JetRec *rec;  /* per-jet record: everything a gather or a rescan reads */
   double4 *p4;
   int *hist;
-  int *lab2;    /* label -> physical */
-  int *pool;    /* tile segments */
+  unsigned short *lab2; /* label -> physical */
+  unsigned short *pool; /* tile segments */
   int pool_size;
 };
 
@@ -299,8 +298,7 @@ __device__ void rebuild_segments(SH &sh, const Ev &ev, int n, int ntiles, int la
 }
 
 template <class SH>
-__device__ __forceinline__ int gathered(const SH &sh, const Gather &G, const Ev &ev, int ng, int j,
-                                        int *tile = nullptr);
+__device__ __forceinline__ int gathered(const SH &sh, const Gather &G, const Ev &ev, int ng, int j);
 
 /* Recomputes the (diJ, label) minimum of tiles listed in G.tiles[0..ng) from
  * all their members. */
@@ -412,9 +410,9 @@ __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, i
 
 /* Member j of the gathered tiles: physical index (or -1 past the end). */
 template <class SH>
-__device__ __forceinline__ int gathered(const SH &sh, const Gather &G, const Ev &ev, int ng, int j, int *tile) {
+__device__ __forceinline__ int gathered(const SH &sh, const Gather &G, const Ev &ev, int ng, int j) {
   if (j >= G.pref[ng]) return -1;
-  if (!tile && j < W) return ev.pool[G.gpos[j]];
+  if (j < W) return ev.pool[G.gpos[j]];
   int lo = 0, hi = ng - 1; /* largest g with g_pref[g] <= j */
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -424,7 +422,6 @@ __device__ __forceinline__ int gathered(const SH &sh, const Gather &G, const Ev 
       hi = mid - 1;
   }
   const int t = G.tiles[lo];
-  if (tile) *tile = t;
   return ev.pool[sh.seg_start[t] + (j - G.pref[lo])];
 }
 

@@ -22,7 +22,8 @@ static int run_event(const double *p4, int N, int p, double R, int tile_q, bool 
   using namespace fjmi;
   std::vector<JetRec> rec(N > 0 ? N : 1);
   std::vector<double> p4s(4 * (size_t)(N > 0 ? N : 1));
-  std::vector<int> hist(N > 0 ? N : 1), lab2(N > 0 ? N : 1), pool(3 * (size_t)N + kTiledPoolPerEvent);
+  std::vector<int> hist(N > 0 ? N : 1);
+  std::vector<unsigned short> lab2(N > 0 ? N : 1), pool(3 * (size_t)N + kTiledPoolPerEvent);
   TiledScratch ts{rec.data(), p4s.data(), hist.data(), lab2.data(), pool.data(), kTiledPoolPerEvent};
   std::vector<int32_t> h1(N > 0 ? N : 1, -7), h2(N > 0 ? N : 1, -7);
   std::vector<double> hd(N > 0 ? N : 1, -7.0), jets(4 * (size_t)(N > 0 ? N : 1), 0.0);
