@@ -83,9 +83,15 @@ struct Gather {
 template <int MAXT>
 struct Shared {
   TMin tmin[MAXT];
-  unsigned long long mt_d[W]; /* member table of a single-chunk gather: diJ bits, */
-  int mt_l[W];                /* keys tkey(label, nearest neighbour) */
-  int mt_p[W];                /* and physical indices, by gather index */
+  /* Member table.  Single-chunk gathers: every member's diJ bits, key
+   * tkey(label, nearest neighbour) and physical index, by gather index.
+   * Multi-chunk gathers: the nchg jets whose entry changed in the step, with
+   * their tiles (mt_t). */
+  unsigned long long mt_d[W];
+  int mt_l[W];
+  int mt_p[W];
+  int mt_t[W];
+  int nchg;
   int seg_start[MAXT], seg_count[MAXT], seg_cap[MAXT];
   Gather g[2]; /* [0]: the step's gather, [1]: orphan rescans */
   int orph[MAXO];
@@ -369,6 +375,48 @@ __device__ void refresh_from_table(SH &sh, const Gather &G, int ng, int total, i
   wave_sync();
 }
 
+template <class SH>
+__device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, int &stampv);
+
+/* Tile minima of a several-chunk gather, incrementally.  The step changed the
+ * entries of the jets listed in the member table (nchg of them: jets that took
+ * the merged jet as their neighbour, the relabelled jet, the rescanned orphans,
+ * the merged jet in its new tile) and removed phys_hi.  A tile whose minimum
+ * jet is one of these is recomputed from all its members (GF gathers them):
+ * its entry may have risen (an orphan's neighbour is farther; for kt the
+ * merged jet's weight is larger) or left the tile.  In every other tile the
+ * old minimum still holds against the unchanged members, so the listed
+ * entries enter the tiles' minima by comparison. */
+template <class SH>
+__device__ void refresh_changed(SH &sh, const Gather &G, Gather &GF, const Ev &ev, int ng, int phys_hi,
+                                int lane, int &stampv) {
+  int cand = -1;
+  if (lane < ng) {
+    const int t = G.tiles[lane];
+    const int pm = sh.tmin[t].p;
+    bool full = pm >= 0 && pm == phys_hi;
+    const int nc = sh.nchg;
+    for (int i = 0; i < nc && !full; ++i) full = sh.mt_p[i] == pm;
+    if (full) cand = t;
+  }
+  const int nf = gather_tiles(sh, GF, cand, lane, stampv);
+  if (nf > 0) refresh_minima(sh, GF, ev, nf, lane);
+  if (lane == 0) {
+    const int nc = sh.nchg;
+    for (int i = 0; i < nc; ++i) {
+      const int t = sh.mt_t[i];
+      const unsigned long long d = sh.mt_d[i];
+      const int k = sh.mt_l[i];
+      if (d < sh.tmin[t].d || (d == sh.tmin[t].d && k < sh.tmin[t].l)) {
+        sh.tmin[t].d = d;
+        sh.tmin[t].l = k;
+        sh.tmin[t].p = sh.mt_p[i];
+      }
+    }
+  }
+  wave_sync();
+}
+
 /* Sets G.tiles to the distinct union of the 3x3 neighbourhoods of the given
  * tiles (centres < 0 are skipped) plus `extra` (if >= 0), and G.pref to the
  * prefix sums of their member counts.  Lane i < 27 proposes neighbour i%9 of
@@ -378,8 +426,6 @@ __device__ void refresh_from_table(SH &sh, const Gather &G, int ng, int total, i
 template <class SH>
 __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, int c2, int extra, int lane,
                             int &stampv) {
-  stampv += 1;
-  wave_sync(); /* the callers' LDS writes (segment counts) come first */
   int cand = -1;
   if (lane < 27) {
     const int c = lane < 9 ? c0 : (lane < 18 ? c1 : c2);
@@ -391,6 +437,14 @@ __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, i
   } else if (lane == 27) {
     cand = extra;
   }
+  return gather_tiles(sh, G, cand, lane, stampv);
+}
+
+/* The gather of the tiles the lanes propose (cand >= 0; duplicates dropped). */
+template <class SH>
+__device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, int &stampv) {
+  stampv += 1;
+  wave_sync(); /* the callers' LDS writes (segment counts) come first */
   bool keep = false;
   if (cand >= 0) keep = atomicExch(&sh.stamp[cand], stampv) != stampv;
   const unsigned long long kb = __ballot(keep);
@@ -1050,7 +1104,10 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       const int ng = build_gather(sh, G, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane, stampv);
       const int total = G.pref[ng];
       const int main_stamp = stampv;
-      if (lane == 0) sh.norph = 0;
+      if (lane == 0) {
+        sh.norph = 0;
+        sh.nchg = 0;
+      }
       wave_sync();
       double cbd = R2, cbw = 0.0;
       int cbl = BIG, cbp = -1;
@@ -1075,10 +1132,13 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           myrp = r.rp;
           myw = r.wn.x;
         }
+        bool chg = false; /* multi-chunk gathers: the member's tile-minimum entry changed */
+        int chg_key = 0, chg_tile = -1;
         if (q >= 0 && q != phys_lo) {
           int4 mq = ev.rec[q].meta;
           double dq = ev.rec[q].dij;
           orph = (mq.x == phys_hi) || (phys_lo >= 0 && mq.x == phys_lo);
+          chg = q == pt && !orph; /* relabelled */
           if (phys_lo >= 0) {
             const double2 rq = ev.rec[q].rp;
             const double2 wq = ev.rec[q].wn;
@@ -1089,6 +1149,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
               ev.rec[q].wn = make_double2(wq.x, d);
               ev.rec[q].dij = dq;
               mynn = phys_lo;
+              chg = true;
             }
             if (d < R2 && (d < cbd || (d == cbd && mq.y < cbl))) {
               cbd = d;
@@ -1098,6 +1159,24 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
             }
           }
           mybits = (unsigned long long)__double_as_longlong(dq);
+          chg_key = tkey(mq.y, mynn);
+          chg_tile = mq.z;
+        }
+        if (!single) { /* list the changed entries for the incremental refresh */
+          const unsigned long long cb = __ballot(chg);
+          if (cb) {
+            int cbase = 0;
+            if (lane == 0) cbase = atomicAdd(&sh.nchg, (int)__popcll(cb));
+            cbase = uni(cbase);
+            const int idx = cbase + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(cb >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)cb, 0u));
+            if (chg && idx < W) {
+              sh.mt_d[idx] = mybits;
+              sh.mt_l[idx] = chg_key;
+              sh.mt_p[idx] = q;
+              sh.mt_t[idx] = chg_tile;
+            }
+          }
         }
         const unsigned long long ob = __ballot(orph);
         orph_lanes = ob;
@@ -1125,6 +1204,17 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         if (myq == phys_lo) {
           mybits = (unsigned long long)__double_as_longlong(dB);
           mynn = nnB;
+        }
+        const int lab_lo = phys_lo == pt ? lab_hi : uni(a_hi ? mB.y : mA.y); /* after the relabelling */
+        if (!single && lane == 0) {
+          const int i = sh.nchg;
+          sh.nchg = i + 1;
+          if (i < W) {
+            sh.mt_d[i] = (unsigned long long)__double_as_longlong(dB);
+            sh.mt_l[i] = tkey(lab_lo, nnB);
+            sh.mt_p[i] = phys_lo;
+            sh.mt_t[i] = tile_new;
+          }
         }
       }
       wave_sync();
@@ -1194,6 +1284,14 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
             ev.rec[o].meta.x = nno;
             ev.rec[o].wn = make_double2(wo, od);
             ev.rec[o].dij = dO;
+            const int c = sh.nchg;
+            sh.nchg = c + 1;
+            if (c < W) {
+              sh.mt_d[c] = (unsigned long long)__double_as_longlong(dO);
+              sh.mt_l[c] = tkey(mo.y, nno);
+              sh.mt_p[c] = o;
+              sh.mt_t[c] = mo.z;
+            }
           }
           if (myq == o) {
             mybits = (unsigned long long)__double_as_longlong(dO);
@@ -1230,9 +1328,13 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         }
       }
 
-      /* 5. tile minima of every gathered tile */
+      /* 5. tile minima of the gathered tiles: from the member table (single
+       * chunk), incrementally from the changed entries (several chunks), or
+       * from all members */
       if (single && norph <= MAXO)
         refresh_from_table(sh, G, ng, total, lane, myq, tkey(mylab, mynn), mybits);
+      else if (norph <= MAXO && uni(sh.nchg) <= W)
+        refresh_changed(sh, G, GO, ev, ng, phys_hi, lane, stampv);
       else
         refresh_minima(sh, G, ev, ng, lane);
     }

@@ -83,6 +83,24 @@ def test_high_multiplicity_tiled(engine, p, alg):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("p,alg", [(1, "kt"), (-1, "antikt")])
+def test_high_pileup_batch(engine, p, alg):
+    """The first 24 events of workload 4 (N ~ U[5000, 20000]) in one batch, each
+    bit-exact against the oracle: gathers of several chunks and their
+    incremental tile-minimum refresh (for kt, the merged jet's larger weight can
+    raise a neighbour's diJ; event 17 exercises it)."""
+    import fastjet_gpu_amd as fj
+    p4, off = fj.gen_workload(4, 24)
+    print(f"[gpu parity] workload4 batch of 24 events, {len(p4)} particles, {alg}", flush=True)
+    h1, h2, hd, jets, nj = engine.cluster_batch(p4, off, p, 0.4)
+    for e in range(24):
+        a, b = off[e], off[e + 1]
+        want = orc.tiled(p4[a:b], p, 0.4)
+        got = (h1[a:b], h2[a:b], hd[a:b], jets[a:a + nj[e]])
+        assert same(want, got), f"event {e}: {first_diff(want, got)}"
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("var", [2, 3])
 def test_high_multiplicity_small_tile_table(engine, var):
     """Workload-4 events through a batch on the 256- and 128-tile kernels (tiles

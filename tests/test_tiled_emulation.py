@@ -41,6 +41,9 @@ def run(emu_bin, *args):
 @pytest.mark.parametrize("workload,e0,e1,variant,p,R", [
     (1, 0, 1, 0, -1, 0.4), (1, 0, 1, 0, 0, 0.4), (1, 0, 1, 0, 1, 0.4),
     (2, 0, 1, 0, 1, 1.0), (2, 1, 2, 3, 0, 0.8), (3, 0, 8, 1, -1, 0.4), (3, 8, 16, 2, 1, 0.6),
+    # high pileup: gathers of several chunks and the incremental tile-minimum refresh; for kt
+    # (event 17, N = 9867) a jet that takes the merged jet as its neighbour can get a larger diJ
+    (4, 1, 2, 0, -1, 0.4), (4, 17, 18, 0, 1, 0.4),
 ])
 def test_generated_events(emu_bin, workload, e0, e1, variant, p, R):
     out = run(emu_bin, workload, e0, e1, variant, p, R, 0)
