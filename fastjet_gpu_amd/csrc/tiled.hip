@@ -318,9 +318,37 @@ __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, in
   }
   wave_sync();
   const int total = G.pref[ng];
-  /* three passes over all members: the diJ minimum, then the lowest label among
+  /* three passes over all members: the diJ minimum, then the lowest key among
    * members at that minimum, then its index (each pass must see the final result
-   * of the previous one across every chunk of 64 members) */
+   * of the previous one across every chunk of 64 members).  Up to two chunks
+   * are loaded once and kept in registers; larger gathers reload per pass. */
+  if (total <= 2 * W) {
+    unsigned long long bits[2];
+    int key[2], t[2], p[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) p[c] = gathered(sh, G, ev, ng, c * W + lane);
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      if (p[c] >= 0) {
+        const int4 m = ev.rec[p[c]].meta;
+        t[c] = m.z;
+        key[c] = tkey(m.y, m.x);
+        bits[c] = (unsigned long long)__double_as_longlong(ev.rec[p[c]].dij);
+      }
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      if (p[c] >= 0) atomicMin(&sh.tmin[t[c]].d, bits[c]);
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      if (p[c] >= 0 && sh.tmin[t[c]].d == bits[c]) atomicMin(&sh.tmin[t[c]].l, key[c]);
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      if (p[c] >= 0 && sh.tmin[t[c]].d == bits[c] && sh.tmin[t[c]].l == key[c]) sh.tmin[t[c]].p = p[c];
+    wave_sync();
+    return;
+  }
   for (int phase = 0; phase < 3; ++phase) {
     for (int j0 = 0; j0 < total; j0 += W) {
       const int p = gathered(sh, G, ev, ng, j0 + lane);
@@ -508,26 +536,41 @@ __device__ void seg_insert(SH &sh, const Ev &ev, int t, int p) {
 
 /* Lexicographic (distance, label) nearest neighbour of physical o among the
  * gathered tiles (all lanes). Returns the winner's physical index or -1. */
-template <class SH>
+template <int NCH, class SH>
 __device__ int nn_scan(const SH &sh, const Gather &G, const Ev &ev, int ng, int o, double orap, double ophi, double R2,
                        int lane,
                        double *best_d, double *best_w) {
   const int total = G.pref[ng];
   double bd = R2, bw = 0.0;
   int bl = BIG, bp = -1;
-  for (int j0 = 0; j0 < total; j0 += W) {
-    const int q = gathered(sh, G, ev, ng, j0 + lane);
-    if (q >= 0 && q != o) {
-      const double2 rp = ev.rec[q].rp;
-      const double d = fjmi_dr2(orap, ophi, rp.x, rp.y);
-      const int lab = ev.rec[q].meta.y;
-      if (d < R2 && (d < bd || (d == bd && lab < bl))) {
-        bd = d;
-        bl = lab;
-        bp = q;
-        bw = ev.rec[q].wn.x;
+  /* NC chunks of 64 members at a time: their pool entries load together,
+   * then their records (one chunk at a time where registers are short) */
+  constexpr int NC = NCH;
+  for (int j0 = 0; j0 < total; j0 += NC * W) {
+    int q[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) q[c] = j0 + c * W < total ? gathered(sh, G, ev, ng, j0 + c * W + lane) : -1;
+    double2 rp[NC];
+    double wq[NC];
+    int lab[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (q[c] >= 0) {
+        rp[c] = ev.rec[q[c]].rp;
+        wq[c] = ev.rec[q[c]].wn.x;
+        lab[c] = ev.rec[q[c]].meta.y;
       }
-    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (q[c] >= 0 && q[c] != o) {
+        const double d = fjmi_dr2(orap, ophi, rp[c].x, rp[c].y);
+        if (d < R2 && (d < bd || (d == bd && lab[c] < bl))) {
+          bd = d;
+          bl = lab[c];
+          bp = q[c];
+          bw = wq[c];
+        }
+      }
   }
   double md;
   int ml;
@@ -721,6 +764,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
     /* loads in flight per lane in the prologue's passes (PB particles) and
      * scans (NB candidates); fewer where more waves share the registers */
     constexpr int PB = WPE <= 2 ? 4 : (WPE == 3 ? 2 : 1);
+    constexpr int kScanChunks = WPE <= 2 ? 4 : 1; /* chunks per batch in the orphan rescans */
     constexpr int NB = WPE <= 2 ? 8 : (WPE == 3 ? 4 : 2);
     double ylo = 1e300, yhi = -1e300;
     /* the next particle's momentum is loaded before this one's kinematics */
@@ -1119,8 +1163,10 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       double2 myrp = make_double2(0.0, 0.0);
       double myw = 0.0;
       unsigned long long orph_lanes = 0; /* single-chunk gathers: lanes holding orphans */
+      int q_next = gathered(sh, G, ev, ng, lane);
       for (int j0 = 0; j0 < total; j0 += W) {
-        const int q = gathered(sh, G, ev, ng, j0 + lane);
+        const int q = q_next;
+        if (j0 + W < total) q_next = gathered(sh, G, ev, ng, j0 + W + lane); /* the next chunk's pool entries load meanwhile */
         bool orph = false;
         if (q >= 0) { /* every member's record, kept in its lane for the orphan rescans */
           const JetRec &r = ev.rec[q];
@@ -1255,7 +1301,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
             owb = ln < 0 ? 0.0 : readlane_d(myw, ln);
           } else {
             const int ngo = build_gather(sh, GO, geo, otile, -1, -1, -1, lane, stampv);
-            nno = nn_scan(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &od, &owb);
+            nno = nn_scan<kScanChunks>(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &od, &owb);
           }
           const double dO = diJ_of(od, ow, nno, owb);
           if (lane == 0) {
@@ -1278,7 +1324,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           const double wo = ev.rec[o].wn.x;
           const int ngo = build_gather(sh, GO, geo, mo.z, -1, -1, -1, lane, stampv);
           double od, owb;
-          const int nno = nn_scan(sh, GO, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
+          const int nno = nn_scan<kScanChunks>(sh, GO, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
           const double dO = diJ_of(od, wo, nno, owb);
           if (lane == 0) {
             ev.rec[o].meta.x = nno;
@@ -1316,7 +1362,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
               const double wo = ev.rec[o].wn.x;
               const int ngo = build_gather(sh, GO, geo, mo.z, -1, -1, -1, lane, stampv);
               double od, owb;
-              const int nno = nn_scan(sh, GO, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
+              const int nno = nn_scan<kScanChunks>(sh, GO, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
               if (lane == 0) {
                 ev.rec[o].meta.x = nno;
                 ev.rec[o].wn = make_double2(wo, od);
