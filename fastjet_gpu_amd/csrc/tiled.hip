@@ -429,18 +429,31 @@ __device__ void refresh_changed(SH &sh, const Gather &G, Gather &GF, const Ev &e
   }
   const int nf = gather_tiles(sh, GF, cand, lane, stampv);
   if (nf > 0) refresh_minima(sh, GF, ev, nf, lane);
-  if (lane == 0) {
-    const int nc = sh.nchg;
-    for (int i = 0; i < nc; ++i) {
-      const int t = sh.mt_t[i];
-      const unsigned long long d = sh.mt_d[i];
-      const int k = sh.mt_l[i];
-      if (d < sh.tmin[t].d || (d == sh.tmin[t].d && k < sh.tmin[t].l)) {
-        sh.tmin[t].d = d;
-        sh.tmin[t].l = k;
-        sh.tmin[t].p = sh.mt_p[i];
-      }
-    }
+  /* lane i holds listed entry i; the best entry of each tile (every lane
+   * compares its entry with all listed ones) then meets the tile's minimum */
+  const int nc = uni(sh.nchg);
+  const bool mine = lane < nc;
+  unsigned long long d = ~0ull;
+  int k = BIG, t = -1, pe = -1;
+  if (mine) {
+    d = sh.mt_d[lane];
+    k = sh.mt_l[lane];
+    t = sh.mt_t[lane];
+    pe = sh.mt_p[lane];
+  }
+  bool best = mine;
+  for (int i = 0; i < nc; ++i) {
+    const int ti = __builtin_amdgcn_readlane(t, i);
+    const unsigned long long di = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(d >> 32), i) << 32) |
+                                  (unsigned)__builtin_amdgcn_readlane((int)d, i);
+    const int ki = __builtin_amdgcn_readlane(k, i);
+    /* another entry of the same tile that is smaller, or equal and listed first */
+    if (ti == t && i != lane && (di < d || (di == d && (ki < k || (ki == k && i < lane))))) best = false;
+  }
+  if (best && (d < sh.tmin[t].d || (d == sh.tmin[t].d && k < sh.tmin[t].l))) {
+    sh.tmin[t].d = d;
+    sh.tmin[t].l = k;
+    sh.tmin[t].p = pe;
   }
   wave_sync();
 }
