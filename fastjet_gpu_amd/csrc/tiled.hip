@@ -1157,7 +1157,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         rebuild_segments(sh, ev, n, ntiles, lane);
       }
 
-      /* 3. gather the 3x3 tiles around the old and new positions and the moved jet's tile */
+      /* 3. gather the 3x3 tiles around the old and new positions and the relabelled jet's tile */
       const int ng = build_gather(sh, G, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane, stampv);
       const int total = G.pref[ng];
       const int main_stamp = stampv;
