@@ -253,6 +253,27 @@ def test_batch_matches_oracle(engine):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("workload,ne,first", [(3, 512, 3000), (5, 128, 500), (2, 64, 10)])
+def test_batch_mixed_algorithms_and_radii(engine, workload, ne, first):
+    """One batch whose events each draw their algorithm from {kt, C/A, anti-kt} and R from
+    {0.2, 0.4, 0.7, 1.0, 1.5} (seeded): every event bit-exact against the oracle."""
+    import fastjet_gpu_amd as fj
+    p4, off = fj.gen_workload(workload, ne, first=first)
+    rng = np.random.default_rng(11 + workload)
+    ps = rng.choice([-1, 0, 1], size=ne).astype(np.int32)
+    Rs = rng.choice([0.2, 0.4, 0.7, 1.0, 1.5], size=ne).astype(np.float64)
+    print(f"[gpu parity] workload {workload} events {first}..{first + ne - 1}, {len(p4)} particles, mixed p and R",
+          flush=True)
+    h1, h2, hd, jets, nj = engine.cluster_batch(p4, off, ps, Rs)
+    for e in range(ne):
+        a, b = off[e], off[e + 1]
+        pe, re = int(ps[e]), float(Rs[e])
+        want = orc.n2plain(p4[a:b], pe, re) if b - a <= 3000 else orc.tiled(p4[a:b], pe, re)
+        got = (h1[a:b], h2[a:b], hd[a:b], jets[a:a + nj[e]])
+        assert same(want, got), f"event {first + e} (N={b - a}, p={pe}, R={re}): {first_diff(want, got)}"
+
+
+@pytest.mark.gpu
 def test_batch_alone_equals_in_mix(engine):
     """Each event's result is the same clustered alone and inside a batch."""
     import fastjet_gpu_amd as fj
