@@ -1292,23 +1292,36 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           const int o = __builtin_amdgcn_readlane(myq, lo);
           const int otile = __builtin_amdgcn_readlane(mytile, lo);
           const double oy = readlane_d(myrp.x, lo), ophi = readlane_d(myrp.y, lo), ow = readlane_d(myw, lo);
-          bool outside = false;
+          /* the nearest gathered candidate, from the lanes' registers */
+          const bool cand = myq >= 0 && myq != o;
+          const double dc = cand ? fjmi_dr2(oy, ophi, myrp.x, myrp.y) : R2;
+          const bool in = cand && dc < R2;
+          double md;
+          int ml;
+          const int ln = wave_argmin(in ? dc : R2, in ? mylab : BIG, &md, &ml);
+          /* It is the orphan's nearest neighbour unless a tile of its 3x3
+           * neighbourhood that the gather missed could hold a jet at distance
+           * <= md: lane i < 9 bounds from below the distance to neighbour tile
+           * i (the orphan's distance to its own tile's edges on that side, less
+           * a margin far above the rounding of the tile boundaries). */
+          bool open = false;
           if (lane < 9) {
-            const int y = otile / geo.nphi + lane / 3 - 1;
+            const int ty = otile / geo.nphi, tp = otile % geo.nphi, dy = lane / 3 - 1, dp = lane % 3 - 1;
+            const int y = ty + dy;
             if (y >= 0 && y < geo.ny) {
-              const int t = y * geo.nphi + (otile % geo.nphi + lane % 3 - 1 + geo.nphi) % geo.nphi;
-              outside = sh.stamp[t] != main_stamp;
+              const int t = y * geo.nphi + (tp + dp + geo.nphi) % geo.nphi;
+              if (sh.stamp[t] != main_stamp) {
+                double by = dy < 0 ? oy - (geo.y0 + ty * geo.sy) : (dy > 0 ? geo.y0 + (ty + 1) * geo.sy - oy : 0.0);
+                double bp = dp < 0 ? ophi - tp * geo.sphi : (dp > 0 ? (tp + 1) * geo.sphi - ophi : 0.0);
+                by = dy != 0 ? (by - 1e-10 > 0.0 ? by - 1e-10 : 0.0) : 0.0;
+                bp = dp != 0 ? (bp - 1e-10 > 0.0 ? bp - 1e-10 : 0.0) : 0.0;
+                open = !(by * by + bp * bp > md);
+              }
             }
           }
           double od, owb;
           int nno;
-          if (__ballot(outside) == 0) {
-            const bool cand = myq >= 0 && myq != o;
-            const double d = cand ? fjmi_dr2(oy, ophi, myrp.x, myrp.y) : R2;
-            const bool in = cand && d < R2;
-            double md;
-            int ml;
-            const int ln = wave_argmin(in ? d : R2, in ? mylab : BIG, &md, &ml);
+          if (__ballot(open) == 0) {
             od = md;
             nno = ln < 0 ? -1 : __builtin_amdgcn_readlane(myq, ln);
             owb = ln < 0 ? 0.0 : readlane_d(myw, ln);
