@@ -170,7 +170,7 @@ def main():
     dom_flops = flops(dom["sum_n2"], dom["particles"])
     achieved = dom_flops / (dom["ms"] * 1e-3) / 1e12 if dom["ms"] > 0 else None
     all_flops = flops(float(np.sum(sizes.astype(np.float64) ** 2)), float(tot))
-    variants = {0: "tiled_kernel<512, 2, false>", 1: "tiled_kernel<256, 3, false>", 2: "tiled_kernel<256, 4, false>",
+    variants = {0: "tiled_kernel<512, 2, false>", 1: "tiled_kernel<384, 3, false>", 2: "tiled_kernel<256, 4, false>",
                 3: "tiled_kernel<128, 4, false>"}
     traffic = measured_traffic(variants.get(dom["p4_global"], "?")) if dom["engine"] == 2 and ne == 4096 else None
 

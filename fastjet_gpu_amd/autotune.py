@@ -8,16 +8,16 @@ current GPU, and keeps the fastest:
                 configuration whose capacity threads*slots is at least hi
                 (and at most 16*hi, larger ones only waste LDS and lanes);
   tiled engine  tile side R, 1.25 R and 1.5 R, with each compiled kernel
-                variant (tile tables of 512, 256 or 128 tiles, and so 8, 12 or
-                16 events per CU).
+                variant (tile tables of 512, 384, 256 or 128 tiles, and so 8,
+                10, 16 or 16 events per CU).
 
 Adjacent intervals with the same winner are merged; the last row is always a
 tiled row covering every larger event.
 
 A second, whole-batch stage then times complete tables on a mixed batch (by
-default the 4096-event N ~ U[50, 2000] workload): the merged per-interval table
-and simpler ones (the per-interval dense winners up to a cut, then one tiled
-row), and keeps the fastest.  Per-interval throughput does not decide alone:
+default the 4096-event N ~ U[50, 2000] workload): each tiled variant alone, the
+merged per-interval table and simpler ones (the per-interval dense winners up
+to a cut, then the best tiled variant), and keeps the fastest.  Per-interval throughput does not decide alone:
 in a mixed batch the kernels of all buckets share the GPU, the batch ends with
 its largest events, and every distinct kernel competes for instruction cache.
 
@@ -144,14 +144,18 @@ def merge(choices):
     return [tuple(r) for r in rows]
 
 
-def simple_tables(choices):
-    """Dense per-interval winners up to a cut, then the tiled winner of the largest interval."""
-    tiled = [c for _, c in choices if c[0] == 2]
-    last = tiled[-1] if tiled else (2, 64, 0, 0)
+def tiled_alone(configs):
+    """One tiled row for every event, per compiled kernel variant (tile side R)."""
+    variants = sorted({G for (cap, eng, T, K, G) in configs if eng == 2})
+    return [(f"tiled variant {v} alone", [(LAST_ROW, 2, 64, 4, v)]) for v in variants]
+
+
+def simple_tables(choices, last):
+    """Dense per-interval winners up to a cut, then the tiled row `last` (engine, threads, slots, variant)."""
     out = []
-    for cut in (0, 128, 256, 512):
+    for cut in (128, 256, 512):
         dense = [(hi, c) for hi, c in choices if hi <= cut and c[0] == 1]
-        if cut and (not dense or dense[-1][0] != cut):
+        if not dense or dense[-1][0] != cut:
             continue
         rows = [r for r in merge(dense + [(LAST_ROW, last)])]
         out.append((f"dense<= {cut} + tiled", rows))
@@ -245,9 +249,20 @@ def main(argv=None):
         # whole-batch stage
         p4, off = fj.gen_workload(a.batch_workload, a.batch_events)
         d = device_buffers(torch, dev, p4, off)
-        tables = [("per-interval winners", merge(choices))] + simple_tables(choices)
-        tables.append(("library default", saved))
         whole = []
+        alone = tiled_alone(configs)
+        for name, rows in alone:
+            rate = time_table(eng, rows, p4, off, d, a.p, a.R, a.reps)
+            whole.append(dict(name=name, table=[list(r) for r in rows], events_per_s=rate))
+            print(f"batch of {len(off) - 1} (workload {a.batch_workload}): {name:32s} {rate:10.0f} ev/s  {rows}",
+                  flush=True)
+        last = tuple(max(whole, key=lambda w: w["events_per_s"])["table"][0][1:])
+        tables = [("per-interval winners", merge(choices))] + simple_tables(choices, last)
+        if len(choices) > 1 and choices[-1][1][0] == 2 and choices[-1][1] != last:
+            # the largest events keep their own winner (the whole batch rarely has them)
+            tables.append((f"best tiled, largest interval's above {choices[-2][0]}",
+                           merge([(choices[-2][0], last), (LAST_ROW, choices[-1][1])])))
+        tables.append(("library default", saved))
         for name, rows in tables:
             rate = time_table(eng, rows, p4, off, d, a.p, a.R, a.reps)
             whole.append(dict(name=name, table=[list(r) for r in rows], events_per_s=rate))

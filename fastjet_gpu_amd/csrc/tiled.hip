@@ -692,6 +692,7 @@ __device__ __forceinline__ void serve_finish(const LaunchArgs &a, int slot, int6
  * event's results are copied to host memory and announced on completion. */
 template <int MAXT, int WPE, bool SERVE>
 __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScratch ts) {
+  static_assert(MAXT % W == 0, "the argmin loads MAXT / W tile minima per lane");
   __shared__ Shared<MAXT> sh;
   const int lane = threadIdx.x;
   for (int t = lane; t < MAXT; t += W) sh.stamp[t] = 0;
@@ -1418,15 +1419,16 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
   }
 }
 
-/* 512 tiles: LDS holds 8 workgroups per CU; 256 tiles: registers hold 3 waves
- * per SIMD, or 4 with a few spills; 128 tiles: 4 waves per SIMD. */
+/* 512 tiles: LDS holds 8 workgroups per CU; 384 tiles (25 rows of 15 at
+ * R = 0.4, |y| < 5): 10 workgroups per CU, registers for 3 waves per SIMD;
+ * 256 tiles: 4 waves per SIMD with a few spills; 128 tiles: 4 waves per SIMD. */
 const void *const kTiledKernels[kNumTiledKernels] = {
-    reinterpret_cast<const void *>(&tiled_kernel<512, 2, false>), reinterpret_cast<const void *>(&tiled_kernel<256, 3, false>),
+    reinterpret_cast<const void *>(&tiled_kernel<512, 2, false>), reinterpret_cast<const void *>(&tiled_kernel<384, 3, false>),
     reinterpret_cast<const void *>(&tiled_kernel<256, 4, false>), reinterpret_cast<const void *>(&tiled_kernel<128, 4, false>)};
 /* serving launches: the same variants, events from the host-published ring */
 const void *const kTiledServeKernels[kNumTiledKernels] = {
-    reinterpret_cast<const void *>(&tiled_kernel<512, 2, true>), reinterpret_cast<const void *>(&tiled_kernel<256, 3, true>),
+    reinterpret_cast<const void *>(&tiled_kernel<512, 2, true>), reinterpret_cast<const void *>(&tiled_kernel<384, 3, true>),
     reinterpret_cast<const void *>(&tiled_kernel<256, 4, true>), reinterpret_cast<const void *>(&tiled_kernel<128, 4, true>)};
-const int kTiledMaxTiles[kNumTiledKernels] = {512, 256, 256, 128};
+const int kTiledMaxTiles[kNumTiledKernels] = {512, 384, 256, 128};
 
 }  // namespace fjmi

@@ -83,7 +83,7 @@ static int run_file(const char *path, int variant, int p, double R) {
   const int n = (int)(p4.size() / 4);
   switch (variant) {
     case 0: return run_event<512, 2>(p4.data(), n, p, R, 0, true);
-    case 1: return run_event<256, 3>(p4.data(), n, p, R, 0, true);
+    case 1: return run_event<384, 3>(p4.data(), n, p, R, 0, true);
     case 2: return run_event<256, 4>(p4.data(), n, p, R, 0, true);
     default: return run_event<128, 4>(p4.data(), n, p, R, 0, true);
   }
@@ -157,7 +157,7 @@ int main(int argc, char **argv) {
     int b = 0;
     switch (variant) {
       case 0: b = run_event<512, 2>(p4.data(), n, p, R, tq, false); break;
-      case 1: b = run_event<256, 3>(p4.data(), n, p, R, tq, false); break;
+      case 1: b = run_event<384, 3>(p4.data(), n, p, R, tq, false); break;
       case 2: b = run_event<256, 4>(p4.data(), n, p, R, tq, false); break;
       default: b = run_event<128, 4>(p4.data(), n, p, R, tq, false); break;
     }
