@@ -274,7 +274,7 @@ class Server:
     event slots are full; poll() returns the completed events (results copied
     into numpy arrays, then released)."""
 
-    def __init__(self, eng, device_budget=0, host_budget=0, max_events=0, variant=0, tile_q=0, idle_us=0):
+    def __init__(self, eng, device_budget=0, host_budget=0, max_events=0, variant=1, tile_q=0, idle_us=0):
         self.eng = eng
         cfg = ServerConfig(device_budget, host_budget, max_events, variant, tile_q, idle_us)
         h = _vp()

@@ -166,7 +166,9 @@ typedef struct {
   int64_t device_budget; /* bytes of device memory for event state (0: 16 GiB) */
   int64_t host_budget;   /* bytes of pinned host memory for inputs and results (0: 8 GiB) */
   int32_t max_events;    /* event slots, i.e. events admitted at once (0: 65536) */
-  int32_t variant;       /* tiled kernel variant, as the p4_global field of a tiled bucket row */
+  int32_t variant;       /* tiled kernel variant, as the p4_global field of a tiled bucket row
+                          * (1 serves the most events per second; 0 has a slightly shorter
+                          * per-event latency at light load) */
   int32_t tile_q;        /* tile side in quarters of R (0: R) */
   int32_t idle_us;       /* microseconds without any claimed event before the GPU workgroups leave (0: 20000) */
 } fjmi_server_config;

@@ -44,18 +44,20 @@ def drive(srv, requests, deadline_s=120.0):
 
 
 @pytest.mark.gpu
-def test_serving_matches_oracle(engine):
+@pytest.mark.parametrize("variant", [1, 0])
+def test_serving_matches_oracle(engine, variant):
     """Mixed sizes (workload 3 and lognormal workload-5 events), per-event algorithm
-    and radius, a budget small enough that admission has to wait."""
+    and radius, a budget small enough that admission has to wait; the default
+    tiled kernel variant (384 tiles) and the 512-tile one."""
     import fastjet_gpu_amd as fj
     reqs = []
     for k in range(160):
         ev = fj.gen_event(3, 2000 + k) if k % 4 else fj.gen_event(5, k)
         p, R, _ = fj.gen_serving_attrs(k)
         reqs.append((k, ev, p, R))
-    print(f"[gpu serving] {len(reqs)} events, sizes {min(len(r[1]) for r in reqs)}..{max(len(r[1]) for r in reqs)}",
-          flush=True)
-    srv = engine.server(device_budget=48 << 20, host_budget=24 << 20, max_events=48)
+    print(f"[gpu serving] variant {variant}: {len(reqs)} events, sizes {min(len(r[1]) for r in reqs)}.."
+          f"{max(len(r[1]) for r in reqs)}", flush=True)
+    srv = engine.server(device_budget=48 << 20, host_budget=24 << 20, max_events=48, variant=variant)
     try:
         done, rejected = drive(srv, reqs)
         info = srv.info()
