@@ -83,10 +83,9 @@ struct Gather {
 template <int MAXT>
 struct Shared {
   TMin tmin[MAXT];
-  /* Member table.  Single-chunk gathers: every member's diJ bits, key
-   * tkey(label, nearest neighbour) and physical index, by gather index.
-   * Multi-chunk gathers: the nchg jets whose entry changed in the step, with
-   * their tiles (mt_t). */
+  /* Changed-entry table of multi-chunk gathers: the nchg jets whose entry
+   * changed in the step, with their diJ bits, key tkey(label, nearest
+   * neighbour), physical index and tile. */
   unsigned long long mt_d[W];
   int mt_l[W];
   int mt_p[W];
@@ -371,35 +370,27 @@ __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, in
   }
 }
 
-/* Tile minima of the gathered tiles from the member table (gathers of at most
- * 64 members): lane j < total wrote member j; lane g < ng scans tile g's range. */
+/* Tile minima of the gathered tiles (gathers of at most 64 members) from the
+ * members' entries in their lanes' registers: lane j < total holds member j
+ * (physical myq in tile mytile).  The lexicographic (diJ, key) minimum per tile
+ * in three passes of LDS atomics, as refresh_minima does with loaded entries. */
 template <class SH>
-__device__ void refresh_from_table(SH &sh, const Gather &G, int ng, int total, int lane, int myq, int mykey,
-                                   unsigned long long mybits) {
-  if (lane < total) {
-    sh.mt_d[lane] = mybits;
-    sh.mt_l[lane] = mykey;
-    sh.mt_p[lane] = myq;
-  }
-  wave_sync();
+__device__ void refresh_from_regs(SH &sh, const Gather &G, int ng, int total, int lane, int myq, int mytile,
+                                  int mykey, unsigned long long mybits) {
   if (lane < ng) {
     const int t = G.tiles[lane];
-    const int j1 = G.pref[lane + 1];
-    TMin best;
-    best.d = ~0ull;
-    best.l = BIG;
-    best.p = -1;
-    for (int j = G.pref[lane]; j < j1; ++j) {
-      const unsigned long long d = sh.mt_d[j];
-      const int l = sh.mt_l[j];
-      if (d < best.d || (d == best.d && l < best.l)) {
-        best.d = d;
-        best.l = l;
-        best.p = sh.mt_p[j];
-      }
-    }
-    sh.tmin[t] = best;
+    sh.tmin[t].d = ~0ull;
+    sh.tmin[t].l = BIG;
+    sh.tmin[t].p = -1;
   }
+  wave_sync();
+  const bool mem = lane < total;
+  if (mem) atomicMin(&sh.tmin[mytile].d, mybits);
+  wave_sync();
+  const bool atd = mem && sh.tmin[mytile].d == mybits;
+  if (atd) atomicMin(&sh.tmin[mytile].l, mykey);
+  wave_sync();
+  if (atd && sh.tmin[mytile].l == mykey) sh.tmin[mytile].p = myq;
   wave_sync();
 }
 
@@ -1240,7 +1231,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         }
         const unsigned long long ob = __ballot(orph);
         orph_lanes = ob;
-        if (ob) {
+        if (!single && ob) { /* single-chunk gathers keep their orphans in orph_lanes */
           int obase = 0;
           if (lane == 0) obase = atomicAdd(&sh.norph, (int)__popcll(ob));
           obase = uni(obase);
@@ -1284,7 +1275,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
        * are already in the lanes' registers: one distance per lane and one
        * wave argmin (gathered jets outside the neighbourhood are farther than
        * R and drop out).  Otherwise the neighbourhood is gathered afresh. */
-      const int norph = uni(sh.norph);
+      const int norph = single ? (int)__popcll(orph_lanes) : uni(sh.norph);
       if (norph > 0 && single) {
         unsigned long long pend = orph_lanes;
         while (pend) {
@@ -1405,7 +1396,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
        * chunk), incrementally from the changed entries (several chunks), or
        * from all members */
       if (single && norph <= MAXO)
-        refresh_from_table(sh, G, ng, total, lane, myq, tkey(mylab, mynn), mybits);
+        refresh_from_regs(sh, G, ng, total, lane, myq, mytile, tkey(mylab, mynn), mybits);
       else if (norph <= MAXO && uni(sh.nchg) <= W)
         refresh_changed(sh, G, GO, ev, ng, phys_hi, lane, stampv);
       else
