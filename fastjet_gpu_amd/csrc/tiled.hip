@@ -197,7 +197,7 @@ __device__ __forceinline__ int wave_incl_scan(int v, int) {
 #endif
 __device__ __forceinline__ void wave_sync() {
 #if FJMI_TILED_WG_FENCES
-  wave_sync();
+  __syncthreads();
 #else
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 #endif
@@ -206,13 +206,27 @@ __device__ __forceinline__ void wave_sync() {
 struct Geo {
   int ny, nphi;
   double y0, sy, sphi;
+  double rsy, rsphi; /* 1 / sy, 1 / sphi */
+  float rnphi;       /* 1 / nphi */
+  /* A tile boundary may fall either way within rounding of the reciprocals; the
+   * tiles' side exceeds R by far more, so every 3x3 neighbourhood still holds
+   * every jet within R of its centre tile's members. */
   __device__ int tile(double y, double phi) const {
-    const double fy = (y - y0) / sy;
+    const double fy = (y - y0) * rsy;
     const int ty = fy < 0.0 ? 0 : (fy >= (double)ny ? ny - 1 : (int)fy);
-    int tp = (int)(phi / sphi);
+    int tp = (int)(phi * rsphi);
     tp = tp >= nphi ? nphi - 1 : (tp < 0 ? 0 : tp);
     return ty * nphi + tp;
   }
+  /* row of tile t: an FP32 quotient estimate (off by less than one for
+   * t < 2^16), corrected once */
+  __device__ int row(int t) const {
+    const int ty = (int)(((float)t + 0.5f) * rnphi);
+    const int tp = t - ty * nphi;
+    return tp < 0 ? ty - 1 : (tp >= nphi ? ty + 1 : ty);
+  }
+  /* column tp + dp, |dp| <= 1, wrapped in phi */
+  __device__ int wrap(int tp) const { return tp < 0 ? tp + nphi : (tp >= nphi ? tp - nphi : tp); }
 };
 
 /* The per-event views of the global scratch. */
@@ -462,9 +476,9 @@ __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, i
   if (lane < 27) {
     const int c = lane < 9 ? c0 : (lane < 18 ? c1 : c2);
     if (c >= 0) {
-      const int ty = c / geo.nphi, tp = c % geo.nphi;
+      const int ty = geo.row(c), tp = c - ty * geo.nphi;
       const int y = ty + (lane % 9) / 3 - 1;
-      if (y >= 0 && y < geo.ny) cand = y * geo.nphi + (tp + (lane % 9) % 3 - 1 + geo.nphi) % geo.nphi;
+      if (y >= 0 && y < geo.ny) cand = y * geo.nphi + geo.wrap(tp + (lane % 9) % 3 - 1);
     }
   } else if (lane == 27) {
     cand = extra;
@@ -815,6 +829,9 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
     geo.y0 = uni_d(sh.y0);
     geo.sy = uni_d(sh.sy);
     geo.sphi = uni_d(sh.sphi);
+    geo.rsy = 1.0 / geo.sy;
+    geo.rsphi = 1.0 / geo.sphi;
+    geo.rnphi = 1.0f / (float)geo.nphi;
     const int ntiles = geo.ny * geo.nphi;
     for (int t = lane; t < ntiles; t += W) sh.seg_count[t] = 0;
     wave_sync();
@@ -890,7 +907,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       const double2 rp = ev.rec[i].rp;
       const double wi = ev.rec[i].wn.x;
       const int ti = ev.rec[i].meta.z;
-      const int ty0 = ti / geo.nphi, tp0 = ti - ty0 * geo.nphi;
+      const int ty0 = geo.row(ti), tp0 = ti - ty0 * geo.nphi;
       double bd = R2;
       int bl = BIG, bp = -1;
       for (int r = 0; r < 6; ++r) {
@@ -1298,10 +1315,10 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
            * a margin far above the rounding of the tile boundaries). */
           bool open = false;
           if (lane < 9) {
-            const int ty = otile / geo.nphi, tp = otile % geo.nphi, dy = lane / 3 - 1, dp = lane % 3 - 1;
+            const int ty = geo.row(otile), tp = otile - ty * geo.nphi, dy = lane / 3 - 1, dp = lane % 3 - 1;
             const int y = ty + dy;
             if (y >= 0 && y < geo.ny) {
-              const int t = y * geo.nphi + (tp + dp + geo.nphi) % geo.nphi;
+              const int t = y * geo.nphi + geo.wrap(tp + dp);
               if (sh.stamp[t] != main_stamp) {
                 double by = dy < 0 ? oy - (geo.y0 + ty * geo.sy) : (dy > 0 ? geo.y0 + (ty + 1) * geo.sy - oy : 0.0);
                 double bp = dp < 0 ? ophi - tp * geo.sphi : (dp > 0 ? (tp + 1) * geo.sphi - ophi : 0.0);
