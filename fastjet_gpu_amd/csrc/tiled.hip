@@ -276,11 +276,26 @@ __device__ __forceinline__ double diJ_of(double nnd, double w, int nn, double wn
   return nnd * ((nn >= 0 && wnn < w) ? wnn : w);
 }
 
+/* Segment capacity = count + (count >> FJMI_TILED_SLACK_SHIFT) +
+ * FJMI_TILED_SLACK_ADD: 50 % slack plus two.  Tests build with less slack (and
+ * a small pool) so that segments relocate and the pool runs out;
+ * FJMI_TILED_ON_REBUILD() lets them count the rebuilds that follow. */
+#ifndef FJMI_TILED_SLACK_SHIFT
+#define FJMI_TILED_SLACK_SHIFT 1
+#endif
+#ifndef FJMI_TILED_SLACK_ADD
+#define FJMI_TILED_SLACK_ADD 2
+#endif
+#ifndef FJMI_TILED_ON_REBUILD
+#define FJMI_TILED_ON_REBUILD() ((void)0)
+#endif
+
 /* Rebuilds every tile segment from the live jets (labels 0..n-1): counting
- * sort into the pool with 50% slack.  Used at the start of an event and when a
- * relocation finds the pool exhausted. */
+ * sort into the pool with slack.  Used when a relocation finds the pool
+ * exhausted. */
 template <class SH>
 __device__ void rebuild_segments(SH &sh, const Ev &ev, int n, int ntiles, int lane) {
+  if (lane == 0) FJMI_TILED_ON_REBUILD();
   for (int t = lane; t < ntiles; t += W) sh.seg_count[t] = 0;
   wave_sync();
   for (int l = lane; l < n; l += W) {
@@ -292,7 +307,7 @@ __device__ void rebuild_segments(SH &sh, const Ev &ev, int n, int ntiles, int la
   for (int t0 = 0; t0 < ntiles; t0 += W) {
     const int t = t0 + lane;
     const int c = t < ntiles ? sh.seg_count[t] : 0;
-    const int cap = c + (c >> 1) + 2;
+    const int cap = c + (c >> FJMI_TILED_SLACK_SHIFT) + FJMI_TILED_SLACK_ADD;
     const int incl = wave_incl_scan(cap, lane);
     if (t < ntiles) {
       sh.seg_start[t] = carry + incl - cap;
@@ -409,7 +424,7 @@ __device__ void refresh_from_regs(SH &sh, const Gather &G, int ng, int total, in
 }
 
 template <class SH>
-__device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, int &stampv);
+__device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, int &stampv, int *total = nullptr);
 
 /* Tile minima of a several-chunk gather, incrementally.  The step changed the
  * entries of the jets listed in the member table (nchg of them: jets that took
@@ -471,7 +486,7 @@ __device__ void refresh_changed(SH &sh, const Gather &G, Gather &GF, const Ev &e
  * written into every listed tile's stamp. */
 template <class SH>
 __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, int c2, int extra, int lane,
-                            int &stampv) {
+                            int &stampv, int *total = nullptr) {
   int cand = -1;
   if (lane < 27) {
     const int c = lane < 9 ? c0 : (lane < 18 ? c1 : c2);
@@ -483,12 +498,13 @@ __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, i
   } else if (lane == 27) {
     cand = extra;
   }
-  return gather_tiles(sh, G, cand, lane, stampv);
+  return gather_tiles(sh, G, cand, lane, stampv, total);
 }
 
-/* The gather of the tiles the lanes propose (cand >= 0; duplicates dropped). */
+/* The gather of the tiles the lanes propose (cand >= 0; duplicates dropped);
+ * returns the number of tiles and, through `total`, of members. */
 template <class SH>
-__device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, int &stampv) {
+__device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, int &stampv, int *total) {
   stampv += 1;
   wave_sync(); /* the callers' LDS writes (segment counts) come first */
   bool keep = false;
@@ -505,6 +521,7 @@ __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, int &stampv) 
   }
   if (lane == 0) G.pref[0] = 0;
   wave_sync();
+  if (total) *total = __builtin_amdgcn_readlane(incl, 63);
   return (int)__popcll(kb);
 }
 
@@ -526,15 +543,15 @@ __device__ __forceinline__ int gathered(const SH &sh, const Gather &G, const Ev 
 }
 
 /* Appends physical p to tile t's segment; relocates the segment to the end of
- * the pool with twice the capacity when full, or flags a rebuild when the pool
- * is exhausted (p is then recorded only in its metadata). */
+ * the pool with twice the capacity when full, or returns true (a rebuild is
+ * needed) when the pool is exhausted (p is then recorded only in its metadata). */
 template <class SH>
-__device__ void seg_insert(SH &sh, const Ev &ev, int t, int p) {
+__device__ bool seg_insert(SH &sh, const Ev &ev, int t, int p) {
   if (sh.seg_count[t] == sh.seg_cap[t]) {
     const int ncap = 2 * sh.seg_cap[t] + 2;
     if (sh.pool_top + ncap > ev.pool_size) {
       sh.rebuild = 1;
-      return;
+      return true;
     }
     const int ns = sh.pool_top;
     for (int i = 0; i < sh.seg_count[t]; ++i) {
@@ -550,6 +567,7 @@ __device__ void seg_insert(SH &sh, const Ev &ev, int t, int p) {
   ev.pool[pos] = p;
   ev.rec[p].meta.w = pos;
   sh.seg_count[t] += 1;
+  return false;
 }
 
 /* Lexicographic (distance, label) nearest neighbour of physical o among the
@@ -858,7 +876,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       for (int t0 = 0; t0 < ntiles; t0 += W) {
         const int t = t0 + lane;
         const int c = t < ntiles ? sh.seg_count[t] : 0;
-        const int cap = c + (c >> 1) + 2;
+        const int cap = c + (c >> FJMI_TILED_SLACK_SHIFT) + FJMI_TILED_SLACK_ADD;
         const int incl = wave_incl_scan(cap, lane);
         const int pincl = wave_incl_scan(c, lane);
         if (t < ntiles) {
@@ -1002,7 +1020,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
        * pairwise tree of lexicographic (diJ, label) compares */
       constexpr int KT = MAXT / W;
       double kd[KT];
-      int kl[KT], kt[KT];
+      int kl[KT], kt[KT], kp[KT];
       uint4 raw[KT];
 #pragma unroll
       for (int k = 0; k < KT; ++k) raw[k] = *reinterpret_cast<const uint4 *>(&sh.tmin[k * W + lane]);
@@ -1013,6 +1031,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         kd[k] = ok ? __hiloint2double((int)raw[k].y, (int)raw[k].x) : __builtin_inf();
         kl[k] = ok ? l : BIG;
         kt[k] = k * W + lane;
+        kp[k] = (int)raw[k].w;
       }
 #pragma unroll
       for (int w = 1; w < KT; w *= 2) {
@@ -1022,6 +1041,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           kd[k] = c ? kd[k + w] : kd[k];
           kl[k] = c ? kl[k + w] : kl[k];
           kt[k] = c ? kt[k + w] : kt[k];
+          kp[k] = c ? kp[k + w] : kp[k];
         }
       }
       const double bd = kd[0];
@@ -1030,7 +1050,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       int lA;
       const int wl = wave_argmin(bd, bl, &dmin, &lA);
       const int tA = wl < 0 ? -1 : uni(__builtin_amdgcn_readlane(bt, wl));
-      const int A = tA < 0 ? -1 : uni(sh.tmin[tA].p);
+      const int A = tA < 0 ? -1 : uni(__builtin_amdgcn_readlane(kp[0], wl));
       const int B = tA < 0 ? -1 : uni(tkey_nn(lA));
       if (A < 0) { /* no live jet found: inconsistent state, stop this event */
         if (lane == 0) atomicOr(a.status, 2u);
@@ -1104,6 +1124,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
        * removal 1 and the N2Plain compaction (the last label moves), then, if
        * the merged jet changes tile, removal 2 (from the segment as removal 1
        * left it) and its insertion */
+      bool need_rebuild = false; /* lane 0's segment insertion exhausted the pool */
       if (lane == 0) {
         if (phys_lo >= 0) {
           a.hist_p1[base + step] = min(hhi, hlo);
@@ -1144,7 +1165,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           }
           sh.seg_count[tile_lo_old] -= 1;
           ev.rec[phys_lo].meta.z = tile_new;
-          seg_insert(sh, ev, tile_new, phys_lo);
+          need_rebuild = seg_insert(sh, ev, tile_new, phys_lo);
         }
         if (a.pj_out) {
           unsigned char *rec = a.pj_out + 40 * (size_t)(base + tail);
@@ -1162,13 +1183,13 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       wave_sync();
       const int n = n_old - 1;
       if (n == 0) break;
-      if (sh.rebuild) { /* pool exhausted by a relocation: lay all segments out again */
+      if (__builtin_amdgcn_readlane((int)need_rebuild, 0)) { /* pool exhausted by a relocation: lay all segments out again */
         rebuild_segments(sh, ev, n, ntiles, lane);
       }
 
       /* 3. gather the 3x3 tiles around the old and new positions and the relabelled jet's tile */
-      const int ng = build_gather(sh, G, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane, stampv);
-      const int total = G.pref[ng];
+      int total;
+      const int ng = build_gather(sh, G, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane, stampv, &total);
       const int main_stamp = stampv;
       if (lane == 0) {
         sh.norph = 0;

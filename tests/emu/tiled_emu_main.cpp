@@ -1,8 +1,15 @@
 // Runs the tiled kernel's source (fastjet_gpu_amd/csrc/tiled.hip) on the CPU as
 // one emulated wavefront and checks every result bit-exactly against the oracle
 // (N2Plain, or its tiled form above 3000 particles).  Test infrastructure only.
-//   tiled_emu WORKLOAD E0 E1 [VARIANT P R TILE_Q NMAX]   generated events
+//   tiled_emu WORKLOAD E0 E1 [VARIANT P R TILE_Q NMAX POOL_EXTRA POOL_PER_N]   generated events
 //   tiled_emu file PATH VARIANT P R                       raw little-endian px,py,pz,E doubles
+// The tile pool of an event holds POOL_PER_N * N + POOL_EXTRA entries (default
+// 3 N + kTiledPoolPerEvent, as the library allocates); a small pool, with a
+// build with less segment slack (-DFJMI_TILED_SLACK_SHIFT=30 -DFJMI_TILED_SLACK_ADD=0),
+// makes segments relocate until the pool is
+// exhausted and exercises the kernel's segment rebuild (counted and printed).
+static long g_rebuilds = 0;
+#define FJMI_TILED_ON_REBUILD() (++g_rebuilds)
 #include "tiled.hip"
 
 #include <vector>
@@ -17,14 +24,16 @@ namespace emu {
 void run_wave(const std::function<void()> &fn);
 }
 
+static int g_pool_extra = fjmi::kTiledPoolPerEvent, g_pool_per_n = 3;
+
 template <int MAXT, int WPE>
 static int run_event(const double *p4, int N, int p, double R, int tile_q, bool verbose) {
   using namespace fjmi;
   std::vector<JetRec> rec(N > 0 ? N : 1);
   std::vector<double> p4s(4 * (size_t)(N > 0 ? N : 1));
   std::vector<int> hist(N > 0 ? N : 1);
-  std::vector<unsigned short> lab2(N > 0 ? N : 1), pool(3 * (size_t)N + kTiledPoolPerEvent);
-  TiledScratch ts{rec.data(), p4s.data(), hist.data(), lab2.data(), pool.data(), kTiledPoolPerEvent};
+  std::vector<unsigned short> lab2(N > 0 ? N : 1), pool((size_t)g_pool_per_n * N + g_pool_extra);
+  TiledScratch ts{rec.data(), p4s.data(), hist.data(), lab2.data(), pool.data(), (g_pool_per_n - 3) * N + g_pool_extra};
   std::vector<int32_t> h1(N > 0 ? N : 1, -7), h2(N > 0 ? N : 1, -7);
   std::vector<double> hd(N > 0 ? N : 1, -7.0), jets(4 * (size_t)(N > 0 ? N : 1), 0.0);
   int32_t nj = -1;
@@ -146,6 +155,8 @@ int main(int argc, char **argv) {
   const double R = argc > 6 ? std::atof(argv[6]) : 0.4;
   const int tq = argc > 7 ? std::atoi(argv[7]) : 0;
   const int nmax = argc > 8 ? std::atoi(argv[8]) : 1 << 30;
+  if (argc > 9) g_pool_extra = std::atoi(argv[9]);
+  if (argc > 10) g_pool_per_n = std::atoi(argv[10]);
   int fails = 0;
   for (long e = e0; e < e1; ++e) {
     const int n = fjmi_gen_event_size(workload, e);
@@ -164,6 +175,7 @@ int main(int argc, char **argv) {
     fails += b;
     std::fflush(stdout);
   }
+  std::printf("rebuilds: %ld\n", g_rebuilds);
   std::printf("done: %d mismatches\n", fails);
   return fails ? 1 : 0;
 }

@@ -16,20 +16,32 @@ LIB = os.path.join(ROOT, "fastjet_gpu_amd", "lib")
 ORACLE = os.path.join(ROOT, "oracle", "_build")
 
 
-@pytest.fixture(scope="module")
-def emu_bin(tmp_path_factory):
+def _build_emu(tmp_path_factory, name, defines=()):
     if not (os.path.exists(os.path.join(LIB, "libfjmi_gen.so")) and os.path.exists(os.path.join(ORACLE, "libfj_oracle.so"))):
         pytest.skip("generator or oracle library not built")
     if shutil.which("g++") is None:
         pytest.skip("g++ not available")
-    out = str(tmp_path_factory.mktemp("emu") / "tiled_emu")
+    out = str(tmp_path_factory.mktemp("emu") / name)
     cmd = ["g++", "-std=c++20", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-ffp-contract=off",
+           *defines,
            f"-I{EMU}", f"-I{os.path.join(ROOT, 'fastjet_gpu_amd', 'csrc')}", f"-I{os.path.join(ROOT, 'include')}",
            f"-I{os.path.join(ROOT, 'oracle')}", "-x", "c++", os.path.join(EMU, "tiled_emu_main.cpp"),
            os.path.join(EMU, "wave_fibers.cpp"), "-x", "none", f"-L{LIB}", "-lfjmi_gen", f"-L{ORACLE}", "-lfj_oracle",
            f"-Wl,-rpath,{LIB}", f"-Wl,-rpath,{ORACLE}", "-o", out]
     subprocess.run(cmd, check=True, capture_output=True)
     return out
+
+
+@pytest.fixture(scope="module")
+def emu_bin(tmp_path_factory):
+    return _build_emu(tmp_path_factory, "tiled_emu")
+
+
+@pytest.fixture(scope="module")
+def emu_tight_bin(tmp_path_factory):
+    """No segment slack: insertions relocate segments and a pool without spare
+    room runs out, so the kernel's segment rebuild runs."""
+    return _build_emu(tmp_path_factory, "tiled_emu_tight", ("-DFJMI_TILED_SLACK_SHIFT=30", "-DFJMI_TILED_SLACK_ADD=0"))
 
 
 def run(emu_bin, *args):
@@ -85,3 +97,13 @@ def test_kernel_kinematics_bit_identical(emu_bin, seed):
     and on random momenta over 160 binary decades."""
     out = run(emu_bin, "kin", 100000, seed)
     assert "kinematics: 0 of" in out
+
+
+@pytest.mark.parametrize("workload,e0,e1,variant,p,R", [(3, 0, 6, 0, -1, 0.4), (3, 0, 6, 1, 1, 0.4)])
+def test_pool_exhaustion_rebuilds(emu_tight_bin, workload, e0, e1, variant, p, R):
+    """A tile pool of exactly N entries (no room for relocated segments): every
+    exhaustion lays all segments out again, and the results stay bit-exact."""
+    out = run(emu_tight_bin, workload, e0, e1, variant, p, R, 0, 1 << 30, 0, 1)
+    assert "done: 0 mismatches" in out
+    rebuilds = int(out.split("rebuilds:")[1].split()[0])
+    assert rebuilds > 0, out[-500:]
