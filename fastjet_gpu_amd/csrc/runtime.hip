@@ -180,7 +180,9 @@ int launch_dense(const KernelDesc &kd, int grid, size_t lds, const LaunchArgs &a
 int cluster_one_device(fjmi_context *ctx, int engine, const double *d_in, int in_stride, int in_off, int n, int p,
                        double R, int32_t *h1, int32_t *h2, double *hd, double *jets, int32_t *nj, unsigned char *pj,
                        hipStream_t stream) {
-  const fjmi_bucket *b = engine == FJMI_ENGINE_TILED ? &kTiledRow : bucket_for(ctx, n, engine);
+  /* engine forced to tiled: the table's tiled row for n, else tile side R with the 512-tile variant */
+  const fjmi_bucket *b = bucket_for(ctx, n, engine);
+  if (!b && engine == FJMI_ENGINE_TILED) b = &kTiledRow;
   fjmi_bucket forced;
   if (!b && engine == FJMI_ENGINE_DENSE && dense_row_for(n, &forced)) b = &forced;
   if (!b) return fail(FJMI_ECAPACITY, "event with %d particles exceeds the largest bucket of engine %d", n, engine);

@@ -46,7 +46,8 @@ enum fjmi_status {
 enum fjmi_engine {
   FJMI_ENGINE_AUTO = 0,  /* the engine of the event's multiplicity bucket */
   FJMI_ENGINE_DENSE = 1, /* replaces tri_matrix.cu / n_array.cu */
-  FJMI_ENGINE_TILED = 2, /* replaces grid.cu */
+  FJMI_ENGINE_TILED = 2, /* replaces grid.cu; a single event takes the table's tiled row for its
+                          * size, or tile side R and variant 0 when the table has none */
   FJMI_ENGINE_EMPTY = 3  /* replaces tri_matrix_empty.cu: per-step launch floor, no result */
 };
 

@@ -253,6 +253,34 @@ def test_batch_matches_oracle(engine):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("var", [1, 0])
+def test_stretched_tiles(engine, var):
+    """Events whose tile grid exceeds the kernel's tile table, so that tiles get
+    taller than R in rapidity: R = 0.05 and 0.1 (125 and 62 columns in phi) and
+    rapidities up to +-9 at R = 0.4 (45 rows at side R), with the 384-tile and
+    the 512-tile variant."""
+    import fastjet_gpu_amd as fj
+    saved = engine.buckets()
+    try:
+        engine.set_buckets([(1000000000, 2, 64, 0, var)])
+        p4, off = fj.gen_workload(3, 6, first=4200)
+        pr = [(-1, 0.05), (1, 0.1), (0, 0.1), (-1, 0.1), (1, 0.05), (0, 0.05)]
+        cases = [(p4[off[e]:off[e + 1]], p, R) for e, (p, R) in enumerate(pr)]
+        rng = np.random.default_rng(5)
+        for n in (300, 900):
+            pt = 0.3 + rng.exponential(1.0, n)
+            eta = rng.uniform(-9.0, 9.0, n)
+            phi = rng.uniform(0.0, 2.0 * np.pi, n)
+            wide = np.stack([pt * np.cos(phi), pt * np.sin(phi), pt * np.sinh(eta), pt * np.cosh(eta)], 1)
+            cases += [(wide, -1, 0.4), (wide, 1, 0.4)]
+        for k, (ev, p, R) in enumerate(cases):
+            check_event(engine, np.ascontiguousarray(ev), p, R, f"stretched tiles, variant {var}, case {k}", eng=2)
+            check_event(engine, np.ascontiguousarray(ev), p, R, f"stretched tiles, variant {var}, case {k} (auto)")
+    finally:
+        engine.set_buckets(saved)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("workload,ne,first", [(3, 512, 3000), (5, 128, 500), (2, 64, 10)])
 def test_batch_mixed_algorithms_and_radii(engine, workload, ne, first):
     """One batch whose events each draw their algorithm from {kt, C/A, anti-kt} and R from
