@@ -30,10 +30,12 @@
  *   3. one gather of the 3x3 tiles around the two old positions and the new one:
  *      orphans (jets whose nearest neighbour was removed or merged), jets now
  *      closer to the new jet, and the new jet's own nearest neighbour;
- *   4. a 3x3 rescan per orphan;
- *   5. the minima of the gathered tiles recomputed from a member table in LDS
- *      (or, for large gathers, with LDS atomic minima: diJ bits, then key, then
- *      the winner's index).
+ *   4. a 3x3 rescan per orphan (from the lanes' registers when the gather
+ *      covers, or provably bounds, its neighbourhood);
+ *   5. the minima of the gathered tiles recomputed with LDS atomic minima (diJ
+ *      bits, then key, then the winner's index) from the members' entries in
+ *      registers, or, for gathers of several chunks, incrementally from the
+ *      entries that changed.
  */
 #include <hip/hip_runtime.h>
 
