@@ -577,7 +577,7 @@ __device__ bool seg_insert(SH &sh, const Ev &ev, int t, int p) {
 template <int NCH, class SH>
 __device__ int nn_scan(const SH &sh, const Gather &G, const Ev &ev, int ng, int o, double orap, double ophi, double R2,
                        int lane,
-                       double *best_d, double *best_w) {
+                       double *best_d, double *best_w, int *best_l = nullptr) {
   const int total = G.pref[ng];
   double bd = R2, bw = 0.0;
   int bl = BIG, bp = -1;
@@ -614,12 +614,55 @@ __device__ int nn_scan(const SH &sh, const Gather &G, const Ev &ev, int ng, int 
   int ml;
   const int ln = wave_argmin(bd, bl, &md, &ml);
   *best_d = md;
+  if (best_l) *best_l = ml;
   if (ln < 0) {
     *best_w = 0.0;
     return -1;
   }
   *best_w = readlane_d(bw, ln);
   return __builtin_amdgcn_readlane(bp, ln);
+}
+
+/* Nearest neighbour (lexicographic (distance, label), within R) of jet o at
+ * (oy, ophi) in tile otile over its 3x3 neighbourhood: its own tile first, then
+ * only the neighbour tiles that could hold a jet at most as far as the best so
+ * far (lane i < 9 bounds the distance to neighbour tile i from below by the
+ * jet's distance to its own tile's edges on that side, less a margin far above
+ * the rounding of the tile boundaries).  For dense events, whose 3x3
+ * neighbourhoods span several chunks of 64; a sparse neighbourhood is one
+ * chunk, scanned faster in one pass. */
+template <int NCH, class SH>
+__device__ int nn_rescan(SH &sh, Gather &GO, const Geo &geo, const Ev &ev, int o, int otile, double oy, double ophi,
+                         double R2, int lane, int &stampv, double *best_d, double *best_w) {
+  int ngo = gather_tiles(sh, GO, lane == 0 ? otile : -1, lane, stampv);
+  double d1, w1;
+  int l1;
+  const int n1 = nn_scan<NCH>(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &d1, &w1, &l1);
+  int cand = -1;
+  if (lane < 9 && lane != 4) {
+    const int ty = geo.row(otile), tp = otile - ty * geo.nphi, dy = lane / 3 - 1, dp = lane % 3 - 1;
+    const int y = ty + dy;
+    if (y >= 0 && y < geo.ny) {
+      double by = dy < 0 ? oy - (geo.y0 + ty * geo.sy) : (dy > 0 ? geo.y0 + (ty + 1) * geo.sy - oy : 0.0);
+      double bp = dp < 0 ? ophi - tp * geo.sphi : (dp > 0 ? (tp + 1) * geo.sphi - ophi : 0.0);
+      by = dy != 0 ? (by - 1e-10 > 0.0 ? by - 1e-10 : 0.0) : 0.0;
+      bp = dp != 0 ? (bp - 1e-10 > 0.0 ? bp - 1e-10 : 0.0) : 0.0;
+      if (!(by * by + bp * bp > d1)) cand = y * geo.nphi + geo.wrap(tp + dp);
+    }
+  }
+  if (__ballot(cand >= 0) == 0) {
+    *best_d = d1;
+    *best_w = w1;
+    return n1;
+  }
+  ngo = gather_tiles(sh, GO, cand, lane, stampv);
+  double d2, w2;
+  int l2;
+  const int n2 = nn_scan<NCH>(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &d2, &w2, &l2);
+  const bool take2 = n2 >= 0 && (n1 < 0 || d2 < d1 || (d2 == d1 && l2 < l1));
+  *best_d = take2 ? d2 : d1;
+  *best_w = take2 ? w2 : w1;
+  return take2 ? n2 : n1;
 }
 
 }  // namespace tiled
@@ -1380,9 +1423,8 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           const int4 mo = ev.rec[o].meta;
           const double2 ro = ev.rec[o].rp;
           const double wo = ev.rec[o].wn.x;
-          const int ngo = build_gather(sh, GO, geo, mo.z, -1, -1, -1, lane, stampv);
           double od, owb;
-          const int nno = nn_scan<kScanChunks>(sh, GO, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
+          const int nno = nn_rescan<kScanChunks>(sh, GO, geo, ev, o, mo.z, ro.x, ro.y, R2, lane, stampv, &od, &owb);
           const double dO = diJ_of(od, wo, nno, owb);
           if (lane == 0) {
             ev.rec[o].meta.x = nno;
