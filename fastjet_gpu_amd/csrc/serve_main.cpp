@@ -21,7 +21,7 @@
  * One JSON line per rate.  --events 0 (default) uses min(200000, rate * S)
  * requests with S = --seconds (20).  --variant: the tiled kernel variant
  * (default 1, 384 tiles and 10 workgroups per CU: the higher served rate near
- * saturation; variant 0 gives a few percent lower p99 at light load).
+ * saturation; variant 0 gives up to 2 % lower p99 at light load).
  */
 #include <hip/hip_runtime_api.h>
 
