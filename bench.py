@@ -9,19 +9,36 @@ persistent workgroups cluster all 4096 events (inputs already resident in HBM).
 `value` is events/s over the K timed steps.  Beside it the line reports:
   plain        the one-event-per-launch path (one kernel launch + sync per event)
                on a sample of the same events, and the batched/plain ratio;
-  host_memory_path  the same batch from host arrays (PCIe copies and planning
-               included; never `value`);
+  launch_floor the reference's tri_matrix_empty schedule (2N+3 empty launches
+               per event, tri_matrix_empty.cu:257-292) on a sample of events;
+  host_memory_path  the same batch from host arrays (PCIe copies included; never
+               `value`);
   latency_ms   p50/p99 per-event latency of the plain path and per-batch latency;
-  roofline     the dominant (slowest-finishing) bucket kernel's algorithmic FP64
-               rate (dense-equivalent operation count) against MI355X's FP64 peak;
-  cpu_baseline the CPU oracle (tiled N2Plain, FastJet N2Tiled-style) on the host cores.
+  serving      the continuous-batching server on the serving stream (lognormal
+               multiplicities, mixed algorithms and radii) at two Poisson rates:
+               served events/s and per-event p50/p99 latency;
+  configs      the single-event configurations (N = 354; N = 808 under three
+               algorithms and two radii): the GPU's plain per-event time beside
+               the CPU oracle on one thread; the high-pileup batch on the GPU;
+  roofline     the tiled kernel against MI355X peaks: algorithmic FP64 work of
+               the tiled method (pair distances counted by the oracle) per
+               launch, the dense-equivalent count, and rocprofv3 counters
+               (executed FP64 FLOPs, VALU busy, LDS and HBM bytes) from the
+               committed profile of the same kernel source;
+  cpu_baseline the CPU oracle (tiled N2Plain, FastJet N2Tiled-style) on all the
+               host's cores, best of 3; its results also check every event of
+               the timed batch bit for bit (`parity`).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --summarize DIR [DIR ...]   (rocprofv3 output -> profile summary)
 """
 import argparse
+import csv
+import glob
 import hashlib
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -31,27 +48,260 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "events/s batched vs per-launch at N~U[50,2000], anti-kt R=0.4; p99 latency"
-FP64_PEAK_TFLOPS = 78.6  # MI355X published FP64 peak (vector and matrix)
+# MI355X (gfx950) published peaks and clock
+FP64_PEAK_TFLOPS = 78.6          # vector FP64, FMA counted as 2
+HBM_PEAK_GBPS = 8000.0
+N_CU, N_SIMD, SCLK_GHZ = 256, 1024, 2.4
+LDS_PEAK_GBPS = N_CU * 128 * SCLK_GHZ  # 128 bytes per clock per CU
+COUNTER_PROFILE = os.path.join(ROOT, "profiles", "tiled_kernel_counters_workload3.md")
+KERNEL_SOURCES = ["fastjet_gpu_amd/csrc/tiled.hip", "fastjet_gpu_amd/csrc/fjmi_math.h",
+                  "fastjet_gpu_amd/csrc/fjmi_math_tables.h", "fastjet_gpu_amd/csrc/fjmi_internal.h",
+                  "fastjet_gpu_amd/csrc/tune_table.inc", "fastjet_gpu_amd/csrc/runtime.hip"]
 
 
-def measured_traffic(kernel_tag):
-    """HBM bytes per launch of the dominant kernel on this workload, from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/), or None."""
-    path = os.path.join(ROOT, "profiles", "tiled_kernel_traffic_workload3.md")
+def source_sha():
+    """Hash of the sources that define the tiled kernel and its launch on this workload."""
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def dense_equivalent_flops(sum_n2, sum_n):
+    """FP64 operations of the dense nearest-neighbour method: 8 per pair
+    evaluation for N(N-1)/2 initial pairs plus N(N-1)/2 merged-row refreshes,
+    plus N(N+1)/2 argmin compares: 8 N(N-1) + N(N+1)/2 per event."""
+    return 8.0 * (sum_n2 - sum_n) + 0.5 * (sum_n2 + sum_n)
+
+
+# ---------------------------------------------------------------------------
+# rocprofv3 output -> committed summary (bench.py --summarize), and back
+# ---------------------------------------------------------------------------
+def _read_csvs(dirs, name):
+    rows = []
+    for d in dirs:
+        for path in glob.glob(os.path.join(d, "**", f"*{name}"), recursive=True):
+            with open(path, newline="") as fh:
+                rows += list(csv.DictReader(fh))
+    return rows
+
+
+# the rocprofv3 runs the committed counter profile comes from (one pass each)
+PROFILE_BENCH = "python3 bench.py --steps 4 --warmup 1 --no-extras --no-cpu"
+PMC_PASSES = [
+    ("A", "SQ_INSTS_VALU_FLOPS_FP64 SQ_INSTS_VALU_FMA_F64 SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 "
+          "SQ_INSTS_VALU_TRANS_F64 SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_VALU_INT64", ""),
+    ("B", "SQ_INSTS_LDS_LOAD_BANDWIDTH SQ_INSTS_LDS_STORE_BANDWIDTH SQ_INSTS_LDS_ATOMIC_BANDWIDTH SQ_INSTS_LDS "
+          "SQ_LDS_BANK_CONFLICT SQ_ACTIVE_INST_VALU SQ_BUSY_CU_CYCLES SQ_ACTIVE_INST_LDS", ""),
+    ("C", "SQ_WAVE_CYCLES SQ_WAIT_INST_ANY SQ_WAIT_ANY SQ_ACTIVE_INST_ANY SQ_WAVES SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR "
+          "SQ_ACTIVE_INST_SCA", ""),
+    ("D", "FETCH_SIZE", "--calibrate-copy"),
+    ("E", "WRITE_SIZE", "--calibrate-copy"),
+    ("F", "TCC_HIT_sum TCC_MISS_sum", ""),
+]
+
+
+def profile_commands(out="gpurun_out/prof"):
+    cmds = [f"timeout -k 10 300 rocprofv3 --kernel-trace --stats -d {out}/kt -o run --output-format csv -- "
+            f"{PROFILE_BENCH}"]
+    for tag, ctrs, extra in PMC_PASSES:
+        cmds.append(f"timeout -s KILL 180 rocprofv3 --pmc {ctrs} -d {out}/{tag} -o run --output-format csv -- "
+                    f"{PROFILE_BENCH} {extra}".rstrip())
+    cmds.append(f"python3 bench.py --summarize {out}/kt " + " ".join(f"{out}/{t}" for t, _, _ in PMC_PASSES))
+    return cmds
+
+
+def summarize(dirs, kernel_key="tiled_kernel<384, 3, false>"):
+    """Per-launch counter values of the tiled kernel (and the calibration copy
+    kernel's FETCH_SIZE / WRITE_SIZE) from rocprofv3 counter_collection.csv files,
+    and its average duration from kernel_stats.csv, as machine-readable lines."""
+    per = {}   # (kernel, dispatch) -> {counter: value}
+    for r in _read_csvs(dirs, "counter_collection.csv"):
+        k = (r.get("Kernel_Name", ""), r.get("Dispatch_Id", ""))
+        per.setdefault(k, {})
+        c = r["Counter_Name"]
+        per[k][c] = per[k].get(c, 0.0) + float(r["Counter_Value"])
+    want = {}
+    copy = {}
+    for (name, _), vals in per.items():
+        short = name.replace("fjmi::", "")
+        if kernel_key in short:
+            for c, v in vals.items():
+                want.setdefault(c, []).append(v)
+        elif "copy" in name.lower():
+            # the 256 MiB device-to-device copy is the largest copy dispatch of the run
+            for c, v in vals.items():
+                copy[c] = [max(copy.get(c, [0.0])[0], v)]
+    out = [f"source_sha: {source_sha()}", f"kernel: {kernel_key}"]
+    steps = 4146882  # clustering steps per launch = particles of the 4096-event batch
+    for r in _read_csvs(dirs, "kernel_stats.csv"):
+        if kernel_key in r.get("Name", "").replace("fjmi::", ""):
+            out.append(f"kernel_avg_ms: {float(r['AverageNs']) / 1e6:.4f} (calls {r.get('Calls', '?')})")
+    for c, vs in sorted(want.items()):
+        out.append(f"counter {c}: {np.mean(vs):.6g} (dispatches {len(vs)})")
+    for c, vs in sorted(copy.items()):
+        out.append(f"copy_counter {c}: {np.mean(vs):.6g} (256 MiB read and 256 MiB written by the copy kernel)")
+    # HBM bytes per launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by
+    # what the same counter reads for the copy kernel of known traffic (256 MiB each way)
+    if "FETCH_SIZE" in want and "WRITE_SIZE" in want and "FETCH_SIZE" in copy and "WRITE_SIZE" in copy:
+        ff = (256 << 20) / (np.mean(copy["FETCH_SIZE"]) * 1024.0)
+        wf = (256 << 20) / (np.mean(copy["WRITE_SIZE"]) * 1024.0)
+        fb = np.mean(want["FETCH_SIZE"]) * 1024.0 * ff
+        wb = np.mean(want["WRITE_SIZE"]) * 1024.0 * wf
+        out.append(f"fetch_factor: {ff:.4f}")
+        out.append(f"write_factor: {wf:.4f}")
+        out.append(f"fetch_bytes_per_launch: {fb:.6g}")
+        out.append(f"write_bytes_per_launch: {wb:.6g}")
+        out.append(f"traffic_bytes_per_launch: {fb + wb:.6g}")
+    g = {c: float(np.mean(v)) for c, v in want.items()}
+    rows = []
+    for label, c in (("VALU instructions", "SQ_INSTS_VALU"), ("  of which FP64 add/sub", "SQ_INSTS_VALU_ADD_F64"),
+                     ("  FP64 mul", "SQ_INSTS_VALU_MUL_F64"), ("  FP64 fma", "SQ_INSTS_VALU_FMA_F64"),
+                     ("  FP64 transcendental (rcp)", "SQ_INSTS_VALU_TRANS_F64"), ("  64-bit integer", "SQ_INSTS_VALU_INT64"),
+                     ("SALU instructions", "SQ_INSTS_SALU"), ("LDS instructions", "SQ_INSTS_LDS"),
+                     ("vector memory reads", "SQ_INSTS_VMEM_RD"), ("vector memory writes", "SQ_INSTS_VMEM_WR")):
+        if c in g:
+            rows.append(f"| {label} | {g[c]:.4g} | {g[c] / steps:.1f} |")
+    extra = []
+    if "SQ_WAVE_CYCLES" in g:
+        for label, c in (("waiting (SQ_WAIT_ANY)", "SQ_WAIT_ANY"), ("issuing (SQ_ACTIVE_INST_ANY)", "SQ_ACTIVE_INST_ANY"),
+                         ("ready, not issued (SQ_WAIT_INST_ANY)", "SQ_WAIT_INST_ANY")):
+            if c in g:
+                extra.append(f"| share of wave cycles {label} | {g[c] / g['SQ_WAVE_CYCLES']:.3f} |")
+    if "TCC_HIT_sum" in g and "TCC_MISS_sum" in g:
+        extra.append(f"| L2 hit rate | {g['TCC_HIT_sum'] / (g['TCC_HIT_sum'] + g['TCC_MISS_sum']):.3f} |")
+    doc = [f"# {kernel_key} counters, 4096-event batch (N ~ U[50, 2000], anti-kt R = 0.4)", "",
+           "Commands (repository root on one MI355X, after `cd /tmp && export TMPDIR=/tmp`), one pass each:", "",
+           "```", *profile_commands(), "```", "",
+           "Machine-readable lines (read by bench.py; counter values per launch, averaged over the profiled "
+           "launches; `source_sha` hashes the kernel's sources as profiled, and bench.py derives no fraction "
+           "from counters of other sources):", "", "```", *out, "```", "",
+           f"Per launch and per clustering step ({steps} steps per launch, one per particle):", "",
+           "| counter | per launch | per step |", "|---|---|---|", *rows, "", "| ratio | value |", "|---|---|",
+           *extra, ""]
+    print("\n".join(doc))
+    return 0
+
+
+def read_profile(path=COUNTER_PROFILE):
+    """Counters per launch from the committed summary: (dict, sha, kernel_avg_ms)."""
+    vals, sha, avg = {}, None, None
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:") and kernel_tag in line:
-                return int(line.split()[1])
+            line = line.strip().strip("`")
+            if line.startswith("source_sha:"):
+                sha = line.split()[1]
+            elif line.startswith("kernel_avg_ms:"):
+                avg = float(line.split()[1])
+            elif line.startswith("counter "):
+                name, v = line[len("counter "):].split(":", 1)
+                vals[name.strip()] = float(v.split()[0])
+            elif line.startswith("traffic_bytes_per_launch:"):
+                vals["traffic_bytes_per_launch"] = float(line.split()[1])
     except OSError:
         pass
-    return None
+    return vals, sha, avg
 
 
-def flops(sum_n2, sum_n):
-    """Algorithmic FP64 operations of the dense nearest-neighbour method:
-    8 per pair evaluation for N(N-1)/2 initial pairs plus N(N-1)/2 merged-row
-    refreshes, plus N(N+1)/2 argmin compares:  8 N(N-1) + N(N+1)/2 per event."""
-    return 8.0 * (sum_n2 - sum_n) + 0.5 * (sum_n2 + sum_n)
+def counter_fields(t_s, n_steps_total):
+    """Fractions of MI355X peaks from the committed counters at kernel time t_s."""
+    vals, sha, avg = read_profile()
+    cur = source_sha()
+    if not vals:
+        return {"source": None, "stale": None}
+    stale = sha != cur
+    f = {"source": os.path.relpath(COUNTER_PROFILE, ROOT), "profiled_source_sha": sha, "current_source_sha": cur,
+         "stale": stale, "profiled_kernel_ms": avg}
+    if stale:  # counters of another build of the kernel: not combined with this run's time
+        return f
+    fl = vals.get("SQ_INSTS_VALU_FLOPS_FP64")  # per wave instruction (FMA = 2), not per lane
+    if fl is not None:
+        f["fp64_flops_executed"] = fl * 64.0  # all 64 lanes counted: an upper bound of the lane work
+        f["fp64_tflops_executed"] = fl * 64.0 / t_s / 1e12
+        f["fp64_issue_frac"] = fl * 64.0 / t_s / 1e12 / FP64_PEAK_TFLOPS
+        f["fp64_insts_per_step"] = sum(vals.get(c, 0.0) for c in (
+            "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_TRANS_F64")
+        ) / n_steps_total if n_steps_total else None
+    av = vals.get("SQ_ACTIVE_INST_VALU")  # quad-cycles, summed over waves
+    if av is not None:
+        f["valu_busy_frac"] = av * 4.0 / (t_s * SCLK_GHZ * 1e9 * N_SIMD)
+    iv = vals.get("SQ_INSTS_VALU")
+    isa = vals.get("SQ_INSTS_SALU")
+    if iv is not None and n_steps_total:
+        f["valu_insts_per_step"] = iv / n_steps_total
+    if isa is not None and n_steps_total:
+        f["salu_insts_per_step"] = isa / n_steps_total
+    lds = sum(vals.get(c, 0.0) for c in ("SQ_INSTS_LDS_LOAD_BANDWIDTH", "SQ_INSTS_LDS_STORE_BANDWIDTH",
+                                         "SQ_INSTS_LDS_ATOMIC_BANDWIDTH")) * 64.0
+    if lds:
+        f["lds_bytes"] = lds
+        f["lds_gbps"] = lds / t_s / 1e9
+        f["lds_peak_gbps"] = LDS_PEAK_GBPS
+        f["lds_frac"] = lds / t_s / 1e9 / LDS_PEAK_GBPS
+    tr = vals.get("traffic_bytes_per_launch")
+    if tr:
+        f["hbm_gbps"] = tr / t_s / 1e9
+        f["hbm_peak_gbps"] = HBM_PEAK_GBPS
+        f["hbm_frac"] = tr / t_s / 1e9 / HBM_PEAK_GBPS
+    wc = vals.get("SQ_WAVE_CYCLES")
+    if wc:
+        for key, c in (("wait_frac", "SQ_WAIT_ANY"), ("wait_issue_frac", "SQ_WAIT_INST_ANY"),
+                       ("issuing_frac", "SQ_ACTIVE_INST_ANY")):
+            if vals.get(c) is not None:
+                f[key] = vals[c] / wc
+    h, m = vals.get("TCC_HIT_sum"), vals.get("TCC_MISS_sum")
+    if h is not None and m:
+        f["l2_hit_frac"] = h / (h + m)
+    return f
+
+
+# ---------------------------------------------------------------------------
+def host_info():
+    cores = len(os.sched_getaffinity(0))
+    model = "?"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return cores, model
+
+
+def timed_plain(eng, d, ev_off, sizes, idx, p, R, stream, engine=0):
+    import torch
+    lat = np.zeros(len(idx))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k, e in enumerate(idx):
+        lo = int(ev_off[e])
+        t = time.perf_counter()
+        eng.cluster_event_device(d["p4"].data_ptr() + 32 * lo, int(sizes[e]), p, R, d["h1"].data_ptr() + 4 * lo,
+                                 d["h2"].data_ptr() + 4 * lo, d["hd"].data_ptr() + 8 * lo,
+                                 d["j"].data_ptr() + 32 * lo, d["nj"].data_ptr() + 4 * e, stream, engine=engine)
+        torch.cuda.current_stream().synchronize()
+        lat[k] = time.perf_counter() - t
+    return time.perf_counter() - t0, lat
+
+
+def serving_leg(rates, seconds):
+    exe = os.path.join(ROOT, "fastjet_gpu_amd", "bin", "fjmi_serve")
+    if not os.path.exists(exe):
+        return {"error": "bin/fjmi_serve not built"}
+    r = subprocess.run([exe, "--rates", ",".join(f"{x:g}" for x in rates), "--seconds", str(seconds),
+                        "--unique", "1024"], capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        return {"error": r.stderr[-500:]}
+    out = []
+    for line in r.stdout.splitlines():
+        if line.startswith("{"):
+            j = json.loads(line)
+            out.append({"offered_per_s": j["rate"], "events_per_s": j["events_per_s"], "requests": j["requests"],
+                        "p50_ms": j["latency_ms"]["p50"], "p99_ms": j["latency_ms"]["p99"],
+                        "refused_submits": j["refused_submits"]})
+    return out
 
 
 def main():
@@ -61,14 +311,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--events", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=256)
-    ap.add_argument("--cpu-sample", type=int, default=4096)
+    ap.add_argument("--floor-sample", type=int, default=32)
     ap.add_argument("--cpu-repeat", type=int, default=3)
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core this process may run on")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the timed batch (for profiling runs): no plain, floor, host, serving, configs legs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="before the timed region, fill and copy a 256 MiB buffer: reference kernels of known HBM "
                          "traffic for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--summarize", nargs="+", metavar="DIR", help="summarise rocprofv3 output directories and exit")
     a = ap.parse_args()
+    if a.summarize:
+        return summarize(a.summarize)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: measures one GPU only (events are independent, so several GPUs would each run "
               "this same batch on their own share of the events; that is not run here). Use --gpus 1.",
@@ -87,13 +343,10 @@ def main():
     dev = torch.device("cuda:0")
     eng = fj.Engine(0)
     tot = len(p4)
-    d_p4 = torch.from_numpy(p4).to(dev)
-    d_off = torch.from_numpy(off).to(dev)
-    d_h1 = torch.zeros(tot, dtype=torch.int32, device=dev)
-    d_h2 = torch.zeros(tot, dtype=torch.int32, device=dev)
-    d_hd = torch.zeros(tot, dtype=torch.float64, device=dev)
-    d_j = torch.zeros(tot * 4, dtype=torch.float64, device=dev)
-    d_nj = torch.zeros(ne, dtype=torch.int32, device=dev)
+    d = dict(p4=torch.from_numpy(p4).to(dev), off=torch.from_numpy(off).to(dev),
+             h1=torch.zeros(tot, dtype=torch.int32, device=dev), h2=torch.zeros(tot, dtype=torch.int32, device=dev),
+             hd=torch.zeros(tot, dtype=torch.float64, device=dev), j=torch.zeros(tot * 4, dtype=torch.float64, device=dev),
+             nj=torch.zeros(ne, dtype=torch.int32, device=dev))
     if a.calibrate_copy:
         src = torch.empty(64 << 20, dtype=torch.float32, device=dev)
         src.fill_(1.0)  # writes 256 MiB
@@ -103,13 +356,13 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     stream = torch.cuda.current_stream().cuda_stream
-    batch = eng.batch_create(off, d_p4.data_ptr(), d_off.data_ptr(), None, None, p, R, d_h1.data_ptr(),
-                             d_h2.data_ptr(), d_hd.data_ptr(), d_j.data_ptr(), d_nj.data_ptr())
+    batch = eng.batch_create(off, d["p4"].data_ptr(), d["off"].data_ptr(), None, None, p, R, d["h1"].data_ptr(),
+                             d["h2"].data_ptr(), d["hd"].data_ptr(), d["j"].data_ptr(), d["nj"].data_ptr())
     for _ in range(a.warmup):
         batch.launch(stream)
     batch.sync(stream)
 
-    # ---- timed region: K graph launches, inputs resident ----
+    # ---- timed region: K graph launches on one stream, inputs resident ----
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps)]
     torch.cuda.synchronize()
@@ -125,73 +378,152 @@ def main():
     step_ms = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
     value = ne * a.steps / wall
     bstats = batch.stats()
+    print(f"[bench] timed {a.steps} launches: {value:.0f} events/s", file=sys.stderr, flush=True)
 
-    # ---- correctness spot check of the timed outputs against the oracle ----
-    parity = "not checked"
+    g = dict(h1=d["h1"].cpu().numpy(), h2=d["h2"].cpu().numpy(), hd=d["hd"].cpu().numpy(),
+             j=d["j"].cpu().numpy().reshape(-1, 4), nj=d["nj"].cpu().numpy())
+
+    # ---- CPU leg: the oracle on every host core (baseline), whose results check every timed event ----
+    cores, model = host_info()
+    threads = a.cpu_threads or cores
+    cpu, parity, pairs = None, "not checked", None
+    orc = None
     try:
         from oracle import oracle as orc
-        h1, h2 = d_h1.cpu().numpy(), d_h2.cpu().numpy()
-        hd, jj, nj = d_hd.cpu().numpy(), d_j.cpu().numpy().reshape(-1, 4), d_nj.cpu().numpy()
-        ok = 0
-        picks = list(range(0, ne, max(1, ne // 8)))[:8]
-        for e in picks:
-            lo, hi = off[e], off[e + 1]
-            w = orc.n2plain(p4[lo:hi], p, R)
-            g = (h1[lo:hi], h2[lo:hi], hd[lo:hi], jj[lo:lo + nj[e]])
-            ok += all(np.asarray(x).tobytes() == np.asarray(y).tobytes() for x, y in zip(w, g))
-        parity = f"bit-exact history and jets on {ok}/{len(picks)} sampled events"
-    except Exception as ex:  # the oracle is optional at bench time
+    except Exception as ex:  # the oracle is test infrastructure; the bench runs without it
         parity = f"not checked ({ex})"
+    if orc is not None:
+        runs = []
+        for k in range(max(1, a.cpu_repeat) if not a.no_cpu else 1):
+            runs.append(orc.cluster_batch(p4, off, p, R, threads, "tiled"))
+        w = runs[0]
+        bad = 0
+        for e in range(ne):
+            lo, hi = off[e], off[e + 1]
+            if (w[4][e] != g["nj"][e] or w[0][lo:hi].tobytes() != g["h1"][lo:hi].tobytes()
+                    or w[1][lo:hi].tobytes() != g["h2"][lo:hi].tobytes()
+                    or w[2][lo:hi].tobytes() != g["hd"][lo:hi].tobytes()
+                    or w[3][lo:lo + w[4][e]].tobytes() != g["j"][lo:lo + w[4][e]].tobytes()):
+                bad += 1
+        parity = f"bit-exact history and jets on {ne - bad}/{ne} events of the timed batch"
+        pairs = float(np.sum(w[5]))
+        if not a.no_cpu:
+            secs = min(r[6] for r in runs)
+            cpu = {"value": ne / secs, "unit": "events/s", "cores": threads, "kind": "port",
+                   "host_cores": cores, "host_cpu": model,
+                   "sample": f"all {ne} events of the timed batch clustered by oracle/'s tiled N2Plain "
+                             f"(FastJet N2Tiled-style, g++ -O3) on {threads} threads of {cores} host cores, "
+                             f"best of {len(runs)}: {secs:.2f} s wall"}
+        print(f"[bench] CPU leg done: {parity}", file=sys.stderr, flush=True)
 
-    # ---- plain one-event-per-launch path on a sample ----
-    ns = min(a.plain_sample, ne)
-    lat = np.zeros(ns)
-    torch.cuda.synchronize()
-    tp0 = time.perf_counter()
-    for e in range(ns):
-        lo = int(off[e])
-        t = time.perf_counter()
-        eng.cluster_event_device(d_p4.data_ptr() + 32 * lo, int(sizes[e]), p, R, d_h1.data_ptr() + 4 * lo,
-                                 d_h2.data_ptr() + 4 * lo, d_hd.data_ptr() + 8 * lo, d_j.data_ptr() + 32 * lo,
-                                 d_nj.data_ptr() + 4 * e, stream)
-        torch.cuda.current_stream().synchronize()
-        lat[e] = time.perf_counter() - t
-    plain_wall = time.perf_counter() - tp0
-    plain_value = ns / plain_wall
+    extras = {}
+    if not a.no_extras:
+        # ---- plain one-event-per-launch path on a sample ----
+        ns = min(a.plain_sample, ne)
+        plain_wall, lat = timed_plain(eng, d, off, sizes, range(ns), p, R, stream)
+        eng.stream_check(stream)
+        extras["plain"] = {"value": ns / plain_wall, "unit": "events/s", "sample_events": ns,
+                           "desc": "one kernel launch + stream sync per event, inputs resident"}
+        extras["latency_ms"] = {"plain_p50": float(np.percentile(lat, 50) * 1e3),
+                                "plain_p99": float(np.percentile(lat, 99) * 1e3),
+                                "batch_p50": float(np.percentile(step_ms, 50)),
+                                "batch_p99": float(np.percentile(step_ms, 99))}
 
-    # ---- host-memory path (PCIe-inclusive): copy in, plan, one graph launch, copy out ----
-    eng.cluster_batch(p4, off, p, R)  # warm
-    th0 = time.perf_counter()
-    eng.cluster_batch(p4, off, p, R)
-    host_path = ne / (time.perf_counter() - th0)
+        # ---- the reference's per-step launch floor: 2N+3 empty launches per event ----
+        nf = min(a.floor_sample, ne)
+        rec = torch.zeros(40 * int(sizes[:nf].max()), dtype=torch.uint8, device=dev)
+        torch.cuda.synchronize()
+        tf0 = time.perf_counter()
+        for e in range(nf):
+            eng.cluster_pseudojets(rec.data_ptr(), int(sizes[e]), p, R, engine=fj.ENGINE_EMPTY, stream=stream)
+            torch.cuda.current_stream().synchronize()
+        floor_wall = time.perf_counter() - tf0
+        launches = int(np.sum(2 * sizes[:nf] + 3))
+        extras["launch_floor"] = {"value": nf / floor_wall, "unit": "events/s", "sample_events": nf,
+                                  "us_per_launch": floor_wall / launches * 1e6,
+                                  "desc": "tri_matrix_empty schedule: 2N+3 empty kernel launches per event, "
+                                          "one sync per event (no clustering)"}
 
-    # ---- roofline of the dominant bucket kernel ----
+        # ---- host-memory path (PCIe-inclusive) ----
+        eng.cluster_batch(p4, off, p, R)  # warm
+        th0 = time.perf_counter()
+        eng.cluster_batch(p4, off, p, R)
+        extras["host_memory_path"] = {"value": ne / (time.perf_counter() - th0), "unit": "events/s",
+                                      "desc": "fjmi_cluster_batch from host arrays: H2D, plan, launch, D2H"}
+
+        # ---- single-event configurations and the high-pileup batch ----
+        cfg = {}
+        ev1 = fj.gen_event(1, 0)
+        ev2 = fj.gen_event(2, 0)
+        runs2 = [(pp, RR) for pp in (1, 0, -1) for RR in (0.4, 1.0)]
+
+        def gpu_event_ms(ev, pp, RR, reps=20):
+            n = len(ev)
+            dd = dict(p4=torch.from_numpy(np.ascontiguousarray(ev)).to(dev),
+                      h1=torch.zeros(n, dtype=torch.int32, device=dev), h2=torch.zeros(n, dtype=torch.int32, device=dev),
+                      hd=torch.zeros(n, dtype=torch.float64, device=dev),
+                      j=torch.zeros(4 * n, dtype=torch.float64, device=dev), nj=torch.zeros(1, dtype=torch.int32, device=dev))
+            one = np.array([0, n], np.int64)
+            timed_plain(eng, dd, one, np.array([n]), [0], pp, RR, stream)
+            _, lt = timed_plain(eng, dd, one, np.array([n]), [0] * reps, pp, RR, stream)
+            return float(np.mean(lt) * 1e3)
+
+        cfg["N354_antikt_0.4"] = {"gpu_plain_ms": gpu_event_ms(ev1, -1, 0.4)}
+        for pp, RR in runs2:
+            cfg[f"N808_{ {1: 'kt', 0: 'cam', -1: 'antikt'}[pp]}_{RR}"] = {"gpu_plain_ms": gpu_event_ms(ev2, pp, RR)}
+        if orc is not None:
+            def cpu_event_ms(ev, pp, RR, reps):
+                evs = np.tile(ev, (reps, 1))
+                o = np.arange(reps + 1, dtype=np.int64) * len(ev)
+                return orc.cluster_batch(evs, o, pp, RR, 1, "n2plain")[6] / reps * 1e3
+            cfg["N354_antikt_0.4"]["cpu_1thread_ms"] = cpu_event_ms(ev1, -1, 0.4, 400)
+            for pp, RR in runs2:
+                cfg[f"N808_{ {1: 'kt', 0: 'cam', -1: 'antikt'}[pp]}_{RR}"]["cpu_1thread_ms"] = cpu_event_ms(ev2, pp, RR, 100)
+        hp4, hoff = fj.gen_workload(4, 64)
+        for pp, name in ((1, "kt"), (-1, "antikt")):
+            eng.cluster_batch(hp4, hoff, pp, 0.4)
+            th = time.perf_counter()
+            eng.cluster_batch(hp4, hoff, pp, 0.4)
+            cfg[f"highpileup64_{name}_0.4"] = {"gpu_batch_events_per_s_host_path": 64 / (time.perf_counter() - th)}
+        extras["configs"] = cfg
+        print("[bench] plain, floor, host path and configs done", file=sys.stderr, flush=True)
+
+    # ---- roofline of the dominant kernel ----
     dom = max(bstats, key=lambda b: b["ms"])
-    dom_flops = flops(dom["sum_n2"], dom["particles"])
-    achieved = dom_flops / (dom["ms"] * 1e-3) / 1e12 if dom["ms"] > 0 else None
-    all_flops = flops(float(np.sum(sizes.astype(np.float64) ** 2)), float(tot))
+    t_kernel = float(np.mean(step_ms)) * 1e-3  # HIP events on the launch stream around each graph launch
     variants = {0: "tiled_kernel<512, 2, false>", 1: "tiled_kernel<384, 3, false>", 2: "tiled_kernel<256, 4, false>",
                 3: "tiled_kernel<128, 4, false>"}
-    traffic = measured_traffic(variants.get(dom["p4_global"], "?")) if dom["engine"] == 2 and ne == 4096 else None
+    kname = (f"dense_kernel<{dom['threads']}, {dom['slots']}, {str(bool(dom['p4_global'])).lower()}, 1>"
+             if dom["engine"] == 1 else variants.get(dom["p4_global"], "tiled_kernel"))
+    dense_f = dense_equivalent_flops(float(np.sum(sizes.astype(np.float64) ** 2)), float(tot))
+    alg_f = 8.0 * pairs if pairs is not None else None
+    ctr = counter_fields(t_kernel, float(tot)) if (len(bstats) == 1 and ne == 4096) else {"source": None}
+    achieved = alg_f / t_kernel / 1e12 if alg_f else None
+    roof = {"bound": "latency",
+            "limiter": "the per-step dependent chain of one wave per event (argmin, merge, gather, neighbour "
+                       "repair); the batch ends with its largest events",
+            "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS if achieved else None,
+            "traffic": None,
+            "kernel": f"{kname} (bucket N<={min(dom['max_n'], int(sizes.max()))}, {dom['n_events']} events, "
+                      f"{dom['grid']} workgroups)",
+            "kernel_ms": t_kernel * 1e3, "kernel_ms_device_clock": dom["ms"],
+            "algorithmic": {"pair_evals": pairs, "fp64_ops": alg_f,
+                            "per_unit": "8 FP64 operations per pair distance; pair distances of the tiled method "
+                                        "counted by the oracle's tiled restatement on the same events"},
+            "dense_equivalent": {"fp64_ops": dense_f, "tflops": dense_f / t_kernel / 1e12,
+                                 "frac": dense_f / t_kernel / 1e12 / FP64_PEAK_TFLOPS,
+                                 "per_unit": "8 N(N-1) + N(N+1)/2 per event (dense nearest-neighbour method)"},
+            "counters": ctr}
+    vals, psha, _ = read_profile()
+    if vals.get("traffic_bytes_per_launch") and not ctr.get("stale", True):
+        roof["traffic"] = vals["traffic_bytes_per_launch"]
 
-    # ---- CPU baseline: the oracle on the host cores, bounded sample ----
-    cpu = None
-    if not a.no_cpu:
-        try:
-            from oracle import oracle as orc
-            cs = min(a.cpu_sample, ne)
-            threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-            secs = min(orc.time_batch(p4[:off[cs]], off[:cs + 1], p, R, threads, "tiled")
-                       for _ in range(max(1, a.cpu_repeat)))
-            cs2 = min(cs, 1024)
-            secs_plain = orc.time_batch(p4[:off[cs2]], off[:cs2 + 1], p, R, threads, "n2plain")
-            cpu = {"value": cs / secs, "unit": "events/s", "cores": threads, "kind": "port",
-                   "sample": f"the first {cs} events of the same batch clustered by oracle/'s tiled N2Plain "
-                             f"(FastJet N2Tiled-style) on {threads} host threads, best of {max(1, a.cpu_repeat)}: "
-                             f"{secs:.2f} s wall; the plain N2Plain restatement did {cs2} events in "
-                             f"{secs_plain:.2f} s ({cs2 / secs_plain:.0f} events/s)"}
-        except Exception as ex:
-            cpu = {"value": None, "unit": "events/s", "cores": 0, "kind": "port", "sample": f"failed: {ex}"}
+    serving = None
+    if not a.no_extras and a.serve_seconds > 0:  # the server runs in its own process, after the legs above
+        serving = {"stream": "lognormal N (median 1500, sigma 0.6, cap 20000); {ak 0.4, ak 0.8, CA 0.8, kt 0.4}; "
+                             "16 GiB device budget", "rates": serving_leg([1e4, 1e5], a.serve_seconds)}
+        print("[bench] serving leg done", file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC,
@@ -209,27 +541,21 @@ def main():
         "config": {"workload": f"{ne} events, N~U[50,2000] (mean {sizes.mean():.0f}), anti-kt R=0.4, "
                                f"one hipGraph launch of persistent bucket kernels",
                    "events": ne, "particles": int(tot), "algorithm": "antikt", "R": R},
-        "plain": {"value": plain_value, "unit": "events/s", "sample_events": ns,
-                  "desc": "one kernel launch + stream sync per event, inputs resident"},
-        "ratio_batched_vs_plain": value / plain_value,
-        "host_memory_path": {"value": host_path, "unit": "events/s",
-                             "desc": "fjmi_cluster_batch from host arrays: H2D, plan + graph capture, launch, D2H"},
-        "latency_ms": {"plain_p50": float(np.percentile(lat, 50) * 1e3),
-                       "plain_p99": float(np.percentile(lat, 99) * 1e3),
-                       "batch_p50": float(np.percentile(step_ms, 50)),
-                       "batch_p99": float(np.percentile(step_ms, 99))},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": traffic,
-                     "kernel": (f"dense_kernel<{dom['threads']}, {dom['slots']}, {str(bool(dom['p4_global'])).lower()}, 1>"
-                                if dom["engine"] == 1 else variants.get(dom["p4_global"], "tiled_kernel"))
-                               + f" (bucket N<={min(dom['max_n'], int(sizes.max()))}, {dom['n_events']} events, "
-                               f"{dom['ms']:.2f} ms)",
-                     "fp64_ops_per_launch": dom_flops, "batch_fp64_ops": all_flops,
-                     "batch_rate_tflops": all_flops / (np.median(step_ms) * 1e-3) / 1e12},
-        "buckets": bstats,
-        "parity": parity,
-        "cpu_baseline": cpu,
     }
+    if "plain" in extras:
+        line["plain"] = extras["plain"]
+        line["ratio_batched_vs_plain"] = value / extras["plain"]["value"]
+    for k in ("launch_floor", "host_memory_path", "latency_ms"):
+        if k in extras:
+            line[k] = extras[k]
+    line["roofline"] = roof
+    line["buckets"] = bstats
+    line["parity"] = parity
+    line["cpu_baseline"] = cpu
+    if serving is not None:
+        line["serving"] = serving
+    if "configs" in extras:
+        line["configs"] = extras["configs"]
     print(json.dumps(line), flush=True)
     batch.close()
     eng.close()

@@ -105,6 +105,7 @@ def _declare(lib):
         "fjmi_cluster_event": ([_vp, _vp, i, i, d, i, _vp, _vp, _vp, _vp, _vp], i),
         "fjmi_cluster_event_device": ([_vp, _vp, i, i, d, i, _vp, _vp, _vp, _vp, _vp, _vp], i),
         "fjmi_cluster_pseudojets": ([_vp, _vp, i, i, d, i, _vp], i),
+        "fjmi_stream_check": ([_vp, _vp], i),
         "fjmi_batch_create": ([_vp, i, _vp, _vp, _vp, _vp, _vp, i, d, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)],
                               i),
         "fjmi_batch_launch": ([_vp, _vp], i),
@@ -216,6 +217,10 @@ class Engine:
     def cluster_pseudojets(self, d_particles, n, p, R, engine=ENGINE_AUTO, stream=0):
         _check(self.lib, self.lib.fjmi_cluster_pseudojets(self.ctx, d_particles, n, int(p), float(R), engine,
                                                           stream or None), "fjmi_cluster_pseudojets")
+
+    def stream_check(self, stream=0):
+        """Synchronises `stream`; raises if a one-event device call on it reported a failure."""
+        _check(self.lib, self.lib.fjmi_stream_check(self.ctx, stream or None), "fjmi_stream_check")
 
     def server(self, **cfg):
         """A continuous-batching Server on this device (see Server)."""

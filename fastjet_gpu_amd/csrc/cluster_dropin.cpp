@@ -11,8 +11,9 @@
  *
  * The engine is chosen when this file is compiled, as the reference chooses it
  * at link time (Makefile:31-50): FJMI_LINKED_ENGINE is one of the fjmi_engine
- * values.  The call runs on the null stream, so the caller's next hipMemcpy sees
- * the result.  One library context per process and device, created on first use.
+ * values.  The call runs on the null stream and returns once the kernel has
+ * finished, so the caller's next hipMemcpy sees the result.  One library context
+ * per process and device, created on first use.
  */
 #include <hip/hip_runtime_api.h>
 
@@ -53,4 +54,7 @@ void cluster(PseudoJet *particles, int size, Algorithm algo, double r) {
     g_ctx_device = device;
   }
   FJMI_CHECK(fjmi_cluster_pseudojets(g_ctx, particles, size, static_cast<int>(algo), r, FJMI_LINKED_ENGINE, nullptr));
+  /* waits for the kernel, as the reference's closing cudaFree calls do (tri_matrix.cu:332-335),
+   * and aborts if it reported a failure */
+  FJMI_CHECK(fjmi_stream_check(g_ctx, nullptr));
 }

@@ -201,12 +201,15 @@ __global__ __launch_bounds__(T, OCC) void dense_kernel(LaunchArgs a) {
   double rap[K], phi[K], w[K], nnd[K], dij[K];
   int nn[K];
 
+  /* a launch without a queue head (one event) takes queue entry 0 and leaves */
+  int pass = 0;
   for (;;) {
     __syncthreads(); /* previous event (or queue fetch) fully consumed */
-    if (t == 0) sh.ev = (int)(atomicAdd(a.head, 1u) - a.head_base);
+    if (t == 0) sh.ev = a.head ? (int)(atomicAdd(a.head, 1u) - a.head_base) : pass;
+    ++pass;
     __syncthreads();
     const int qi = uni(sh.ev);
-    if (qi >= a.n_queue) {
+    if (qi < 0 || qi >= a.n_queue) {
       if (a.t_end && threadIdx.x == 0) atomicMax(a.t_end, (unsigned long long)wall_clock64());
       break;
     }
@@ -214,7 +217,10 @@ __global__ __launch_bounds__(T, OCC) void dense_kernel(LaunchArgs a) {
     const int64_t base = a.offsets ? a.offsets[e] : 0;
     const int N = a.offsets ? uni((int)(a.offsets[e + 1] - base)) : a.n_single;
     if (N > CAP) {
-      if (t == 0) atomicOr(a.status, (unsigned)STATUS_CAPACITY);
+      if (t == 0) {
+        atomicOr(a.status, (unsigned)STATUS_CAPACITY);
+        a.njets[e] = -1; /* failed */
+      }
       continue;
     }
     if (N <= 0) {

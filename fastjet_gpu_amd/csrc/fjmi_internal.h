@@ -20,8 +20,18 @@ struct ServeDesc {
   int32_t p;
   double R;
   int32_t slot; /* event slot: result njets index, completion token - 1 */
-  int32_t pad;
+  /* written by the host as (descriptor number mod 2^31-1) + 1, and set to 0 by the
+   * GPU once it has copied the descriptor: the host writes a cell again only
+   * after that, so a claimant never reads a cell that has been reused */
+  uint32_t seq;
 };
+
+/* Serving: the sequence tag of descriptor number h (never 0, which marks a
+ * consumed cell). */
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline unsigned serve_seq(unsigned long long h) { return (unsigned)(h % 0x7FFFFFFFull) + 1u; }
 
 /* Serving: words the host writes and the kernels read (fine-grained host memory). */
 struct ServeHost {
@@ -73,7 +83,7 @@ struct LaunchArgs {
   /* event queue */
   const int32_t *queue; /* event indices, processed in order; nullptr: identity */
   int n_queue;
-  unsigned int *head;      /* queue head counter */
+  unsigned int *head;      /* queue head counter; nullptr: each workgroup takes entry 0 once (one event) */
   unsigned int head_base;  /* value of *head when the launch starts */
   /* per-event parameters (nullptr -> uniform values) */
   const int32_t *p_arr;
@@ -137,7 +147,7 @@ constexpr size_t kTiledBytesPerParticle =
     sizeof(JetRec) + 32 + 4 + sizeof(unsigned short) + kTiledPoolPerParticle * kTiledPoolEntryBytes;
 
 /* Bytes of tiled scratch for `particles` particles in `events` events. */
-inline size_t tiled_scratch_bytes(int64_t particles, int64_t events, int pool_per_event) {
+constexpr size_t tiled_scratch_bytes(int64_t particles, int64_t events, int pool_per_event) {
   return (size_t)particles * kTiledBytesPerParticle + (size_t)events * pool_per_event * kTiledPoolEntryBytes + 1024;
 }
 

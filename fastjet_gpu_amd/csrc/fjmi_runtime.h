@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -58,16 +59,32 @@ struct DevBuf {
 }  // namespace rt
 }  // namespace fjmi
 
+namespace fjmi {
+namespace rt {
+
+/* Device state of the one-event entry points on one stream: their scratch and
+ * outputs, and the status word the kernels raise on failure.  Launches on one
+ * stream are ordered, so they may share it; launches on different streams each
+ * get their own. */
+struct StreamState {
+  DevBuf status; /* 4 bytes, zeroed at creation and by fjmi_stream_check */
+  DevBuf hist1, hist2, hdij, jets, njets, p4s, coords, p4in, tscratch;
+  void release_all() {
+    for (DevBuf *b : {&status, &hist1, &hist2, &hdij, &jets, &njets, &p4s, &coords, &p4in, &tscratch}) b->release();
+  }
+};
+
+}  // namespace rt
+}  // namespace fjmi
+
 /* Per-device library context. */
 struct fjmi_context {
   int device = 0;
   int n_cu = 0;
   std::vector<fjmi_bucket> buckets;
   std::mutex mtx;
-  /* one-event path */
-  fjmi::rt::DevBuf ctrl;
-  uint32_t head_base = 0;
-  fjmi::rt::DevBuf hist1, hist2, hdij, jets, njets, p4s, coords, p4in, tscratch;
+  /* one-event paths: per-stream state (key: the hipStream_t, nullptr for the null stream) */
+  std::map<void *, fjmi::rt::StreamState> streams;
   int max_lds = 0;
 };
 

@@ -63,13 +63,6 @@ size_t dyn_lds_bytes(const KernelDesc &d) {
   return 32 * ch + 4 * cap;
 }
 
-/* Control block of the one-event-per-launch path: the queue head (monotonic;
- * each launch consumes 2 counts) and the status word. */
-struct Ctrl {
-  uint32_t head;
-  uint32_t status;
-};
-
 }  // namespace
 
 
@@ -176,10 +169,43 @@ int launch_dense(const KernelDesc &kd, int grid, size_t lds, const LaunchArgs &a
   return FJMI_OK;
 }
 
-/* One event on the device: one kernel launch with one workgroup. */
-int cluster_one_device(fjmi_context *ctx, int engine, const double *d_in, int in_stride, int in_off, int n, int p,
-                       double R, int32_t *h1, int32_t *h2, double *hd, double *jets, int32_t *nj, unsigned char *pj,
-                       hipStream_t stream) {
+/* The state of the one-event entry points on `stream`, created on its first use
+ * (its status word zeroed synchronously there).  Called with ctx->mtx held. */
+int stream_state(fjmi_context *ctx, hipStream_t stream, fjmi::rt::StreamState **out) {
+  auto it = ctx->streams.find(static_cast<void *>(stream));
+  if (it == ctx->streams.end()) {
+    fjmi::rt::StreamState st;
+    int rc = st.status.ensure(sizeof(uint32_t));
+    if (rc) return rc;
+    hipError_t e = hipMemset(st.status.ptr, 0, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      st.release_all();
+      return fail(FJMI_EHIP, "status word init: %s", hipGetErrorString(e));
+    }
+    it = ctx->streams.emplace(static_cast<void *>(stream), st).first;
+  }
+  *out = &it->second;
+  return FJMI_OK;
+}
+
+/* Zeroes n 32-bit words on `stream` with a fill kernel (no memset node, so a
+ * caller may capture the call into a graph). */
+int zero_words(void *p, int n, hipStream_t stream) {
+  unsigned *w = static_cast<unsigned *>(p);
+  unsigned long long *t = nullptr;
+  int nt = 0;
+  void *params[] = {&w, &n, &t, &nt};
+  HIPCHK(hipLaunchKernel(fjmi::kZeroKernel, dim3(1), dim3(64), params, 0, stream));
+  return FJMI_OK;
+}
+
+/* One event on the device: one kernel launch with one workgroup and no queue
+ * head (the kernel takes the single queue entry and leaves), using the scratch
+ * of `st`, the state of `stream`. */
+int cluster_one_device(fjmi_context *ctx, fjmi::rt::StreamState &st, int engine, const double *d_in, int in_stride,
+                       int in_off, int n, int p, double R, int32_t *h1, int32_t *h2, double *hd, double *jets,
+                       int32_t *nj, unsigned char *pj, hipStream_t stream) {
   /* engine forced to tiled: the table's tiled row for n, else tile side R with the 512-tile variant */
   const fjmi_bucket *b = bucket_for(ctx, n, engine);
   if (!b && engine == FJMI_ENGINE_TILED) b = &kTiledRow;
@@ -189,18 +215,16 @@ int cluster_one_device(fjmi_context *ctx, int engine, const double *d_in, int in
   if (b->engine == FJMI_ENGINE_TILED && n > fjmi::kTiledMaxN)
     return fail(FJMI_ECAPACITY, "event with %d particles: the tiled engine takes at most %d", n, fjmi::kTiledMaxN);
   int rc;
-  if ((rc = ctx->ctrl.ensure(sizeof(Ctrl)))) return rc;
   const bool tiled = b->engine == FJMI_ENGINE_TILED;
   const KernelDesc *kd = tiled ? nullptr : find_kernel(b->threads, b->slots, b->p4_global);
   fjmi::TiledScratch ts{};
   if (tiled) {
-    if ((rc = ctx->tscratch.ensure(fjmi::tiled_scratch_bytes(n, 1, fjmi::kTiledPoolPerEvent)))) return rc;
-    ts = fjmi::tiled_scratch_carve(ctx->tscratch.ptr, n, fjmi::kTiledPoolPerEvent);
+    if ((rc = st.tscratch.ensure(fjmi::tiled_scratch_bytes(n, 1, fjmi::kTiledPoolPerEvent)))) return rc;
+    ts = fjmi::tiled_scratch_carve(st.tscratch.ptr, n, fjmi::kTiledPoolPerEvent);
   } else if (kd->p4g) {
-    if ((rc = ctx->p4s.ensure(sizeof(double) * 4 * (size_t)n))) return rc;
-    if ((rc = ctx->coords.ensure(sizeof(double) * 4 * (size_t)n))) return rc;
+    if ((rc = st.p4s.ensure(sizeof(double) * 4 * (size_t)n))) return rc;
+    if ((rc = st.coords.ensure(sizeof(double) * 4 * (size_t)n))) return rc;
   }
-  Ctrl *c = ctx->ctrl.as<Ctrl>();
   LaunchArgs a = base_args();
   a.in = d_in;
   a.in_stride = in_stride;
@@ -209,8 +233,8 @@ int cluster_one_device(fjmi_context *ctx, int engine, const double *d_in, int in
   a.n_single = n;
   a.queue = nullptr;
   a.n_queue = 1;
-  a.head = &c->head;
-  a.head_base = ctx->head_base;
+  a.head = nullptr; /* one pass: queue entry 0 */
+  a.head_base = 0;
   a.p_uni = p;
   a.R_uni = R;
   a.hist_p1 = h1;
@@ -219,9 +243,9 @@ int cluster_one_device(fjmi_context *ctx, int engine, const double *d_in, int in
   a.jets = jets;
   a.njets = nj;
   a.pj_out = pj;
-  a.p4_scratch = ctx->p4s.as<double>();
-  a.coord_scratch = ctx->coords.as<double>();
-  a.status = &c->status;
+  a.p4_scratch = st.p4s.as<double>();
+  a.coord_scratch = st.coords.as<double>();
+  a.status = st.status.as<unsigned int>();
   a.tile_q = tiled ? b->slots : 0;
   if (tiled) {
     void *params[] = {&a, &ts};
@@ -229,8 +253,14 @@ int cluster_one_device(fjmi_context *ctx, int engine, const double *d_in, int in
   } else if ((rc = launch_dense(*kd, 1, dyn_lds_bytes(*kd), a, stream))) {
     return rc;
   }
-  ctx->head_base += 2; /* one successful and one failing queue pop */
   return FJMI_OK;
+}
+
+/* The fjmi_status of a kernel status word. */
+int status_error(uint32_t status) {
+  if (status & fjmi::STATUS_INTERNAL)
+    return fail(FJMI_EINTERNAL, "a clustering kernel found an inconsistent state (status 0x%x)", status);
+  return fail(FJMI_ECAPACITY, "a clustering kernel reported an event beyond its capacity (status 0x%x)", status);
 }
 
 int check_engine(int engine) {
@@ -308,16 +338,6 @@ int fjmi_create(int device, fjmi_context **out) {
     return rc;
   }
   ctx->buckets.assign(kDefaultBuckets, kDefaultBuckets + nb);
-  if ((rc = ctx->ctrl.ensure(sizeof(Ctrl)))) {
-    delete ctx;
-    return rc;
-  }
-  e = hipMemset(ctx->ctrl.ptr, 0, sizeof(Ctrl));
-  if (e == hipSuccess) e = hipDeviceSynchronize();
-  if (e != hipSuccess) {
-    delete ctx;
-    return fail(FJMI_EHIP, "control block init: %s", hipGetErrorString(e));
-  }
   *out = ctx;
   return FJMI_OK;
 }
@@ -326,9 +346,7 @@ int fjmi_destroy(fjmi_context *ctx) {
   if (!ctx) return FJMI_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
-  for (DevBuf *b : {&ctx->ctrl, &ctx->hist1, &ctx->hist2, &ctx->hdij, &ctx->jets, &ctx->njets, &ctx->p4s,
-                    &ctx->coords, &ctx->p4in, &ctx->tscratch})
-    b->release();
+  for (auto &kv : ctx->streams) kv.second.release_all();
   delete ctx;
   return FJMI_OK;
 }
@@ -358,14 +376,14 @@ int fjmi_cluster_event_device(fjmi_context *ctx, const double *d_p4, int n, int 
   if (rc) return rc;
   if ((rc = check_engine(engine))) return rc;
   if (n > 0 && (!d_p4 || !d_h1 || !d_h2 || !d_hd || !d_jets || !d_nj)) return fail(FJMI_EINVAL, "null buffer");
+  if (n == 0 && !d_nj) return fail(FJMI_EINVAL, "null njets");
   std::lock_guard<std::mutex> lk(ctx->mtx);
   if ((rc = set_device(ctx))) return rc;
-  if (n == 0) {
-    HIPCHK(hipMemsetAsync(d_nj, 0, sizeof(int32_t), (hipStream_t)stream));
-    return FJMI_OK;
-  }
-  return cluster_one_device(ctx, engine, d_p4, 4, 0, n, p, R, d_h1, d_h2, d_hd, d_jets, d_nj, nullptr,
-                            (hipStream_t)stream);
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (n == 0) return zero_words(d_nj, 1, s);
+  fjmi::rt::StreamState *st = nullptr;
+  if ((rc = stream_state(ctx, s, &st))) return rc;
+  return cluster_one_device(ctx, *st, engine, d_p4, 4, 0, n, p, R, d_h1, d_h2, d_hd, d_jets, d_nj, nullptr, s);
 }
 
 int fjmi_cluster_pseudojets(fjmi_context *ctx, void *d_particles, int n, int p, double R, int engine, void *stream) {
@@ -389,14 +407,33 @@ int fjmi_cluster_pseudojets(fjmi_context *ctx, void *d_particles, int n, int p, 
     return FJMI_OK;
   }
   if ((rc = check_engine(engine))) return rc;
+  fjmi::rt::StreamState *st = nullptr;
+  if ((rc = stream_state(ctx, s, &st))) return rc;
   const size_t nn = (size_t)n;
-  if ((rc = ctx->hist1.ensure(4 * nn)) || (rc = ctx->hist2.ensure(4 * nn)) || (rc = ctx->hdij.ensure(8 * nn)) ||
-      (rc = ctx->jets.ensure(32 * nn)) || (rc = ctx->njets.ensure(4)))
+  if ((rc = st->hist1.ensure(4 * nn)) || (rc = st->hist2.ensure(4 * nn)) || (rc = st->hdij.ensure(8 * nn)) ||
+      (rc = st->jets.ensure(32 * nn)) || (rc = st->njets.ensure(4)))
     return rc;
-  return cluster_one_device(ctx, engine, static_cast<const double *>(d_particles), 5, 1, n, p, R,
-                            ctx->hist1.as<int32_t>(),
-                            ctx->hist2.as<int32_t>(), ctx->hdij.as<double>(), ctx->jets.as<double>(),
-                            ctx->njets.as<int32_t>(), static_cast<unsigned char *>(d_particles), s);
+  return cluster_one_device(ctx, *st, engine, static_cast<const double *>(d_particles), 5, 1, n, p, R,
+                            st->hist1.as<int32_t>(), st->hist2.as<int32_t>(), st->hdij.as<double>(),
+                            st->jets.as<double>(), st->njets.as<int32_t>(), static_cast<unsigned char *>(d_particles),
+                            s);
+}
+
+int fjmi_stream_check(fjmi_context *ctx, void *stream) {
+  if (!ctx) return fail(FJMI_EINVAL, "null context");
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  HIPCHK(hipStreamSynchronize(s));
+  auto it = ctx->streams.find(static_cast<void *>(s));
+  if (it == ctx->streams.end()) return FJMI_OK; /* nothing launched on it */
+  uint32_t status = 0;
+  HIPCHK(hipMemcpy(&status, it->second.status.ptr, sizeof status, hipMemcpyDeviceToHost));
+  if (!status) return FJMI_OK;
+  const uint32_t zero = 0;
+  HIPCHK(hipMemcpy(it->second.status.ptr, &zero, sizeof zero, hipMemcpyHostToDevice));
+  return status_error(status);
 }
 
 int fjmi_cluster_event(fjmi_context *ctx, const double *p4, int n, int p, double R, int engine, int32_t *h1,
@@ -413,27 +450,31 @@ int fjmi_cluster_event(fjmi_context *ctx, const double *p4, int n, int p, double
   if ((rc = check_engine(engine))) return rc;
   std::lock_guard<std::mutex> lk(ctx->mtx);
   if ((rc = set_device(ctx))) return rc;
+  fjmi::rt::StreamState *st = nullptr;
+  if ((rc = stream_state(ctx, nullptr, &st))) return rc;
   const size_t nn = (size_t)n;
-  if ((rc = ctx->p4in.ensure(32 * nn)) || (rc = ctx->hist1.ensure(4 * nn)) || (rc = ctx->hist2.ensure(4 * nn)) ||
-      (rc = ctx->hdij.ensure(8 * nn)) || (rc = ctx->jets.ensure(32 * nn)) || (rc = ctx->njets.ensure(4)))
+  if ((rc = st->p4in.ensure(32 * nn)) || (rc = st->hist1.ensure(4 * nn)) || (rc = st->hist2.ensure(4 * nn)) ||
+      (rc = st->hdij.ensure(8 * nn)) || (rc = st->jets.ensure(32 * nn)) || (rc = st->njets.ensure(4)))
     return rc;
-  HIPCHK(hipMemcpy(ctx->p4in.ptr, p4, 32 * nn, hipMemcpyHostToDevice));
-  Ctrl *c = ctx->ctrl.as<Ctrl>();
-  HIPCHK(hipMemsetAsync(&c->status, 0, sizeof(uint32_t), nullptr));
-  if ((rc = cluster_one_device(ctx, engine, ctx->p4in.as<double>(), 4, 0, n, p, R, ctx->hist1.as<int32_t>(),
-                               ctx->hist2.as<int32_t>(), ctx->hdij.as<double>(), ctx->jets.as<double>(),
-                               ctx->njets.as<int32_t>(), nullptr, nullptr)))
+  HIPCHK(hipMemcpy(st->p4in.ptr, p4, 32 * nn, hipMemcpyHostToDevice));
+  if ((rc = cluster_one_device(ctx, *st, engine, st->p4in.as<double>(), 4, 0, n, p, R, st->hist1.as<int32_t>(),
+                               st->hist2.as<int32_t>(), st->hdij.as<double>(), st->jets.as<double>(),
+                               st->njets.as<int32_t>(), nullptr, nullptr)))
     return rc;
   HIPCHK(hipStreamSynchronize(nullptr));
   uint32_t status = 0;
-  HIPCHK(hipMemcpy(&status, &c->status, sizeof status, hipMemcpyDeviceToHost));
-  if (status) return fail(FJMI_ECAPACITY, "kernel reported status 0x%x", status);
-  HIPCHK(hipMemcpy(h1, ctx->hist1.ptr, 4 * nn, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(h2, ctx->hist2.ptr, 4 * nn, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(hd, ctx->hdij.ptr, 8 * nn, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(nj, ctx->njets.ptr, 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&status, st->status.ptr, sizeof status, hipMemcpyDeviceToHost));
+  if (status) {
+    const uint32_t zero = 0;
+    HIPCHK(hipMemcpy(st->status.ptr, &zero, sizeof zero, hipMemcpyHostToDevice));
+    return status_error(status);
+  }
+  HIPCHK(hipMemcpy(h1, st->hist1.ptr, 4 * nn, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(h2, st->hist2.ptr, 4 * nn, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(hd, st->hdij.ptr, 8 * nn, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(nj, st->njets.ptr, 4, hipMemcpyDeviceToHost));
   if (*nj < 0 || *nj > n) return fail(FJMI_EHIP, "corrupt jet count %d", *nj);
-  HIPCHK(hipMemcpy(jets, ctx->jets.ptr, 32 * (size_t)(*nj), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(jets, st->jets.ptr, 32 * (size_t)(*nj), hipMemcpyDeviceToHost));
   return FJMI_OK;
 }
 
@@ -609,7 +650,7 @@ int fjmi_batch_sync(fjmi_batch *b, void *stream) {
   HIPCHK(hipStreamSynchronize((hipStream_t)stream));
   uint32_t st = 0;
   HIPCHK(hipMemcpy(&st, b->d_status, sizeof st, hipMemcpyDeviceToHost));
-  if (st) return fail(FJMI_ECAPACITY, "kernel reported status 0x%x", st);
+  if (st) return status_error(st);
   return FJMI_OK;
 }
 

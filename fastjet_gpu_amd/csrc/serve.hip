@@ -27,8 +27,12 @@
  *            clear the entries; relaunch the kernel if it has left (idle
  *            timeout) while published events are still unfinished;
  *   release  return the event's ranges and slot.
- * At most `slots` events are admitted at once, so neither ring can wrap onto an
- * entry still in use.
+ * At most `slots` events are admitted at once, so the completion ring cannot
+ * wrap onto an entry still in use.  The descriptor ring could, if a workgroup
+ * stalled between claiming descriptor h and copying it while later events
+ * completed and were released: each descriptor therefore carries a sequence tag
+ * that the GPU clears after its copy, and submit() reuses a cell only once it
+ * has been cleared (FJMI_EAGAIN until then).
  */
 #include <hip/hip_runtime.h>
 
@@ -326,6 +330,9 @@ int fjmi_server_submit(fjmi_server *s, const double *p4, int n, int p, double R,
   if (n > fjmi::kTiledMaxN)
     return fail(FJMI_ECAPACITY, "event of %d particles: the tiled engine takes at most %d", n, fjmi::kTiledMaxN);
   if (s->free_slots.empty()) return fail(FJMI_EAGAIN, "all %u event slots in use", s->E);
+  fjmi::ServeDesc &d = s->ring[s->published % s->E];
+  if (__atomic_load_n(&d.seq, __ATOMIC_ACQUIRE) != 0u) /* the GPU has not yet copied the descriptor E back */
+    return fail(FJMI_EAGAIN, "descriptor ring full");
   const int64_t base = s->parts.alloc(alloc_len(n));
   if (base < 0) return fail(FJMI_EAGAIN, "particle arena full");
   const int64_t pool_off = s->pool.alloc(pool_len(n));
@@ -342,14 +349,13 @@ int fjmi_server_submit(fjmi_server *s, const double *p4, int n, int p, double R,
   sl.n = n;
   sl.state = 1;
   if (n > 0) std::memcpy(s->in_p4 + 4 * base, p4, 32 * (size_t)n);
-  fjmi::ServeDesc &d = s->ring[s->published % s->E];
   d.base = base;
   d.pool_off = pool_off;
   d.n = n;
   d.p = p;
   d.R = R;
   d.slot = slot;
-  d.pad = 0;
+  __atomic_store_n(&d.seq, fjmi::serve_seq(s->published), __ATOMIC_RELAXED); /* ordered by the release below */
   s->published += 1;
   __atomic_store_n(&s->host->tail, (unsigned long long)s->published, __ATOMIC_RELEASE);
   s->in_flight += 1;

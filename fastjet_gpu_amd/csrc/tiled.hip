@@ -51,6 +51,14 @@ constexpr int W = 64;       /* one wave per event */
 constexpr int MAXG = 32;    /* tiles gathered per step (3 x 3x3 + 1 <= 28) */
 constexpr int MAXO = 128;   /* orphans listed per step (more: rescan in rounds) */
 constexpr int BIG = 0x7fffffff;
+/* The gather counter restarts (every tile's stamp cleared) when it reaches this
+ * value, so a long event cannot wrap it onto stamps still in the table; a
+ * restart within a step only makes the orphan rescans' coverage test (stamp ==
+ * the step's gather) fail, i.e. take the fresh-gather path. */
+#ifndef FJMI_TILED_STAMP_LIMIT
+#define FJMI_TILED_STAMP_LIMIT 0xFFFFFF00u
+#endif
+constexpr unsigned kStampLimit = FJMI_TILED_STAMP_LIMIT;
 
 /* A tile minimum's tie-break key: the minimum jet's label in the high half and
  * its nearest neighbour's index (0xFFFF: none) in the low half, offset so that
@@ -97,7 +105,7 @@ struct Shared {
   Gather g[2]; /* [0]: the step's gather, [1]: orphan rescans */
   int orph[MAXO];
   int norph;
-  int stamp[MAXT]; /* last gather that listed the tile (dedup) */
+  unsigned stamp[MAXT]; /* last gather that listed the tile (dedup) */
   int pool_top;
   int rebuild;
   int ev;
@@ -426,7 +434,7 @@ __device__ void refresh_from_regs(SH &sh, const Gather &G, int ng, int total, in
 }
 
 template <class SH>
-__device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, int &stampv, int *total = nullptr);
+__device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &stampv, int *total = nullptr);
 
 /* Tile minima of a several-chunk gather, incrementally.  The step changed the
  * entries of the jets listed in the member table (nchg of them: jets that took
@@ -439,7 +447,7 @@ __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, int &stampv, 
  * entries enter the tiles' minima by comparison. */
 template <class SH>
 __device__ void refresh_changed(SH &sh, const Gather &G, Gather &GF, const Ev &ev, int ng, int phys_hi,
-                                int lane, int &stampv) {
+                                int lane, unsigned &stampv) {
   int cand = -1;
   if (lane < ng) {
     const int t = G.tiles[lane];
@@ -488,7 +496,7 @@ __device__ void refresh_changed(SH &sh, const Gather &G, Gather &GF, const Ev &e
  * written into every listed tile's stamp. */
 template <class SH>
 __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, int c2, int extra, int lane,
-                            int &stampv, int *total = nullptr) {
+                            unsigned &stampv, int *total = nullptr) {
   int cand = -1;
   if (lane < 27) {
     const int c = lane < 9 ? c0 : (lane < 18 ? c1 : c2);
@@ -506,7 +514,12 @@ __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, i
 /* The gather of the tiles the lanes propose (cand >= 0; duplicates dropped);
  * returns the number of tiles and, through `total`, of members. */
 template <class SH>
-__device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, int &stampv, int *total) {
+__device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &stampv, int *total) {
+  if (stampv >= kStampLimit) { /* the counter starts again from zero before it could wrap */
+    wave_sync();
+    for (int t = lane; t < (int)(sizeof sh.stamp / sizeof sh.stamp[0]); t += W) sh.stamp[t] = 0u;
+    stampv = 0;
+  }
   stampv += 1;
   wave_sync(); /* the callers' LDS writes (segment counts) come first */
   bool keep = false;
@@ -633,7 +646,7 @@ __device__ int nn_scan(const SH &sh, const Gather &G, const Ev &ev, int ng, int 
  * chunk, scanned faster in one pass. */
 template <int NCH, class SH>
 __device__ int nn_rescan(SH &sh, Gather &GO, const Geo &geo, const Ev &ev, int o, int otile, double oy, double ophi,
-                         double R2, int lane, int &stampv, double *best_d, double *best_w) {
+                         double R2, int lane, unsigned &stampv, double *best_d, double *best_w) {
   int ngo = gather_tiles(sh, GO, lane == 0 ? otile : -1, lane, stampv);
   double d1, w1;
   int l1;
@@ -763,8 +776,8 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
   static_assert(MAXT % W == 0, "the argmin loads MAXT / W tile minima per lane");
   __shared__ Shared<MAXT> sh;
   const int lane = threadIdx.x;
-  for (int t = lane; t < MAXT; t += W) sh.stamp[t] = 0;
-  int stampv = 0; /* gather counter; the stamps start below it */
+  unsigned stampv = 0; /* gather counter; the stamps are cleared below it at each event */
+  int pass = 0;        /* a launch without a queue head (one event) takes queue entry 0 and leaves */
   for (;;) {
     wave_sync();
     if (lane == 0) {
@@ -772,21 +785,29 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         const long long h = serve_claim(a.serve);
         sh.ev = h < 0 ? -1 : 0;
         if (h >= 0) {
-          const ServeDesc *d = a.serve.ring + (unsigned long long)h % a.serve.ring_cap;
+          ServeDesc *d = const_cast<ServeDesc *>(a.serve.ring) + (unsigned long long)h % a.serve.ring_cap;
+          const unsigned want = serve_seq((unsigned long long)h);
+          const unsigned got = __hip_atomic_load(&d->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
           sh.sd.base = d->base;
           sh.sd.pool_off = d->pool_off;
           sh.sd.n = d->n;
           sh.sd.p = d->p;
           sh.sd.R = d->R;
           sh.sd.slot = d->slot;
+          if (got != want) { /* the host broke the ring protocol: fail the launch's status, skip nothing */
+            atomicOr(a.status, (unsigned)STATUS_INTERNAL);
+          }
+          /* the cell may be written again once the copy above is complete */
+          __hip_atomic_store(&d->seq, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
       } else {
-        sh.ev = (int)(atomicAdd(a.head, 1u) - a.head_base);
+        sh.ev = a.head ? (int)(atomicAdd(a.head, 1u) - a.head_base) : pass;
       }
     }
+    ++pass;
     wave_sync();
     const int qi = uni(sh.ev);
-    if (SERVE ? qi < 0 : qi >= a.n_queue) {
+    if (SERVE ? qi < 0 : (qi < 0 || qi >= a.n_queue)) {
       if (a.t_end && threadIdx.x == 0) atomicMax(a.t_end, (unsigned long long)wall_clock64());
       break;
     }
@@ -813,7 +834,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       if constexpr (SERVE) {
         serve_finish(a, e, base, 0, -1, lane);
       } else if (lane == 0) {
-        a.njets[e] = 0;
+        a.njets[e] = -1; /* failed */
       }
       continue;
     }
@@ -827,6 +848,8 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
     }
     const double R2 = R * R;
     const double invR2 = 1.0 / R2;
+    for (int t = lane; t < MAXT; t += W) sh.stamp[t] = 0;
+    stampv = 0;
     Ev ev;
     ev.rec = ts.rec + base;
     ev.p4 = reinterpret_cast<double4 *>(ts.p4) + base;
@@ -1235,7 +1258,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       /* 3. gather the 3x3 tiles around the old and new positions and the relabelled jet's tile */
       int total;
       const int ng = build_gather(sh, G, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane, stampv, &total);
-      const int main_stamp = stampv;
+      const unsigned main_stamp = stampv;
       if (lane == 0) {
         sh.norph = 0;
         sh.nchg = 0;
@@ -1487,7 +1510,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
     if constexpr (SERVE) {
       serve_finish(a, e, base, N, bad ? -1 : nj, lane);
     } else if (lane == 0) {
-      a.njets[e] = nj;
+      a.njets[e] = bad ? -1 : nj; /* -1: the clustering failed */
     }
   }
 }

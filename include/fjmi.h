@@ -12,7 +12,8 @@
  *   beam step:   (parent, -1), dij = d_iB / R^2;
  * history indices are FastJet's: particles 0..N-1, the jet made at step s is N+s.
  * jets[4j..4j+3] = (px, py, pz, E) of the j-th jet promoted to the beam, and
- * njets = number of jets.  In the batched calls every per-particle array is
+ * njets = number of jets (-1 if a kernel failed on the event: then the calls
+ * that report errors also return one).  In the batched calls every per-particle array is
  * indexed like the input particles (event e starts at offsets[e]), and njets has
  * one entry per event.
  *
@@ -40,7 +41,8 @@ enum fjmi_status {
   FJMI_ECAPACITY = -3,   /* an event is larger than any configured kernel */
   FJMI_ENOMEM = -4,      /* allocation failed / over the memory budget */
   FJMI_EUNSUPPORTED = -5, /* engine not available */
-  FJMI_EAGAIN = -6        /* serving: the memory budget or the event slots are full; poll, release, retry */
+  FJMI_EAGAIN = -6,       /* serving: the memory budget or the event slots are full; poll, release, retry */
+  FJMI_EINTERNAL = -7     /* a kernel found an inconsistent clustering state (its event's output is invalid) */
 };
 
 enum fjmi_engine {
@@ -100,7 +102,12 @@ int fjmi_cluster_event(fjmi_context *ctx, const double *p4, int n, int p, double
                        int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets);
 
 /* Single event, device memory in and out, asynchronous on `stream`: one kernel
- * launch (the "plain" one-event-per-launch path).  d_p4 holds 4n doubles. */
+ * launch (the "plain" one-event-per-launch path).  d_p4 holds 4n doubles.
+ * The one-event device entry points (this one and fjmi_cluster_pseudojets) keep
+ * their scratch and a status word per stream: calls on different streams may
+ * overlap, calls on one stream run in order.  A failure inside the kernel
+ * (an event beyond the engine's capacity, an inconsistent state) is reported by
+ * the next fjmi_stream_check on that stream, not by the call itself. */
 int fjmi_cluster_event_device(fjmi_context *ctx, const double *d_p4, int n, int p, double R, int engine,
                               int32_t *d_hist_p1, int32_t *d_hist_p2, double *d_hist_dij, double *d_jets,
                               int32_t *d_njets, void *stream);
@@ -110,6 +117,13 @@ int fjmi_cluster_event_device(fjmi_context *ctx, const double *d_p4, int n, int 
  * of tri_matrix.cu:240-245, 272-284): afterwards records with isJet hold the jets.
  * Asynchronous on `stream`. */
 int fjmi_cluster_pseudojets(fjmi_context *ctx, void *d_particles, int n, int p, double R, int engine, void *stream);
+
+/* Synchronises `stream` and reports (then clears) any failure raised by the
+ * kernels of the one-event device entry points launched on it since the last
+ * check: FJMI_OK, FJMI_ECAPACITY or FJMI_EINTERNAL.  The reference's cluster()
+ * has no error return; its engines abort on CUDA errors (cudaCheck.h:11-47) and
+ * the drop-in cluster() calls this to do the same. */
+int fjmi_stream_check(fjmi_context *ctx, void *stream);
 
 /* Batched clustering of many events in ONE graph launch.  Not in the reference
  * (it clusters one event per call, main.cc:269-368).  The plan sorts events into

@@ -456,6 +456,10 @@ static inline double tj_dij(const tj_t *J, int p) {
   return J[p].nnd * kt2;
 }
 
+/* pair distances evaluated by the tiled restatement on this thread (the
+ * algorithmic work of the tiled method; read by fjo_tiled_pair_evals) */
+static _Thread_local int64_t t_pair_evals;
+
 /* lexicographic (distance, label) nearest neighbour of p among the 3x3 tiles */
 static void tj_rescan(tj_t *J, const int *head, const tgeom_t *g, int p, double R2) {
   int nb[9];
@@ -465,6 +469,7 @@ static void tj_rescan(tj_t *J, const int *head, const tgeom_t *g, int p, double 
   for (int m = 0; m < k; ++m)
     for (int q = head[nb[m]]; q >= 0; q = J[q].next) {
       if (q == p) continue;
+      ++t_pair_evals;
       double d = fjmi_dr2(J[p].eta, J[p].phi, J[q].eta, J[q].phi);
       if (d < best || (d == best && bn >= 0 && J[q].label < bl)) {
         best = d;
@@ -612,6 +617,7 @@ int fjo_cluster_tiled(const double *p4, int n, int p, double R, int mm, int32_t 
       for (int m = 0; m < k; ++m)
         for (int q = head[nb[m]]; q >= 0; q = J[q].next) {
           if (q == plo) continue;
+          ++t_pair_evals;
           double d = fjmi_dr2(J[q].eta, J[q].phi, J[plo].eta, J[plo].phi);
           if (d < J[q].nnd) { /* orphans were rescanned against the new jet already */
             J[q].nnd = d;
@@ -638,14 +644,37 @@ int fjo_cluster_tiled(const double *p4, int n, int p, double R, int mm, int32_t 
   return 0;
 }
 
+int64_t fjo_tiled_pair_evals(const double *p4, int n, int p, double R) {
+  if (n <= 0) return 0;
+  int32_t *h1 = (int32_t *)malloc(sizeof(int32_t) * (size_t)n), *h2 = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+  double *hd = (double *)malloc(sizeof(double) * (size_t)n), *jt = (double *)malloc(sizeof(double) * 4 * (size_t)n);
+  int32_t nj = 0;
+  int64_t r = -1;
+  if (h1 && h2 && hd && jt) {
+    t_pair_evals = 0;
+    if (fjo_cluster_tiled(p4, n, p, R, FJO_MATH_SHARED, h1, h2, hd, jt, &nj) == 0) r = t_pair_evals;
+  }
+  free(h1);
+  free(h2);
+  free(hd);
+  free(jt);
+  return r;
+}
+
 /* ---------------------------------------------------------------------------
- * CPU baseline timing: events spread over threads, n2plain or tiled, shared math.
+ * Batches on host threads: the CPU baseline (timed) and the parity reference
+ * for whole GPU batches (outputs kept).
  * ------------------------------------------------------------------------- */
 typedef struct {
   const double *p4;
   const int64_t *off;
-  int n_events, p, nthreads, tid, method;
+  const int32_t *p_arr;
+  const double *R_arr;
+  int n_events, p, nthreads, tid, method, mm;
   double R;
+  int32_t *o1, *o2, *onj;
+  double *od, *oj;
+  int64_t *opairs;
   int err;
 } tb_arg_t;
 
@@ -663,11 +692,19 @@ static void *tb_worker(void *vp) {
   if (!h1 || !h2 || !hd || !jt) a->err = 1;
   for (int e = a->tid; e < a->n_events && !a->err; e += a->nthreads) {
     int32_t nj;
-    const double *q = a->p4 + 4 * a->off[e];
-    const int n = (int)(a->off[e + 1] - a->off[e]);
-    int rc = a->method ? fjo_cluster_tiled(q, n, a->p, a->R, FJO_MATH_SHARED, h1, h2, hd, jt, &nj)
-                       : fjo_cluster_n2plain(q, n, a->p, a->R, FJO_MATH_SHARED, h1, h2, hd, jt, &nj);
+    const int64_t b = a->off[e];
+    const double *q = a->p4 + 4 * b;
+    const int n = (int)(a->off[e + 1] - b);
+    const int pe = a->p_arr ? a->p_arr[e] : a->p;
+    const double Re = a->R_arr ? a->R_arr[e] : a->R;
+    t_pair_evals = 0;
+    int32_t *w1 = a->o1 ? a->o1 + b : h1, *w2 = a->o2 ? a->o2 + b : h2;
+    double *wd = a->od ? a->od + b : hd, *wj = a->oj ? a->oj + 4 * b : jt;
+    int rc = a->method ? fjo_cluster_tiled(q, n, pe, Re, a->mm, w1, w2, wd, wj, &nj)
+                       : fjo_cluster_n2plain(q, n, pe, Re, a->mm, w1, w2, wd, wj, &nj);
     if (rc) a->err = 1;
+    if (a->onj) a->onj[e] = nj;
+    if (a->opairs) a->opairs[e] = a->method ? t_pair_evals : -1;
   }
   free(h1);
   free(h2);
@@ -676,16 +713,23 @@ static void *tb_worker(void *vp) {
   return NULL;
 }
 
-double fjo_time_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R, int nthreads,
-                      int method) {
-  if (nthreads < 1 || nthreads > 1024) return -1.0;
+double fjo_cluster_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R,
+                         const int32_t *p_arr, const double *R_arr, int nthreads, int method, int math_mode,
+                         int32_t *hist_p1, int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets,
+                         int64_t *pair_evals) {
+  if (nthreads < 1 || nthreads > 1024 || n_events < 0) return -1.0;
   pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
   tb_arg_t *args = (tb_arg_t *)malloc(sizeof(tb_arg_t) * (size_t)nthreads);
-  if (!th || !args) return -1.0;
+  if (!th || !args) {
+    free(th);
+    free(args);
+    return -1.0;
+  }
   struct timespec t0, t1;
   clock_gettime(CLOCK_MONOTONIC, &t0);
   for (int t = 0; t < nthreads; ++t) {
-    args[t] = (tb_arg_t){p4, offsets, n_events, p, nthreads, t, method, R, 0};
+    args[t] = (tb_arg_t){p4, offsets, p_arr, R_arr, n_events, p, nthreads, t, method, math_mode, R,
+                         hist_p1, hist_p2, njets, hist_dij, jets, pair_evals, 0};
     pthread_create(&th[t], NULL, tb_worker, &args[t]);
   }
   int err = 0;
@@ -698,4 +742,10 @@ double fjo_time_batch(const double *p4, const int64_t *offsets, int n_events, in
   free(args);
   if (err) return -1.0;
   return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+double fjo_time_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R, int nthreads,
+                      int method) {
+  return fjo_cluster_batch(p4, offsets, n_events, p, R, NULL, NULL, nthreads, method, FJO_MATH_SHARED, NULL, NULL,
+                           NULL, NULL, NULL, NULL);
 }

@@ -63,11 +63,26 @@ double fjo_log(double x);
 double fjo_atan2(double y, double x);
 void fjo_set_jet(const double *p4, int p, int math_mode, double *rap, double *phi, double *w);
 
-/* CPU baseline: clusters n_events events (p4 packed, offsets[n_events+1] in
- * particles) on nthreads threads with n2plain (method 0) or tiled (method 1);
+/* Clusters n_events events (p4 packed, offsets[n_events+1] in particles) on
+ * nthreads threads with n2plain (method 0) or tiled (method 1) in math mode
+ * math_mode; per-event p_arr / R_arr, or NULL for p / R.  Outputs are indexed
+ * like the particles (jets of event e from 4*offsets[e]), njets per event, and
+ * for the tiled method the pair distances each event evaluated (-1 for
+ * n2plain); any output may be NULL (discarded).  Returns wall seconds, or -1. */
+double fjo_cluster_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R,
+                         const int32_t *p_arr, const double *R_arr, int nthreads, int method, int math_mode,
+                         int32_t *hist_p1, int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets,
+                         int64_t *pair_evals);
+
+/* CPU baseline: fjo_cluster_batch with shared math and discarded outputs;
  * returns wall seconds, or -1. */
 double fjo_time_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R, int nthreads,
                       int method);
+
+/* Pair distances the tiled restatement evaluates clustering one event (its
+ * initial nearest neighbours, orphan rescans and merged-jet updates): the
+ * algorithmic work of the tiled method.  -1 on error. */
+int64_t fjo_tiled_pair_evals(const double *p4, int n, int p, double R);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,11 @@ def lib():
         L.fjo_set_jet.restype = None
         L.fjo_time_batch.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.fjo_time_batch.restype = ctypes.c_double
+        L.fjo_cluster_batch.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _vp,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.fjo_cluster_batch.restype = ctypes.c_double
+        L.fjo_tiled_pair_evals.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.fjo_tiled_pair_evals.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -78,6 +83,41 @@ def time_batch(p4, offsets, p, R, nthreads, method="tiled"):
     offsets = np.ascontiguousarray(offsets, dtype=np.int64)
     return lib().fjo_time_batch(p4.ctypes.data_as(_vp), offsets.ctypes.data_as(_vp), len(offsets) - 1, int(p),
                                 float(R), int(nthreads), 1 if method == "tiled" else 0)
+
+
+def cluster_batch(p4, offsets, p, R, nthreads, method="tiled", math_mode=MATH_SHARED):
+    """Clusters every event of a packed batch on `nthreads` host threads; p and R
+    are scalars or per-event arrays.  Returns (hist_p1, hist_p2, hist_dij, jets,
+    njets, pair_evals, seconds) with per-particle arrays indexed like the input
+    and pair_evals per event (tiled method; -1 for n2plain)."""
+    p4 = np.ascontiguousarray(p4, dtype=np.float64).reshape(-1, 4)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    ne, tot = len(offsets) - 1, len(p4)
+    pa = np.ascontiguousarray(p, dtype=np.int32) if np.ndim(p) else None
+    Ra = np.ascontiguousarray(R, dtype=np.float64) if np.ndim(R) else None
+    h1 = np.zeros(max(tot, 1), np.int32)
+    h2 = np.zeros(max(tot, 1), np.int32)
+    hd = np.zeros(max(tot, 1), np.float64)
+    jets = np.zeros((max(tot, 1), 4), np.float64)
+    nj = np.zeros(max(ne, 1), np.int32)
+    pairs = np.zeros(max(ne, 1), np.int64)
+    P = lambda a: None if a is None else a.ctypes.data_as(_vp)
+    secs = lib().fjo_cluster_batch(P(p4), P(offsets), ne, 0 if pa is not None else int(p),
+                                   1.0 if Ra is not None else float(R), P(pa), P(Ra), int(nthreads),
+                                   1 if method == "tiled" else 0, math_mode, P(h1), P(h2), P(hd), P(jets), P(nj),
+                                   P(pairs))
+    if secs < 0:
+        raise ValueError("fjo_cluster_batch failed")
+    return h1[:tot], h2[:tot], hd[:tot], jets[:tot], nj[:ne], pairs[:ne], secs
+
+
+def tiled_pair_evals(p4, p, R):
+    """Pair distances the tiled restatement evaluates for one event."""
+    p4 = np.ascontiguousarray(p4, dtype=np.float64).reshape(-1, 4)
+    r = lib().fjo_tiled_pair_evals(p4.ctypes.data_as(_vp), len(p4), int(p), float(R))
+    if r < 0:
+        raise ValueError("fjo_tiled_pair_evals failed")
+    return int(r)
 
 
 def set_jet(p4row, p, math_mode=MATH_SHARED):

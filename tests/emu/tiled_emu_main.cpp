@@ -3,6 +3,8 @@
 // (N2Plain, or its tiled form above 3000 particles).  Test infrastructure only.
 //   tiled_emu WORKLOAD E0 E1 [VARIANT P R TILE_Q NMAX POOL_EXTRA POOL_PER_N]   generated events
 //   tiled_emu file PATH VARIANT P R                       raw little-endian px,py,pz,E doubles
+//   tiled_emu qbound                                      queue indices outside [0, n_queue) leave at once
+// FJMI_EMU_HEADLESS=1 launches without a queue head (the one-event form).
 // The tile pool of an event holds POOL_PER_N * N + POOL_EXTRA entries (default
 // 3 N + kTiledPoolPerEvent, as the library allocates); a small pool, with a
 // build with less segment slack (-DFJMI_TILED_SLACK_SHIFT=30 -DFJMI_TILED_SLACK_ADD=0),
@@ -25,6 +27,7 @@ void run_wave(const std::function<void()> &fn);
 }
 
 static int g_pool_extra = fjmi::kTiledPoolPerEvent, g_pool_per_n = 3;
+static bool g_headless = false;
 
 template <int MAXT, int WPE>
 static int run_event(const double *p4, int N, int p, double R, int tile_q, bool verbose) {
@@ -48,7 +51,7 @@ static int run_event(const double *p4, int N, int p, double R, int tile_q, bool 
   a.n_single = N;
   a.queue = nullptr;
   a.n_queue = 1;
-  a.head = &head;
+  a.head = g_headless ? nullptr : &head;
   a.head_base = 0;
   a.p_uni = p;
   a.R_uni = R;
@@ -140,7 +143,53 @@ static int run_kin(long n, unsigned seed) {
   return bad ? 1 : 0;
 }
 
+/* Launches whose queue head yields an index below 0 or at/after n_queue must
+ * leave without touching any output. */
+static int run_qbound() {
+  using namespace fjmi;
+  const double p4[8] = {1.0, 0.0, 0.0, 1.0, 1.0, 0.01, 0.0, 1.0001};
+  std::vector<JetRec> rec(2);
+  std::vector<double> p4s(8);
+  std::vector<int> hist(2);
+  std::vector<unsigned short> lab2(2), pool(6 + kTiledPoolPerEvent);
+  TiledScratch ts{rec.data(), p4s.data(), hist.data(), lab2.data(), pool.data(), kTiledPoolPerEvent};
+  int fails = 0;
+  const unsigned cases[][2] = {{0u, 3u}, {5u, 0u}, {0x80000000u, 0u}};
+  for (const auto &c : cases) {
+    int32_t h1[2] = {-7, -7}, h2[2] = {-7, -7}, nj = -1;
+    double hd[2] = {-7.0, -7.0}, jets[8] = {0};
+    unsigned head = c[0], status = 0;
+    int64_t offs[2] = {0, 2};
+    LaunchArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.in = p4;
+    a.in_stride = 4;
+    a.offsets = offs;
+    a.n_single = 2;
+    a.n_queue = 1;
+    a.head = &head;
+    a.head_base = c[1];
+    a.p_uni = -1;
+    a.R_uni = 0.4;
+    a.hist_p1 = h1;
+    a.hist_p2 = h2;
+    a.hist_dij = hd;
+    a.jets = jets;
+    a.njets = &nj;
+    a.status = &status;
+    a.stage_chunk = kStageChunk;
+    emu::run_wave([&] { tiled_kernel<128, 4, false>(a, ts); });
+    const bool ok = nj == -1 && h1[0] == -7 && h1[1] == -7 && h2[0] == -7 && status == 0;
+    std::printf("qbound head=%u base=%u: %s\n", c[0], c[1], ok ? "ok" : "TOUCHED");
+    fails += !ok;
+  }
+  std::printf("done: %d mismatches\n", fails);
+  return fails ? 1 : 0;
+}
+
 int main(int argc, char **argv) {
+  if (const char *h = std::getenv("FJMI_EMU_HEADLESS")) g_headless = std::atoi(h) != 0;
+  if (argc >= 2 && std::strcmp(argv[1], "qbound") == 0) return run_qbound();
   if (argc >= 2 && std::strcmp(argv[1], "kin") == 0) return run_kin(std::atol(argv[2]), (unsigned)std::atol(argv[3]));
   if (argc >= 2 && std::strcmp(argv[1], "file") == 0)
     return run_file(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atof(argv[5]));

@@ -335,3 +335,57 @@ def test_pseudojet_dropin_layout(engine):
     assert sorted(np.nonzero(out["isJet"])[0].tolist()) == sorted(jet_slots)
     for j, slot in enumerate(jet_slots):
         assert out["p4"][slot].tobytes() == jets[j].tobytes()
+
+
+@pytest.mark.gpu
+def test_one_event_calls_on_several_streams(engine):
+    """One-event device calls enqueued on three streams back to back, with no
+    synchronisation between them: each stream has its own scratch and status
+    word, so the overlapping launches do not share state; every result is
+    bit-exact and each stream's check reports no failure."""
+    import torch
+    import fastjet_gpu_amd as fj
+    evs = [fj.gen_event(3, 7), fj.gen_event(3, 11), fj.gen_event(2, 0), fj.gen_event(3, 12)]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream(), torch.cuda.Stream()]
+    bufs = []
+    for ev in evs:
+        n = len(ev)
+        bufs.append(dict(p4=torch.from_numpy(np.ascontiguousarray(ev)).cuda(),
+                         h1=torch.full((n,), -7, dtype=torch.int32, device="cuda"),
+                         h2=torch.full((n,), -7, dtype=torch.int32, device="cuda"),
+                         hd=torch.zeros(n, dtype=torch.float64, device="cuda"),
+                         jets=torch.zeros(4 * n, dtype=torch.float64, device="cuda"),
+                         nj=torch.full((1,), -1, dtype=torch.int32, device="cuda")))
+    torch.cuda.synchronize()
+    print(f"[gpu parity] {len(evs)} one-event calls on {len(streams)} streams, N={[len(e) for e in evs]}", flush=True)
+    for k, (ev, b) in enumerate(zip(evs, bufs)):
+        st = streams[k % len(streams)]
+        engine.cluster_event_device(b["p4"].data_ptr(), len(ev), -1, 0.4, b["h1"].data_ptr(), b["h2"].data_ptr(),
+                                    b["hd"].data_ptr(), b["jets"].data_ptr(), b["nj"].data_ptr(), st.cuda_stream)
+    for st in streams:
+        engine.stream_check(st.cuda_stream)
+    for ev, b in zip(evs, bufs):
+        nj = int(b["nj"].cpu()[0])
+        got = (b["h1"].cpu().numpy(), b["h2"].cpu().numpy(), b["hd"].cpu().numpy(),
+               b["jets"].cpu().numpy().reshape(-1, 4)[:nj])
+        want = orc.n2plain(ev, -1, 0.4)
+        assert same(want, got), f"N={len(ev)}: {first_diff(want, got)}"
+
+
+@pytest.mark.gpu
+def test_stream_check_without_failures(engine):
+    """fjmi_stream_check on a stream that never ran a one-event call, and after a
+    successful one, returns without error."""
+    import torch
+    s = torch.cuda.Stream()
+    engine.stream_check(s.cuda_stream)
+    ev = orc.read_events(os.path.join(GOLDEN, "354_fastjet.dat"))[0]
+    n = len(ev)
+    d = torch.from_numpy(ev).cuda()
+    out = [torch.zeros(n, dtype=torch.int32, device="cuda"), torch.zeros(n, dtype=torch.int32, device="cuda"),
+           torch.zeros(n, dtype=torch.float64, device="cuda"), torch.zeros(4 * n, dtype=torch.float64, device="cuda"),
+           torch.zeros(1, dtype=torch.int32, device="cuda")]
+    torch.cuda.synchronize()
+    engine.cluster_event_device(d.data_ptr(), n, 1, 0.4, *[t.data_ptr() for t in out], s.cuda_stream)
+    engine.stream_check(s.cuda_stream)
+    engine.stream_check(0)

@@ -38,14 +38,23 @@ def emu_bin(tmp_path_factory):
 
 
 @pytest.fixture(scope="module")
+def emu_stamp_bin(tmp_path_factory):
+    """The gather counter restarts every 40 gathers (instead of near 2^32), so
+    every event restarts it many times, also between a step's gather and its
+    orphan rescans."""
+    return _build_emu(tmp_path_factory, "tiled_emu_stamp", ("-DFJMI_TILED_STAMP_LIMIT=40u",))
+
+
+@pytest.fixture(scope="module")
 def emu_tight_bin(tmp_path_factory):
     """No segment slack: insertions relocate segments and a pool without spare
     room runs out, so the kernel's segment rebuild runs."""
     return _build_emu(tmp_path_factory, "tiled_emu_tight", ("-DFJMI_TILED_SLACK_SHIFT=30", "-DFJMI_TILED_SLACK_ADD=0"))
 
 
-def run(emu_bin, *args):
-    r = subprocess.run([emu_bin, *map(str, args)], capture_output=True, text=True, timeout=600)
+def run(emu_bin, *args, env=None):
+    r = subprocess.run([emu_bin, *map(str, args)], capture_output=True, text=True, timeout=600,
+                       env=None if env is None else {**os.environ, **env})
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     return r.stdout
 
@@ -107,3 +116,26 @@ def test_pool_exhaustion_rebuilds(emu_tight_bin, workload, e0, e1, variant, p, R
     assert "done: 0 mismatches" in out
     rebuilds = int(out.split("rebuilds:")[1].split()[0])
     assert rebuilds > 0, out[-500:]
+
+
+def test_queue_index_bounds(emu_bin):
+    """A queue head yielding an index below 0 or at/after the queue's end (the
+    one-event form pops entry 0 only) leaves the launch without writing."""
+    out = run(emu_bin, "qbound")
+    assert "done: 0 mismatches" in out and out.count(": ok") == 3, out
+
+
+@pytest.mark.parametrize("variant,p", [(1, -1), (0, 1)])
+def test_headless_single_pass(emu_bin, variant, p):
+    """Launches without a queue head (how the one-event entry points launch):
+    one pass over queue entry 0, bit-exact."""
+    out = run(emu_bin, 3, 20, 24, variant, p, 0.4, env={"FJMI_EMU_HEADLESS": "1"})
+    assert "done: 0 mismatches" in out
+
+
+@pytest.mark.parametrize("workload,e0,e1,variant,p,R", [(3, 0, 4, 1, -1, 0.4), (4, 1, 2, 0, 1, 0.4)])
+def test_gather_counter_restarts(emu_stamp_bin, workload, e0, e1, variant, p, R):
+    """Restarting the per-event gather counter (all stamps cleared) keeps every
+    result bit-exact, single-chunk and several-chunk gathers alike."""
+    out = run(emu_stamp_bin, workload, e0, e1, variant, p, R, 0)
+    assert "done: 0 mismatches" in out
