@@ -106,6 +106,7 @@ def _declare(lib):
         "fjmi_cluster_event_device": ([_vp, _vp, i, i, d, i, _vp, _vp, _vp, _vp, _vp, _vp], i),
         "fjmi_cluster_pseudojets": ([_vp, _vp, i, i, d, i, _vp], i),
         "fjmi_stream_check": ([_vp, _vp], i),
+        "fjmi_cluster_event_device_noalloc": ([_vp, _vp, i, i, d, _vp, _vp, _vp, _vp, _vp, _vp], i),
         "fjmi_batch_create": ([_vp, i, _vp, _vp, _vp, _vp, _vp, i, d, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)],
                               i),
         "fjmi_batch_launch": ([_vp, _vp], i),
@@ -217,6 +218,13 @@ class Engine:
     def cluster_pseudojets(self, d_particles, n, p, R, engine=ENGINE_AUTO, stream=0):
         _check(self.lib, self.lib.fjmi_cluster_pseudojets(self.ctx, d_particles, n, int(p), float(R), engine,
                                                           stream or None), "fjmi_cluster_pseudojets")
+
+    def cluster_event_device_noalloc(self, d_p4, n, p, R, d_h1, d_h2, d_hd, d_jets, d_nj, stream=0):
+        """fjmi_cluster_event_device_noalloc: the tiled engine with the library's static scratch (no
+        allocation; events of up to 20000 particles; calls must not overlap on one device)."""
+        _check(self.lib, self.lib.fjmi_cluster_event_device_noalloc(self.ctx, d_p4, n, int(p), float(R), d_h1, d_h2,
+                                                                    d_hd, d_jets, d_nj, stream or None),
+               "fjmi_cluster_event_device_noalloc")
 
     def stream_check(self, stream=0):
         """Synchronises `stream`; raises if a one-event device call on it reported a failure."""

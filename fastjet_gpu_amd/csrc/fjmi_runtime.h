@@ -86,6 +86,9 @@ struct fjmi_context {
   /* one-event paths: per-stream state (key: the hipStream_t, nullptr for the null stream) */
   std::map<void *, fjmi::rt::StreamState> streams;
   int max_lds = 0;
+  /* device address of this device's copy of the library's static scratch array
+   * (fjmi_cluster_event_device_noalloc), from hipGetSymbolAddress at creation */
+  unsigned char *static_scratch = nullptr;
 };
 
 #endif

@@ -65,6 +65,19 @@ size_t dyn_lds_bytes(const KernelDesc &d) {
 
 }  // namespace
 
+/* Scratch of fjmi_cluster_event_device_noalloc: the tiled engine's per-event
+ * scratch for up to kNoAllocMaxN particles, then one status word.  Each device
+ * holds its own copy for the life of the process; host code never uses this
+ * name as an address (it is a host-side stand-in): fjmi_create fetches each
+ * device's address with hipGetSymbolAddress and the entry point passes that. */
+namespace {
+constexpr int kNoAllocMaxN = 20000;
+constexpr size_t kNoAllocTiledBytes =
+    (fjmi::tiled_scratch_bytes(kNoAllocMaxN, 1, fjmi::kTiledPoolPerEvent) + 255) / 256 * 256;
+constexpr size_t kNoAllocBytes = kNoAllocTiledBytes + 256;
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 
 struct BucketRun {
   bool tiled;
@@ -323,6 +336,11 @@ int fjmi_create(int device, fjmi_context **out) {
   }
   ctx->n_cu = prop.multiProcessorCount;
   ctx->max_lds = (int)prop.sharedMemPerBlock;
+  e = hipGetSymbolAddress(reinterpret_cast<void **>(&ctx->static_scratch), HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ctx->static_scratch) {
+    delete ctx;
+    return fail(FJMI_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+  }
   for (int i = 0; i < fjmi::kNumDenseKernels; ++i) {
     const KernelDesc &d = fjmi::kDenseKernels[i];
     size_t lds = dyn_lds_bytes(d);
@@ -417,6 +435,48 @@ int fjmi_cluster_pseudojets(fjmi_context *ctx, void *d_particles, int n, int p, 
                             st->hist1.as<int32_t>(), st->hist2.as<int32_t>(), st->hdij.as<double>(),
                             st->jets.as<double>(), st->njets.as<int32_t>(), static_cast<unsigned char *>(d_particles),
                             s);
+}
+
+int fjmi_cluster_event_device_noalloc(fjmi_context *ctx, const double *d_p4, int n, int p, double R,
+                                      int32_t *d_h1, int32_t *d_h2, double *d_hd, double *d_jets, int32_t *d_nj,
+                                      void *stream) {
+  if (!ctx) return fail(FJMI_EINVAL, "null context");
+  int rc = check_args(n, p, R);
+  if (rc) return rc;
+  if (!d_nj || (n > 0 && (!d_p4 || !d_h1 || !d_h2 || !d_hd || !d_jets))) return fail(FJMI_EINVAL, "null buffer");
+  if (n > kNoAllocMaxN)
+    return fail(FJMI_ECAPACITY, "event with %d particles: the no-allocation entry point takes at most %d", n,
+                kNoAllocMaxN);
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  if ((rc = set_device(ctx))) return rc;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (n == 0) return zero_words(d_nj, 1, s);
+  /* the table's tiled row for n, else tile side R with the 512-tile variant */
+  const fjmi_bucket *b = bucket_for(ctx, n, FJMI_ENGINE_TILED);
+  if (!b) b = &kTiledRow;
+  /* every byte of the tiled scratch that the kernel reads it has written for this
+   * event first, so nothing is cleared; the status word is written on failure only
+   * (the caller sees njets = -1) and is never read */
+  fjmi::TiledScratch ts = fjmi::tiled_scratch_carve(ctx->static_scratch, n, fjmi::kTiledPoolPerEvent);
+  LaunchArgs a = base_args();
+  a.in = d_p4;
+  a.offsets = nullptr;
+  a.n_single = n;
+  a.queue = nullptr;
+  a.n_queue = 1;
+  a.head = nullptr;
+  a.p_uni = p;
+  a.R_uni = R;
+  a.hist_p1 = d_h1;
+  a.hist_p2 = d_h2;
+  a.hist_dij = d_hd;
+  a.jets = d_jets;
+  a.njets = d_nj;
+  a.status = reinterpret_cast<unsigned int *>(ctx->static_scratch + kNoAllocTiledBytes);
+  a.tile_q = b->slots;
+  void *params[] = {&a, &ts};
+  HIPCHK(hipLaunchKernel(fjmi::kTiledKernels[b->p4_global], dim3(1), dim3(64), params, 0, s));
+  return FJMI_OK;
 }
 
 int fjmi_stream_check(fjmi_context *ctx, void *stream) {

@@ -118,6 +118,20 @@ int fjmi_cluster_event_device(fjmi_context *ctx, const double *d_p4, int n, int 
  * Asynchronous on `stream`. */
 int fjmi_cluster_pseudojets(fjmi_context *ctx, void *d_particles, int n, int p, double R, int engine, void *stream);
 
+/* Same as fjmi_cluster_event_device on the tiled engine, but allocates nothing,
+ * ever: no device or host allocation on any call (the context's creation set it
+ * up).  No counterpart in the reference; for callers that cannot allocate, such
+ * as code under graph capture or tight serving loops.  Its scratch is a static
+ * array of the library, one per device, sized for events of up to 20000
+ * particles (about 2.2 MB); larger events return FJMI_ECAPACITY.  Every call on
+ * a device shares that device's array, so calls that overlap on one device (on
+ * different streams, or from several threads or contexts) race and must not be
+ * issued; calls in order on one stream are safe.  A kernel failure shows as
+ * *d_njets == -1.  Asynchronous on `stream`. */
+int fjmi_cluster_event_device_noalloc(fjmi_context *ctx, const double *d_p4, int n, int p, double R,
+                                      int32_t *d_hist_p1, int32_t *d_hist_p2, double *d_hist_dij, double *d_jets,
+                                      int32_t *d_njets, void *stream);
+
 /* Synchronises `stream` and reports (then clears) any failure raised by the
  * kernels of the one-event device entry points launched on it since the last
  * check: FJMI_OK, FJMI_ECAPACITY or FJMI_EINTERNAL.  The reference's cluster()

@@ -33,9 +33,9 @@ def test_header_parse_finds_every_entry_point():
     for must in ("fjmi_create", "fjmi_cluster_event_device", "fjmi_cluster_pseudojets", "fjmi_batch_create",
                  "fjmi_batch_launch", "fjmi_batch_stats", "fjmi_cluster_batch", "fjmi_server_create",
                  "fjmi_server_submit", "fjmi_server_poll", "fjmi_server_release", "fjmi_server_destroy",
-                 "fjmi_stream_check"):
+                 "fjmi_stream_check", "fjmi_cluster_event_device_noalloc"):
         assert must in names
-    assert len(names) == 23
+    assert len(names) == 24
 
 
 @pytest.mark.parametrize("header,lib", [("fjmi.h", "libfjmi.so"), ("fjmi_gen.h", "libfjmi_gen.so")])
