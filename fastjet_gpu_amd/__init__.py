@@ -17,6 +17,7 @@ __all__ = ["Engine", "Server", "FjmiError", "KT", "CAMBRIDGE_AACHEN", "ANTIKT", 
            "ENGINE_TILED", "ENGINE_EMPTY", "gen_event", "gen_serving_attrs", "gen_workload", "lib_path", "load_lib"]
 
 KT, CAMBRIDGE_AACHEN, ANTIKT = 1, 0, -1
+EXCLUSIVE_NJETS, EXCLUSIVE_DCUT = 0, 1
 ENGINE_AUTO, ENGINE_DENSE, ENGINE_TILED, ENGINE_EMPTY = 0, 1, 2, 3
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
@@ -106,6 +107,10 @@ def _declare(lib):
         "fjmi_cluster_event_device": ([_vp, _vp, i, i, d, i, _vp, _vp, _vp, _vp, _vp, _vp], i),
         "fjmi_cluster_pseudojets": ([_vp, _vp, i, i, d, i, _vp], i),
         "fjmi_stream_check": ([_vp, _vp], i),
+        "fjmi_inclusive_jets": ([_vp, i, _vp, _vp, _vp, d, _vp, _vp, _vp], i),
+        "fjmi_inclusive_pseudojets": ([_vp, _vp, i, d, _vp, _vp, _vp], i),
+        "fjmi_history_view": ([_vp, i, _vp, i, _vp, _vp, _vp, _vp, _vp, _vp], i),
+        "fjmi_exclusive_jets": ([_vp, i, _vp, i, _vp, _vp, _vp, _vp, i, d, _vp, _vp, _vp, _vp], i),
         "fjmi_cluster_event_device_noalloc": ([_vp, _vp, i, i, d, _vp, _vp, _vp, _vp, _vp, _vp], i),
         "fjmi_batch_create": ([_vp, i, _vp, _vp, _vp, _vp, _vp, i, d, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)],
                               i),
@@ -225,6 +230,26 @@ class Engine:
         _check(self.lib, self.lib.fjmi_cluster_event_device_noalloc(self.ctx, d_p4, n, int(p), float(R), d_h1, d_h2,
                                                                     d_hd, d_jets, d_nj, stream or None),
                "fjmi_cluster_event_device_noalloc")
+
+    # -- results derived on the device (device pointers as ints; d_offsets 0 for one event) --
+    def inclusive_jets(self, n_events, d_offsets, d_jets, d_njets, ptmin, d_out, d_out_n, stream=0):
+        _check(self.lib, self.lib.fjmi_inclusive_jets(self.ctx, n_events, d_offsets or None, d_jets, d_njets,
+                                                      float(ptmin), d_out, d_out_n, stream or None),
+               "fjmi_inclusive_jets")
+
+    def inclusive_pseudojets(self, d_particles, n, ptmin, d_out, d_count, stream=0):
+        _check(self.lib, self.lib.fjmi_inclusive_pseudojets(self.ctx, d_particles, n, float(ptmin), d_out, d_count,
+                                                            stream or None), "fjmi_inclusive_pseudojets")
+
+    def history_view(self, n_events, d_offsets, n_single, d_h1, d_h2, d_hd, d_child, d_max_dij, stream=0):
+        _check(self.lib, self.lib.fjmi_history_view(self.ctx, n_events, d_offsets or None, n_single, d_h1, d_h2,
+                                                    d_hd, d_child, d_max_dij, stream or None), "fjmi_history_view")
+
+    def exclusive_jets(self, n_events, d_offsets, n_single, d_p4, d_h1, d_h2, d_hd, mode, param, d_work, d_out,
+                       d_out_n, stream=0):
+        _check(self.lib, self.lib.fjmi_exclusive_jets(self.ctx, n_events, d_offsets or None, n_single, d_p4, d_h1,
+                                                      d_h2, d_hd, int(mode), float(param), d_work, d_out, d_out_n,
+                                                      stream or None), "fjmi_exclusive_jets")
 
     def stream_check(self, stream=0):
         """Synchronises `stream`; raises if a one-event device call on it reported a failure."""

@@ -22,6 +22,7 @@
 
 #include "cluster.h"
 #include "fjmi.h"
+#include "fjmi_dropin.h"
 
 #ifndef FJMI_LINKED_ENGINE
 #define FJMI_LINKED_ENGINE FJMI_ENGINE_AUTO
@@ -42,9 +43,8 @@ int g_ctx_device = -1;
     if ((call) != 0) die(__FILE__, __LINE__, #call, fjmi_last_error()); \
   } while (0)
 
-}  // namespace
-
-void cluster(PseudoJet *particles, int size, Algorithm algo, double r) {
+/* The library context of the current device, created on first use. */
+fjmi_context *context() {
   int device = 0;
   if (hipGetDevice(&device) != hipSuccess) device = 0;
   if (!g_ctx || g_ctx_device != device) {
@@ -53,8 +53,20 @@ void cluster(PseudoJet *particles, int size, Algorithm algo, double r) {
     FJMI_CHECK(fjmi_create(device, &g_ctx));
     g_ctx_device = device;
   }
-  FJMI_CHECK(fjmi_cluster_pseudojets(g_ctx, particles, size, static_cast<int>(algo), r, FJMI_LINKED_ENGINE, nullptr));
+  return g_ctx;
+}
+
+}  // namespace
+
+void inclusive_jets(const PseudoJet *particles, int size, double ptmin, PseudoJet *jets, int *count) {
+  fjmi_context *ctx = context();
+  FJMI_CHECK(fjmi_inclusive_pseudojets(ctx, particles, size, ptmin, jets, count, nullptr));
+}
+
+void cluster(PseudoJet *particles, int size, Algorithm algo, double r) {
+  fjmi_context *ctx = context();
+  FJMI_CHECK(fjmi_cluster_pseudojets(ctx, particles, size, static_cast<int>(algo), r, FJMI_LINKED_ENGINE, nullptr));
   /* waits for the kernel, as the reference's closing cudaFree calls do (tri_matrix.cu:332-335),
    * and aborts if it reported a failure */
-  FJMI_CHECK(fjmi_stream_check(g_ctx, nullptr));
+  FJMI_CHECK(fjmi_stream_check(ctx, nullptr));
 }

@@ -31,6 +31,12 @@ extern const void *const kTiledKernels[kNumTiledKernels];
 extern const void *const kTiledServeKernels[kNumTiledKernels]; /* same, serving mode (LaunchArgs::serve) */
 extern const int kTiledMaxTiles[kNumTiledKernels];
 
+/* post-processing (postproc.hip), 256 threads, one workgroup per event */
+extern const void *const kInclusiveKernel;   /* (jets, njets, offsets, ptmin, out, n_out) */
+extern const void *const kInclusivePjKernel; /* (records, n, ptmin, out, n_out) */
+extern const void *const kHistoryKernel;     /* (offsets, n_single, p1, p2, dij, child, max_dij) */
+extern const void *const kExclusiveKernel;   /* (offsets, n_single, p4, p1, p2, dij, mode, param, work, out, n_out) */
+
 /* Pool entries per event of the tiled engine beyond 3 per particle (>= 2 per tile). */
 constexpr int kTiledPoolPerEvent = 2048;
 

@@ -479,6 +479,80 @@ int fjmi_cluster_event_device_noalloc(fjmi_context *ctx, const double *d_p4, int
   return FJMI_OK;
 }
 
+int fjmi_inclusive_jets(fjmi_context *ctx, int n_events, const int64_t *d_offsets, const double *d_jets,
+                        const int32_t *d_njets, double ptmin, double *d_out_jets, int32_t *d_out_njets,
+                        void *stream) {
+  if (!ctx) return fail(FJMI_EINVAL, "null context");
+  if (n_events < 0 || (n_events > 1 && !d_offsets)) return fail(FJMI_EINVAL, "offsets needed for several events");
+  if (!(ptmin >= 0.0)) return fail(FJMI_EINVAL, "ptmin must be >= 0");
+  if (n_events == 0) return FJMI_OK;
+  if (!d_jets || !d_njets || !d_out_jets || !d_out_njets) return fail(FJMI_EINVAL, "null buffer");
+  if (d_out_jets == d_jets) return fail(FJMI_EINVAL, "the output must not overlap the input");
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  void *params[] = {&d_jets, &d_njets, &d_offsets, &ptmin, &d_out_jets, &d_out_njets};
+  HIPCHK(hipLaunchKernel(fjmi::kInclusiveKernel, dim3(n_events), dim3(256), params, 0, (hipStream_t)stream));
+  return FJMI_OK;
+}
+
+int fjmi_inclusive_pseudojets(fjmi_context *ctx, const void *d_particles, int n, double ptmin, void *d_out,
+                              int32_t *d_count, void *stream) {
+  if (!ctx) return fail(FJMI_EINVAL, "null context");
+  if (n < 0 || !(ptmin >= 0.0)) return fail(FJMI_EINVAL, "bad argument");
+  if (!d_count || (n > 0 && (!d_particles || !d_out))) return fail(FJMI_EINVAL, "null buffer");
+  if (d_out == d_particles && n > 0) return fail(FJMI_EINVAL, "the output must not overlap the input");
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (n == 0) return zero_words(d_count, 1, s);
+  const unsigned char *in = static_cast<const unsigned char *>(d_particles);
+  unsigned char *out = static_cast<unsigned char *>(d_out);
+  void *params[] = {&in, &n, &ptmin, &out, &d_count};
+  HIPCHK(hipLaunchKernel(fjmi::kInclusivePjKernel, dim3(1), dim3(256), params, 0, s));
+  return FJMI_OK;
+}
+
+int fjmi_history_view(fjmi_context *ctx, int n_events, const int64_t *d_offsets, int n_single,
+                      const int32_t *d_hist_p1, const int32_t *d_hist_p2, const double *d_hist_dij, int32_t *d_child,
+                      double *d_max_dij, void *stream) {
+  if (!ctx) return fail(FJMI_EINVAL, "null context");
+  if (n_events < 0 || (!d_offsets && (n_events > 1 || n_single < 0))) return fail(FJMI_EINVAL, "bad event layout");
+  if (n_events == 0) return FJMI_OK;
+  if (!d_hist_p1 || !d_hist_p2 || !d_hist_dij || !d_child || !d_max_dij) return fail(FJMI_EINVAL, "null buffer");
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  void *params[] = {&d_offsets, &n_single, &d_hist_p1, &d_hist_p2, &d_hist_dij, &d_child, &d_max_dij};
+  HIPCHK(hipLaunchKernel(fjmi::kHistoryKernel, dim3(n_events), dim3(256), params, 0, (hipStream_t)stream));
+  return FJMI_OK;
+}
+
+int fjmi_exclusive_jets(fjmi_context *ctx, int n_events, const int64_t *d_offsets, int n_single, const double *d_p4,
+                        const int32_t *d_hist_p1, const int32_t *d_hist_p2, const double *d_hist_dij, int mode,
+                        double param, double *d_work, double *d_out_jets, int32_t *d_out_njets, void *stream) {
+  if (!ctx) return fail(FJMI_EINVAL, "null context");
+  if (n_events < 0 || (!d_offsets && (n_events > 1 || n_single < 0))) return fail(FJMI_EINVAL, "bad event layout");
+  if (mode == FJMI_EXCLUSIVE_NJETS) {
+    if (!(param >= 0.0) || param != (double)(int)param) return fail(FJMI_EINVAL, "njets must be an integer >= 0");
+  } else if (mode == FJMI_EXCLUSIVE_DCUT) {
+    if (!(param >= 0.0)) return fail(FJMI_EINVAL, "dcut must be >= 0");
+  } else {
+    return fail(FJMI_EINVAL, "mode must be FJMI_EXCLUSIVE_NJETS or FJMI_EXCLUSIVE_DCUT");
+  }
+  if (n_events == 0) return FJMI_OK;
+  if (!d_p4 || !d_hist_p1 || !d_hist_p2 || !d_hist_dij || !d_work || !d_out_jets || !d_out_njets)
+    return fail(FJMI_EINVAL, "null buffer");
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  void *params[] = {&d_offsets, &n_single, &d_p4, &d_hist_p1, &d_hist_p2, &d_hist_dij, &mode, &param, &d_work,
+                    &d_out_jets, &d_out_njets};
+  HIPCHK(hipLaunchKernel(fjmi::kExclusiveKernel, dim3(n_events), dim3(256), params, 0, (hipStream_t)stream));
+  return FJMI_OK;
+}
+
 int fjmi_stream_check(fjmi_context *ctx, void *stream) {
   if (!ctx) return fail(FJMI_EINVAL, "null context");
   std::lock_guard<std::mutex> lk(ctx->mtx);

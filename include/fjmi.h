@@ -139,6 +139,52 @@ int fjmi_cluster_event_device_noalloc(fjmi_context *ctx, const double *d_p4, int
  * the drop-in cluster() calls this to do the same. */
 int fjmi_stream_check(fjmi_context *ctx, void *stream);
 
+/* ---- Results derived from a clustering on the device --------------------------
+ * All asynchronous on `stream`, device memory in and out.  A batch passes
+ * d_offsets (n_events + 1 entries, as for the batch calls); a single event passes
+ * n_events = 1 and d_offsets = NULL (with n_single particles where needed). */
+
+/* Inclusive jets above ptmin, sorted by decreasing pt^2 = px^2 + py^2: the jets
+ * with px^2 + py^2 >= ptmin^2 of each event (d_njets[e] jets at 4*offsets[e] of
+ * d_jets, as every entry point writes them) go to 4*offsets[e] of d_out_jets,
+ * their count to d_out_njets[e] (-1 for an event whose njets is -1).  Equal pt^2
+ * keep promotion order.  Replaces the host-side filter and sort of the
+ * reference's driver after each cluster() call (main.cc:309-327). */
+int fjmi_inclusive_jets(fjmi_context *ctx, int n_events, const int64_t *d_offsets, const double *d_jets,
+                        const int32_t *d_njets, double ptmin, double *d_out_jets, int32_t *d_out_njets,
+                        void *stream);
+
+/* The same on n PseudoJet records after the drop-in cluster(): the records with
+ * isJet and pt^2 >= ptmin^2, whole and sorted, to d_out (n records of room), and
+ * their count to *d_count (main.cc:309-327). */
+int fjmi_inclusive_pseudojets(fjmi_context *ctx, const void *d_particles, int n, double ptmin, void *d_out,
+                              int32_t *d_count, void *stream);
+
+/* FastJet's ClusterSequence history fields implied by the step arrays.  History
+ * entries 0..N-1 are the particles, N+s the result of step s.  d_child gets 2
+ * entries per particle (event e's 2N entries from 2*offsets[e]): the entry whose
+ * step consumed entry k, or -3 (FastJet's Invalid) if none.  d_max_dij gets one
+ * per particle: max(dij[0..s]) (FastJet's max_dij_so_far; its particle entries
+ * hold 0).  Not in the reference, which exposes no history. */
+int fjmi_history_view(fjmi_context *ctx, int n_events, const int64_t *d_offsets, int n_single,
+                      const int32_t *d_hist_p1, const int32_t *d_hist_p2, const double *d_hist_dij, int32_t *d_child,
+                      double *d_max_dij, void *stream);
+
+/* FastJet's exclusive jets from a complete history (ClusterSequence::
+ * exclusive_jets): mode FJMI_EXCLUSIVE_NJETS with param = njets (events with
+ * fewer particles get d_out_njets[e] = -1), or FJMI_EXCLUSIVE_DCUT with param =
+ * dcut >= 0 (compared with the normalised dij of the history, as FastJet's
+ * n_exclusive_jets does).  The jets are the history entries alive at the stop
+ * point, in the order of the steps that consume them; their momenta are the
+ * E-scheme sums of the history replayed in step order, bit-identical to the
+ * clustering's.  d_p4 is the event's input (4 doubles per particle), d_work
+ * holds 8 doubles per particle, d_out_jets 4 per particle.  Not in the
+ * reference (it has inclusive jets only). */
+enum { FJMI_EXCLUSIVE_NJETS = 0, FJMI_EXCLUSIVE_DCUT = 1 };
+int fjmi_exclusive_jets(fjmi_context *ctx, int n_events, const int64_t *d_offsets, int n_single, const double *d_p4,
+                        const int32_t *d_hist_p1, const int32_t *d_hist_p2, const double *d_hist_dij, int mode,
+                        double param, double *d_work, double *d_out_jets, int32_t *d_out_njets, void *stream);
+
 /* Batched clustering of many events in ONE graph launch.  Not in the reference
  * (it clusters one event per call, main.cc:269-368).  The plan sorts events into
  * multiplicity buckets (fjmi_set_buckets), orders each bucket's queue by cost,

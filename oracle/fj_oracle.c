@@ -644,6 +644,106 @@ int fjo_cluster_tiled(const double *p4, int n, int p, double R, int mm, int32_t 
   return 0;
 }
 
+/* ---------------------------------------------------------------------------
+ * Results derived from a clustering history (the same outputs as the GPU's
+ * post-processing kernels).
+ * ------------------------------------------------------------------------- */
+
+/* main.cc:309-327: keep the jets with px^2 + py^2 >= ptmin^2 (the filter tests
+ * "px*px + py*py < ptmin*ptmin" for removal) and sort them by decreasing
+ * px^2 + py^2.  The reference's std::sort leaves equal keys in an unspecified
+ * order; here they keep their promotion order. */
+int fjo_inclusive_jets(const double *jets, int njets, double ptmin, double *out, int32_t *n_out) {
+  if (njets < 0 || !n_out) return -1;
+  const double pt2min = ptmin * ptmin;
+  int k = 0;
+  for (int i = 0; i < njets; ++i) {
+    const double *q = jets + 4 * (size_t)i;
+    const double key = q[0] * q[0] + q[1] * q[1];
+    if (key < pt2min) continue;
+    int rank = 0; /* kept jets before it in the sorted order */
+    for (int j = 0; j < njets; ++j) {
+      const double *r = jets + 4 * (size_t)j;
+      const double kj = r[0] * r[0] + r[1] * r[1];
+      if (kj < pt2min) continue;
+      if (kj > key || (kj == key && j < i)) ++rank;
+    }
+    memcpy(out + 4 * (size_t)rank, q, sizeof(double) * 4);
+    ++k;
+  }
+  *n_out = k;
+  return 0;
+}
+
+/* FastJet's ClusterSequence history view of our N steps: history entries
+ * 0..N-1 are the particles and N+s the result of step s.  child[k] is the entry
+ * whose step consumed entry k (-3, FastJet's Invalid, for none: the results of
+ * beam steps and nothing else once the clustering is complete);
+ * max_dij[s] = max(dij[0..s]) (FastJet's max_dij_so_far, whose particle
+ * entries hold 0). */
+int fjo_history_children(const int32_t *hp1, const int32_t *hp2, const double *hdij, int n, int32_t *child,
+                         double *max_dij) {
+  if (n < 0) return -1;
+  for (int k = 0; k < 2 * n; ++k) child[k] = -3;
+  double m = 0.0;
+  for (int s = 0; s < n; ++s) {
+    if (hp1[s] < 0 || hp1[s] >= n + s || hp2[s] >= n + s) return -2;
+    child[hp1[s]] = n + s;
+    if (hp2[s] >= 0) child[hp2[s]] = n + s;
+    m = hdij[s] > m ? hdij[s] : m;
+    max_dij[s] = m;
+  }
+  return 0;
+}
+
+/* FastJet's exclusive jets from the history: ClusterSequence::exclusive_jets(njets)
+ * (mode 0, param = njets <= N) or exclusive_jets(dcut) (mode 1, param = dcut, via
+ * n_exclusive_jets: the last entry whose max_dij_so_far <= dcut is the stop
+ * point).  The jets are the entries before the stop point consumed at or after
+ * it, in history order of the consuming step (parent1 then parent2).  Their
+ * momenta are the E-scheme sums of the history, replayed in step order. */
+int fjo_exclusive_jets(const double *p4, const int32_t *hp1, const int32_t *hp2, const double *hdij, int n, int mode,
+                       double param, double *out, int32_t *n_out) {
+  if (n < 0 || !n_out) return -1;
+  int njets;
+  if (mode == 0) {
+    njets = (int)param;
+    if (!(param >= 0.0) || njets > n || (double)njets != param) return -1;
+  } else {
+    int i = 2 * n - 1;
+    double m = 0.0;
+    double *mx = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    if (!mx) return -2;
+    for (int s = 0; s < n; ++s) {
+      m = hdij[s] > m ? hdij[s] : m;
+      mx[s] = m;
+    }
+    while (i >= 0) {
+      const double ms = i >= n ? mx[i - n] : 0.0;
+      if (ms <= param) break;
+      --i;
+    }
+    free(mx);
+    njets = 2 * n - (i + 1);
+  }
+  const int stop = 2 * n - njets;
+  double *h = (double *)malloc(sizeof(double) * 4 * (size_t)(2 * n + 1));
+  if (!h) return -2;
+  memcpy(h, p4, sizeof(double) * 4 * (size_t)n);
+  for (int s = 0; s + n < stop; ++s)
+    if (hp2[s] >= 0)
+      for (int c = 0; c < 4; ++c) h[4 * (size_t)(n + s) + c] = h[4 * (size_t)hp1[s] + c] + h[4 * (size_t)hp2[s] + c];
+  int k = 0;
+  for (int e = stop; e < 2 * n; ++e) {
+    const int s = e - n;
+    if (hp1[s] < stop) memcpy(out + 4 * (size_t)k++, h + 4 * (size_t)hp1[s], sizeof(double) * 4);
+    if (hp2[s] < stop && hp2[s] > 0) memcpy(out + 4 * (size_t)k++, h + 4 * (size_t)hp2[s], sizeof(double) * 4);
+  }
+  free(h);
+  *n_out = k;
+  return 0;
+}
+
 int64_t fjo_tiled_pair_evals(const double *p4, int n, int p, double R) {
   if (n <= 0) return 0;
   int32_t *h1 = (int32_t *)malloc(sizeof(int32_t) * (size_t)n), *h2 = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);

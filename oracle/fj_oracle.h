@@ -79,6 +79,20 @@ double fjo_cluster_batch(const double *p4, const int64_t *offsets, int n_events,
 double fjo_time_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R, int nthreads,
                       int method);
 
+/* Inclusive jets above ptmin sorted by decreasing pt^2 (main.cc:309-327; equal
+ * keys keep promotion order): writes n_out jets to out. */
+int fjo_inclusive_jets(const double *jets, int njets, double ptmin, double *out, int32_t *n_out);
+
+/* FastJet's ClusterSequence history view: child[2n] (-3 = none) and max_dij[n]
+ * (max_dij_so_far of the step entries). */
+int fjo_history_children(const int32_t *hp1, const int32_t *hp2, const double *hdij, int n, int32_t *child,
+                         double *max_dij);
+
+/* FastJet's exclusive jets from a complete history: mode 0 njets = param,
+ * mode 1 dcut = param.  out holds up to n jets. */
+int fjo_exclusive_jets(const double *p4, const int32_t *hp1, const int32_t *hp2, const double *hdij, int n, int mode,
+                       double param, double *out, int32_t *n_out);
+
 /* Pair distances the tiled restatement evaluates clustering one event (its
  * initial nearest neighbours, orphan rescans and merged-jet updates): the
  * algorithmic work of the tiled method.  -1 on error. */

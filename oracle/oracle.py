@@ -40,6 +40,12 @@ def lib():
         L.fjo_cluster_batch.restype = ctypes.c_double
         L.fjo_tiled_pair_evals.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.fjo_tiled_pair_evals.restype = ctypes.c_int64
+        L.fjo_inclusive_jets.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _vp, _vp]
+        L.fjo_inclusive_jets.restype = ctypes.c_int
+        L.fjo_history_children.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, _vp]
+        L.fjo_history_children.restype = ctypes.c_int
+        L.fjo_exclusive_jets.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _vp]
+        L.fjo_exclusive_jets.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -118,6 +124,44 @@ def tiled_pair_evals(p4, p, R):
     if r < 0:
         raise ValueError("fjo_tiled_pair_evals failed")
     return int(r)
+
+
+def _P(a):
+    return a.ctypes.data_as(_vp)
+
+
+def inclusive_jets(jets, ptmin):
+    """Jets with pt^2 >= ptmin^2 sorted by decreasing pt^2 (main.cc:309-327)."""
+    jets = np.ascontiguousarray(jets, dtype=np.float64).reshape(-1, 4)
+    out = np.zeros((max(len(jets), 1), 4))
+    n = np.zeros(1, np.int32)
+    if lib().fjo_inclusive_jets(_P(jets), len(jets), float(ptmin), _P(out), _P(n)) != 0:
+        raise ValueError("fjo_inclusive_jets failed")
+    return out[:n[0]].copy()
+
+
+def history_children(h1, h2, hd):
+    """(child[2N], max_dij[N]) of a history (FastJet's ClusterSequence view)."""
+    h1, h2 = np.ascontiguousarray(h1, np.int32), np.ascontiguousarray(h2, np.int32)
+    hd = np.ascontiguousarray(hd, np.float64)
+    n = len(h1)
+    child = np.zeros(max(2 * n, 1), np.int32)
+    md = np.zeros(max(n, 1), np.float64)
+    if lib().fjo_history_children(_P(h1), _P(h2), _P(hd), n, _P(child), _P(md)) != 0:
+        raise ValueError("fjo_history_children failed")
+    return child[:2 * n], md[:n]
+
+
+def exclusive_jets(p4, h1, h2, hd, mode, param):
+    """FastJet's exclusive_jets(njets) (mode 0) or exclusive_jets(dcut) (mode 1) from a history."""
+    p4 = np.ascontiguousarray(p4, np.float64).reshape(-1, 4)
+    h1, h2 = np.ascontiguousarray(h1, np.int32), np.ascontiguousarray(h2, np.int32)
+    hd = np.ascontiguousarray(hd, np.float64)
+    out = np.zeros((max(len(p4), 1), 4))
+    k = np.zeros(1, np.int32)
+    if lib().fjo_exclusive_jets(_P(p4), _P(h1), _P(h2), _P(hd), len(p4), int(mode), float(param), _P(out), _P(k)) != 0:
+        raise ValueError("fjo_exclusive_jets failed")
+    return out[:k[0]].copy()
 
 
 def set_jet(p4row, p, math_mode=MATH_SHARED):

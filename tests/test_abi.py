@@ -33,9 +33,10 @@ def test_header_parse_finds_every_entry_point():
     for must in ("fjmi_create", "fjmi_cluster_event_device", "fjmi_cluster_pseudojets", "fjmi_batch_create",
                  "fjmi_batch_launch", "fjmi_batch_stats", "fjmi_cluster_batch", "fjmi_server_create",
                  "fjmi_server_submit", "fjmi_server_poll", "fjmi_server_release", "fjmi_server_destroy",
-                 "fjmi_stream_check", "fjmi_cluster_event_device_noalloc"):
+                 "fjmi_stream_check", "fjmi_cluster_event_device_noalloc", "fjmi_inclusive_jets",
+                 "fjmi_inclusive_pseudojets", "fjmi_history_view", "fjmi_exclusive_jets"):
         assert must in names
-    assert len(names) == 24
+    assert len(names) == 28
 
 
 @pytest.mark.parametrize("header,lib", [("fjmi.h", "libfjmi.so"), ("fjmi_gen.h", "libfjmi_gen.so")])
@@ -52,6 +53,7 @@ def test_dropin_cluster_symbol():
         pytest.skip("libfjmi_cluster.so not built")
     out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
     assert "_Z7clusterP9PseudoJeti9Algorithmd" in out
+    assert "_Z14inclusive_jetsPK9PseudoJetidPS_Pi" in out  # include/fjmi_dropin.h
 
 
 def test_calls_without_gpu_are_errors_not_crashes():

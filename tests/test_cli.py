@@ -124,3 +124,82 @@ def test_empty_engine_runs():
     r = run("tri_matrix_empty", ["-antikt", "-r", "0.4", "--csv", "-f", path])
     assert r.returncode == 0, r.stderr
     assert re.search(r"^354,\d+,\d+\.\d$", r.stdout, re.M)
+
+
+def _g(x, prec):
+    """std::cout << x under std::defaultfloat with precision prec (C's %g)."""
+    return f"{x:.{max(prec, 1)}g}"
+
+
+def expected_stdout(groups, p, R, ptmin, cartesian=False, csv=False, repeat=1, combine=1):
+    """The whole stdout of main.cc:269-368 after the banner, with the timings as
+    <t> and <s>.  Keeps the reference's stream state: print_jets leaves precision
+    8 set (main.cc:87), which the next groups' "found ... ms" lines inherit."""
+    out, prec = [], 6
+    for grp in groups:
+        n, jl = expected_lines(grp, p, R, ptmin, cartesian)
+        if not csv:
+            k = len(grp)
+            out.append(f"found {n} particles" + (f" in {k} event{'' if k == 1 else 's'}" if combine != 1 else ""))
+            for _ in range(repeat):
+                out.append(f"found {len(jl)} jets above {_g(ptmin, prec)} GeV in <t> ms" if ptmin > 0
+                           else f"found {len(jl)} jets in <t> ms")
+            out += jl + [""]
+            prec = 8
+            out.append(f"clustered {n} particles into {len(jl)} jets above {_g(ptmin, prec)} GeV in <t>"
+                       + (" +/- <s>" if repeat > 1 else "") + " ms")
+        else:
+            out.append(f"{n},{len(jl)},<t>" + (",<s>" if repeat > 1 else ""))
+    return out
+
+
+def normalised(stdout, csv=False, repeat=1):
+    """stdout without the 3 banner lines, timings replaced by <t> / <s>; the mean and
+    sigma of several repetitions must share the fixed precision main.cc:350-358 picks."""
+    lines = stdout.splitlines()
+    assert lines[0].startswith("Running on HIP device ")
+    assert re.fullmatch(r"  - maximum shared memory per block: \d+ kB", lines[1])
+    assert re.fullmatch(r"  - kernel printf buffer size:       \d+ kB", lines[2])
+    out = []
+    for l in lines[3:]:
+        m = re.fullmatch(r"(.* in )(\d+(?:\.\d+)?) \+/- (\d+(?:\.\d+)?) ms", l)
+        c = re.fullmatch(r"(\d+),(\d+),(\d+(?:\.\d+)?)(?:,(\d+(?:\.\d+)?))?", l) if csv else None
+        if m:
+            a, b = m.group(2), m.group(3)
+            assert len(a.partition(".")[2]) == len(b.partition(".")[2]), l
+            out.append(m.group(1) + "<t> +/- <s> ms")
+        elif c:
+            if c.group(4) is not None:
+                assert len(c.group(3).partition(".")[2]) == len(c.group(4).partition(".")[2]), l
+            elif repeat == 1:
+                assert re.fullmatch(r"\d+\.\d", c.group(3)), l  # precision 1 for a single repetition
+            out.append(f"{c.group(1)},{c.group(2)},<t>" + (",<s>" if c.group(4) is not None else ""))
+        else:
+            out.append(re.sub(r" in \S+ ms$", " in <t> ms", l))
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("binary,args,fname,groups,p,R,ptmin,cart,csv,rep,comb", [
+    ("tri_matrix", ["-antikt", "-r", "0.4", "-p", "1.0"], "808_cms.dat", None, -1, 0.4, 1.0, False, False, 1, 1),
+    ("grid", ["-antikt", "-r", "0.7", "--cartesian", "--combine", "2", "--repeat", "3"], "multi-event.dat", 2,
+     -1, 0.7, 0.0, True, False, 3, 2),
+    ("n_array", ["-kt", "-r", "1.0", "-p", "0.5", "--repeat", "2"], "354_fastjet.dat", None, 1, 1.0, 0.5, False,
+     False, 2, 1),
+    ("tri_matrix", ["-cam", "-r", "0.4", "--csv"], "multi-event.dat", 1, 0, 0.4, 0.0, False, True, 1, 1),
+    ("grid", ["-kt", "-r", "0.4", "-p", "2.25", "--combine", "3"], "multi-event.dat", 3, 1, 0.4, 2.25, False,
+     False, 1, 3),
+])
+def test_whole_stdout(binary, args, fname, groups, p, R, ptmin, cart, csv, rep, comb):
+    """Every stdout line of the CLI equals main.cc's output for the same input, the
+    timings aside: banner, per-event and per-repetition lines, jets, blank lines,
+    summary or CSV, stream precision carried across event groups."""
+    path = os.path.join(GOLDEN, fname)
+    events = orc.read_events(path)
+    grp = [events[k:k + (groups or 1)] for k in range(0, len(events), groups or 1)]
+    print(f"[gpu cli] {binary} {' '.join(args)} -f {fname}", flush=True)
+    r = run(binary, [*args, "-f", path])
+    assert r.returncode == 0, r.stderr
+    want = expected_stdout(grp, p, R, ptmin, cart, csv, rep, comb)
+    got = normalised(r.stdout, csv, rep)
+    assert got == want, next(((k, g, w) for k, (g, w) in enumerate(zip(got, want)) if g != w), (len(got), len(want)))
