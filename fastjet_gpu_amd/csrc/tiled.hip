@@ -105,6 +105,9 @@ struct Shared {
   Gather g[2]; /* [0]: the step's gather, [1]: orphan rescans */
   int orph[MAXO];
   int norph;
+  /* the last steps' history entries, written to global memory W at a time */
+  int hb_p1[W], hb_p2[W];
+  double hb_dij[W];
   unsigned stamp[MAXT]; /* last gather that listed the tile (dedup) */
   int pool_top;
   int rebuild;
@@ -1077,7 +1080,14 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
     /* ---- the N recombination steps ---- */
     int nj = 0;
     bool bad = false;
-    for (int step = 0; step < N; ++step) {
+    int step = 0;
+    for (; step < N; ++step) {
+      if ((step & (W - 1)) == 0 && step > 0) { /* the previous W history entries, one coalesced store each */
+        wave_sync();
+        a.hist_p1[base + step - W + lane] = sh.hb_p1[lane];
+        a.hist_p2[base + step - W + lane] = sh.hb_p2[lane];
+        a.hist_dij[base + step - W + lane] = sh.hb_dij[lane];
+      }
       const int n_old = N - step;
       /* the jet with the last label moves in this step (N2Plain compaction); its
        * index is loaded first so the load overlaps the argmin */
@@ -1188,41 +1198,36 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
       const int q_last2_v = uni(q_last2_raw);
       const double4 qt = qt_raw;
 
-      /* the stores: the history entry and the new jet (or the beam jet), then
-       * removal 1 and the N2Plain compaction (the last label moves), then, if
-       * the merged jet changes tile, removal 2 (from the segment as removal 1
-       * left it) and its insertion */
+      /* the stores: the history entry (to the LDS block) and the new jet's
+       * record, then removal 1 and the N2Plain compaction (the last label
+       * moves), then, if the merged jet changes tile, removal 2 (from the
+       * segment as removal 1 left it) and its insertion.  The stores nothing
+       * later in this step reads (the new jet's four-momentum and history
+       * index, the beam jet, the removed jet's record, the label map, the
+       * drop-in record) wait until the members' loads have been issued, so
+       * that those loads do not wait for them (loads and stores share one
+       * counter). */
       bool need_rebuild = false; /* lane 0's segment insertion exhausted the pool */
+      const int jet_slot = nj;
       if (lane == 0) {
         if (phys_lo >= 0) {
-          a.hist_p1[base + step] = min(hhi, hlo);
-          a.hist_p2[base + step] = max(hhi, hlo);
-          a.hist_dij[base + step] = dmin * invR2;
-          ev.p4[phys_lo] = qn;
-          ev.hist[phys_lo] = N + step;
+          sh.hb_p1[step & (W - 1)] = min(hhi, hlo);
+          sh.hb_p2[step & (W - 1)] = max(hhi, hlo);
+          sh.hb_dij[step & (W - 1)] = dmin * invR2;
           ev.rec[phys_lo].rp = make_double2(nrap, nphi);
           ev.rec[phys_lo].wn = make_double2(nw, R2);
           ev.rec[phys_lo].meta.x = -1;
         } else {
-          a.hist_p1[base + step] = hhi;
-          a.hist_p2[base + step] = -1;
-          a.hist_dij[base + step] = dmin * invR2;
-          double *jo = a.jets + 4 * (base + nj);
-          jo[0] = qhi.x;
-          jo[1] = qhi.y;
-          jo[2] = qhi.z;
-          jo[3] = qhi.w;
+          sh.hb_p1[step & (W - 1)] = hhi;
+          sh.hb_p2[step & (W - 1)] = -1;
+          sh.hb_dij[step & (W - 1)] = dmin * invR2;
         }
         if (pos_hi != last_hi) {
           ev.pool[pos_hi] = q_last_hi;
           ev.rec[q_last_hi].meta.w = pos_hi;
         }
         sh.seg_count[tile_hi] = cnt_hi - 1;
-        ev.rec[phys_hi].meta = make_int4(-1, -1, -1, -1);
-        if (pt != phys_hi) {
-          ev.rec[pt].meta.y = lab_hi;
-          ev.lab2[lab_hi] = pt;
-        }
+        if (pt != phys_hi) ev.rec[pt].meta.y = lab_hi;
         if (phys_lo >= 0 && tile_new != tile_lo_old) {
           const bool moved = same_tile && pos_lo == last_hi && pos_hi != last_hi;
           const int pos2 = moved ? pos_hi : pos_lo;
@@ -1235,6 +1240,21 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           ev.rec[phys_lo].meta.z = tile_new;
           need_rebuild = seg_insert(sh, ev, tile_new, phys_lo);
         }
+      }
+      auto deferred_stores = [&]() {
+        if (lane != 0) return;
+        if (phys_lo >= 0) {
+          ev.p4[phys_lo] = qn;
+          ev.hist[phys_lo] = N + step;
+        } else {
+          double *jo = a.jets + 4 * (base + jet_slot);
+          jo[0] = qhi.x;
+          jo[1] = qhi.y;
+          jo[2] = qhi.z;
+          jo[3] = qhi.w;
+        }
+        ev.rec[phys_hi].meta = make_int4(-1, -1, -1, -1);
+        if (pt != phys_hi) ev.lab2[lab_hi] = pt;
         if (a.pj_out) {
           unsigned char *rec = a.pj_out + 40 * (size_t)(base + tail);
           reinterpret_cast<int *>(rec)[0] = tail;
@@ -1246,12 +1266,20 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           dp[2] = o.z;
           dp[3] = o.w;
         }
-      }
+      };
       if (phys_lo < 0) ++nj;
       wave_sync();
       const int n = n_old - 1;
-      if (n == 0) break;
+      if (n == 0) {
+        deferred_stores();
+        ++step; /* this step's entry is recorded */
+        break;
+      }
+      bool deferred_done = false;
       if (__builtin_amdgcn_readlane((int)need_rebuild, 0)) { /* pool exhausted by a relocation: lay all segments out again */
+        deferred_stores(); /* the rebuild reads the label map */
+        deferred_done = true;
+        wave_sync();
         rebuild_segments(sh, ev, n, ntiles, lane);
       }
 
@@ -1346,6 +1374,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
           if (orph && idx < MAXO) sh.orph[idx] = q;
         }
       }
+      if (!deferred_done) deferred_stores();
       if (phys_lo >= 0) { /* the new jet's own nearest neighbour */
         double md;
         int ml;
@@ -1506,6 +1535,16 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         refresh_changed(sh, G, GO, ev, ng, phys_hi, lane, stampv);
       else
         refresh_minima(sh, G, ev, ng, lane);
+    }
+    { /* the history entries recorded since the last full block */
+      wave_sync();
+      const int done = step;
+      const int first = (done - 1) & ~(W - 1);
+      if (done > 0 && first + lane < done) {
+        a.hist_p1[base + first + lane] = sh.hb_p1[lane];
+        a.hist_p2[base + first + lane] = sh.hb_p2[lane];
+        a.hist_dij[base + first + lane] = sh.hb_dij[lane];
+      }
     }
     if constexpr (SERVE) {
       serve_finish(a, e, base, N, bad ? -1 : nj, lane);
