@@ -35,7 +35,9 @@ extern const int kTiledMaxTiles[kNumTiledKernels];
 extern const void *const kInclusiveKernel;   /* (jets, njets, offsets, ptmin, out, n_out) */
 extern const void *const kInclusivePjKernel; /* (records, n, ptmin, out, n_out) */
 extern const void *const kHistoryKernel;     /* (offsets, n_single, p1, p2, dij, child, max_dij) */
-extern const void *const kExclusiveKernel;   /* (offsets, n_single, p4, p1, p2, dij, mode, param, work, out, n_out) */
+extern const void *const kExclusiveKernel;
+extern const void *const kJetsScanKernel; /* (njets, n, joff[n+1]), one workgroup of 256 */
+extern const void *const kJetsPackKernel; /* (jets, offsets, njets, joff, out), one workgroup of 64 per event */   /* (offsets, n_single, p4, p1, p2, dij, mode, param, work, out, n_out) */
 
 /* Pool entries per event of the tiled engine beyond 3 per particle (>= 2 per tile). */
 constexpr int kTiledPoolPerEvent = 2048;

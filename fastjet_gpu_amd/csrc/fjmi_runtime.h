@@ -77,6 +77,8 @@ struct StreamState {
 }  // namespace rt
 }  // namespace fjmi
 
+struct HostPath; /* host-memory batch pipeline (runtime.hip) */
+
 /* Per-device library context. */
 struct fjmi_context {
   int device = 0;
@@ -89,6 +91,7 @@ struct fjmi_context {
   /* device address of this device's copy of the library's static scratch array
    * (fjmi_cluster_event_device_noalloc), from hipGetSymbolAddress at creation */
   unsigned char *static_scratch = nullptr;
+  HostPath *host = nullptr; /* fjmi_cluster_batch's staging, created on first use */
 };
 
 #endif

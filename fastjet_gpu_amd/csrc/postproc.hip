@@ -240,6 +240,41 @@ __global__ __launch_bounds__(256) void exclusive_kernel(const int64_t *offsets, 
   n_out[e] = k;
 }
 
+/* Jets of a batch packed back to back (for copies to the host): joff[e] = sum
+ * of max(njets, 0) over the events before e, joff[n] the total.  One workgroup. */
+__global__ __launch_bounds__(256) void jets_scan_kernel(const int32_t *njets, int n, int32_t *joff) {
+  __shared__ int sm[T];
+  int carry = 0;
+  for (int e0 = 0; e0 < n; e0 += T) {
+    const int e = e0 + (int)threadIdx.x;
+    const int v = e < n && njets[e] > 0 ? njets[e] : 0;
+    __syncthreads();
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < T; d <<= 1) { /* inclusive scan of the chunk */
+      const int o = (int)threadIdx.x >= d ? sm[threadIdx.x - d] : 0;
+      __syncthreads();
+      sm[threadIdx.x] += o;
+      __syncthreads();
+    }
+    if (e < n) joff[e] = carry + sm[threadIdx.x] - v;
+    carry += sm[T - 1];
+  }
+  if (threadIdx.x == 0) joff[n] = carry;
+}
+
+/* Copies event e's njets[e] jets from 4*offsets[e] of `jets` to 4*joff[e] of `out`. */
+__global__ __launch_bounds__(64) void jets_pack_kernel(const double *jets, const int64_t *offsets,
+                                                       const int32_t *njets, const int32_t *joff, double *out) {
+  const int e = blockIdx.x;
+  const int nd = 4 * (njets[e] > 0 ? njets[e] : 0);
+  const double *src = jets + 4 * offsets[e];
+  double *dst = out + 4 * (int64_t)joff[e];
+  for (int i = threadIdx.x; i < nd; i += 64) dst[i] = src[i];
+}
+
+const void *const kJetsScanKernel = reinterpret_cast<const void *>(&jets_scan_kernel);
+const void *const kJetsPackKernel = reinterpret_cast<const void *>(&jets_pack_kernel);
 const void *const kInclusiveKernel = reinterpret_cast<const void *>(&inclusive_kernel);
 const void *const kInclusivePjKernel = reinterpret_cast<const void *>(&inclusive_pj_kernel);
 const void *const kHistoryKernel = reinterpret_cast<const void *>(&history_kernel);

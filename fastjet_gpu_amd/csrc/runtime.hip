@@ -12,12 +12,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fjmi.h"
@@ -96,14 +99,76 @@ struct BucketRun {
   double sum_n2;
 };
 
+/* The device side of one batch of events: multiplicity buckets, their
+ * longest-first queues and scratch, and one persistent-kernel launch per
+ * bucket (BucketRun).  Built by make_plan; launched inside a captured graph
+ * (fjmi_batch_*) or directly on a stream (the host-memory pipeline). */
+struct Plan {
+  fjmi::rt::DevBuf queue, heads, p4s, coords, tscratch, times;
+  std::vector<BucketRun> runs;
+  std::vector<int32_t> flat; /* the queues, host copy */
+  uint32_t *d_status = nullptr;
+  int nb = 0;
+  void release() {
+    for (fjmi::rt::DevBuf *b : {&queue, &heads, &p4s, &coords, &tscratch, &times}) b->release();
+    runs.clear();
+  }
+};
+
+/* Host-memory batches (fjmi_cluster_batch): chunks of events in flight, each
+ * in a slot with its own stream, device buffers and pinned staging, kept by
+ * the context for the next call. */
+constexpr int kHostSlots = 4;
+constexpr int kHostMaxChunks = 4;                 /* chunks per call, at most */
+constexpr int64_t kHostChunkParticles = 1 << 18;  /* and at least this many particles each */
+
+struct HostSlot {
+  Plan plan;
+  fjmi::rt::DevBuf d_in, d_off, d_p, d_R, d_h1, d_h2, d_hd, d_jets, d_nj, d_joff, d_jc;
+  char *pin = nullptr; /* pinned staging */
+  size_t pin_cap = 0;
+  int32_t *h_h1 = nullptr, *h_h2 = nullptr, *h_nj = nullptr, *h_joff = nullptr;
+  double *h_hd = nullptr, *h_jc = nullptr;
+  hipStream_t s = nullptr;
+  std::vector<hipStream_t> fork; /* further buckets' kernels */
+  std::vector<hipEvent_t> ev_join;
+  hipEvent_t ev_fork = nullptr, ev_out = nullptr;
+  std::vector<int32_t> events; /* the chunk's events (caller's indices) */
+  std::vector<int64_t> off;    /* their particle offsets in the chunk */
+  int n_events = 0;
+  int64_t particles = 0;
+  uint32_t *h_status = nullptr;
+  bool busy = false;
+  void release() {
+    if (s) (void)hipStreamSynchronize(s);
+    plan.release();
+    for (fjmi::rt::DevBuf *b : {&d_in, &d_off, &d_p, &d_R, &d_h1, &d_h2, &d_hd, &d_jets, &d_nj, &d_joff, &d_jc})
+      b->release();
+    if (pin) (void)hipHostFree(pin);
+    pin = nullptr;
+    pin_cap = 0;
+    for (hipStream_t f : fork) (void)hipStreamDestroy(f);
+    for (hipEvent_t j : ev_join) (void)hipEventDestroy(j);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_out) (void)hipEventDestroy(ev_out);
+    if (s) (void)hipStreamDestroy(s);
+    fork.clear();
+    ev_join.clear();
+    s = nullptr;
+    ev_fork = ev_out = nullptr;
+  }
+};
+
+struct HostPath {
+  HostSlot slot[kHostSlots];
+};
+
 struct fjmi_batch {
   fjmi_context *ctx = nullptr;
   int n_events = 0;
-  DevBuf queue, heads, p4s, coords, tscratch, times;
-  std::vector<BucketRun> runs;
+  Plan plan;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  uint32_t *d_status = nullptr;
   /* timing events recorded inside the graph: [0] after the queue reset, [i+1] after bucket i */
   std::vector<hipEvent_t> tev;
 };
@@ -282,6 +347,296 @@ int check_engine(int engine) {
   return FJMI_OK;
 }
 
+/* Buckets, longest-first queues, scratch and per-bucket launch arguments of a
+ * batch whose event e holds particles [h_offsets[e], h_offsets[e+1]); the
+ * device buffers of `pl` grow as needed and are kept for the next plan.  The
+ * queues are left in pl.flat for the caller to upload to pl.queue. */
+int make_plan(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const double *d_p4, const int64_t *d_offsets,
+              const int32_t *d_p, const double *d_R, int p_uni, double R_uni, int32_t *d_h1, int32_t *d_h2,
+              double *d_hd, double *d_jets, int32_t *d_nj, Plan &pl) {
+  const int nb = (int)ctx->buckets.size();
+  pl.nb = nb;
+  pl.runs.clear();
+  pl.flat.clear();
+  std::vector<std::vector<int32_t>> lists(nb);
+  int64_t max_p4g_total = 0;
+  bool any_tiled = false;
+  for (int e = 0; e < n_events; ++e) {
+    const int64_t n = h_offsets[e + 1] - h_offsets[e];
+    if (n < 0 || h_offsets[e] < 0) return fail(FJMI_EINVAL, "offsets not monotonic at event %d", e);
+    int b = 0;
+    while (b < nb && n > ctx->buckets[b].max_n) ++b;
+    if (b == nb) return fail(FJMI_ECAPACITY, "event %d has %lld particles, above the largest bucket", e, (long long)n);
+    if (ctx->buckets[b].engine == FJMI_ENGINE_TILED && n > fjmi::kTiledMaxN)
+      return fail(FJMI_ECAPACITY, "event %d has %lld particles: the tiled engine takes at most %d", e, (long long)n,
+                  fjmi::kTiledMaxN);
+    lists[b].push_back(e);
+    if (ctx->buckets[b].engine == FJMI_ENGINE_TILED)
+      any_tiled = true;
+    else if (ctx->buckets[b].p4_global)
+      max_p4g_total = h_offsets[n_events];
+  }
+  /* cost-ordered queues (longest first): the last events to start are the cheapest */
+  std::vector<size_t> start(nb);
+  for (int b = 0; b < nb; ++b) {
+    std::stable_sort(lists[b].begin(), lists[b].end(), [&](int32_t x, int32_t y) {
+      return h_offsets[x + 1] - h_offsets[x] > h_offsets[y + 1] - h_offsets[y];
+    });
+    start[b] = pl.flat.size();
+    pl.flat.insert(pl.flat.end(), lists[b].begin(), lists[b].end());
+  }
+  int rc;
+  if ((rc = pl.queue.ensure(sizeof(int32_t) * std::max<size_t>(pl.flat.size(), 1)))) return rc;
+  if ((rc = pl.heads.ensure(sizeof(uint32_t) * (nb + 1)))) return rc;
+  if ((rc = pl.times.ensure(sizeof(unsigned long long) * (nb + 1)))) return rc;
+  if (max_p4g_total > 0) {
+    if ((rc = pl.p4s.ensure(32 * (size_t)max_p4g_total))) return rc;
+    if ((rc = pl.coords.ensure(32 * (size_t)max_p4g_total))) return rc;
+  }
+  if (any_tiled) {
+    const int64_t P = h_offsets[n_events];
+    if ((rc = pl.tscratch.ensure(fjmi::tiled_scratch_bytes(P, n_events, fjmi::kTiledPoolPerEvent)))) return rc;
+  }
+  uint32_t *heads = pl.heads.as<uint32_t>();
+  pl.d_status = heads + nb;
+  for (int b = nb - 1; b >= 0; --b) { /* largest multiplicities first */
+    if (lists[b].empty()) continue;
+    const fjmi_bucket &bk = ctx->buckets[b];
+    BucketRun r;
+    r.tiled = bk.engine == FJMI_ENGINE_TILED;
+    r.kd = r.tiled ? nullptr : find_kernel(bk.threads, bk.slots, bk.p4_global);
+    r.fn = r.tiled ? fjmi::kTiledKernels[bk.p4_global] : r.kd->fn;
+    r.threads = r.tiled ? 64 : r.kd->T;
+    r.lds = r.tiled ? 0 : dyn_lds_bytes(*r.kd);
+    r.ts = r.tiled ? fjmi::tiled_scratch_carve(pl.tscratch.ptr, h_offsets[n_events], fjmi::kTiledPoolPerEvent)
+                   : fjmi::TiledScratch{};
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, r.fn, r.threads, r.lds);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    /* FJMI_MAX_WG_PER_CU caps the persistent workgroups per CU (fewer events in
+     * flight: a smaller working set for the caches) */
+    if (const char *cap = std::getenv("FJMI_MAX_WG_PER_CU")) {
+      const int c = std::atoi(cap);
+      if (c >= 1 && c < per_cu) per_cu = c;
+    }
+    r.n_events = (int)lists[b].size();
+    r.max_n = bk.max_n;
+    r.tile_q = r.tiled ? bk.slots : 0;
+    r.tile_table = r.tiled ? bk.p4_global : 0;
+    r.particles = 0;
+    r.sum_n2 = 0.0;
+    for (int32_t ev : lists[b]) {
+      const double nev = (double)(h_offsets[ev + 1] - h_offsets[ev]);
+      r.particles += (int64_t)nev;
+      r.sum_n2 += nev * nev;
+    }
+    r.grid = (int)std::min<int64_t>(r.n_events, (int64_t)per_cu * ctx->n_cu);
+    LaunchArgs a = base_args();
+    a.in = d_p4;
+    a.offsets = d_offsets;
+    a.queue = pl.queue.as<int32_t>() + start[b];
+    a.n_queue = r.n_events;
+    a.head = heads + b;
+    a.head_base = 0;
+    a.p_arr = d_p;
+    a.R_arr = d_R;
+    a.p_uni = p_uni;
+    a.R_uni = R_uni;
+    a.hist_p1 = d_h1;
+    a.hist_p2 = d_h2;
+    a.hist_dij = d_hd;
+    a.jets = d_jets;
+    a.njets = d_nj;
+    a.p4_scratch = pl.p4s.as<double>();
+    a.coord_scratch = pl.coords.as<double>();
+    a.status = pl.d_status;
+    a.t_end = pl.times.as<unsigned long long>() + 1 + pl.runs.size();
+    a.tile_q = r.tile_q;
+    r.args = a;
+    pl.runs.push_back(r);
+  }
+  return FJMI_OK;
+}
+
+/* ---- host-memory batches: a pipeline of chunks through pinned staging ---- */
+
+/* Runs f(i) for i in [0, n) on up to 8 host threads (the staging copies). */
+template <class F>
+void parallel_for(int n, const F &f) {
+  const int T = std::max(1, std::min(8, (int)std::thread::hardware_concurrency()));
+  if (n < 256 || T == 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (;;) {
+      const int i0 = next.fetch_add(64);
+      if (i0 >= n) return;
+      for (int i = i0; i < std::min(n, i0 + 64); ++i) f(i);
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < T; ++t) th.emplace_back(work);
+  work();
+  for (std::thread &x : th) x.join();
+}
+
+int host_alloc_pinned(HostSlot &sl, size_t bytes) {
+  if (bytes <= sl.pin_cap) return FJMI_OK;
+  if (sl.pin) HIPCHK(hipHostFree(sl.pin));
+  sl.pin = nullptr;
+  sl.pin_cap = 0;
+  void *q = nullptr;
+  HIPCHK(hipHostMalloc(&q, bytes, hipHostMallocDefault));
+  sl.pin = static_cast<char *>(q);
+  sl.pin_cap = bytes;
+  return FJMI_OK;
+}
+
+/* Stages the chunk's events (original indices ev[0..n)) and enqueues its
+ * copies in, its kernels and its copies out on the slot's stream. */
+int host_start(fjmi_context *ctx, HostSlot &sl, const int32_t *ev, int n, const int64_t *offsets, const double *p4,
+               const int32_t *p, const double *R, int p_uni, double R_uni) {
+  int rc;
+  if (!sl.s) {
+    HIPCHK(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&sl.ev_out, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&sl.ev_fork, hipEventDisableTiming));
+  }
+  sl.events.assign(ev, ev + n);
+  sl.n_events = n;
+  sl.off.resize(n + 1);
+  sl.off[0] = 0;
+  for (int i = 0; i < n; ++i) sl.off[i + 1] = sl.off[i] + (offsets[ev[i] + 1] - offsets[ev[i]]);
+  const int64_t P = sl.off[n];
+  sl.particles = P;
+  const size_t Pn = (size_t)std::max<int64_t>(P, 1), En = (size_t)n;
+  /* pinned layout: in (32 P) | h1 (4 P) | h2 (4 P) | hd (8 P) | jc (32 P) | offsets (8 (E+1)) | p (4 E) |
+   * R (8 E) | queue (4 E) | nj (4 E) | joff (4 (E+1)) | status (4) */
+  const size_t o_in = 0, o_h1 = o_in + 32 * Pn, o_h2 = o_h1 + 4 * Pn, o_hd = o_h2 + 4 * Pn, o_jc = o_hd + 8 * Pn,
+               o_off = o_jc + 32 * Pn, o_p = o_off + 8 * (En + 1), o_R = o_p + 4 * En, o_q = o_R + 8 * En,
+               o_nj = o_q + 4 * En, o_jo = o_nj + 4 * En, o_st = o_jo + 4 * (En + 1), o_end = o_st + 4;
+  if ((rc = host_alloc_pinned(sl, o_end))) return rc;
+  double *in = reinterpret_cast<double *>(sl.pin + o_in);
+  int64_t *off = reinterpret_cast<int64_t *>(sl.pin + o_off);
+  int32_t *pp = reinterpret_cast<int32_t *>(sl.pin + o_p);
+  double *RR = reinterpret_cast<double *>(sl.pin + o_R);
+  std::memcpy(off, sl.off.data(), 8 * (En + 1));
+  parallel_for(n, [&](int i) {
+    const int e = ev[i];
+    std::memcpy(in + 4 * off[i], p4 + 4 * offsets[e], 32 * (size_t)(offsets[e + 1] - offsets[e]));
+    if (p) pp[i] = p[e];
+    if (R) RR[i] = R[e];
+  });
+  sl.h_h1 = reinterpret_cast<int32_t *>(sl.pin + o_h1);
+  sl.h_h2 = reinterpret_cast<int32_t *>(sl.pin + o_h2);
+  sl.h_hd = reinterpret_cast<double *>(sl.pin + o_hd);
+  sl.h_jc = reinterpret_cast<double *>(sl.pin + o_jc);
+  sl.h_nj = reinterpret_cast<int32_t *>(sl.pin + o_nj);
+  sl.h_joff = reinterpret_cast<int32_t *>(sl.pin + o_jo);
+  sl.h_status = reinterpret_cast<uint32_t *>(sl.pin + o_st);
+  /* device buffers */
+  if ((rc = sl.d_in.ensure(32 * Pn)) || (rc = sl.d_off.ensure(8 * (En + 1))) || (rc = sl.d_p.ensure(4 * En)) ||
+      (rc = sl.d_R.ensure(8 * En)) || (rc = sl.d_h1.ensure(4 * Pn)) || (rc = sl.d_h2.ensure(4 * Pn)) ||
+      (rc = sl.d_hd.ensure(8 * Pn)) || (rc = sl.d_jets.ensure(32 * Pn)) || (rc = sl.d_nj.ensure(4 * En)) ||
+      (rc = sl.d_joff.ensure(4 * (En + 1))) || (rc = sl.d_jc.ensure(32 * Pn)))
+    return rc;
+  if ((rc = make_plan(ctx, n, sl.off.data(), sl.d_in.as<double>(), sl.d_off.as<int64_t>(),
+                      p ? sl.d_p.as<int32_t>() : nullptr, R ? sl.d_R.as<double>() : nullptr, p_uni, R_uni,
+                      sl.d_h1.as<int32_t>(), sl.d_h2.as<int32_t>(), sl.d_hd.as<double>(), sl.d_jets.as<double>(),
+                      sl.d_nj.as<int32_t>(), sl.plan)))
+    return rc;
+  int32_t *q = reinterpret_cast<int32_t *>(sl.pin + o_q);
+  std::memcpy(q, sl.plan.flat.data(), 4 * sl.plan.flat.size());
+  hipStream_t s = sl.s;
+  HIPCHK(hipMemcpyAsync(sl.d_in.ptr, in, 32 * (size_t)P, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(sl.d_off.ptr, off, 8 * (En + 1), hipMemcpyHostToDevice, s));
+  if (p) HIPCHK(hipMemcpyAsync(sl.d_p.ptr, pp, 4 * En, hipMemcpyHostToDevice, s));
+  if (R) HIPCHK(hipMemcpyAsync(sl.d_R.ptr, RR, 8 * En, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(sl.plan.queue.ptr, q, 4 * sl.plan.flat.size(), hipMemcpyHostToDevice, s));
+  /* the kernels: queue heads and status cleared, then each bucket's persistent
+   * kernel (buckets after the first on streams of their own, joined again) */
+  {
+    unsigned *hq = sl.plan.heads.as<unsigned>();
+    int cnt = sl.plan.nb + 1;
+    unsigned long long *tp = sl.plan.times.as<unsigned long long>();
+    int tcnt = (int)sl.plan.runs.size() + 1;
+    void *zp[] = {&hq, &cnt, &tp, &tcnt};
+    HIPCHK(hipLaunchKernel(fjmi::kZeroKernel, dim3(1), dim3(64), zp, 0, s));
+  }
+  const size_t nr = sl.plan.runs.size();
+  if (nr > 1) {
+    HIPCHK(hipEventRecord(sl.ev_fork, s));
+    while (sl.fork.size() < nr - 1) {
+      hipStream_t f = nullptr;
+      hipEvent_t j = nullptr;
+      HIPCHK(hipStreamCreateWithFlags(&f, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+      sl.fork.push_back(f);
+      sl.ev_join.push_back(j);
+    }
+  }
+  for (size_t i = 0; i < nr; ++i) {
+    BucketRun &r = sl.plan.runs[i];
+    void *params[] = {&r.args, &r.ts};
+    hipStream_t rs = i == 0 ? s : sl.fork[i - 1];
+    if (i > 0) HIPCHK(hipStreamWaitEvent(rs, sl.ev_fork, 0));
+    HIPCHK(hipLaunchKernel(r.fn, dim3(r.grid), dim3(r.threads), params, r.lds, rs));
+    if (i > 0) HIPCHK(hipEventRecord(sl.ev_join[i - 1], rs));
+  }
+  for (size_t i = 1; i < nr; ++i) HIPCHK(hipStreamWaitEvent(s, sl.ev_join[i - 1], 0));
+  /* jets packed back to back, then the copies out of everything but the jets */
+  {
+    const int32_t *dn = sl.d_nj.as<int32_t>();
+    int32_t *jo = sl.d_joff.as<int32_t>();
+    int nn = n;
+    void *sp[] = {&dn, &nn, &jo};
+    HIPCHK(hipLaunchKernel(fjmi::kJetsScanKernel, dim3(1), dim3(256), sp, 0, s));
+    if (n > 0) {
+      const double *jets = sl.d_jets.as<double>();
+      const int64_t *doff = sl.d_off.as<int64_t>();
+      double *out = sl.d_jc.as<double>();
+      void *pk[] = {&jets, &doff, &dn, &jo, &out};
+      HIPCHK(hipLaunchKernel(fjmi::kJetsPackKernel, dim3(n), dim3(64), pk, 0, s));
+    }
+  }
+  HIPCHK(hipMemcpyAsync(sl.h_h1, sl.d_h1.ptr, 4 * (size_t)P, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(sl.h_h2, sl.d_h2.ptr, 4 * (size_t)P, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(sl.h_hd, sl.d_hd.ptr, 8 * (size_t)P, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(sl.h_nj, sl.d_nj.ptr, 4 * En, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(sl.h_joff, sl.d_joff.ptr, 4 * (En + 1), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(sl.h_status, sl.plan.d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(sl.ev_out, s));
+  sl.busy = true;
+  return FJMI_OK;
+}
+
+/* Waits for a slot's chunk, copies its packed jets out, and writes its results
+ * into the caller's arrays at the events' own positions. */
+int host_finish(HostSlot &sl, const int64_t *offsets, int32_t *h1, int32_t *h2, double *hd, double *jets,
+                int32_t *nj) {
+  sl.busy = false;
+  HIPCHK(hipEventSynchronize(sl.ev_out));
+  const int n = sl.n_events;
+  const int64_t njt = sl.h_joff[n];
+  if (njt > 0)
+    HIPCHK(hipMemcpyAsync(sl.h_jc, sl.d_jc.ptr, 32 * (size_t)njt, hipMemcpyDeviceToHost, sl.s));
+  HIPCHK(hipStreamSynchronize(sl.s));
+  if (*sl.h_status) return status_error(*sl.h_status);
+  parallel_for(n, [&](int i) {
+    const int e = sl.events[i];
+    const int64_t lo = sl.off[i], m = sl.off[i + 1] - lo, dst = offsets[e];
+    std::memcpy(h1 + dst, sl.h_h1 + lo, 4 * (size_t)m);
+    std::memcpy(h2 + dst, sl.h_h2 + lo, 4 * (size_t)m);
+    std::memcpy(hd + dst, sl.h_hd + lo, 8 * (size_t)m);
+    const int k = sl.h_nj[i];
+    nj[e] = k;
+    if (k > 0) std::memcpy(jets + 4 * dst, sl.h_jc + 4 * (size_t)sl.h_joff[i], 32 * (size_t)k);
+  });
+  return FJMI_OK;
+}
+
 int check_args(int n, int p, double R) {
   if (n < 0) return fail(FJMI_EINVAL, "negative particle count");
   if (p < -1 || p > 1) return fail(FJMI_EINVAL, "exponent p must be -1, 0 or 1");
@@ -365,6 +720,10 @@ int fjmi_destroy(fjmi_context *ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
   for (auto &kv : ctx->streams) kv.second.release_all();
+  if (ctx->host) {
+    for (HostSlot &sl : ctx->host->slot) sl.release();
+    delete ctx->host;
+  }
   delete ctx;
   return FJMI_OK;
 }
@@ -624,25 +983,6 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
   std::lock_guard<std::mutex> lk(ctx->mtx);
   int rc;
   if ((rc = set_device(ctx))) return rc;
-  const int nb = (int)ctx->buckets.size();
-  std::vector<std::vector<int32_t>> lists(nb);
-  int64_t max_p4g_total = 0;
-  bool any_tiled = false;
-  for (int e = 0; e < n_events; ++e) {
-    const int64_t n = h_offsets[e + 1] - h_offsets[e];
-    if (n < 0 || h_offsets[e] < 0) return fail(FJMI_EINVAL, "offsets not monotonic at event %d", e);
-    int b = 0;
-    while (b < nb && n > ctx->buckets[b].max_n) ++b;
-    if (b == nb) return fail(FJMI_ECAPACITY, "event %d has %lld particles, above the largest bucket", e, (long long)n);
-    if (ctx->buckets[b].engine == FJMI_ENGINE_TILED && n > fjmi::kTiledMaxN)
-      return fail(FJMI_ECAPACITY, "event %d has %lld particles: the tiled engine takes at most %d", e, (long long)n,
-                  fjmi::kTiledMaxN);
-    lists[b].push_back(e);
-    if (ctx->buckets[b].engine == FJMI_ENGINE_TILED)
-      any_tiled = true;
-    else if (ctx->buckets[b].p4_global)
-      max_p4g_total = h_offsets[n_events];
-  }
   auto *bt = new fjmi_batch();
   bt->ctx = ctx;
   bt->n_events = n_events;
@@ -650,110 +990,34 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
     fjmi_batch_destroy(bt);
     return code;
   };
-  /* cost-ordered queues (longest first): the last events to start are the cheapest */
-  std::vector<int32_t> flat;
-  std::vector<size_t> start(nb);
-  for (int b = 0; b < nb; ++b) {
-    std::stable_sort(lists[b].begin(), lists[b].end(), [&](int32_t x, int32_t y) {
-      return h_offsets[x + 1] - h_offsets[x] > h_offsets[y + 1] - h_offsets[y];
-    });
-    start[b] = flat.size();
-    flat.insert(flat.end(), lists[b].begin(), lists[b].end());
-  }
-  if ((rc = bt->queue.ensure(sizeof(int32_t) * std::max<size_t>(flat.size(), 1)))) return cleanup(rc);
-  if ((rc = bt->heads.ensure(sizeof(uint32_t) * (nb + 1)))) return cleanup(rc);
-  if ((rc = bt->times.ensure(sizeof(unsigned long long) * (nb + 1)))) return cleanup(rc);
-  if (max_p4g_total > 0) {
-    if ((rc = bt->p4s.ensure(32 * (size_t)max_p4g_total))) return cleanup(rc);
-    if ((rc = bt->coords.ensure(32 * (size_t)max_p4g_total))) return cleanup(rc);
-  }
-  if (any_tiled) {
-    const int64_t P = h_offsets[n_events];
-    if ((rc = bt->tscratch.ensure(fjmi::tiled_scratch_bytes(P, n_events, fjmi::kTiledPoolPerEvent))))
-      return cleanup(rc);
-  }
-  if (!flat.empty()) {
-    hipError_t e = hipMemcpy(bt->queue.ptr, flat.data(), sizeof(int32_t) * flat.size(), hipMemcpyHostToDevice);
+  Plan &pl = bt->plan;
+  if ((rc = make_plan(ctx, n_events, h_offsets, d_p4, d_offsets, d_p, d_R, p_uni, R_uni, d_h1, d_h2, d_hd, d_jets,
+                      d_nj, pl)))
+    return cleanup(rc);
+  if (!pl.flat.empty()) {
+    hipError_t e = hipMemcpy(pl.queue.ptr, pl.flat.data(), sizeof(int32_t) * pl.flat.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return cleanup(fail(FJMI_EHIP, "queue upload: %s", hipGetErrorString(e)));
-  }
-  uint32_t *heads = bt->heads.as<uint32_t>();
-  bt->d_status = heads + nb;
-  for (int b = nb - 1; b >= 0; --b) { /* largest multiplicities first */
-    if (lists[b].empty()) continue;
-    const fjmi_bucket &bk = ctx->buckets[b];
-    BucketRun r;
-    r.tiled = bk.engine == FJMI_ENGINE_TILED;
-    r.kd = r.tiled ? nullptr : find_kernel(bk.threads, bk.slots, bk.p4_global);
-    r.fn = r.tiled ? fjmi::kTiledKernels[bk.p4_global] : r.kd->fn;
-    r.threads = r.tiled ? 64 : r.kd->T;
-    r.lds = r.tiled ? 0 : dyn_lds_bytes(*r.kd);
-    r.ts = r.tiled ? fjmi::tiled_scratch_carve(bt->tscratch.ptr, h_offsets[n_events], fjmi::kTiledPoolPerEvent)
-                   : fjmi::TiledScratch{};
-    int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, r.fn, r.threads, r.lds);
-    if (e != hipSuccess || per_cu < 1) per_cu = 1;
-    /* FJMI_MAX_WG_PER_CU caps the persistent workgroups per CU (fewer events in
-     * flight: a smaller working set for the caches) */
-    if (const char *cap = std::getenv("FJMI_MAX_WG_PER_CU")) {
-      const int c = std::atoi(cap);
-      if (c >= 1 && c < per_cu) per_cu = c;
-    }
-    r.n_events = (int)lists[b].size();
-    r.max_n = bk.max_n;
-    r.tile_q = r.tiled ? bk.slots : 0;
-    r.tile_table = r.tiled ? bk.p4_global : 0;
-    r.particles = 0;
-    r.sum_n2 = 0.0;
-    for (int32_t ev : lists[b]) {
-      const double nev = (double)(h_offsets[ev + 1] - h_offsets[ev]);
-      r.particles += (int64_t)nev;
-      r.sum_n2 += nev * nev;
-    }
-    r.grid = (int)std::min<int64_t>(r.n_events, (int64_t)per_cu * ctx->n_cu);
-    LaunchArgs a = base_args();
-    a.in = d_p4;
-    a.offsets = d_offsets;
-    a.queue = bt->queue.as<int32_t>() + start[b];
-    a.n_queue = r.n_events;
-    a.head = heads + b;
-    a.head_base = 0;
-    a.p_arr = d_p;
-    a.R_arr = d_R;
-    a.p_uni = p_uni;
-    a.R_uni = R_uni;
-    a.hist_p1 = d_h1;
-    a.hist_p2 = d_h2;
-    a.hist_dij = d_hd;
-    a.jets = d_jets;
-    a.njets = d_nj;
-    a.p4_scratch = bt->p4s.as<double>();
-    a.coord_scratch = bt->coords.as<double>();
-    a.status = bt->d_status;
-    a.t_end = bt->times.as<unsigned long long>() + 1 + bt->runs.size();
-    a.tile_q = r.tile_q;
-    r.args = a;
-    bt->runs.push_back(r);
   }
   /* capture: zero the heads and status, then every bucket's persistent kernel on
    * its own forked stream so the buckets run concurrently */
   hipStream_t s0 = nullptr;
-  std::vector<hipStream_t> ss(bt->runs.size(), nullptr);
-  bt->tev.assign(bt->runs.size() + 1, nullptr);
+  std::vector<hipStream_t> ss(pl.runs.size(), nullptr);
+  bt->tev.assign(pl.runs.size() + 1, nullptr);
   hipError_t e = hipStreamCreateWithFlags(&s0, hipStreamNonBlocking);
   for (size_t i = 0; e == hipSuccess && i < ss.size(); ++i) e = hipStreamCreateWithFlags(&ss[i], hipStreamNonBlocking);
   for (size_t i = 0; e == hipSuccess && i < bt->tev.size(); ++i) e = hipEventCreate(&bt->tev[i]);
   if (e == hipSuccess) e = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    unsigned *hp = heads;
-    int cnt = nb + 1;
-    unsigned long long *tp = bt->times.as<unsigned long long>();
-    int tcnt = (int)bt->runs.size() + 1;
+    unsigned *hp = pl.heads.as<unsigned>();
+    int cnt = pl.nb + 1;
+    unsigned long long *tp = pl.times.as<unsigned long long>();
+    int tcnt = (int)pl.runs.size() + 1;
     void *zp[] = {&hp, &cnt, &tp, &tcnt};
     e = hipLaunchKernel(fjmi::kZeroKernel, dim3(1), dim3(64), zp, 0, s0);
     if (e == hipSuccess) e = hipEventRecord(bt->tev[0], s0);
-    for (size_t i = 0; e == hipSuccess && i < bt->runs.size(); ++i) {
+    for (size_t i = 0; e == hipSuccess && i < pl.runs.size(); ++i) {
       e = hipStreamWaitEvent(ss[i], bt->tev[0], 0);
-      BucketRun &r = bt->runs[i];
+      BucketRun &r = pl.runs[i];
       void *params[] = {&r.args, &r.ts};
       if (e == hipSuccess) e = hipLaunchKernel(r.fn, dim3(r.grid), dim3(r.threads), params, r.lds, ss[i]);
       if (e == hipSuccess) e = hipEventRecord(bt->tev[i + 1], ss[i]);
@@ -763,8 +1027,8 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
     if (e == hipSuccess) e = e2;
   }
   if (e == hipSuccess) e = hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0);
-  for (hipStream_t s : ss)
-    if (s) (void)hipStreamDestroy(s);
+  for (hipStream_t st : ss)
+    if (st) (void)hipStreamDestroy(st);
   if (s0) (void)hipStreamDestroy(s0);
   if (e != hipSuccess) return cleanup(fail(FJMI_EHIP, "graph capture: %s", hipGetErrorString(e)));
   *out = bt;
@@ -783,7 +1047,7 @@ int fjmi_batch_sync(fjmi_batch *b, void *stream) {
   HIPCHK(hipSetDevice(b->ctx->device));
   HIPCHK(hipStreamSynchronize((hipStream_t)stream));
   uint32_t st = 0;
-  HIPCHK(hipMemcpy(&st, b->d_status, sizeof st, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&st, b->plan.d_status, sizeof st, hipMemcpyDeviceToHost));
   if (st) return status_error(st);
   return FJMI_OK;
 }
@@ -791,9 +1055,9 @@ int fjmi_batch_sync(fjmi_batch *b, void *stream) {
 int fjmi_batch_stats(fjmi_batch *b, fjmi_bucket_stats *out, int cap, int *n) {
   if (!b || !n) return fail(FJMI_EINVAL, "null argument");
   HIPCHK(hipSetDevice(b->ctx->device));
-  *n = (int)b->runs.size();
+  *n = (int)b->plan.runs.size();
   for (int i = 0; i < *n && i < cap; ++i) {
-    const BucketRun &r = b->runs[i];
+    const BucketRun &r = b->plan.runs[i];
     fjmi_bucket_stats &o = out[i];
     o.bucket.max_n = r.max_n;
     o.bucket.engine = r.tiled ? FJMI_ENGINE_TILED : FJMI_ENGINE_DENSE;
@@ -807,8 +1071,8 @@ int fjmi_batch_stats(fjmi_batch *b, fjmi_bucket_stats *out, int cap, int *n) {
     o.ms = -1.0f;
   }
   /* completion times measured by the kernels themselves (100 MHz wall clock) */
-  std::vector<unsigned long long> t(b->runs.size() + 1, 0);
-  HIPCHK(hipMemcpy(t.data(), b->times.ptr, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost));
+  std::vector<unsigned long long> t(b->plan.runs.size() + 1, 0);
+  HIPCHK(hipMemcpy(t.data(), b->plan.times.ptr, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost));
   for (int i = 0; i < *n && i < cap; ++i)
     if (t[i + 1] >= t[0]) out[i].ms = (float)((double)(t[i + 1] - t[0]) / 100e3);
   return FJMI_OK;
@@ -822,12 +1086,7 @@ int fjmi_batch_destroy(fjmi_batch *b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (hipEvent_t ev : b->tev)
     if (ev) (void)hipEventDestroy(ev);
-  b->queue.release();
-  b->heads.release();
-  b->p4s.release();
-  b->coords.release();
-  b->tscratch.release();
-  b->times.release();
+  b->plan.release();
   delete b;
   return FJMI_OK;
 }
@@ -837,52 +1096,86 @@ int fjmi_cluster_batch(fjmi_context *ctx, int n_events, const int64_t *offsets, 
                        int32_t *nj) {
   if (!ctx || !offsets || n_events < 0) return fail(FJMI_EINVAL, "bad argument");
   if (n_events == 0) return FJMI_OK;
-  const size_t total = (size_t)offsets[n_events];
+  const int64_t total = offsets[n_events];
   if (total > 0 && (!p4 || !h1 || !h2 || !hd || !jets)) return fail(FJMI_EINVAL, "null buffer");
   if (!nj) return fail(FJMI_EINVAL, "null njets");
-  HIPCHK(hipSetDevice(ctx->device));
-  DevBuf dp4, doff, dpp, dR, d1, d2, dd, dj, dn;
-  auto rel = [&]() {
-    for (DevBuf *x : {&dp4, &doff, &dpp, &dR, &d1, &d2, &dd, &dj, &dn}) x->release();
-  };
-  int rc = 0;
-  const size_t tb = std::max<size_t>(total, 1);
-  if ((rc = dp4.ensure(32 * tb)) || (rc = doff.ensure(8 * (size_t)(n_events + 1))) || (rc = d1.ensure(4 * tb)) ||
-      (rc = d2.ensure(4 * tb)) || (rc = dd.ensure(8 * tb)) || (rc = dj.ensure(32 * tb)) ||
-      (rc = dn.ensure(4 * (size_t)n_events)) || (p && (rc = dpp.ensure(4 * (size_t)n_events))) ||
-      (R && (rc = dR.ensure(8 * (size_t)n_events)))) {
-    rel();
-    return rc;
+  if (!p && (p_uni < -1 || p_uni > 1)) return fail(FJMI_EINVAL, "exponent p must be -1, 0 or 1");
+  if (!R && !(R_uni > 0.0)) return fail(FJMI_EINVAL, "radius must be positive");
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  /* every event must fit a bucket before anything is started */
+  for (int e = 0; e < n_events; ++e) {
+    const int64_t n = offsets[e + 1] - offsets[e];
+    if (n < 0 || offsets[e] < 0) return fail(FJMI_EINVAL, "offsets not monotonic at event %d", e);
+    const fjmi_bucket *b = bucket_for(ctx, (int)std::min<int64_t>(n, INT32_MAX), FJMI_ENGINE_AUTO);
+    if (!b) return fail(FJMI_ECAPACITY, "event %d has %lld particles, above the largest bucket", e, (long long)n);
+    if (b->engine == FJMI_ENGINE_TILED && n > fjmi::kTiledMaxN)
+      return fail(FJMI_ECAPACITY, "event %d has %lld particles: the tiled engine takes at most %d", e, (long long)n,
+                  fjmi::kTiledMaxN);
   }
-  hipError_t e = hipSuccess;
-  if (total) e = hipMemcpy(dp4.ptr, p4, 32 * total, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(doff.ptr, offsets, 8 * (size_t)(n_events + 1), hipMemcpyHostToDevice);
-  if (e == hipSuccess && p) e = hipMemcpy(dpp.ptr, p, 4 * (size_t)n_events, hipMemcpyHostToDevice);
-  if (e == hipSuccess && R) e = hipMemcpy(dR.ptr, R, 8 * (size_t)n_events, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    rel();
-    return fail(FJMI_EHIP, "upload: %s", hipGetErrorString(e));
-  }
-  fjmi_batch *b = nullptr;
-  rc = fjmi_batch_create(ctx, n_events, offsets, dp4.as<double>(), doff.as<int64_t>(), p ? dpp.as<int32_t>() : nullptr,
-                         R ? dR.as<double>() : nullptr, p_uni, R_uni, d1.as<int32_t>(), d2.as<int32_t>(),
-                         dd.as<double>(), dj.as<double>(), dn.as<int32_t>(), &b);
-  if (!rc) rc = fjmi_batch_launch(b, nullptr);
-  if (!rc) rc = fjmi_batch_sync(b, nullptr);
-  if (b) fjmi_batch_destroy(b);
-  if (!rc) {
-    e = hipSuccess;
-    if (total) {
-      e = hipMemcpy(h1, d1.ptr, 4 * total, hipMemcpyDeviceToHost);
-      if (e == hipSuccess) e = hipMemcpy(h2, d2.ptr, 4 * total, hipMemcpyDeviceToHost);
-      if (e == hipSuccess) e = hipMemcpy(hd, dd.ptr, 8 * total, hipMemcpyDeviceToHost);
-      if (e == hipSuccess) e = hipMemcpy(jets, dj.ptr, 32 * total, hipMemcpyDeviceToHost);
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if (!ctx->host) ctx->host = new HostPath();
+  HostPath &hp = *ctx->host;
+
+  /* events longest first, cut into chunks of about equal particle counts */
+  std::vector<int32_t> order(n_events);
+  for (int e = 0; e < n_events; ++e) order[e] = e;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+    return offsets[x + 1] - offsets[x] > offsets[y + 1] - offsets[y];
+  });
+  const int n_chunks = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)kHostMaxChunks, (int64_t)n_events,
+                                                                     (total + kHostChunkParticles - 1) /
+                                                                         kHostChunkParticles}));
+  std::vector<int> cut(n_chunks + 1, n_events);
+  cut[0] = 0;
+  {
+    int64_t acc = 0;
+    int c = 1;
+    for (int i = 0; i < n_events && c < n_chunks; ++i) {
+      acc += offsets[order[i] + 1] - offsets[order[i]];
+      if (acc * n_chunks >= total * c) cut[c++] = i + 1;
     }
-    if (e == hipSuccess) e = hipMemcpy(nj, dn.ptr, 4 * (size_t)n_events, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) rc = fail(FJMI_EHIP, "download: %s", hipGetErrorString(e));
   }
-  rel();
-  return rc;
+  /* on an error, the chunks still in flight are waited for and dropped, so
+   * that no later call writes their results */
+  auto abort_all = [&](int code) {
+    for (HostSlot &sl : hp.slot)
+      if (sl.busy) {
+        (void)hipStreamSynchronize(sl.s);
+        sl.busy = false;
+      }
+    return code;
+  };
+  for (int c = 0; c < n_chunks; ++c) { /* a slot's earlier chunk must be finished before it is reused */
+    if (cut[c + 1] == cut[c]) continue;
+    HostSlot &sl = hp.slot[c % kHostSlots];
+    if (sl.busy && (rc = host_finish(sl, offsets, h1, h2, hd, jets, nj))) return abort_all(rc);
+    if ((rc = host_start(ctx, sl, order.data() + cut[c], cut[c + 1] - cut[c], offsets, p4, p, R, p_uni, R_uni)))
+      return abort_all(rc);
+  }
+  /* finished in completion order: the first chunk holds the largest events and
+   * ends last, while the others' copies out overlap its kernels */
+  for (;;) {
+    HostSlot *ready = nullptr;
+    bool any = false;
+    for (HostSlot &sl : hp.slot) {
+      if (!sl.busy) continue;
+      any = true;
+      const hipError_t q = hipEventQuery(sl.ev_out);
+      if (q == hipSuccess) {
+        ready = &sl;
+        break;
+      }
+      if (q != hipErrorNotReady) return abort_all(fail(FJMI_EHIP, "batch chunk: %s", hipGetErrorString(q)));
+    }
+    if (!any) break;
+    if (!ready) {
+      std::this_thread::sleep_for(std::chrono::microseconds(50));
+      continue;
+    }
+    if ((rc = host_finish(*ready, offsets, h1, h2, hd, jets, nj))) return abort_all(rc);
+  }
+  return FJMI_OK;
 }
 
 }  // extern "C"

@@ -215,8 +215,14 @@ typedef struct {
 } fjmi_bucket_stats;
 int fjmi_batch_stats(fjmi_batch *b, fjmi_bucket_stats *out, int cap, int *n_buckets);
 
-/* Host-memory convenience for a batch: copies in, plans, launches, copies out,
- * synchronises.  Per-particle outputs are sized like p4/4 (total particles). */
+/* Host-memory batch: copies in, clusters, copies out, synchronises.
+ * Per-particle outputs are sized like p4/4 (total particles).  The events are
+ * taken longest first in up to four chunks, each on its own stream with pinned
+ * staging that the context keeps: a chunk's copies overlap the others' kernels,
+ * and only the history and each event's njets jets come back.  For a stream of
+ * batches already in device memory, alternate two fjmi_batch plans (own
+ * outputs) on two streams, so that one batch's tail overlaps the next one's
+ * start. */
 int fjmi_cluster_batch(fjmi_context *ctx, int n_events, const int64_t *offsets, const double *p4, const int32_t *p,
                        const double *R, int p_uni, double R_uni, int32_t *hist_p1, int32_t *hist_p2, double *hist_dij,
                        double *jets, int32_t *njets);
