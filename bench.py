@@ -302,7 +302,8 @@ def serving_leg(rates, seconds):
     for line in r.stdout.splitlines():
         if line.startswith("{"):
             j = json.loads(line)
-            out.append({"offered_per_s": j["rate"], "events_per_s": j["events_per_s"], "requests": j["requests"],
+            out.append({"offered_per_s": j["rate"], "events_per_s": j["events_per_s"],
+                        "mid_run_events_per_s": j["mid_events_per_s"], "requests": j["requests"],
                         "p50_ms": j["latency_ms"]["p50"], "p99_ms": j["latency_ms"]["p99"],
                         "refused_submits": j["refused_submits"]})
     return out
@@ -329,6 +330,9 @@ def main():
                     help="before the timed region, fill and copy a 256 MiB buffer: reference kernels of known HBM "
                          "traffic for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--summarize", nargs="+", metavar="DIR", help="summarise rocprofv3 output directories and exit")
+    ap.add_argument("--events-file", metavar="FILE",
+                    help="time the events of a binary event file (fjmi_events_write) instead of the generated "
+                         "workload, with the file's per-event exponent and radius where it holds them")
     a = ap.parse_args()
     if a.summarize:
         return summarize(a.summarize)
@@ -342,8 +346,15 @@ def main():
     import fastjet_gpu_amd as fj
 
     p, R = fj.ANTIKT, 0.4
-    p4, off = fj.gen_workload(3, a.events)
+    pa = Ra = None
+    if a.events_file:
+        p4, off, pa, Ra = fj.read_events(a.events_file)
+    else:
+        p4, off = fj.gen_workload(3, a.events)
     ne = len(off) - 1
+    if ne < 1:
+        print("bench.py: no events to time", file=sys.stderr)
+        return 2
     sizes = np.diff(off)
     sha = hashlib.sha256(p4.tobytes()).hexdigest()
 
@@ -354,6 +365,9 @@ def main():
              h1=torch.zeros(tot, dtype=torch.int32, device=dev), h2=torch.zeros(tot, dtype=torch.int32, device=dev),
              hd=torch.zeros(tot, dtype=torch.float64, device=dev), j=torch.zeros(tot * 4, dtype=torch.float64, device=dev),
              nj=torch.zeros(ne, dtype=torch.int32, device=dev))
+    d["p"] = torch.from_numpy(pa).to(dev) if pa is not None else None
+    d["R"] = torch.from_numpy(Ra).to(dev) if Ra is not None else None
+    dptr = lambda t: t.data_ptr() if t is not None else None
     if a.calibrate_copy:
         src = torch.empty(64 << 20, dtype=torch.float32, device=dev)
         src.fill_(1.0)  # writes 256 MiB
@@ -363,7 +377,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     stream = torch.cuda.current_stream().cuda_stream
-    batch = eng.batch_create(off, d["p4"].data_ptr(), d["off"].data_ptr(), None, None, p, R, d["h1"].data_ptr(),
+    batch = eng.batch_create(off, d["p4"].data_ptr(), d["off"].data_ptr(), dptr(d["p"]), dptr(d["R"]), p, R,
+                             d["h1"].data_ptr(),
                              d["h2"].data_ptr(), d["hd"].data_ptr(), d["j"].data_ptr(), d["nj"].data_ptr())
     for _ in range(a.warmup):
         batch.launch(stream)
@@ -375,7 +390,8 @@ def main():
         dk = dict(d)
         for key in ("h1", "h2", "hd", "j", "nj"):
             dk[key] = torch.zeros_like(d[key])
-        bk = eng.batch_create(off, d["p4"].data_ptr(), d["off"].data_ptr(), None, None, p, R, dk["h1"].data_ptr(),
+        bk = eng.batch_create(off, d["p4"].data_ptr(), d["off"].data_ptr(), dptr(d["p"]), dptr(d["R"]), p, R,
+                              dk["h1"].data_ptr(),
                               dk["h2"].data_ptr(), dk["hd"].data_ptr(), dk["j"].data_ptr(), dk["nj"].data_ptr())
         sk = torch.cuda.Stream()
         bk.launch(sk.cuda_stream)
@@ -426,7 +442,8 @@ def main():
     if orc is not None:
         runs = []
         for k in range(max(1, a.cpu_repeat) if not a.no_cpu else 1):
-            runs.append(orc.cluster_batch(p4, off, p, R, threads, "tiled"))
+            runs.append(orc.cluster_batch(p4, off, p if pa is None else pa, R if Ra is None else Ra, threads,
+                                          "tiled"))
         w = runs[0]
         bad = 0
         for g in outs:
@@ -451,7 +468,7 @@ def main():
         print(f"[bench] CPU leg done: {parity}", file=sys.stderr, flush=True)
 
     extras = {}
-    if not a.no_extras:
+    if not a.no_extras and not a.events_file:
         # ---- plain one-event-per-launch path on a sample ----
         ns = min(a.plain_sample, ne)
         plain_wall, lat = timed_plain(eng, d, off, sizes, range(ns), p, R, stream)
@@ -536,7 +553,8 @@ def main():
              if dom["engine"] == 1 else variants.get(dom["p4_global"], "tiled_kernel"))
     dense_f = dense_equivalent_flops(float(np.sum(sizes.astype(np.float64) ** 2)), float(tot))
     alg_f = 8.0 * pairs if pairs is not None else None
-    ctr = counter_fields(t_kernel, float(tot)) if (len(bstats) == 1 and ne == 4096) else {"source": None}
+    ctr = (counter_fields(t_kernel, float(tot)) if (len(bstats) == 1 and ne == 4096 and not a.events_file)
+           else {"source": None})
     achieved = alg_f / t_kernel / 1e12 if alg_f else None
     roof = {"bound": "latency",
             "limiter": "the per-step dependent chain of one wave per event (argmin, merge, gather, neighbour "
@@ -559,9 +577,9 @@ def main():
         roof["traffic"] = vals["traffic_bytes_per_launch"]
 
     serving = None
-    if not a.no_extras and a.serve_seconds > 0:  # the server runs in its own process, after the legs above
+    if not a.no_extras and not a.events_file and a.serve_seconds > 0:  # own process, after the legs above
         serving = {"stream": "lognormal N (median 1500, sigma 0.6, cap 20000); {ak 0.4, ak 0.8, CA 0.8, kt 0.4}; "
-                             "16 GiB device budget", "rates": serving_leg([1e4, 1e5], a.serve_seconds)}
+                             "16 GiB device budget", "rates": serving_leg([1e4, 1e5, 1e6], a.serve_seconds)}
         print("[bench] serving leg done", file=sys.stderr, flush=True)
 
     line = {
@@ -581,6 +599,14 @@ def main():
                                f"one hipGraph launch of persistent bucket kernels",
                    "events": ne, "particles": int(tot), "algorithm": "antikt", "R": R},
     }
+    if a.events_file:
+        line["data"] = f"event file {os.path.basename(a.events_file)}, sha256 of four-momenta {sha[:16]}"
+        line["config"] = {"workload": f"{ne} events of {os.path.basename(a.events_file)} (mean N "
+                                      f"{sizes.mean():.0f}, max {int(sizes.max())}), "
+                                      + ("per-event p" if pa is not None else "anti-kt") + ", "
+                                      + ("per-event R" if Ra is not None else "R=0.4")
+                                      + ", one hipGraph launch of persistent bucket kernels",
+                          "events": ne, "particles": int(tot)}
     line["single_stream"] = {"value": ne * a.steps / single_wall, "unit": "events/s",
                              "ms_per_step": single_wall * 1e3 / a.steps,
                              "desc": "the same K launches on one stream: each batch alone on the GPU"}

@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 __all__ = ["Engine", "Server", "FjmiError", "KT", "CAMBRIDGE_AACHEN", "ANTIKT", "ENGINE_AUTO", "ENGINE_DENSE",
-           "ENGINE_TILED", "ENGINE_EMPTY", "gen_event", "gen_serving_attrs", "gen_workload", "lib_path", "load_lib"]
+           "ENGINE_TILED", "ENGINE_EMPTY", "gen_event", "gen_serving_attrs", "gen_workload", "lib_path", "load_lib",
+           "write_events", "read_events", "read_text_events"]
 
 KT, CAMBRIDGE_AACHEN, ANTIKT = 1, 0, -1
 EXCLUSIVE_NJETS, EXCLUSIVE_DCUT = 0, 1
@@ -107,6 +108,9 @@ def _declare(lib):
         "fjmi_cluster_event_device": ([_vp, _vp, i, i, d, i, _vp, _vp, _vp, _vp, _vp, _vp], i),
         "fjmi_cluster_pseudojets": ([_vp, _vp, i, i, d, i, _vp], i),
         "fjmi_stream_check": ([_vp, _vp], i),
+        "fjmi_events_write": ([ctypes.c_char_p, i, _vp, _vp, _vp, _vp], i),
+        "fjmi_events_info": ([ctypes.c_char_p, _c_int_p, ctypes.POINTER(ctypes.c_int64), _c_int_p, _c_int_p], i),
+        "fjmi_events_read": ([ctypes.c_char_p, _vp, _vp, _vp, _vp], i),
         "fjmi_inclusive_jets": ([_vp, i, _vp, _vp, _vp, d, _vp, _vp, _vp], i),
         "fjmi_inclusive_pseudojets": ([_vp, _vp, i, d, _vp, _vp, _vp], i),
         "fjmi_history_view": ([_vp, i, _vp, i, _vp, _vp, _vp, _vp, _vp, _vp], i),
@@ -362,6 +366,62 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# -- binary event files (fjmi_events_*, include/fjmi.h) --
+def write_events(path, p4, offsets, p=None, R=None):
+    """Writes a batch (packed (P, 4) four-momenta, int64 offsets[E+1], optional per-event p and R)."""
+    lib = load_lib()
+    p4 = np.ascontiguousarray(p4, dtype=np.float64).reshape(-1, 4)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    pa = None if p is None else np.ascontiguousarray(p, dtype=np.int32)
+    Ra = None if R is None else np.ascontiguousarray(R, dtype=np.float64)
+    _check(lib, lib.fjmi_events_write(os.fsencode(path), len(offsets) - 1, _ptr(offsets), _ptr(p4), _ptr(pa), _ptr(Ra)),
+           "fjmi_events_write")
+
+
+def read_events(path):
+    """(p4 (P, 4), offsets[E+1], p or None, R or None) of an event file."""
+    lib = load_lib()
+    ne, npart, hp, hr = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+    _check(lib, lib.fjmi_events_info(os.fsencode(path), ctypes.byref(ne), ctypes.byref(npart), ctypes.byref(hp),
+                                     ctypes.byref(hr)), "fjmi_events_info")
+    offsets = np.zeros(ne.value + 1, np.int64)
+    p4 = np.zeros((max(npart.value, 1), 4), np.float64)
+    pa = np.zeros(max(ne.value, 1), np.int32) if hp.value else None
+    Ra = np.zeros(max(ne.value, 1), np.float64) if hr.value else None
+    _check(lib, lib.fjmi_events_read(os.fsencode(path), _ptr(offsets), _ptr(p4), _ptr(pa), _ptr(Ra)),
+           "fjmi_events_read")
+    return (p4[:npart.value], offsets, None if pa is None else pa[:ne.value], None if Ra is None else Ra[:ne.value])
+
+
+def read_text_events(path):
+    """Events of a text file in the reference driver's format (main.cc:34-70): "px py pz E" per line (extra
+    fields ignored); a line that is empty or starts with '#' ends an event; any other line that does not parse
+    is an error, as it is there.  Returns (p4 (P, 4), offsets[E+1])."""
+    events, cur = [], []
+    with open(path) as fh:
+        for line in fh:
+            fields = line.split()
+            try:
+                vals = [float(x) for x in fields[:4]] if len(fields) >= 4 else None
+            except ValueError:
+                vals = None
+            if vals is None:
+                if line.rstrip("\r\n") and not line.startswith("#"):
+                    raise ValueError(f"Error while parsing particles:\n{line.rstrip()}")
+                if cur:
+                    events.append(cur)
+                    cur = []
+                continue
+            cur.append(vals)
+    if cur:
+        events.append(cur)
+    sizes = np.array([len(e) for e in events], np.int64)
+    offsets = np.zeros(len(events) + 1, np.int64)
+    np.cumsum(sizes, out=offsets[1:])
+    p4 = np.array([v for e in events for v in e], np.float64).reshape(-1, 4)
+    return p4, offsets
 
 
 # -- generator --

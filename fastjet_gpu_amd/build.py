@@ -60,10 +60,11 @@ def build(jobs=4, cli=True, oracle=True):
         f_rt = ex.submit(hip_obj, "runtime.hip")
         f_sv = ex.submit(hip_obj, "serve.hip")
         f_pp = ex.submit(hip_obj, "postproc.hip")
+        f_ef = ex.submit(hip_obj, "event_file.cpp")
         f_gen = ex.submit(run, ["gcc", *CC_FLAGS, "-shared", "-o", os.path.join(LIB, "libfjmi_gen.so"),
                                 os.path.join(CSRC, "fjmi_gen.c"), "-lm"])
         f_or = ex.submit(run, ["make", "-s", "-C", os.path.join(ROOT, "oracle")]) if oracle else None
-        objs = [f_dense.result(), f_tiled.result(), f_rt.result(), f_sv.result(), f_pp.result()]
+        objs = [f_dense.result(), f_tiled.result(), f_rt.result(), f_sv.result(), f_pp.result(), f_ef.result()]
         f_gen.result()
         if f_or:
             f_or.result()

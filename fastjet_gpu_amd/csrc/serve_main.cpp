@@ -14,14 +14,20 @@
  *
  * Latency of a request = the time its completion is seen by the host loop minus
  * its arrival time (so it includes queueing for admission and on the GPU).
- * Throughput = requests / (last completion - first arrival).
+ * Throughput = requests / (last completion - first arrival); beside it, the rate
+ * between the 10th and 90th percentile completions (without the ramp-up and
+ * the final drain, whose largest events finish alone).
  *
  *   fjmi_serve [--rates 1e3,1e4,1e5,1e6] [--events N] [--seconds S] [--unique U]
  *              [--device-gib G] [--host-gib H] [--slots E] [--variant V]
+ *              [--replay FILE]
  * One JSON line per rate.  --events 0 (default) uses min(200000, rate * S)
  * requests with S = --seconds (20).  --variant: the tiled kernel variant
  * (default 1, 384 tiles and 10 workgroups per CU: the higher served rate near
- * saturation; variant 0 gives up to 2 % lower p99 at light load).
+ * saturation; variant 0 gives up to 2 % lower p99 at light load).  --replay:
+ * the events of a binary event file (fjmi_events_*) replace the generated
+ * ones, request i taking event i mod E and, where the file holds them, that
+ * event's exponent and radius.
  */
 #include <hip/hip_runtime_api.h>
 
@@ -76,6 +82,7 @@ int main(int argc, char **argv) {
   int unique = 2048;
   double device_gib = 16.0, host_gib = 8.0;
   int slots = 0, variant = 1;
+  std::string replay;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     auto next = [&]() -> const char * {
@@ -109,6 +116,8 @@ int main(int argc, char **argv) {
       slots = std::atoi(next());
     } else if (a == "--variant") {
       variant = std::atoi(next());
+    } else if (a == "--replay") {
+      replay = next();
     } else {
       std::fprintf(stderr, "Unrecognized option %s\n", a.c_str());
       return 1;
@@ -124,16 +133,44 @@ int main(int argc, char **argv) {
       return 1;
     }
 
-  /* the distinct events, generated before any clock starts */
-  std::vector<std::vector<double>> pool(unique);
-  double mean_n = 0.0;
-  for (int e = 0; e < unique; ++e) {
-    const int n = fjmi_gen_event_size(5, e);
-    pool[e].resize(4 * (size_t)std::max(n, 1));
-    if (fjmi_gen_event(5, e, pool[e].data(), n) != n) die("generator");
-    pool[e].resize(4 * (size_t)n);
-    mean_n += n;
+  /* the distinct events, generated or read before any clock starts */
+  std::vector<std::vector<double>> pool;
+  std::vector<int32_t> file_p;
+  std::vector<double> file_R;
+  if (!replay.empty()) {
+    int ne = 0, has_p = 0, has_R = 0;
+    int64_t np = 0;
+    if (fjmi_events_info(replay.c_str(), &ne, &np, &has_p, &has_R)) die("fjmi_events_info");
+    if (ne < 1) {
+      std::fprintf(stderr, "fjmi_serve: %s holds no events\n", replay.c_str());
+      return 1;
+    }
+    std::vector<int64_t> off((size_t)ne + 1);
+    std::vector<double> p4(4 * (size_t)std::max<int64_t>(np, 1));
+    if (has_p) file_p.resize(ne);
+    if (has_R) file_R.resize(ne);
+    if (fjmi_events_read(replay.c_str(), off.data(), p4.data(), has_p ? file_p.data() : nullptr,
+                         has_R ? file_R.data() : nullptr))
+      die("fjmi_events_read");
+    for (int e = 0; e < ne; ++e)
+      if (off[e + 1] - off[e] > 2000000000 / 4) {
+        std::fprintf(stderr, "fjmi_serve: event %d of %s is too large\n", e, replay.c_str());
+        return 1;
+      }
+    pool.resize(ne);
+    for (int e = 0; e < ne; ++e) pool[e].assign(p4.begin() + 4 * off[e], p4.begin() + 4 * off[e + 1]);
+    unique = ne;
+  } else {
+    pool.resize(unique);
+    for (int e = 0; e < unique; ++e) {
+      const int n = fjmi_gen_event_size(5, e);
+      pool[e].resize(4 * (size_t)std::max(n, 1));
+      if (fjmi_gen_event(5, e, pool[e].data(), n) != n) die("generator");
+      pool[e].resize(4 * (size_t)n);
+    }
   }
+  double mean_n = 0.0;
+  for (const auto &ev : pool) mean_n += (double)(ev.size() / 4);
   mean_n /= unique;
 
   fjmi_context *ctx = nullptr;
@@ -151,6 +188,7 @@ int main(int argc, char **argv) {
   fjmi_server_info_get(srv, &info0);
 
   std::vector<fjmi_server_result> res(4096);
+  std::vector<double> done_at; /* completion times of the current rate's requests */
   /* no completion for this long while requests are outstanding ends the run */
   const double stall_limit_s = 30.0;
   double last_progress = 0.0;
@@ -172,7 +210,10 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "fjmi_serve: request %llu failed on the GPU\n", (unsigned long long)res[i].tag);
         std::exit(1);
       }
-      if (lat) (*lat)[res[i].tag] = now - (*reqs)[res[i].tag].arrival;
+      if (lat) {
+        (*lat)[res[i].tag] = now - (*reqs)[res[i].tag].arrival;
+        done_at.push_back(now);
+      }
       if (fjmi_server_release(srv, res[i].handle)) die("fjmi_server_release");
     }
     if (k) {
@@ -213,9 +254,14 @@ int main(int argc, char **argv) {
       double R, gap;
       fjmi_gen_serving_attrs(i, &p, &R, &gap);
       t += gap / rate;
-      reqs[i] = Request{(int)(i % unique), p, R, t};
+      const int ev = (int)(i % unique);
+      if (!file_p.empty()) p = file_p[ev];
+      if (!file_R.empty()) R = file_R[ev];
+      reqs[i] = Request{ev, p, R, t};
     }
     std::vector<double> lat(n_req, 0.0), admit_wait(n_req, 0.0);
+    done_at.clear();
+    done_at.reserve(n_req);
     std::deque<long> fifo;
     long next = 0, done = 0, eagain = 0, max_fifo = 0;
     double last_done = 0.0;
@@ -250,14 +296,21 @@ int main(int argc, char **argv) {
     fjmi_server_info after{};
     fjmi_server_info_get(srv, &after);
     const double span = last_done - reqs[0].arrival;
+    /* the served rate between the 10th and the 90th percentile completion: the
+     * server's rate once started and before the last events drain (an overloaded
+     * run ends with a backlog whose largest events finish alone) */
+    std::sort(done_at.begin(), done_at.end());
+    const size_t i10 = done_at.size() / 10, i90 = done_at.size() * 9 / 10;
+    const double mid = (i90 > i10 && done_at[i90] > done_at[i10])
+                           ? (double)(i90 - i10) / (done_at[i90] - done_at[i10]) : 0.0;
     std::printf(
-        "{\"rate\": %.6g, \"requests\": %ld, \"events_per_s\": %.1f, \"offered_per_s\": %.1f, "
+        "{\"rate\": %.6g, \"requests\": %ld, \"events_per_s\": %.1f, \"mid_events_per_s\": %.1f, \"offered_per_s\": %.1f, "
         "\"latency_ms\": {\"p50\": %.3f, \"p90\": %.3f, \"p99\": %.3f, \"p999\": %.3f, \"max\": %.3f}, "
         "\"admission_wait_ms\": {\"p50\": %.3f, \"p99\": %.3f}, \"max_host_queue\": %ld, \"refused_submits\": %ld, "
         "\"peak_particles_in_flight\": %lld, \"particle_capacity\": %lld, \"event_slots\": %d, "
         "\"device_bytes\": %lld, \"host_bytes\": %lld, \"launches\": %lld, \"grid\": %d, "
         "\"mean_n_unique\": %.1f, \"unique_events\": %d, \"server_create_s\": %.3f, \"gpu_peak_busy\": %lld}\n",
-        rate, n_req, (double)n_req / span, (double)n_req / reqs[n_req - 1].arrival, 1e3 * percentile(lat, 0.50),
+        rate, n_req, (double)n_req / span, mid, (double)n_req / reqs[n_req - 1].arrival, 1e3 * percentile(lat, 0.50),
         1e3 * percentile(lat, 0.90), 1e3 * percentile(lat, 0.99), 1e3 * percentile(lat, 0.999),
         1e3 * percentile(lat, 1.0), 1e3 * percentile(admit_wait, 0.50), 1e3 * percentile(admit_wait, 0.99), max_fifo,
         eagain, (long long)after.peak_particles_in_flight, (long long)after.particle_capacity, after.event_capacity,

@@ -227,6 +227,22 @@ int fjmi_cluster_batch(fjmi_context *ctx, int n_events, const int64_t *offsets, 
                        const double *R, int p_uni, double R_uni, int32_t *hist_p1, int32_t *hist_p2, double *hist_dij,
                        double *jets, int32_t *njets);
 
+/* ---- Binary event-batch files ------------------------------------------------
+ * Many events' four-momenta (4 doubles per particle) and int64 offsets, with
+ * optional per-event exponent p and radius R, for throughput runs and serving
+ * replays.  Layout: "FJMIEVT1", uint64 events, uint64 particles, uint32 flags
+ * (bit 0: p present, bit 1: R present), uint32 0, int64 offsets[E+1],
+ * double p4[4P], int32 p[E], double R[E]; little-endian, no padding.  The
+ * reference reads text events only (main.cc:34-85); its text files convert to
+ * this form losslessly.  Host memory; no GPU needed. */
+int fjmi_events_write(const char *path, int n_events, const int64_t *offsets, const double *p4, const int32_t *p,
+                      const double *R);
+/* Sizes of a file's contents; has_p / has_R may be NULL. */
+int fjmi_events_info(const char *path, int *n_events, int64_t *n_particles, int *has_p, int *has_R);
+/* Reads a file into caller buffers sized from fjmi_events_info (p / R may be
+ * NULL to skip them); checks magic, sizes, offsets, exponents and radii. */
+int fjmi_events_read(const char *path, int64_t *offsets, double *p4, int32_t *p, double *R);
+
 /* ---- Serving: continuous batching of arriving events ------------------------
  * Not in the reference (its CLI clusters one event per call, main.cc:269-368).
  * A server keeps a fixed device-memory arena for the state of admitted events

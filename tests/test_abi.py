@@ -36,7 +36,7 @@ def test_header_parse_finds_every_entry_point():
                  "fjmi_stream_check", "fjmi_cluster_event_device_noalloc", "fjmi_inclusive_jets",
                  "fjmi_inclusive_pseudojets", "fjmi_history_view", "fjmi_exclusive_jets"):
         assert must in names
-    assert len(names) == 28
+    assert len(names) == 31
 
 
 @pytest.mark.parametrize("header,lib", [("fjmi.h", "libfjmi.so"), ("fjmi_gen.h", "libfjmi_gen.so")])

@@ -165,3 +165,27 @@ def test_server_argument_errors(engine):
             srv.submit(np.ones((cap + 1, 4)), 1, 0.4, 0)
     finally:
         srv.close()
+
+
+@pytest.mark.gpu
+def test_serve_replays_an_event_file(tmp_path):
+    """fjmi_serve --replay: the load generator serves the events of a binary event file with their own
+    exponent and radius (every result is checked for a GPU failure by fjmi_serve itself)."""
+    import json
+    import os
+    import subprocess
+    import fastjet_gpu_amd as fj
+    print("case: fjmi_serve --replay, 64 events, 200 requests at 2000/s", flush=True)
+    p4, off = fj.gen_workload(5, 64)
+    ne = len(off) - 1
+    path = str(tmp_path / "replay.fjmi")
+    fj.write_events(path, p4, off, np.array([(-1, 0, 1)[e % 3] for e in range(ne)], np.int32),
+                    np.array([(0.4, 0.8)[e % 2] for e in range(ne)]))
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fastjet_gpu_amd", "bin",
+                       "fjmi_serve")
+    r = subprocess.run([exe, "--replay", path, "--rates", "2000", "--events", "200", "--device-gib", "2",
+                        "--host-gib", "1"], capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = [json.loads(x) for x in r.stdout.splitlines() if x.startswith("{")]
+    assert len(lines) == 1 and lines[0]["requests"] == 200 and lines[0]["unique_events"] == ne
+    assert abs(lines[0]["mean_n_unique"] - float(np.mean(np.diff(off)))) < 0.06
