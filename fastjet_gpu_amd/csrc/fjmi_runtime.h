@@ -84,6 +84,9 @@ struct fjmi_context {
   int device = 0;
   int n_cu = 0;
   std::vector<fjmi_bucket> buckets;
+  /* set by fjmi_set_buckets: the caller's table then also decides the engine
+   * of one-event calls with FJMI_ENGINE_AUTO (else the latency rule does) */
+  bool caller_table = false;
   std::mutex mtx;
   /* one-event paths: per-stream state (key: the hipStream_t, nullptr for the null stream) */
   std::map<void *, fjmi::rt::StreamState> streams;

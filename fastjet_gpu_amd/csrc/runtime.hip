@@ -231,6 +231,25 @@ bool dense_row_for(int n, fjmi_bucket *out) {
   return true;
 }
 
+/* The engine that finishes one event alone soonest.  The dense engine's step
+ * costs grow with N (a scan of every slot) and not with R; the tiled engine's
+ * with the jets in a 3x3 tile neighbourhood (about 0.14 N R^2 for tiles of side
+ * R over |y| < 5), on top of a larger fixed cost per step.  Measured on MI355X,
+ * one event alone, both engines at their best shapes, N from 100 to 8000:
+ *   R = 0.2: dense up to N = 354, a tie at 500, tiled from 808;
+ *   R = 0.3, 0.4, 0.45: dense up to N = 1000, tiled at 1500 and 2000;
+ *   R = 0.5: tiled 2 % ahead at N = 1500, dense ahead at 2000 and 5194;
+ *   R >= 0.6: dense at every N it holds (up to 8192).
+ * So: dense up to N_c(R), with N_c = 500 at R <= 0.2 rising linearly to 1200 at
+ * R = 0.3 and staying there, and dense at every N from R = 0.5 (no case measured
+ * more than 2 % from the faster engine).  Both engines give the same results
+ * bit for bit. */
+bool latency_prefers_dense(int n, double R) {
+  if (R >= 0.5) return true;
+  const double nc = R <= 0.2 ? 500.0 : (R >= 0.3 ? 1200.0 : 500.0 + 7000.0 * (R - 0.2));
+  return (double)n <= nc;
+}
+
 LaunchArgs base_args() {
   LaunchArgs a;
   std::memset(&a, 0, sizeof a);
@@ -284,10 +303,15 @@ int zero_words(void *p, int n, hipStream_t stream) {
 int cluster_one_device(fjmi_context *ctx, fjmi::rt::StreamState &st, int engine, const double *d_in, int in_stride,
                        int in_off, int n, int p, double R, int32_t *h1, int32_t *h2, double *hd, double *jets,
                        int32_t *nj, unsigned char *pj, hipStream_t stream) {
-  /* engine forced to tiled: the table's tiled row for n, else tile side R with the 512-tile variant */
-  const fjmi_bucket *b = bucket_for(ctx, n, engine);
-  if (!b && engine == FJMI_ENGINE_TILED) b = &kTiledRow;
+  /* FJMI_ENGINE_AUTO under the default table: the latency rule, the dense shape
+   * with the fewest slots per thread; engine forced to tiled: the table's tiled
+   * row for n, else tile side R with the 512-tile variant */
   fjmi_bucket forced;
+  const fjmi_bucket *b = nullptr;
+  if (engine == FJMI_ENGINE_AUTO && !ctx->caller_table && latency_prefers_dense(n, R) && dense_row_for(n, &forced))
+    b = &forced;
+  if (!b) b = bucket_for(ctx, n, engine);
+  if (!b && engine == FJMI_ENGINE_TILED) b = &kTiledRow;
   if (!b && engine == FJMI_ENGINE_DENSE && dense_row_for(n, &forced)) b = &forced;
   if (!b) return fail(FJMI_ECAPACITY, "event with %d particles exceeds the largest bucket of engine %d", n, engine);
   if (b->engine == FJMI_ENGINE_TILED && n > fjmi::kTiledMaxN)
@@ -734,6 +758,7 @@ int fjmi_set_buckets(fjmi_context *ctx, const fjmi_bucket *rows, int n_rows) {
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(ctx->mtx);
   ctx->buckets.assign(rows, rows + n_rows);
+  ctx->caller_table = true;
   return FJMI_OK;
 }
 

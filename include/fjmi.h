@@ -46,7 +46,11 @@ enum fjmi_status {
 };
 
 enum fjmi_engine {
-  FJMI_ENGINE_AUTO = 0,  /* the engine of the event's multiplicity bucket */
+  FJMI_ENGINE_AUTO = 0,  /* batches: the engine of the event's multiplicity bucket.  One-event calls:
+                          * the engine that finishes one event alone soonest (dense for R >= 0.5 up to
+                          * its 8192-particle capacity; below R = 0.5 dense up to N = 500 at R <= 0.2,
+                          * rising linearly to 1200 at R = 0.3 and above; tiled otherwise), until
+                          * fjmi_set_buckets installs a table, which then decides for them too */
   FJMI_ENGINE_DENSE = 1, /* replaces tri_matrix.cu / n_array.cu */
   FJMI_ENGINE_TILED = 2, /* replaces grid.cu; a single event takes the table's tiled row for its
                           * size, or tile side R and variant 0 when the table has none */
@@ -91,7 +95,9 @@ int fjmi_destroy(fjmi_context *ctx);
  * one row per tile table size) and returns how many exist. */
 int fjmi_kernel_configs(fjmi_bucket *out, int cap);
 
-/* Replace / read the multiplicity-bucket table (rows sorted by max_n). */
+/* Replace / read the multiplicity-bucket table (rows sorted by max_n).  After
+ * fjmi_set_buckets the table also picks the engine of one-event calls with
+ * FJMI_ENGINE_AUTO, in place of the latency rule (see fjmi_engine). */
 int fjmi_set_buckets(fjmi_context *ctx, const fjmi_bucket *rows, int n_rows);
 int fjmi_get_buckets(fjmi_context *ctx, fjmi_bucket *rows, int cap, int *n_rows);
 
