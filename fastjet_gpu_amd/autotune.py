@@ -8,8 +8,8 @@ current GPU, and keeps the fastest:
                 configuration whose capacity threads*slots is at least hi
                 (and at most 16*hi, larger ones only waste LDS and lanes);
   tiled engine  tile side R, 1.25 R and 1.5 R, with each compiled kernel
-                variant (tile tables of 512, 384, 256 or 128 tiles, and so 8,
-                10, 16 or 16 events per CU).
+                variant (tile tables of 512, 384, 320 or 256 tiles, and so 8,
+                10, 12 or 12 events per CU).
 
 Adjacent intervals with the same winner are merged; the last row is always a
 tiled row covering every larger event.
@@ -20,6 +20,10 @@ merged per-interval table and simpler ones (the per-interval dense winners up
 to a cut, then the best tiled variant), and keeps the fastest.  Per-interval throughput does not decide alone:
 in a mixed batch the kernels of all buckets share the GPU, the batch ends with
 its largest events, and every distinct kernel competes for instruction cache.
+The objective of that stage (--objective) is the throughput of a stream of
+batches, two batch plans alternating on two streams so that one batch's tail
+overlaps the next one's start (pipelined, the default), or of one batch at a
+time (single, which favours the variants whose largest events finish first).
 
 The reference picks its launch shapes per call from the occupancy calculator
 instead (launch.h:11-60, grid.cu:597-633).
@@ -28,7 +32,8 @@ Events of an interval come from the seeded generator: events of the 4096-event
 N ~ U[50, 2000] workload whose N falls in the interval, or, outside [50, 2000],
 random subsets of the right size of lognormal-multiplicity events.
 
-  python -m fastjet_gpu_amd.autotune [--events 4096] [--json out.json] [--write]
+  python -m fastjet_gpu_amd.autotune [--events 4096] [--objective pipelined|single]
+                                     [--json out.json] [--write]
 
 --write replaces csrc/tune_table.inc (the library's default table; rebuild with
 python -m fastjet_gpu_amd.build afterwards).  Needs a GPU.
@@ -166,6 +171,7 @@ def simple_tables(choices, last):
 
 
 def time_table(eng, rows, p4, off, d, p, R, reps):
+    """Events/s of one batch launch at a time (best of the repetitions)."""
     eng.set_buckets(rows)
     b = eng.batch_create(off, d["p4"].data_ptr(), d["off"].data_ptr(), None, None, p, R, d["h1"].data_ptr(),
                          d["h2"].data_ptr(), d["hd"].data_ptr(), d["j"].data_ptr(), d["nj"].data_ptr())
@@ -181,6 +187,33 @@ def time_table(eng, rows, p4, off, d, p, R, reps):
         return (len(off) - 1) / best
     finally:
         b.close()
+
+
+def time_table_pipelined(torch, eng, rows, off, ds, streams, p, R, reps, launches=12):
+    """Events/s of a stream of batches: one plan per output set in `ds`, launched in turn on `streams`, so that
+    a batch's tail overlaps the next batch's start (best of the repetitions of `launches` launches)."""
+    eng.set_buckets(rows)
+    plans = [eng.batch_create(off, d["p4"].data_ptr(), d["off"].data_ptr(), None, None, p, R, d["h1"].data_ptr(),
+                              d["h2"].data_ptr(), d["hd"].data_ptr(), d["j"].data_ptr(), d["nj"].data_ptr())
+             for d in ds]
+    try:
+        for b, s in zip(plans, streams):
+            b.launch(s.cuda_stream)
+            b.sync(s.cuda_stream)
+        best = float("inf")
+        for _ in range(max(reps, 3)):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for k in range(launches):
+                plans[k % len(plans)].launch(streams[k % len(plans)].cuda_stream)
+            torch.cuda.synchronize()
+            best = min(best, (time.perf_counter() - t) / launches)
+        for b, s in zip(plans, streams):
+            b.sync(s.cuda_stream)
+        return (len(off) - 1) / best
+    finally:
+        for b in plans:
+            b.close()
 
 
 def device_buffers(torch, dev, p4, off):
@@ -214,6 +247,8 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--batch-workload", type=int, default=3, help="generator workload of the whole-batch stage")
     ap.add_argument("--batch-events", type=int, default=4096)
+    ap.add_argument("--objective", choices=("pipelined", "single"), default="pipelined",
+                    help="whole-batch stage: a stream of batches on two alternating plans, or one batch at a time")
     ap.add_argument("--json", default=None)
     ap.add_argument("--write", action="store_true")
     a = ap.parse_args(argv)
@@ -249,10 +284,20 @@ def main(argv=None):
         # whole-batch stage
         p4, off = fj.gen_workload(a.batch_workload, a.batch_events)
         d = device_buffers(torch, dev, p4, off)
+        if a.objective == "pipelined":
+            d2 = device_buffers(torch, dev, p4, off)
+            d2["p4"], d2["off"] = d["p4"], d["off"]
+            streams = [torch.cuda.current_stream(), torch.cuda.Stream()]
+
+            def timed(rows):
+                return time_table_pipelined(torch, eng, rows, off, [d, d2], streams, a.p, a.R, a.reps)
+        else:
+            def timed(rows):
+                return time_table(eng, rows, p4, off, d, a.p, a.R, a.reps)
         whole = []
         alone = tiled_alone(configs)
         for name, rows in alone:
-            rate = time_table(eng, rows, p4, off, d, a.p, a.R, a.reps)
+            rate = timed(rows)
             whole.append(dict(name=name, table=[list(r) for r in rows], events_per_s=rate))
             print(f"batch of {len(off) - 1} (workload {a.batch_workload}): {name:32s} {rate:10.0f} ev/s  {rows}",
                   flush=True)
@@ -264,26 +309,28 @@ def main(argv=None):
                            merge([(choices[-2][0], last), (LAST_ROW, choices[-1][1])])))
         tables.append(("library default", saved))
         for name, rows in tables:
-            rate = time_table(eng, rows, p4, off, d, a.p, a.R, a.reps)
+            rate = timed(rows)
             whole.append(dict(name=name, table=[list(r) for r in rows], events_per_s=rate))
             print(f"batch of {len(off) - 1} (workload {a.batch_workload}): {name:32s} {rate:10.0f} ev/s  {rows}",
                   flush=True)
         del d
+        if a.objective == "pipelined":
+            del d2
     finally:
         eng.set_buckets(saved)
     best_whole = max(whole, key=lambda w: w["events_per_s"])
     rows = [tuple(r) for r in best_whole["table"]]
     props = torch.cuda.get_device_properties(0)
     dev_name = f"{props.name} ({getattr(props, 'gcnArchName', '?')})"
-    out = dict(device=dev_name, p=a.p, R=a.R, events=a.events, edges=edges, measurements=results,
+    out = dict(device=dev_name, p=a.p, R=a.R, events=a.events, edges=edges, objective=a.objective, measurements=results,
                whole_batch=whole, chosen=best_whole["name"], table=[list(r) for r in rows])
     print(json.dumps(dict(table=out["table"])), flush=True)
     if a.json:
         with open(a.json, "w") as fh:
             json.dump(out, fh, indent=1)
     if a.write:
-        note = (f"whole-batch stage on {a.batch_events} events of workload {a.batch_workload}, p={a.p} R={a.R}: "
-                f"{best_whole['name']}")
+        note = (f"whole-batch stage ({a.objective}) on {a.batch_events} events of workload {a.batch_workload}, "
+                f"p={a.p} R={a.R}: {best_whole['name']}")
         print("wrote", write_inc(rows, f"{note}, {dev_name}"))
     eng.close()
 

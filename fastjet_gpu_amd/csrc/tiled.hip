@@ -101,7 +101,11 @@ struct Shared {
   int mt_p[W];
   int mt_t[W];
   int nchg;
-  int seg_start[MAXT], seg_count[MAXT], seg_cap[MAXT];
+  int seg_start[MAXT], seg_count[MAXT];
+  /* segment capacities, at most 0xFFFF: a tile never holds more than an
+   * event's kTiledMaxN = 65535 jets (16 bits, so 320 tiles fit 12 workgroups
+   * per CU) */
+  unsigned short seg_cap[MAXT];
   Gather g[2]; /* [0]: the step's gather, [1]: orphan rescans */
   int orph[MAXO];
   int norph;
@@ -324,7 +328,7 @@ __device__ void rebuild_segments(SH &sh, const Ev &ev, int n, int ntiles, int la
     const int incl = wave_incl_scan(cap, lane);
     if (t < ntiles) {
       sh.seg_start[t] = carry + incl - cap;
-      sh.seg_cap[t] = cap;
+      sh.seg_cap[t] = (unsigned short)min(cap, 0xFFFF);
       sh.seg_count[t] = 0;
     }
     carry += __builtin_amdgcn_readlane(incl, 63);
@@ -562,11 +566,14 @@ __device__ __forceinline__ int gathered(const SH &sh, const Gather &G, const Ev 
 
 /* Appends physical p to tile t's segment; relocates the segment to the end of
  * the pool with twice the capacity when full, or returns true (a rebuild is
- * needed) when the pool is exhausted (p is then recorded only in its metadata). */
+ * needed) when the pool is exhausted (p is then recorded only in its metadata).
+ * The capacity stays at most 0xFFFF: an appended jet is the merged one, whose
+ * parent lo is in another tile, so the segment then holds at most n - 1 <=
+ * 65534 jets and a full segment has a capacity below 0xFFFF. */
 template <class SH>
 __device__ bool seg_insert(SH &sh, const Ev &ev, int t, int p) {
-  if (sh.seg_count[t] == sh.seg_cap[t]) {
-    const int ncap = 2 * sh.seg_cap[t] + 2;
+  if (sh.seg_count[t] == (int)sh.seg_cap[t]) {
+    const int ncap = min(2 * (int)sh.seg_cap[t] + 2, 0xFFFF);
     if (sh.pool_top + ncap > ev.pool_size) {
       sh.rebuild = 1;
       return true;
@@ -952,7 +959,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
         const int pincl = wave_incl_scan(c, lane);
         if (t < ntiles) {
           sh.seg_start[t] = carry + incl - cap;
-          sh.seg_cap[t] = cap;
+          sh.seg_cap[t] = (unsigned short)min(cap, 0xFFFF);
           sh.tmin[t].p = pcarry + pincl - c;
         }
         carry += __builtin_amdgcn_readlane(incl, 63);
@@ -1554,16 +1561,19 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
   }
 }
 
-/* 512 tiles: LDS holds 8 workgroups per CU; 384 tiles (25 rows of 15 at
- * R = 0.4, |y| < 5): 10 workgroups per CU, registers for 3 waves per SIMD;
- * 256 tiles: 4 waves per SIMD with a few spills; 128 tiles: 4 waves per SIMD. */
+/* Workgroups per CU, from 160 KiB of LDS and the register budget of the
+ * waves per SIMD requested: 512 tiles, 2 waves (the full register budget):
+ * 8 per CU; 384 tiles (25 rows of 15 at R = 0.4, |y| < 5), 3 waves: 10 per CU
+ * (LDS-bound); 320 tiles, 3 waves: 12 per CU; 256 tiles, 3 waves: 12 per CU
+ * (register-bound; more tiles per event than 192 at the same occupancy, and
+ * fewer spills than at 4 waves). */
 const void *const kTiledKernels[kNumTiledKernels] = {
     reinterpret_cast<const void *>(&tiled_kernel<512, 2, false>), reinterpret_cast<const void *>(&tiled_kernel<384, 3, false>),
-    reinterpret_cast<const void *>(&tiled_kernel<256, 4, false>), reinterpret_cast<const void *>(&tiled_kernel<128, 4, false>)};
+    reinterpret_cast<const void *>(&tiled_kernel<320, 3, false>), reinterpret_cast<const void *>(&tiled_kernel<256, 3, false>)};
 /* serving launches: the same variants, events from the host-published ring */
 const void *const kTiledServeKernels[kNumTiledKernels] = {
     reinterpret_cast<const void *>(&tiled_kernel<512, 2, true>), reinterpret_cast<const void *>(&tiled_kernel<384, 3, true>),
-    reinterpret_cast<const void *>(&tiled_kernel<256, 4, true>), reinterpret_cast<const void *>(&tiled_kernel<128, 4, true>)};
-const int kTiledMaxTiles[kNumTiledKernels] = {512, 384, 256, 128};
+    reinterpret_cast<const void *>(&tiled_kernel<320, 3, true>), reinterpret_cast<const void *>(&tiled_kernel<256, 3, true>)};
+const int kTiledMaxTiles[kNumTiledKernels] = {512, 384, 320, 256};
 
 }  // namespace fjmi

@@ -64,9 +64,9 @@ enum fjmi_engine {
  *     global memory (p4_global = 1);
  *   FJMI_ENGINE_TILED: one 64-lane wave per event (threads = 64); `slots` is the
  *     tile side in quarters of R, 4..16 (0 means 4: side R); `p4_global` selects
- *     the kernel variant: 0 = 512 tiles (8 events per CU), 1 = 384 tiles
- *     (10 per CU), 2 = 256 tiles (16 per CU, a few register spills), 3 = 128
- *     tiles (16 per CU).  A smaller tile table takes less LDS, so more events
+ *     the kernel variant: 0 = 512 tiles (8 events per CU, the full register
+ *     budget), 1 = 384 tiles (10 per CU), 2 = 320 tiles (12 per CU), 3 = 256
+ *     tiles (12 per CU).  A smaller tile table takes less LDS, so more events
  *     run per CU; an event whose tiles would not fit gets taller tiles in
  *     rapidity.
  * Replaces the reference's occupancy-derived launch shapes (launch.h:11-60). */

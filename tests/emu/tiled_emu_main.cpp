@@ -96,8 +96,8 @@ static int run_file(const char *path, int variant, int p, double R) {
   switch (variant) {
     case 0: return run_event<512, 2>(p4.data(), n, p, R, 0, true);
     case 1: return run_event<384, 3>(p4.data(), n, p, R, 0, true);
-    case 2: return run_event<256, 4>(p4.data(), n, p, R, 0, true);
-    default: return run_event<128, 4>(p4.data(), n, p, R, 0, true);
+    case 2: return run_event<320, 3>(p4.data(), n, p, R, 0, true);
+    default: return run_event<256, 3>(p4.data(), n, p, R, 0, true);
   }
 }
 
@@ -178,7 +178,7 @@ static int run_qbound() {
     a.njets = &nj;
     a.status = &status;
     a.stage_chunk = kStageChunk;
-    emu::run_wave([&] { tiled_kernel<128, 4, false>(a, ts); });
+    emu::run_wave([&] { tiled_kernel<256, 3, false>(a, ts); });
     const bool ok = nj == -1 && h1[0] == -7 && h1[1] == -7 && h2[0] == -7 && status == 0;
     std::printf("qbound head=%u base=%u: %s\n", c[0], c[1], ok ? "ok" : "TOUCHED");
     fails += !ok;
@@ -218,8 +218,8 @@ int main(int argc, char **argv) {
     switch (variant) {
       case 0: b = run_event<512, 2>(p4.data(), n, p, R, tq, false); break;
       case 1: b = run_event<384, 3>(p4.data(), n, p, R, tq, false); break;
-      case 2: b = run_event<256, 4>(p4.data(), n, p, R, tq, false); break;
-      default: b = run_event<128, 4>(p4.data(), n, p, R, tq, false); break;
+      case 2: b = run_event<320, 3>(p4.data(), n, p, R, tq, false); break;
+      default: b = run_event<256, 3>(p4.data(), n, p, R, tq, false); break;
     }
     fails += b;
     std::fflush(stdout);
