@@ -58,6 +58,10 @@ HBM_PEAK_GBPS = 8000.0
 N_CU, N_SIMD, SCLK_GHZ = 256, 1024, 2.4
 LDS_PEAK_GBPS = N_CU * 128 * SCLK_GHZ  # 128 bytes per clock per CU
 COUNTER_PROFILE = os.path.join(ROOT, "profiles", "tiled_kernel_counters_workload3.md")
+# the tiled kernel of each variant (the p4_global field of a tiled bucket row)
+TILED_VARIANTS = {0: "tiled_kernel<512, 2, false>", 1: "tiled_kernel<384, 3, false>", 2: "tiled_kernel<320, 3, false>",
+                  3: "tiled_kernel<256, 3, false>"}
+PROFILED_KERNEL = TILED_VARIANTS[3]  # the default table's kernel for the 4096-event batch
 KERNEL_SOURCES = ["fastjet_gpu_amd/csrc/tiled.hip", "fastjet_gpu_amd/csrc/fjmi_math.h",
                   "fastjet_gpu_amd/csrc/fjmi_math_tables.h", "fastjet_gpu_amd/csrc/fjmi_internal.h",
                   "fastjet_gpu_amd/csrc/tune_table.inc", "fastjet_gpu_amd/csrc/runtime.hip"]
@@ -116,7 +120,7 @@ def profile_commands(out="gpurun_out/prof"):
     return cmds
 
 
-def summarize(dirs, kernel_key="tiled_kernel<384, 3, false>"):
+def summarize(dirs, kernel_key=PROFILED_KERNEL):
     """Per-launch counter values of the tiled kernel (and the calibration copy
     kernel's FETCH_SIZE / WRITE_SIZE) from rocprofv3 counter_collection.csv files,
     and its average duration from kernel_stats.csv, as machine-readable lines."""
@@ -189,13 +193,15 @@ def summarize(dirs, kernel_key="tiled_kernel<384, 3, false>"):
 
 
 def read_profile(path=COUNTER_PROFILE):
-    """Counters per launch from the committed summary: (dict, sha, kernel_avg_ms)."""
-    vals, sha, avg = {}, None, None
+    """Counters per launch from the committed summary: (dict, sha, kernel_avg_ms, kernel name)."""
+    vals, sha, avg, kern = {}, None, None, None
     try:
         for line in open(path):
             line = line.strip().strip("`")
             if line.startswith("source_sha:"):
                 sha = line.split()[1]
+            elif line.startswith("kernel:"):
+                kern = line.split(":", 1)[1].strip()
             elif line.startswith("kernel_avg_ms:"):
                 avg = float(line.split()[1])
             elif line.startswith("counter "):
@@ -205,19 +211,20 @@ def read_profile(path=COUNTER_PROFILE):
                 vals["traffic_bytes_per_launch"] = float(line.split()[1])
     except OSError:
         pass
-    return vals, sha, avg
+    return vals, sha, avg, kern
 
 
-def counter_fields(t_s, n_steps_total):
-    """Fractions of MI355X peaks from the committed counters at kernel time t_s."""
-    vals, sha, avg = read_profile()
+def counter_fields(t_s, n_steps_total, kname):
+    """Fractions of MI355X peaks from the committed counters at kernel time t_s, if they were taken on the same
+    kernel (`kname`) built from the same sources."""
+    vals, sha, avg, kern = read_profile()
     cur = source_sha()
     if not vals:
         return {"source": None, "stale": None}
-    stale = sha != cur
+    stale = sha != cur or kern != kname
     f = {"source": os.path.relpath(COUNTER_PROFILE, ROOT), "profiled_source_sha": sha, "current_source_sha": cur,
-         "stale": stale, "profiled_kernel_ms": avg}
-    if stale:  # counters of another build of the kernel: not combined with this run's time
+         "profiled_kernel": kern, "stale": stale, "profiled_kernel_ms": avg}
+    if stale:  # counters of another kernel or build: not combined with this run's time
         return f
     fl = vals.get("SQ_INSTS_VALU_FLOPS_FP64")  # per wave instruction (FMA = 2), not per lane
     if fl is not None:
@@ -547,18 +554,17 @@ def main():
     # ---- roofline of the dominant kernel ----
     dom = max(bstats, key=lambda b: b["ms"])
     t_kernel = float(np.mean(step_ms)) * 1e-3  # HIP events on the launch stream around each graph launch
-    variants = {0: "tiled_kernel<512, 2, false>", 1: "tiled_kernel<384, 3, false>", 2: "tiled_kernel<256, 4, false>",
-                3: "tiled_kernel<128, 4, false>"}
     kname = (f"dense_kernel<{dom['threads']}, {dom['slots']}, {str(bool(dom['p4_global'])).lower()}, 1>"
-             if dom["engine"] == 1 else variants.get(dom["p4_global"], "tiled_kernel"))
+             if dom["engine"] == 1 else TILED_VARIANTS.get(dom["p4_global"], "tiled_kernel"))
     dense_f = dense_equivalent_flops(float(np.sum(sizes.astype(np.float64) ** 2)), float(tot))
     alg_f = 8.0 * pairs if pairs is not None else None
-    ctr = (counter_fields(t_kernel, float(tot)) if (len(bstats) == 1 and ne == 4096 and not a.events_file)
+    ctr = (counter_fields(t_kernel, float(tot), kname) if (len(bstats) == 1 and ne == 4096 and not a.events_file)
            else {"source": None})
     achieved = alg_f / t_kernel / 1e12 if alg_f else None
     roof = {"bound": "latency",
             "limiter": "the per-step dependent chain of one wave per event (argmin, merge, gather, neighbour "
-                       "repair); the batch ends with its largest events",
+                       "repair), times the events in flight (workgroups per CU); one batch alone ends with its "
+                       "largest events, which a stream of batches overlaps with the next batch",
             "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS if achieved else None,
             "traffic": None,
@@ -572,7 +578,7 @@ def main():
                                  "frac": dense_f / t_kernel / 1e12 / FP64_PEAK_TFLOPS,
                                  "per_unit": "8 N(N-1) + N(N+1)/2 per event (dense nearest-neighbour method)"},
             "counters": ctr}
-    vals, psha, _ = read_profile()
+    vals, psha, _, _ = read_profile()
     if vals.get("traffic_bytes_per_launch") and not ctr.get("stale", True):
         roof["traffic"] = vals["traffic_bytes_per_launch"]
 
