@@ -250,6 +250,17 @@ bool latency_prefers_dense(int n, double R) {
   return (double)n <= nc;
 }
 
+/* The tiled row for one event alone (tile side R): the 384-tile variant, the
+ * shortest step at 3 waves' registers, except for small R and large N, where
+ * the 512-tile variant's finer grid and full register budget win (measured at
+ * R = 0.2: 9.5 against 9.8 ms at N = 2000, 37.5 against 41.6 ms at N = 5194; at
+ * R = 0.4 the two are within 1 % or the 384-tile variant is ahead).  The
+ * default table's variants hold more events per CU, which a batch needs and a
+ * single event does not. */
+fjmi_bucket latency_tiled_row(int n, double R) {
+  return fjmi_bucket{0x7fffffff, FJMI_ENGINE_TILED, 64, 0, (R < 0.3 && n > 2000) ? 0 : 1};
+}
+
 LaunchArgs base_args() {
   LaunchArgs a;
   std::memset(&a, 0, sizeof a);
@@ -303,13 +314,20 @@ int zero_words(void *p, int n, hipStream_t stream) {
 int cluster_one_device(fjmi_context *ctx, fjmi::rt::StreamState &st, int engine, const double *d_in, int in_stride,
                        int in_off, int n, int p, double R, int32_t *h1, int32_t *h2, double *hd, double *jets,
                        int32_t *nj, unsigned char *pj, hipStream_t stream) {
-  /* FJMI_ENGINE_AUTO under the default table: the latency rule, the dense shape
-   * with the fewest slots per thread; engine forced to tiled: the table's tiled
-   * row for n, else tile side R with the 512-tile variant */
+  /* under the default table: FJMI_ENGINE_AUTO follows the latency rule (the
+   * dense shape with the fewest slots per thread, or the latency tiled row), and
+   * the tiled engine takes the latency tiled row; under a caller's table, its
+   * row for n, else (engine forced) tile side R with the 512-tile variant or the
+   * dense shape with the fewest slots per thread */
   fjmi_bucket forced;
   const fjmi_bucket *b = nullptr;
-  if (engine == FJMI_ENGINE_AUTO && !ctx->caller_table && latency_prefers_dense(n, R) && dense_row_for(n, &forced))
+  if (engine == FJMI_ENGINE_AUTO && !ctx->caller_table) {
+    if (!(latency_prefers_dense(n, R) && dense_row_for(n, &forced))) forced = latency_tiled_row(n, R);
     b = &forced;
+  } else if (engine == FJMI_ENGINE_TILED && !ctx->caller_table) {
+    forced = latency_tiled_row(n, R);
+    b = &forced;
+  }
   if (!b) b = bucket_for(ctx, n, engine);
   if (!b && engine == FJMI_ENGINE_TILED) b = &kTiledRow;
   if (!b && engine == FJMI_ENGINE_DENSE && dense_row_for(n, &forced)) b = &forced;
@@ -835,8 +853,10 @@ int fjmi_cluster_event_device_noalloc(fjmi_context *ctx, const double *d_p4, int
   if ((rc = set_device(ctx))) return rc;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   if (n == 0) return zero_words(d_nj, 1, s);
-  /* the table's tiled row for n, else tile side R with the 512-tile variant */
-  const fjmi_bucket *b = bucket_for(ctx, n, FJMI_ENGINE_TILED);
+  /* under the default table the latency row for one event alone; else the
+   * table's tiled row for n, else tile side R with the 512-tile variant */
+  const fjmi_bucket latency_row = latency_tiled_row(n, R);
+  const fjmi_bucket *b = ctx->caller_table ? bucket_for(ctx, n, FJMI_ENGINE_TILED) : &latency_row;
   if (!b) b = &kTiledRow;
   /* every byte of the tiled scratch that the kernel reads it has written for this
    * event first, so nothing is cleared; the status word is written on failure only

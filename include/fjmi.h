@@ -49,11 +49,13 @@ enum fjmi_engine {
   FJMI_ENGINE_AUTO = 0,  /* batches: the engine of the event's multiplicity bucket.  One-event calls:
                           * the engine that finishes one event alone soonest (dense for R >= 0.5 up to
                           * its 8192-particle capacity; below R = 0.5 dense up to N = 500 at R <= 0.2,
-                          * rising linearly to 1200 at R = 0.3 and above; tiled otherwise), until
-                          * fjmi_set_buckets installs a table, which then decides for them too */
+                          * rising linearly to 1200 at R = 0.3 and above; tiled otherwise, with 384
+                          * tiles, or 512 for R < 0.3 and N > 2000), until fjmi_set_buckets installs
+                          * a table, which then decides for them too */
   FJMI_ENGINE_DENSE = 1, /* replaces tri_matrix.cu / n_array.cu */
-  FJMI_ENGINE_TILED = 2, /* replaces grid.cu; a single event takes the table's tiled row for its
-                          * size, or tile side R and variant 0 when the table has none */
+  FJMI_ENGINE_TILED = 2, /* replaces grid.cu; a single event takes tile side R with 384 tiles (512 for
+                          * R < 0.3 and N > 2000) under the default table, else the caller's table's
+                          * tiled row for its size, or tile side R and variant 0 when it has none */
   FJMI_ENGINE_EMPTY = 3  /* replaces tri_matrix_empty.cu: per-step launch floor, no result */
 };
 

@@ -103,7 +103,7 @@ def test_high_pileup_batch(engine, p, alg):
 @pytest.mark.gpu
 @pytest.mark.parametrize("var", [2, 3])
 def test_high_multiplicity_small_tile_table(engine, var):
-    """Workload-4 events through a batch on the 256- and 128-tile kernels (tiles
+    """Workload-4 events through a batch on the 320- and 256-tile kernels (tiles
     taller in rapidity than R), kt and anti-kt R = 0.4."""
     import fastjet_gpu_amd as fj
     evs = [fj.gen_event(4, e) for e in (0, 5)]
@@ -187,7 +187,7 @@ def test_empty_event(engine, eng):
 def test_batch_all_tiled(engine, tq, var):
     """A batch with every event on the tiled engine (bucket table with one tiled row):
     tile side R (tq 0 or 4), 1.25 R, 1.5 R and 4 R, with each kernel variant (tile
-    tables of 512, 256 and 128 tiles): the history depends on neither."""
+    tables of 512, 384, 320 and 256 tiles): the history depends on neither."""
     import fastjet_gpu_amd as fj
     saved = engine.buckets()
     try:
@@ -253,12 +253,12 @@ def test_batch_matches_oracle(engine):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("var", [1, 0])
+@pytest.mark.parametrize("var", [1, 0, 3])
 def test_stretched_tiles(engine, var):
     """Events whose tile grid exceeds the kernel's tile table, so that tiles get
     taller than R in rapidity: R = 0.05 and 0.1 (125 and 62 columns in phi) and
-    rapidities up to +-9 at R = 0.4 (45 rows at side R), with the 384-tile and
-    the 512-tile variant."""
+    rapidities up to +-9 at R = 0.4 (45 rows at side R), with the 384-, 512- and
+    256-tile variants (the last leaves 2 rows at R = 0.05)."""
     import fastjet_gpu_amd as fj
     saved = engine.buckets()
     try:
