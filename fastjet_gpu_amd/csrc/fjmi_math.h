@@ -141,7 +141,7 @@ FJMI_HD fjmi_dd fjmi_dd_make(double hi, double lo) {
 /* log x for positive normal finite x = 2^e0 * (the argument): the table and
  * polynomial path, written without branches so that a caller's independent
  * work interleaves with it. */
-FJMI_HD double fjmi_log_core(double x, int e0) {
+FJMI_HD fjmi_dd fjmi_log_core_dd(double x, int e0) {
   const uint64_t u = fjmi_d2u(x);
   int e = e0 + (int)(u >> 52) - 1023;
   double m = fjmi_u2d((u & 0x000FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL);
@@ -171,8 +171,11 @@ FJMI_HD double fjmi_log_core(double x, int e0) {
   const fjmi_dd acc1 = fjmi_dd_add(fjmi_dd_make((double)e * FJMI_LN2_HI, (double)e * FJMI_LN2_LO), acc0);
   /* selects per component: a select of whole structs may be lowered through memory */
   const fjmi_dd acc = fjmi_dd_make(e != 0 ? acc1.hi : acc0.hi, e != 0 ? acc1.lo : acc0.lo);
-  return fjmi_dd_add(acc, L).hi;
+  return fjmi_dd_add(acc, L);
 }
+
+/* the rounded logarithm: the high part of the double-double */
+FJMI_HD double fjmi_log_core(double x, int e0) { return fjmi_log_core_dd(x, e0).hi; }
 
 FJMI_HD double fjmi_log(double x) {
   if (!(x > 0.0)) {
@@ -254,6 +257,133 @@ FJMI_HD double fjmi_atan2_core(double y, double x) {
   return yneg ? -hi : hi;
 }
 
+/* ---- exp, pow, sin/cos, sqrt: the functions of general jet definitions ----
+ * (the generalised-kt weight kt^(2p) for any real p, and the recombination
+ * schemes whose merged jet is built from (pt, y, phi)).  Same rules as above:
+ * basic operations and fma only, so host and device round alike. */
+
+/* 2^k as a double for -1022 <= k <= 1023 */
+FJMI_HD double fjmi_pow2i(int k) { return fjmi_u2d((uint64_t)(k + 1023) << 52); }
+
+/* x * 2^k for a double-double x in [0.5, 4): the high part, scaled in two steps
+ * below the normal range (rounded once more there: such results are subnormal) */
+FJMI_HD double fjmi_scale_dd(fjmi_dd x, int k) {
+  if (k > 1023) return x.hi * fjmi_pow2i(1023) * fjmi_pow2i(k - 1023);
+  if (k >= -1021) return x.hi * fjmi_pow2i(k);
+  return (x.hi * fjmi_pow2i(k + 64)) * fjmi_pow2i(-64);
+}
+
+/* exp(x) for a double-double x with |x.hi| < 746, as 2^k * (a double-double
+ * in [1, 2) up to rounding):  n = round(64 x / ln 2), j = n mod 64, k = (n - j) / 64,
+ * r = x - n ln(2)/64 in double-double (|r| <= ln(2)/128), exp(x) = 2^k 2^(j/64)
+ * exp(r) with exp(r) - 1 = r + r^2/2 + r^3 P(r) (P to r^4/5040, the next term
+ * below 2^-75 relative). */
+FJMI_HD fjmi_dd fjmi_exp_core_dd(fjmi_dd x, int *k_out) {
+  const double nf = __builtin_rint(x.hi * FJMI_INV_LN2_64);
+  const int n = (int)nf;
+  const int j = n & 63;
+  *k_out = (n - j) / 64;
+  fjmi_dd r = fjmi_dd_add_d(x, -nf * FJMI_LN2_64_HI); /* the product is exact */
+  r = fjmi_dd_add_d(r, -nf * FJMI_LN2_64_LO);
+  const double rh = r.hi;
+  double P = __builtin_fma(rh, 1.0 / 5040.0, 1.0 / 720.0);
+  P = __builtin_fma(rh, P, 1.0 / 120.0);
+  P = __builtin_fma(rh, P, 1.0 / 24.0);
+  P = __builtin_fma(rh, P, 1.0 / 6.0);
+  fjmi_dd r2 = fjmi_two_prod(rh, rh);
+  r2.lo += 2.0 * rh * r.lo;
+  fjmi_dd E = fjmi_dd_add(r, fjmi_dd_make(0.5 * r2.hi, 0.5 * r2.lo));
+  E = fjmi_dd_add_d(E, (r2.hi * rh) * P);
+  const fjmi_dd T = fjmi_dd_make(fjmi_exp2_tab[j][0], fjmi_exp2_tab[j][1]);
+  return fjmi_dd_add(T, fjmi_dd_mul(T, E));
+}
+
+/* e^x for a double-double argument: +inf above the overflow threshold, 0 below
+ * the smallest subnormal's */
+FJMI_HD double fjmi_exp_dd(fjmi_dd x) {
+  if (x.hi != x.hi) return x.hi;
+  if (x.hi > 709.782712893384) return __builtin_inf();
+  if (x.hi < -745.1332191019412) return 0.0;
+  int k;
+  const fjmi_dd v = fjmi_exp_core_dd(x, &k);
+  return fjmi_scale_dd(v, k);
+}
+
+FJMI_HD double fjmi_exp(double x) { return fjmi_exp_dd(fjmi_dd_make(x, 0.0)); }
+
+/* x^p for finite x > 0 (normal or subnormal) and finite p: exp(p log x) with the
+ * logarithm and the product in double-double */
+FJMI_HD double fjmi_pow_pos(double x, double p) {
+  const fjmi_dd L = fjmi_d2u(x) < 0x0010000000000000ULL ? fjmi_log_core_dd(x * 0x1p54, -54) : fjmi_log_core_dd(x, 0);
+  return fjmi_exp_dd(fjmi_dd_mul_d(L, p));
+}
+
+/* sqrt(x), correctly rounded: x = m 4^e with m in [1, 4), Newton's iteration
+ * y = (y + m/y)/2 from (1 + m)/2 (five steps reach 2^-60), then the neighbour of
+ * y whose square is nearest m (the residual m - c^2 is exact in one fma). */
+FJMI_HD double fjmi_sqrt(double x) {
+  if (x != x || x < 0.0) return __builtin_nan("");
+  if (x == 0.0 || x == __builtin_inf()) return x;
+  int eb = 0;
+  double xs = x;
+  if (fjmi_d2u(xs) < 0x0010000000000000ULL) {
+    xs *= 0x1p108;
+    eb = -108;
+  }
+  const int ex = (int)(fjmi_d2u(xs) >> 52) - 1023 + eb;
+  const int h = (ex >= 0 ? ex : ex - 1) / 2; /* floor(ex / 2) */
+  const double m = fjmi_u2d((fjmi_d2u(xs) & 0x000FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL) *
+                   ((ex - 2 * h) ? 2.0 : 1.0); /* [1, 4) */
+  double y = 0.5 * (1.0 + m);
+  for (int i = 0; i < 5; ++i) y = 0.5 * (y + m / y);
+  const double r0 = __builtin_fma(-y, y, m);
+  const double up = fjmi_u2d(fjmi_d2u(y) + 1), dn = fjmi_u2d(fjmi_d2u(y) - 1);
+  const double c = r0 > 0.0 ? up : dn;
+  const double r1 = __builtin_fma(-c, c, m);
+  y = (r0 != 0.0 && __builtin_fabs(r1) < __builtin_fabs(r0)) ? c : y;
+  return y * fjmi_pow2i(h);
+}
+
+/* sin and cos of |x| < 16 pi: r = x - n pi/2 in double-double (|r| <= pi/4);
+ * sin r = r + r^3 (-1/6 + z (1/120 + z S(z))) and cos r = 1 - z/2 + z^2 (1/24 +
+ * z (-1/720 + z C(z))) with z = r^2, the first two coefficients of each series
+ * in double-double and the Taylor tails S, C (to r^17 and r^18, the next terms
+ * below 2^-62 relative) in double; then the quadrant of n. */
+FJMI_HD void fjmi_sincos(double x, double *s_out, double *c_out) {
+  const double nf = __builtin_rint(x * 0x1.45f306dc9c883p-1); /* 2/pi */
+  const int n = (int)nf;
+  const fjmi_dd r = fjmi_dd_add(fjmi_dd_make(x, 0.0), fjmi_dd_mul_d(fjmi_dd_make(FJMI_PIO2_HI, FJMI_PIO2_LO), -nf));
+  const double rh = r.hi;
+  fjmi_dd z = fjmi_two_prod(rh, rh);
+  z.lo += 2.0 * rh * r.lo;
+  z = fjmi_fast_two_sum(z.hi, z.lo);
+  const double zh = z.hi;
+  double S = __builtin_fma(zh, 1.0 / 355687428096000.0, -1.0 / 1307674368000.0);
+  S = __builtin_fma(zh, S, 1.0 / 6227020800.0);
+  S = __builtin_fma(zh, S, -1.0 / 39916800.0);
+  S = __builtin_fma(zh, S, 1.0 / 362880.0);
+  S = __builtin_fma(zh, S, -1.0 / 5040.0);
+  fjmi_dd qs = fjmi_dd_add(fjmi_dd_make(0x1.1111111111111p-7, 0x1.1111111111111p-63), fjmi_dd_mul_d(z, S));
+  qs = fjmi_dd_add(fjmi_dd_make(-0x1.5555555555555p-3, -0x1.5555555555555p-57), fjmi_dd_mul(z, qs));
+  const fjmi_dd sn = fjmi_dd_add(r, fjmi_dd_mul(fjmi_dd_mul(z, r), qs));
+  double C = __builtin_fma(zh, -1.0 / 6402373705728000.0, 1.0 / 20922789888000.0);
+  C = __builtin_fma(zh, C, -1.0 / 87178291200.0);
+  C = __builtin_fma(zh, C, 1.0 / 479001600.0);
+  C = __builtin_fma(zh, C, -1.0 / 3628800.0);
+  C = __builtin_fma(zh, C, 1.0 / 40320.0);
+  fjmi_dd qc = fjmi_dd_add(fjmi_dd_make(-0x1.6c16c16c16c17p-10, 0x1.f49f49f49f49fp-65), fjmi_dd_mul_d(z, C));
+  qc = fjmi_dd_add(fjmi_dd_make(0x1.5555555555555p-5, 0x1.5555555555555p-59), fjmi_dd_mul(z, qc));
+  fjmi_dd cs = fjmi_dd_add(fjmi_dd_make(1.0, 0.0), fjmi_dd_make(-0.5 * z.hi, -0.5 * z.lo));
+  cs = fjmi_dd_add(cs, fjmi_dd_mul(fjmi_dd_mul(z, z), qc));
+  const double sv = sn.hi, cv = cs.hi;
+  switch (n & 3) {
+    case 0: *s_out = sv; *c_out = cv; break;
+    case 1: *s_out = cv; *c_out = -sv; break;
+    case 2: *s_out = -sv; *c_out = -cv; break;
+    default: *s_out = -cv; *c_out = sv; break;
+  }
+}
+
 /* ---- FastJet PseudoJet kinematics (tri_matrix.cu:61-100, grid.cu:112-166) ---- */
 
 /* Azimuth in [0, 2pi); 0 for zero transverse momentum. */
@@ -284,6 +414,22 @@ FJMI_HD double fjmi_weight(double kt2, int p) {
   if (p > 0) return kt2;
   if (p == 0) return 1.0;
   return kt2 > 1e-300 ? 1.0 / kt2 : 1e300;
+}
+
+/* Generalised-kt weight kt^(2p) for any real p (FastJet's genkt algorithm, its
+ * ClusterSequence::jet_scale_for_algorithm: for p <= 0 a kt^2 below 1e-300 is
+ * taken as 1e-300).  p = 1, 0 and -1 are the kt, Cambridge/Aachen and anti-kt
+ * weights above, bit for bit; other p go through fjmi_pow_pos. */
+FJMI_HD double fjmi_weight_gen(double kt2, double p) {
+  if (p == 1.0) return kt2;
+  if (p == 0.0) return 1.0;
+  if (p == -1.0) return kt2 > 1e-300 ? 1.0 / kt2 : 1e300;
+  double k = kt2;
+  if (p <= 0.0 && k < 1e-300) k = 1e-300;
+  if (k != k) return k;
+  if (k == 0.0) return 0.0; /* p > 0 */
+  if (k == __builtin_inf()) return p > 0.0 ? k : 0.0;
+  return fjmi_pow_pos(k, p);
 }
 
 /* Squared (rapidity, phi) distance with the phi wrap at pi (tri_matrix.cu:102-109). */

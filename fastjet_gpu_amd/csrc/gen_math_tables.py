@@ -4,6 +4,8 @@ atan2 in fjmi_math.h, computed with 80-digit decimal arithmetic.
   log:   for j in [90, 182]:  INV[j] = RN(128/j), LOGINV[j] = -log(INV[j]) as a
          double-double (hi + lo), where INV[j] is the exact double value.
   atan:  for j in [0, 64]:    ATAN[j] = atan(j/64) as a double-double.
+  exp:   for j in [0, 63]:    2^(j/64) as a double-double; ln(2)/64 split so that
+         n * hi is exact for |n| < 2^17; pi/2 as a double-double.
 
 Run from the repository root:  python fastjet_gpu_amd/csrc/gen_math_tables.py
 """
@@ -37,6 +39,22 @@ def atan(x):
     return s * (2 ** n)
 
 
+def pi():
+    """pi to the context precision (Machin's formula)."""
+    def arctan_inv(k):
+        x = D(1) / k
+        s, t, n, x2 = D(0), x, 0, x * x
+        while True:
+            term = t / (2 * n + 1)
+            if abs(term) < D(10) ** -79:
+                break
+            s += term if n % 2 == 0 else -term
+            t *= x2
+            n += 1
+        return s
+    return 4 * (4 * arctan_inv(5) - arctan_inv(239))
+
+
 def main():
     ln2 = D(2).ln()
     import struct
@@ -57,6 +75,22 @@ def main():
     lines += ["};", "", "/* {hi, lo} of atan(j/64) for j = 0..64 */", "FJMI_TABLE const double fjmi_atan_tab[][2] = {"]
     for j in range(65):
         hi, lo = dd(atan(D(j) / 64))
+        lines.append(f"    {{{hi.hex()}, {lo.hex()}}}, /* j={j} */")
+    lines += ["};", ""]
+    l64 = ln2 / 64
+    bits = struct.unpack("<Q", struct.pack("<d", float(l64)))[0] & ~((1 << 17) - 1)
+    l64hi = struct.unpack("<d", struct.pack("<Q", bits))[0]
+    l64lo = float(l64 - D(l64hi))
+    pio2 = pi() / 2
+    ph, pl = dd(pio2)
+    lines += ["/* ln(2)/64 = FJMI_LN2_64_HI + FJMI_LN2_64_LO, the high part with 36 significant bits",
+              " * (n * FJMI_LN2_64_HI is exact for |n| < 2^17); 64/ln(2) rounded; pi/2 as a double-double */",
+              f"#define FJMI_LN2_64_HI {l64hi.hex()}", f"#define FJMI_LN2_64_LO {l64lo.hex()}",
+              f"#define FJMI_INV_LN2_64 {float(D(64) / ln2).hex()}",
+              f"#define FJMI_PIO2_HI {ph.hex()}", f"#define FJMI_PIO2_LO {pl.hex()}", "",
+              "/* {hi, lo} of 2^(j/64) for j = 0..63 */", "FJMI_TABLE const double fjmi_exp2_tab[][2] = {"]
+    for j in range(64):
+        hi, lo = dd((D(j) / 64 * ln2).exp())
         lines.append(f"    {{{hi.hex()}, {lo.hex()}}}, /* j={j} */")
     lines += ["};", "", "#endif", ""]
     with open(os.path.join(HERE, "fjmi_math_tables.h"), "w") as fh:

@@ -18,6 +18,20 @@
 
 double fjo_log(double x) { return fjmi_log(x); }
 double fjo_atan2(double y, double x) { return fjmi_atan2(y, x); }
+double fjo_exp(double x) { return fjmi_exp(x); }
+double fjo_pow(double x, double p) { return fjmi_pow_pos(x, p); }
+double fjo_sqrt(double x) { return fjmi_sqrt(x); }
+double fjo_sin(double x) {
+  double s, c;
+  fjmi_sincos(x, &s, &c);
+  return s;
+}
+double fjo_cos(double x) {
+  double s, c;
+  fjmi_sincos(x, &s, &c);
+  return c;
+}
+double fjo_weight_gen(double kt2, double p) { return fjmi_weight_gen(kt2, p); }
 
 /* PseudoJet rap/phi and algorithm weight (tri_matrix.cu:61-100, grid.cu:153-166). */
 static void set_jet(const double *q, int p, int math_mode, double *rap, double *phi, double *w) {

@@ -62,6 +62,13 @@ int fjo_cluster_tiled(const double *p4, int n, int p, double R, int math_mode, i
 double fjo_log(double x);
 double fjo_atan2(double y, double x);
 void fjo_set_jet(const double *p4, int p, int math_mode, double *rap, double *phi, double *w);
+/* The functions of general jet definitions (fjmi_math.h), exported for tests. */
+double fjo_exp(double x);
+double fjo_pow(double x, double p); /* x > 0 finite, p finite */
+double fjo_sqrt(double x);
+double fjo_sin(double x);
+double fjo_cos(double x);
+double fjo_weight_gen(double kt2, double p);
 
 /* Clusters n_events events (p4 packed, offsets[n_events+1] in particles) on
  * nthreads threads with n2plain (method 0) or tiled (method 1) in math mode

@@ -31,6 +31,11 @@ def lib():
         L.fjo_log.restype = ctypes.c_double
         L.fjo_atan2.argtypes = [ctypes.c_double, ctypes.c_double]
         L.fjo_atan2.restype = ctypes.c_double
+        for name, nargs in (("fjo_exp", 1), ("fjo_pow", 2), ("fjo_sqrt", 1), ("fjo_sin", 1), ("fjo_cos", 1),
+                            ("fjo_weight_gen", 2)):
+            f = getattr(L, name)
+            f.argtypes = [ctypes.c_double] * nargs
+            f.restype = ctypes.c_double
         L.fjo_set_jet.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
         L.fjo_set_jet.restype = None
         L.fjo_time_batch.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
