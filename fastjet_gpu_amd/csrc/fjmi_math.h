@@ -432,6 +432,109 @@ FJMI_HD double fjmi_weight_gen(double kt2, double p) {
   return fjmi_pow_pos(k, p);
 }
 
+/* ---- Recombination schemes (FastJet's JetDefinition::DefaultRecombiner, its
+ * RecombinationScheme numbering).  The reference has the E-scheme only
+ * (tri_matrix.cu:247-250, grid.cu:453-456). ---- */
+#define FJMI_SCHEME_E 0        /* four-momentum sum */
+#define FJMI_SCHEME_PT 1       /* massless (pt, y, phi), pt-weighted y and phi; inputs made massless, E = |p| */
+#define FJMI_SCHEME_PT2 2      /* as pt, pt^2-weighted */
+#define FJMI_SCHEME_ET 3       /* as pt; inputs made massless by rescaling |p| to E */
+#define FJMI_SCHEME_ET2 4      /* as Et, pt^2-weighted */
+#define FJMI_SCHEME_BIPT 5     /* as pt, inputs unchanged */
+#define FJMI_SCHEME_BIPT2 6    /* as pt2, inputs unchanged */
+#define FJMI_SCHEME_WTA_PT 7   /* winner takes all: the harder jet's y, phi and mass, pt = pta + ptb */
+#define FJMI_SCHEME_WTA_MODP 8 /* winner takes all: the harder jet's direction, |p| = |pa| + |pb| */
+#define FJMI_NUM_SCHEMES 9
+
+/* An input particle as the scheme takes it (DefaultRecombiner::preprocess). */
+FJMI_HD void fjmi_preprocess(int scheme, double *px, double *py, double *pz, double *E) {
+  if (scheme == FJMI_SCHEME_PT || scheme == FJMI_SCHEME_PT2) {
+    *E = fjmi_sqrt((*px * *px + *py * *py) + *pz * *pz);
+  } else if (scheme == FJMI_SCHEME_ET || scheme == FJMI_SCHEME_ET2) {
+    const double rescale = *E / fjmi_sqrt((*px * *px + *py * *py) + *pz * *pz);
+    *px *= rescale;
+    *py *= rescale;
+    *pz *= rescale;
+  }
+}
+
+/* The momentum of (pt, y, phi, m) (PseudoJet::reset_PtYPhiM) into q[4]. */
+FJMI_HD void fjmi_ptyphim(double pt, double y, double phi, double m, double *q) {
+  const double ptm = (m == 0.0) ? pt : fjmi_sqrt(pt * pt + m * m);
+  const double exprap = fjmi_exp(y);
+  const double pminus = ptm / exprap;
+  const double pplus = ptm * exprap;
+  double s, c;
+  fjmi_sincos(phi, &s, &c);
+  q[0] = pt * c;
+  q[1] = pt * s;
+  q[2] = 0.5 * (pplus - pminus);
+  q[3] = 0.5 * (pplus + pminus);
+}
+
+/* The merged jet of jets a and b (momenta qa, qb; rapidities and azimuths ya,
+ * phia, yb, phib) into qo[4] (DefaultRecombiner::recombine).  Returns 1 when
+ * the scheme fixes the merged jet's rapidity and azimuth (then in *y and *phi,
+ * the azimuth in [0, 2 pi), as PseudoJet::set_cached_rap_phi keeps them), 0
+ * when they follow from its momentum. */
+FJMI_HD int fjmi_recombine(int scheme, const double *qa, double ya, double phia, const double *qb, double yb,
+                           double phib, double *qo, double *y, double *phi) {
+  if (scheme == FJMI_SCHEME_E) {
+    for (int c = 0; c < 4; ++c) qo[c] = qa[c] + qb[c];
+    return 0;
+  }
+  const double kta = qa[0] * qa[0] + qa[1] * qa[1], ktb = qb[0] * qb[0] + qb[1] * qb[1];
+  if (scheme == FJMI_SCHEME_WTA_MODP) {
+    const double ma2 = kta + qa[2] * qa[2], mb2 = ktb + qb[2] * qb[2];
+    const int a_hard = ma2 >= mb2;
+    const double *qh = a_hard ? qa : qb;
+    const double mh2 = a_hard ? ma2 : mb2, ms2 = a_hard ? mb2 : ma2;
+    if (mh2 == 0.0) {
+      for (int c = 0; c < 4; ++c) qo[c] = 0.0;
+      return 0;
+    }
+    const double modp_hard = fjmi_sqrt(mh2);
+    const double modp_ab = modp_hard + fjmi_sqrt(ms2);
+    const double scale = modp_ab / modp_hard;
+    qo[0] = qh[0] * scale;
+    qo[1] = qh[1] * scale;
+    qo[2] = qh[2] * scale;
+    qo[3] = modp_ab;
+    return 0;
+  }
+  if (scheme == FJMI_SCHEME_WTA_PT) {
+    const int a_hard = kta >= ktb;
+    const double *qh = a_hard ? qa : qb;
+    const double kth = a_hard ? kta : ktb;
+    const double m2 = (qh[3] + qh[2]) * (qh[3] - qh[2]) - kth;
+    const double m = m2 < 0.0 ? -fjmi_sqrt(-m2) : fjmi_sqrt(m2);
+    *y = a_hard ? ya : yb;
+    *phi = a_hard ? phia : phib;
+    fjmi_ptyphim(fjmi_sqrt(kta) + fjmi_sqrt(ktb), *y, *phi, m, qo);
+    return 1;
+  }
+  const int by_pt = scheme == FJMI_SCHEME_PT || scheme == FJMI_SCHEME_ET || scheme == FJMI_SCHEME_BIPT;
+  const double pta = fjmi_sqrt(kta), ptb = fjmi_sqrt(ktb);
+  const double wa = by_pt ? pta : kta, wb = by_pt ? ptb : ktb;
+  const double perp_ab = pta + ptb;
+  if (perp_ab == 0.0) {
+    for (int c = 0; c < 4; ++c) qo[c] = 0.0;
+    return 0;
+  }
+  const double y_ab = (wa * ya + wb * yb) / (wa + wb);
+  double pa = phia, pb = phib;
+  if (pa - pb > FJMI_PI) pb += FJMI_TWOPI;
+  if (pa - pb < -FJMI_PI) pb -= FJMI_TWOPI;
+  const double phi_ab = (wa * pa + wb * pb) / (wa + wb);
+  fjmi_ptyphim(perp_ab, y_ab, phi_ab, 0.0, qo);
+  double ph = phi_ab;
+  if (ph >= FJMI_TWOPI) ph -= FJMI_TWOPI;
+  if (ph < 0.0) ph += FJMI_TWOPI;
+  *y = y_ab;
+  *phi = ph;
+  return 1;
+}
+
 /* Squared (rapidity, phi) distance with the phi wrap at pi (tri_matrix.cu:102-109). */
 FJMI_HD double fjmi_dr2(double rap1, double phi1, double rap2, double phi2) {
   double dphi = __builtin_fabs(phi1 - phi2);

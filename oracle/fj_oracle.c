@@ -33,8 +33,8 @@ double fjo_cos(double x) {
 }
 double fjo_weight_gen(double kt2, double p) { return fjmi_weight_gen(kt2, p); }
 
-/* PseudoJet rap/phi and algorithm weight (tri_matrix.cu:61-100, grid.cu:153-166). */
-static void set_jet(const double *q, int p, int math_mode, double *rap, double *phi, double *w) {
+/* PseudoJet rap/phi (tri_matrix.cu:61-100) with the shared or glibc log/atan2. */
+static void rap_phi(const double *q, int math_mode, double *rap, double *phi) {
   const double px = q[0], py = q[1], pz = q[2], E = q[3];
   const double kt2 = px * px + py * py;
   if (math_mode == FJO_MATH_LIBM) {
@@ -56,7 +56,24 @@ static void set_jet(const double *q, int p, int math_mode, double *rap, double *
     *phi = fjmi_phi(px, py, kt2);
     *rap = fjmi_rap(pz, E, kt2);
   }
-  *w = fjmi_weight(kt2, p);
+}
+
+/* PseudoJet rap/phi and algorithm weight (tri_matrix.cu:61-100, grid.cu:153-166). */
+static void set_jet(const double *q, int p, int math_mode, double *rap, double *phi, double *w) {
+  rap_phi(q, math_mode, rap, phi);
+  *w = fjmi_weight(q[0] * q[0] + q[1] * q[1], p);
+}
+
+/* A jet definition: the generalised-kt exponent (1 kt, 0 C/A, -1 anti-kt, any
+ * other real genkt; FastJet's JetDefinition with genkt_algorithm) and the
+ * recombination scheme (FJMI_SCHEME_*, FastJet's DefaultRecombiner). */
+typedef struct {
+  double p;
+  int scheme;
+} jdef_t;
+
+static int jdef_ok(const jdef_t *d) {
+  return d->p == d->p && d->p > -1e300 && d->p < 1e300 && d->scheme >= 0 && d->scheme < FJMI_NUM_SCHEMES;
 }
 
 void fjo_set_jet(const double *p4, int p, int math_mode, double *rap, double *phi, double *w) {
@@ -69,6 +86,7 @@ void fjo_set_jet(const double *p4, int p, int math_mode, double *rap, double *ph
  * ------------------------------------------------------------------------- */
 typedef struct {
   double *p4;      /* 4 * (2N) */
+  double *rap, *phi; /* per stored jet: from its momentum, or as the scheme fixes them */
   int32_t *hist;   /* history index of each jet */
   int njets_store; /* number of jets stored */
   int N;
@@ -77,14 +95,31 @@ typedef struct {
   double *hdij;
   double *out_jets;
   int32_t nout;
+  jdef_t def;
+  int mm;
 } book_t;
 
-static int book_init(book_t *b, const double *p4, int n, int32_t *hp1, int32_t *hp2, double *hdij, double *jets) {
+/* the inputs, as the recombination scheme takes them, with their rap/phi */
+static int book_init(book_t *b, const double *p4, int n, const jdef_t *def, int mm, int32_t *hp1, int32_t *hp2,
+                     double *hdij, double *jets) {
   b->p4 = (double *)malloc(sizeof(double) * 4 * (size_t)(2 * n + 1));
+  b->rap = (double *)malloc(sizeof(double) * (size_t)(2 * n + 1));
+  b->phi = (double *)malloc(sizeof(double) * (size_t)(2 * n + 1));
   b->hist = (int32_t *)malloc(sizeof(int32_t) * (size_t)(2 * n + 1));
-  if (!b->p4 || !b->hist) return -1;
+  if (!b->p4 || !b->hist || !b->rap || !b->phi) {
+    free(b->p4);
+    free(b->rap);
+    free(b->phi);
+    free(b->hist);
+    return -1;
+  }
   memcpy(b->p4, p4, sizeof(double) * 4 * (size_t)n);
-  for (int i = 0; i < n; ++i) b->hist[i] = i;
+  for (int i = 0; i < n; ++i) {
+    double *q = b->p4 + 4 * (size_t)i;
+    fjmi_preprocess(def->scheme, &q[0], &q[1], &q[2], &q[3]);
+    rap_phi(q, mm, &b->rap[i], &b->phi[i]);
+    b->hist[i] = i;
+  }
   b->njets_store = n;
   b->N = n;
   b->step = 0;
@@ -93,18 +128,36 @@ static int book_init(book_t *b, const double *p4, int n, int32_t *hp1, int32_t *
   b->hdij = hdij;
   b->out_jets = jets;
   b->nout = 0;
+  b->def = *def;
+  b->mm = mm;
   return 0;
 }
 
 static void book_free(book_t *b) {
   free(b->p4);
+  free(b->rap);
+  free(b->phi);
   free(b->hist);
 }
 
-/* E-scheme recombination (tri_matrix.cu:247-250): returns the new jet index. */
+/* the algorithm weight of stored jet `jet` */
+static double book_w(const book_t *b, int jet) {
+  const double *q = b->p4 + 4 * (size_t)jet;
+  return fjmi_weight_gen(q[0] * q[0] + q[1] * q[1], b->def.p);
+}
+
+/* Recombination (E-scheme: tri_matrix.cu:247-250; other schemes: FastJet's
+ * DefaultRecombiner, fjmi_recombine): returns the new jet index. */
 static int book_merge(book_t *b, int ja, int jb, double dij) {
   int k = b->njets_store++;
-  for (int c = 0; c < 4; ++c) b->p4[4 * k + c] = b->p4[4 * ja + c] + b->p4[4 * jb + c];
+  double y, ph;
+  if (fjmi_recombine(b->def.scheme, b->p4 + 4 * (size_t)ja, b->rap[ja], b->phi[ja], b->p4 + 4 * (size_t)jb,
+                     b->rap[jb], b->phi[jb], b->p4 + 4 * (size_t)k, &y, &ph)) {
+    b->rap[k] = y;
+    b->phi[k] = ph;
+  } else {
+    rap_phi(b->p4 + 4 * (size_t)k, b->mm, &b->rap[k], &b->phi[k]);
+  }
   int ha = b->hist[ja], hb = b->hist[jb];
   b->hist[k] = b->N + b->step;
   b->hp1[b->step] = ha < hb ? ha : hb;
@@ -141,8 +194,10 @@ static inline double bj_diJ(const bj_t *B, int i) {
   return B[i].nn_dist * kt2;
 }
 
-static void bj_set(bj_t *x, const book_t *b, int jet, int p, int mm, double R2) {
-  set_jet(b->p4 + 4 * (size_t)jet, p, mm, &x->eta, &x->phi, &x->kt2);
+static void bj_set(bj_t *x, const book_t *b, int jet, double R2) {
+  x->eta = b->rap[jet];
+  x->phi = b->phi[jet];
+  x->kt2 = book_w(b, jet);
   x->jet = jet;
   x->nn_dist = R2;
   x->nn = -1;
@@ -164,14 +219,15 @@ static void bj_set_nn_nocross(bj_t *B, int i, int n, double R2) {
   B[i].nn_dist = best;
 }
 
-int fjo_cluster_n2plain(const double *p4, int n, int p, double R, int mm, int32_t *hp1, int32_t *hp2, double *hdij,
+static int cluster_n2plain(const double *p4, int n, const jdef_t *def, double R, int mm, int32_t *hp1, int32_t *hp2,
+                           double *hdij,
                         double *jets, int32_t *njets) {
-  if (n < 0 || !(R > 0.0) || p < -1 || p > 1) return -1;
+  if (n < 0 || !(R > 0.0) || !jdef_ok(def)) return -1;
   *njets = 0;
   if (n == 0) return 0;
   const double R2 = R * R, invR2 = 1.0 / R2;
   book_t bk;
-  if (book_init(&bk, p4, n, hp1, hp2, hdij, jets)) return -2;
+  if (book_init(&bk, p4, n, def, mm, hp1, hp2, hdij, jets)) return -2;
   bj_t *B = (bj_t *)malloc(sizeof(bj_t) * (size_t)n);
   double *diJ = (double *)malloc(sizeof(double) * (size_t)n);
   if (!B || !diJ) {
@@ -180,7 +236,7 @@ int fjo_cluster_n2plain(const double *p4, int n, int p, double R, int mm, int32_
     book_free(&bk);
     return -2;
   }
-  for (int i = 0; i < n; ++i) bj_set(&B[i], &bk, i, p, mm, R2);
+  for (int i = 0; i < n; ++i) bj_set(&B[i], &bk, i, R2);
   /* initial NN by cross-checking each slot against the lower slots */
   for (int a = 1; a < n; ++a) {
     double best = R2;
@@ -219,7 +275,7 @@ int fjo_cluster_n2plain(const double *p4, int n, int p, double R, int mm, int32_
         jb = t;
       }
       int k = book_merge(&bk, B[ja].jet, B[jb].jet, dij);
-      bj_set(&B[jb], &bk, k, p, mm, R2);
+      bj_set(&B[jb], &bk, k, R2);
     } else {
       book_beam(&bk, B[ja].jet, dij);
     }
@@ -258,20 +314,21 @@ int fjo_cluster_n2plain(const double *p4, int n, int p, double R, int mm, int32_
 /* ---------------------------------------------------------------------------
  * Brute force: same slot semantics, NN recomputed from scratch every step.
  * ------------------------------------------------------------------------- */
-int fjo_cluster_brute(const double *p4, int n, int p, double R, int mm, int32_t *hp1, int32_t *hp2, double *hdij,
+static int cluster_brute(const double *p4, int n, const jdef_t *def, double R, int mm, int32_t *hp1, int32_t *hp2,
+                           double *hdij,
                       double *jets, int32_t *njets) {
-  if (n < 0 || !(R > 0.0) || p < -1 || p > 1) return -1;
+  if (n < 0 || !(R > 0.0) || !jdef_ok(def)) return -1;
   *njets = 0;
   if (n == 0) return 0;
   const double R2 = R * R, invR2 = 1.0 / R2;
   book_t bk;
-  if (book_init(&bk, p4, n, hp1, hp2, hdij, jets)) return -2;
+  if (book_init(&bk, p4, n, def, mm, hp1, hp2, hdij, jets)) return -2;
   bj_t *B = (bj_t *)malloc(sizeof(bj_t) * (size_t)n);
   if (!B) {
     book_free(&bk);
     return -2;
   }
-  for (int i = 0; i < n; ++i) bj_set(&B[i], &bk, i, p, mm, R2);
+  for (int i = 0; i < n; ++i) bj_set(&B[i], &bk, i, R2);
   while (n > 0) {
     for (int i = 0; i < n; ++i) bj_set_nn_nocross(B, i, n, R2);
     double dmin = bj_diJ(B, 0);
@@ -292,7 +349,7 @@ int fjo_cluster_brute(const double *p4, int n, int p, double R, int mm, int32_t 
         jb = t;
       }
       int k = book_merge(&bk, B[ja].jet, B[jb].jet, dij);
-      bj_set(&B[jb], &bk, k, p, mm, R2);
+      bj_set(&B[jb], &bk, k, R2);
     } else {
       book_beam(&bk, B[ja].jet, dij);
     }
@@ -308,14 +365,15 @@ int fjo_cluster_brute(const double *p4, int n, int p, double R, int mm, int32_t 
 /* ---------------------------------------------------------------------------
  * Textbook pairwise generalised-kt (no NN bookkeeping).
  * ------------------------------------------------------------------------- */
-int fjo_cluster_pairwise(const double *p4, int n, int p, double R, int mm, int32_t *hp1, int32_t *hp2, double *hdij,
+static int cluster_pairwise(const double *p4, int n, const jdef_t *def, double R, int mm, int32_t *hp1, int32_t *hp2,
+                           double *hdij,
                          double *jets, int32_t *njets) {
-  if (n < 0 || !(R > 0.0) || p < -1 || p > 1) return -1;
+  if (n < 0 || !(R > 0.0) || !jdef_ok(def)) return -1;
   *njets = 0;
   if (n == 0) return 0;
   const double R2 = R * R, invR2 = 1.0 / R2;
   book_t bk;
-  if (book_init(&bk, p4, n, hp1, hp2, hdij, jets)) return -2;
+  if (book_init(&bk, p4, n, def, mm, hp1, hp2, hdij, jets)) return -2;
   int *live = (int *)malloc(sizeof(int) * (size_t)n);
   double *rap = (double *)malloc(sizeof(double) * (size_t)(2 * n));
   double *phi = (double *)malloc(sizeof(double) * (size_t)(2 * n));
@@ -330,7 +388,9 @@ int fjo_cluster_pairwise(const double *p4, int n, int p, double R, int mm, int32
   }
   for (int i = 0; i < n; ++i) {
     live[i] = i;
-    set_jet(bk.p4 + 4 * (size_t)i, p, mm, &rap[i], &phi[i], &w[i]);
+    rap[i] = bk.rap[i];
+    phi[i] = bk.phi[i];
+    w[i] = book_w(&bk, i);
   }
   int nl = n;
   while (nl > 0) {
@@ -360,7 +420,9 @@ int fjo_cluster_pairwise(const double *p4, int n, int p, double R, int mm, int32
     double dij = best * invR2;
     if (bj >= 0) {
       int k = book_merge(&bk, live[bi], live[bj], dij);
-      set_jet(bk.p4 + 4 * (size_t)k, p, mm, &rap[k], &phi[k], &w[k]);
+      rap[k] = bk.rap[k];
+      phi[k] = bk.phi[k];
+      w[k] = book_w(&bk, k);
       live[bi] = k;
       for (int c = bj; c < nl - 1; ++c) live[c] = live[c + 1];
     } else {
@@ -426,10 +488,9 @@ static void tgeom_init(tgeom_t *g, const tj_t *J, int n, double R) {
 
 static int tgeom_tile(const tgeom_t *g, double y, double phi) {
   double fy = (y - g->y0) / g->sy;
-  int ty = fy < 0.0 ? 0 : (fy >= (double)g->ny ? g->ny - 1 : (int)fy);
-  int tp = (int)(phi / g->sphi);
-  if (tp >= g->nphi) tp = g->nphi - 1;
-  if (tp < 0) tp = 0;
+  int ty = fy >= 0.0 ? (fy < (double)g->ny ? (int)fy : g->ny - 1) : 0; /* NaN: tile row 0 */
+  const double fp = phi / g->sphi;
+  int tp = fp >= 0.0 ? (fp < (double)g->nphi ? (int)fp : g->nphi - 1) : 0;
   return ty * g->nphi + tp;
 }
 
@@ -495,14 +556,15 @@ static void tj_rescan(tj_t *J, const int *head, const tgeom_t *g, int p, double 
   J[p].nnd = best;
 }
 
-int fjo_cluster_tiled(const double *p4, int n, int p, double R, int mm, int32_t *hp1, int32_t *hp2, double *hdij,
+static int cluster_tiled(const double *p4, int n, const jdef_t *def, double R, int mm, int32_t *hp1, int32_t *hp2,
+                           double *hdij,
                       double *jets, int32_t *njets) {
-  if (n < 0 || !(R > 0.0) || p < -1 || p > 1) return -1;
+  if (n < 0 || !(R > 0.0) || !jdef_ok(def)) return -1;
   *njets = 0;
   if (n == 0) return 0;
   const double R2 = R * R, invR2 = 1.0 / R2;
   book_t bk;
-  if (book_init(&bk, p4, n, hp1, hp2, hdij, jets)) return -2;
+  if (book_init(&bk, p4, n, def, mm, hp1, hp2, hdij, jets)) return -2;
   tj_t *J = (tj_t *)malloc(sizeof(tj_t) * (size_t)n);
   int *lab2 = (int *)malloc(sizeof(int) * (size_t)n);
   double *dij = (double *)malloc(sizeof(double) * (size_t)n);
@@ -514,7 +576,9 @@ int fjo_cluster_tiled(const double *p4, int n, int p, double R, int mm, int32_t 
     return -2;
   }
   for (int i = 0; i < n; ++i) {
-    set_jet(bk.p4 + 4 * (size_t)i, p, mm, &J[i].eta, &J[i].phi, &J[i].w);
+    J[i].eta = bk.rap[i];
+    J[i].phi = bk.phi[i];
+    J[i].w = book_w(&bk, i);
     J[i].jet = i;
     J[i].label = i;
     lab2[i] = i;
@@ -568,7 +632,9 @@ int fjo_cluster_tiled(const double *p4, int n, int p, double R, int mm, int32_t 
     if (plo >= 0) {
       int k = book_merge(&bk, J[phi_].jet, J[plo].jet, dd);
       tl_remove(J, head, plo);
-      set_jet(bk.p4 + 4 * (size_t)k, p, mm, &J[plo].eta, &J[plo].phi, &J[plo].w);
+      J[plo].eta = bk.rap[k];
+      J[plo].phi = bk.phi[k];
+      J[plo].w = book_w(&bk, k);
       J[plo].jet = k;
       J[plo].nn = -1;
       J[plo].nnd = R2;
@@ -758,6 +824,24 @@ int fjo_exclusive_jets(const double *p4, const int32_t *hp1, const int32_t *hp2,
   return 0;
 }
 
+/* Public entry points: kt / C/A / anti-kt with the E-scheme, and any jet definition. */
+#define FJO_WRAP(name)                                                                                              \
+  int fjo_cluster_##name(const double *p4, int n, int p, double R, int mm, int32_t *hp1, int32_t *hp2, double *hdij, \
+                         double *jets, int32_t *njets) {                                                             \
+    if (p < -1 || p > 1) return -1;                                                                                  \
+    const jdef_t d = {(double)p, FJMI_SCHEME_E};                                                                     \
+    return cluster_##name(p4, n, &d, R, mm, hp1, hp2, hdij, jets, njets);                                            \
+  }                                                                                                                  \
+  int fjo_cluster_##name##_def(const double *p4, int n, double p, int scheme, double R, int mm, int32_t *hp1,      \
+                               int32_t *hp2, double *hdij, double *jets, int32_t *njets) {                          \
+    const jdef_t d = {p, scheme};                                                                                    \
+    return cluster_##name(p4, n, &d, R, mm, hp1, hp2, hdij, jets, njets);                                            \
+  }
+FJO_WRAP(n2plain)
+FJO_WRAP(brute)
+FJO_WRAP(pairwise)
+FJO_WRAP(tiled)
+
 int64_t fjo_tiled_pair_evals(const double *p4, int n, int p, double R) {
   if (n <= 0) return 0;
   int32_t *h1 = (int32_t *)malloc(sizeof(int32_t) * (size_t)n), *h2 = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
@@ -784,7 +868,8 @@ typedef struct {
   const int64_t *off;
   const int32_t *p_arr;
   const double *R_arr;
-  int n_events, p, nthreads, tid, method, mm;
+  int n_events, nthreads, tid, method, mm;
+  jdef_t def;
   double R;
   int32_t *o1, *o2, *onj;
   double *od, *oj;
@@ -809,13 +894,14 @@ static void *tb_worker(void *vp) {
     const int64_t b = a->off[e];
     const double *q = a->p4 + 4 * b;
     const int n = (int)(a->off[e + 1] - b);
-    const int pe = a->p_arr ? a->p_arr[e] : a->p;
+    jdef_t de = a->def;
+    if (a->p_arr) de.p = (double)a->p_arr[e];
     const double Re = a->R_arr ? a->R_arr[e] : a->R;
     t_pair_evals = 0;
     int32_t *w1 = a->o1 ? a->o1 + b : h1, *w2 = a->o2 ? a->o2 + b : h2;
     double *wd = a->od ? a->od + b : hd, *wj = a->oj ? a->oj + 4 * b : jt;
-    int rc = a->method ? fjo_cluster_tiled(q, n, pe, Re, a->mm, w1, w2, wd, wj, &nj)
-                       : fjo_cluster_n2plain(q, n, pe, Re, a->mm, w1, w2, wd, wj, &nj);
+    int rc = a->method ? cluster_tiled(q, n, &de, Re, a->mm, w1, w2, wd, wj, &nj)
+                       : cluster_n2plain(q, n, &de, Re, a->mm, w1, w2, wd, wj, &nj);
     if (rc) a->err = 1;
     if (a->onj) a->onj[e] = nj;
     if (a->opairs) a->opairs[e] = a->method ? t_pair_evals : -1;
@@ -827,10 +913,10 @@ static void *tb_worker(void *vp) {
   return NULL;
 }
 
-double fjo_cluster_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R,
-                         const int32_t *p_arr, const double *R_arr, int nthreads, int method, int math_mode,
-                         int32_t *hist_p1, int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets,
-                         int64_t *pair_evals) {
+static double cluster_batch(const double *p4, const int64_t *offsets, int n_events, jdef_t def, double R,
+                            const int32_t *p_arr, const double *R_arr, int nthreads, int method, int math_mode,
+                            int32_t *hist_p1, int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets,
+                            int64_t *pair_evals) {
   if (nthreads < 1 || nthreads > 1024 || n_events < 0) return -1.0;
   pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
   tb_arg_t *args = (tb_arg_t *)malloc(sizeof(tb_arg_t) * (size_t)nthreads);
@@ -842,7 +928,7 @@ double fjo_cluster_batch(const double *p4, const int64_t *offsets, int n_events,
   struct timespec t0, t1;
   clock_gettime(CLOCK_MONOTONIC, &t0);
   for (int t = 0; t < nthreads; ++t) {
-    args[t] = (tb_arg_t){p4, offsets, p_arr, R_arr, n_events, p, nthreads, t, method, math_mode, R,
+    args[t] = (tb_arg_t){p4, offsets, p_arr, R_arr, n_events, nthreads, t, method, math_mode, def, R,
                          hist_p1, hist_p2, njets, hist_dij, jets, pair_evals, 0};
     pthread_create(&th[t], NULL, tb_worker, &args[t]);
   }
@@ -856,6 +942,28 @@ double fjo_cluster_batch(const double *p4, const int64_t *offsets, int n_events,
   free(args);
   if (err) return -1.0;
   return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+double fjo_cluster_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R,
+                         const int32_t *p_arr, const double *R_arr, int nthreads, int method, int math_mode,
+                         int32_t *hist_p1, int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets,
+                         int64_t *pair_evals) {
+  if (p < -1 || p > 1) return -1.0;
+  if (p_arr)
+    for (int e = 0; e < n_events; ++e)
+      if (p_arr[e] < -1 || p_arr[e] > 1) return -1.0;
+  const jdef_t d = {(double)p, FJMI_SCHEME_E};
+  return cluster_batch(p4, offsets, n_events, d, R, p_arr, R_arr, nthreads, method, math_mode, hist_p1, hist_p2,
+                       hist_dij, jets, njets, pair_evals);
+}
+
+double fjo_cluster_batch_def(const double *p4, const int64_t *offsets, int n_events, double p, int scheme, double R,
+                             const double *R_arr, int nthreads, int method, int32_t *hist_p1, int32_t *hist_p2,
+                             double *hist_dij, double *jets, int32_t *njets) {
+  const jdef_t d = {p, scheme};
+  if (!jdef_ok(&d)) return -1.0;
+  return cluster_batch(p4, offsets, n_events, d, R, NULL, R_arr, nthreads, method, FJO_MATH_SHARED, hist_p1, hist_p2,
+                       hist_dij, jets, njets, NULL);
 }
 
 double fjo_time_batch(const double *p4, const int64_t *offsets, int n_events, int p, double R, int nthreads,

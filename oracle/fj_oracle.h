@@ -58,6 +58,25 @@ int fjo_cluster_pairwise(const double *p4, int n, int p, double R, int math_mode
 int fjo_cluster_tiled(const double *p4, int n, int p, double R, int math_mode, int32_t *hist_p1, int32_t *hist_p2,
                       double *hist_dij, double *jets, int32_t *njets);
 
+/* The same four methods for any jet definition: the generalised-kt exponent p
+ * (1 kt, 0 C/A, -1 anti-kt, any other finite real genkt, with FastJet's
+ * kt^2 >= 1e-300 guard for p <= 0) and a recombination scheme FJMI_SCHEME_*
+ * (FastJet's DefaultRecombiner: inputs preprocessed as the scheme requires,
+ * merged jets built by fjmi_recombine).  Neither is in the reference, which has
+ * p in {1, 0, -1} and the E-scheme (cluster.h:4, tri_matrix.cu:247-250). */
+int fjo_cluster_n2plain_def(const double *p4, int n, double p, int scheme, double R, int math_mode, int32_t *hist_p1,
+                            int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets);
+int fjo_cluster_brute_def(const double *p4, int n, double p, int scheme, double R, int math_mode, int32_t *hist_p1,
+                          int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets);
+int fjo_cluster_pairwise_def(const double *p4, int n, double p, int scheme, double R, int math_mode, int32_t *hist_p1,
+                             int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets);
+int fjo_cluster_tiled_def(const double *p4, int n, double p, int scheme, double R, int math_mode, int32_t *hist_p1,
+                          int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets);
+/* A batch with one jet definition (and optional per-event R) on nthreads host threads. */
+double fjo_cluster_batch_def(const double *p4, const int64_t *offsets, int n_events, double p, int scheme, double R,
+                             const double *R_arr, int nthreads, int method, int32_t *hist_p1, int32_t *hist_p2,
+                             double *hist_dij, double *jets, int32_t *njets);
+
 /* Kinematics exported for tests. */
 double fjo_log(double x);
 double fjo_atan2(double y, double x);

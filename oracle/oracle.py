@@ -27,6 +27,15 @@ def lib():
             f = getattr(L, name)
             f.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
             f.restype = ctypes.c_int
+        for name in ("fjo_cluster_n2plain_def", "fjo_cluster_brute_def", "fjo_cluster_pairwise_def",
+                     "fjo_cluster_tiled_def"):
+            f = getattr(L, name)
+            f.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp,
+                          _vp, _vp, _vp]
+            f.restype = ctypes.c_int
+        L.fjo_cluster_batch_def.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                            _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
+        L.fjo_cluster_batch_def.restype = ctypes.c_double
         L.fjo_log.argtypes = [ctypes.c_double]
         L.fjo_log.restype = ctypes.c_double
         L.fjo_atan2.argtypes = [ctypes.c_double, ctypes.c_double]
@@ -68,6 +77,48 @@ def _run(fname, p4, p, R, math_mode):
     if rc != 0:
         raise ValueError(f"{fname} returned {rc}")
     return h1[:n], h2[:n], hd[:n], jets[:nj[0]].copy()
+
+
+def _run_def(fname, p4, p, scheme, R):
+    p4 = np.ascontiguousarray(p4, dtype=np.float64).reshape(-1, 4)
+    n = len(p4)
+    h1 = np.zeros(max(n, 1), np.int32)
+    h2 = np.zeros(max(n, 1), np.int32)
+    hd = np.zeros(max(n, 1), np.float64)
+    jets = np.zeros((max(n, 1), 4), np.float64)
+    nj = np.zeros(1, np.int32)
+    rc = getattr(lib(), fname)(p4.ctypes.data_as(_vp), n, float(p), int(scheme), float(R), MATH_SHARED,
+                               h1.ctypes.data_as(_vp), h2.ctypes.data_as(_vp), hd.ctypes.data_as(_vp),
+                               jets.ctypes.data_as(_vp), nj.ctypes.data_as(_vp))
+    if rc != 0:
+        raise ValueError(f"{fname} failed ({rc})")
+    return h1[:n], h2[:n], hd[:n], jets[:nj[0]]
+
+
+def cluster_def(p4, p, scheme, R, method="n2plain"):
+    """One event under a general jet definition: genkt exponent p (any finite real) and recombination scheme
+    (0 E, 1 pt, 2 pt2, 3 Et, 4 Et2, 5 BIpt, 6 BIpt2, 7 WTA_pt, 8 WTA_modp).  Returns (h1, h2, dij, jets)."""
+    return _run_def(f"fjo_cluster_{method}_def", p4, p, scheme, R)
+
+
+def cluster_batch_def(p4, offsets, p, scheme, R, nthreads, method="tiled"):
+    """A batch under one general jet definition (R scalar or per event).  Returns (h1, h2, hd, jets, nj)."""
+    p4 = np.ascontiguousarray(p4, dtype=np.float64).reshape(-1, 4)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    ne, tot = len(offsets) - 1, len(p4)
+    Ra = np.ascontiguousarray(R, dtype=np.float64) if np.ndim(R) else None
+    h1 = np.zeros(max(tot, 1), np.int32)
+    h2 = np.zeros(max(tot, 1), np.int32)
+    hd = np.zeros(max(tot, 1), np.float64)
+    jets = np.zeros((max(tot, 1), 4), np.float64)
+    nj = np.zeros(max(ne, 1), np.int32)
+    P = lambda a: None if a is None else a.ctypes.data_as(_vp)
+    secs = lib().fjo_cluster_batch_def(P(p4), P(offsets), ne, float(p), int(scheme), 1.0 if Ra is not None else float(R),
+                                       P(Ra), int(nthreads), 1 if method == "tiled" else 0, P(h1), P(h2), P(hd),
+                                       P(jets), P(nj))
+    if secs < 0:
+        raise ValueError("fjo_cluster_batch_def failed")
+    return h1[:tot], h2[:tot], hd[:tot], jets[:tot], nj[:ne]
 
 
 def n2plain(p4, p, R, math_mode=MATH_SHARED):

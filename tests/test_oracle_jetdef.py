@@ -1,0 +1,205 @@
+"""Pins of the oracle's general jet definitions (oracle/fj_oracle.c, the *_def
+entry points): the generalised-kt exponent p for any real p and FastJet's
+recombination schemes (fastjet_gpu_amd/csrc/fjmi_math.h: fjmi_weight_gen,
+fjmi_preprocess, fjmi_recombine).  Neither is in the reference (cluster.h:4,
+tri_matrix.cu:247-250); FastJet, whose JetDefinition defines them, is not
+available here, so the checks are:
+
+  - independent restatements agree bit for bit: N2Plain, brute force (every
+    nearest neighbour recomputed each step) and the tiled form for every
+    scheme, and the textbook pairwise loop where the merged jet does not depend
+    on the order of its parents;
+  - p = 1, 0, -1 with the E-scheme is the kt / C/A / anti-kt clustering itself;
+  - genkt ordering: p > 0 merges the softest pairs first, p < 0 clusters
+    around the hardest particles (an isolated pair's beam order follows kt^(2p));
+  - each scheme's defining property, on the jets and their constituents from
+    the history: pt-like schemes give massless jets whose pt is the sum of the
+    constituents' pt; the pt and pt2 schemes preprocess inputs to E = |p|, Et
+    and Et2 rescale |p| to E; BIpt and BIpt2 leave the inputs alone; WTA_pt
+    keeps the hardest constituent's direction and sums pt; WTA_modp keeps the
+    hardest constituent's direction and sums |p|; the E-scheme sums momenta.
+"""
+import math
+
+import numpy as np
+import pytest
+from hypothesis import given, settings, strategies as st
+
+from oracle import oracle as orc
+
+SCHEMES = list(range(9))
+E, PT, PT2, ET, ET2, BIPT, BIPT2, WTA_PT, WTA_MODP = SCHEMES
+PS = [0.5, -0.5, 1.5, -2.0, 0.25, 2.0]
+
+
+def random_event(rng, n, spread=1.0):
+    pt = 0.5 + rng.exponential(5.0, n)
+    eta = rng.uniform(-2.0 * spread, 2.0 * spread, n)
+    phi = rng.uniform(0, 2 * math.pi, n)
+    m = rng.uniform(0, 0.3, n)
+    px, py, pz = pt * np.cos(phi), pt * np.sin(phi), pt * np.sinh(eta)
+    Ev = np.sqrt(px * px + py * py + pz * pz + m * m)
+    return np.stack([px, py, pz, Ev], axis=1)
+
+
+def same(a, b):
+    return all(np.asarray(x).tobytes() == np.asarray(y).tobytes() for x, y in zip(a, b))
+
+
+def constituents(n, h1, h2):
+    """The particles of each final jet, in the order of the beam steps that promote them."""
+    members = {i: [i] for i in range(n)}
+    out = []
+    for s in range(n):
+        if h2[s] >= 0:
+            members[n + s] = members.pop(h1[s]) + members.pop(h2[s])
+        else:
+            out.append(members.pop(h1[s]))
+    return out
+
+
+@settings(max_examples=40, deadline=None)
+@given(n=st.integers(1, 30), seed=st.integers(0, 2 ** 31 - 1), p=st.sampled_from(PS + [1.0, 0.0, -1.0]),
+       scheme=st.sampled_from(SCHEMES), R=st.sampled_from([0.3, 0.6, 1.0]))
+def test_restatements_agree(n, seed, p, scheme, R):
+    ev = random_event(np.random.default_rng(seed), n)
+    a = orc.cluster_def(ev, p, scheme, R, "n2plain")
+    assert same(a, orc.cluster_def(ev, p, scheme, R, "brute"))
+    assert same(a, orc.cluster_def(ev, p, scheme, R, "tiled"))
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("p,scheme", [(0.5, E), (-0.5, E), (2.0, E), (1.5, WTA_MODP), (-1.0, WTA_MODP)])
+def test_textbook_pairwise_agrees(seed, p, scheme):
+    """The textbook loop keeps no N2Plain slots, so it is compared where the merged jet does not depend on the
+    order of its two parents: the pt-like and WTA_pt schemes wrap the second parent's azimuth next to the first's
+    before averaging (DefaultRecombiner::recombine), and N2Plain passes the parent of the higher slot first
+    (FastJet's _simple_N2_cluster), as the brute-force and tiled forms do."""
+    ev = random_event(np.random.default_rng(400 + seed), 120)
+    assert same(orc.cluster_def(ev, p, scheme, 0.6, "n2plain"), orc.cluster_def(ev, p, scheme, 0.6, "pairwise"))
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_tiled_agrees_on_larger_events(seed):
+    ev = random_event(np.random.default_rng(500 + seed), 1500, spread=2.0)
+    for p, scheme in ((0.5, E), (-0.7, PT), (1.2, WTA_MODP)):
+        assert same(orc.cluster_def(ev, p, scheme, 0.4, "n2plain"), orc.cluster_def(ev, p, scheme, 0.4, "tiled"))
+
+
+@pytest.mark.parametrize("p", [1, 0, -1])
+def test_integer_p_with_e_scheme_is_the_algorithm_itself(p):
+    for seed in range(5):
+        ev = random_event(np.random.default_rng(600 + seed), 200)
+        assert same(orc.n2plain(ev, p, 0.5), orc.cluster_def(ev, float(p), E, 0.5))
+
+
+def test_genkt_beam_order_follows_the_weight():
+    """Particles farther apart than R never merge and go to the beam in increasing kt^(2p): softest first
+    for p > 0, hardest first for p < 0."""
+    pts = np.array([3.0, 1.0, 7.0, 2.0, 5.0])
+    phis = np.arange(5) * 1.2
+    ev = np.stack([pts * np.cos(phis), pts * np.sin(phis), np.zeros(5), pts], 1)
+    for p, order in ((0.5, [1, 3, 0, 4, 2]), (2.0, [1, 3, 0, 4, 2]), (-0.5, [2, 4, 0, 3, 1]), (-3.0, [2, 4, 0, 3, 1])):
+        h1, h2, hd, jets = orc.cluster_def(ev, p, E, 0.5)
+        assert list(h1) == order and all(h2 == -1)
+        w = pts[order] ** (2 * p)
+        assert np.allclose(hd, w, rtol=1e-14)  # d_iB = kt^(2p)
+
+
+def _jets_and_parts(ev, p, scheme, R=0.7):
+    h1, h2, hd, jets = orc.cluster_def(ev, p, scheme, R)
+    return jets, constituents(len(ev), h1, h2)
+
+
+def _pt(q):
+    return math.hypot(q[0], q[1])
+
+
+def _rap(q):
+    return 0.5 * math.log((q[3] + q[2]) / (q[3] - q[2]))
+
+
+@pytest.mark.parametrize("scheme", [PT, PT2, ET, ET2, BIPT, BIPT2])
+def test_pt_like_schemes_sum_pt_into_massless_jets(scheme):
+    ev = random_event(np.random.default_rng(700 + scheme), 80)
+    jets, parts = _jets_and_parts(ev, -1.0, scheme)
+    pre = np.array([_preprocessed(q, scheme) for q in ev])
+    for j, ps in zip(jets, parts):
+        if len(ps) < 2:
+            assert np.array_equal(j, pre[ps[0]])  # a lone particle is the input as the scheme takes it
+            continue
+        assert _pt(j) == pytest.approx(sum(_pt(pre[i]) for i in ps), rel=1e-13)
+        assert j[3] ** 2 - (j[0] ** 2 + j[1] ** 2 + j[2] ** 2) == pytest.approx(0.0, abs=1e-10 * j[3] ** 2)
+
+
+def _preprocessed(q, scheme):
+    px, py, pz, e = q
+    mod = math.sqrt((px * px + py * py) + pz * pz)
+    if scheme in (PT, PT2):
+        return np.array([px, py, pz, mod])
+    if scheme in (ET, ET2):
+        r = e / mod
+        return np.array([px * r, py * r, pz * r, e])
+    return np.array(q)
+
+
+def test_preprocessing_per_scheme():
+    """pt, pt2: E := |p| (the 3-momentum kept); Et, Et2: |p| := E (E kept); others: unchanged — seen on
+    particles that end as jets of their own."""
+    ev = random_event(np.random.default_rng(800), 6)
+    ev[:, :3] *= 1.0
+    far = np.array([[math.cos(k), math.sin(k), 0.3 * k, 0.0] for k in range(6)])
+    far[:, 3] = np.sqrt((far[:, :3] ** 2).sum(1) + 0.04)  # massive, pairwise far apart at R = 0.3
+    for scheme in SCHEMES:
+        h1, h2, hd, jets = orc.cluster_def(far, 1.0, scheme, 0.3)
+        assert all(h2 == -1)
+        for jet, i in zip(jets, h1):
+            want = _preprocessed(far[i], scheme)
+            assert jet.tobytes() == want.tobytes(), (scheme, i)
+
+
+def test_wta_pt_keeps_a_constituent_direction():
+    """Each merge keeps the harder jet's rapidity and azimuth, so a jet points exactly where one of its
+    particles does (to the rounding of rebuilding its momentum), and its pt is the constituents' sum."""
+    ev = random_event(np.random.default_rng(900), 60)
+    jets, parts = _jets_and_parts(ev, 1.0, WTA_PT, R=1.0)
+    merged = 0
+    for j, ps in zip(jets, parts):
+        if len(ps) < 2:
+            continue
+        merged += 1
+        phi = math.atan2(j[1], j[0])
+        assert any(abs(_rap(j) - _rap(ev[i])) < 1e-9
+                   and abs(math.remainder(phi - math.atan2(ev[i][1], ev[i][0]), 2 * math.pi)) < 1e-9 for i in ps)
+        assert _pt(j) == pytest.approx(sum(_pt(ev[i]) for i in ps), rel=1e-13)
+    assert merged > 3
+
+
+def test_wta_modp_keeps_a_constituent_direction_and_sums_modp():
+    ev = random_event(np.random.default_rng(901), 60)
+    jets, parts = _jets_and_parts(ev, 0.0, WTA_MODP, R=1.0)
+    mod = lambda q: math.sqrt(q[0] ** 2 + q[1] ** 2 + q[2] ** 2)
+    merged = 0
+    for j, ps in zip(jets, parts):
+        if len(ps) < 2:
+            continue
+        merged += 1
+        assert mod(j) == pytest.approx(sum(mod(ev[i]) for i in ps), rel=1e-13)
+        assert j[3] == pytest.approx(mod(j), rel=1e-13)  # massless: E = |p|
+        u = j[:3] / mod(j)
+        assert any(np.allclose(u, ev[i][:3] / mod(ev[i]), atol=1e-12) for i in ps)
+    assert merged > 3
+
+
+def test_e_scheme_sums_momenta():
+    ev = random_event(np.random.default_rng(902), 60)
+    jets, parts = _jets_and_parts(ev, 0.5, E, R=0.8)
+    for j, ps in zip(jets, parts):
+        assert np.allclose(j, ev[ps].sum(0), rtol=1e-13, atol=1e-12)
+
+
+def test_invalid_definitions_are_refused():
+    ev = random_event(np.random.default_rng(903), 5)
+    for p, scheme in ((math.nan, E), (math.inf, E), (0.5, 9), (0.5, -1)):
+        with pytest.raises(ValueError):
+            orc.cluster_def(ev, p, scheme, 0.4)
