@@ -15,9 +15,15 @@ import numpy as np
 
 __all__ = ["Engine", "Server", "FjmiError", "KT", "CAMBRIDGE_AACHEN", "ANTIKT", "ENGINE_AUTO", "ENGINE_DENSE",
            "ENGINE_TILED", "ENGINE_EMPTY", "gen_event", "gen_serving_attrs", "gen_workload", "lib_path", "load_lib",
-           "write_events", "read_events", "read_text_events"]
+           "write_events", "read_events", "read_text_events", "JetDef", "JETDEF_DTYPE", "SCHEME_E", "SCHEME_PT",
+           "SCHEME_PT2", "SCHEME_ET", "SCHEME_ET2", "SCHEME_BIPT", "SCHEME_BIPT2", "SCHEME_WTA_PT", "SCHEME_WTA_MODP",
+           "NUM_SCHEMES"]
 
 KT, CAMBRIDGE_AACHEN, ANTIKT = 1, 0, -1
+# recombination schemes of general jet definitions (FastJet's RecombinationScheme numbers, include/fjmi.h)
+SCHEME_E, SCHEME_PT, SCHEME_PT2, SCHEME_ET, SCHEME_ET2, SCHEME_BIPT, SCHEME_BIPT2, SCHEME_WTA_PT, SCHEME_WTA_MODP = \
+    range(9)
+NUM_SCHEMES = 9
 EXCLUSIVE_NJETS, EXCLUSIVE_DCUT = 0, 1
 ENGINE_AUTO, ENGINE_DENSE, ENGINE_TILED, ENGINE_EMPTY = 0, 1, 2, 3
 
@@ -38,6 +44,15 @@ FJMI_EAGAIN = -6
 class Bucket(ctypes.Structure):
     _fields_ = [("max_n", ctypes.c_int32), ("engine", ctypes.c_int32), ("threads", ctypes.c_int32),
                 ("slots", ctypes.c_int32), ("p4_global", ctypes.c_int32)]
+
+
+class JetDef(ctypes.Structure):
+    """fjmi_jetdef: genkt exponent p (any finite real) and recombination scheme SCHEME_*."""
+    _fields_ = [("p", ctypes.c_double), ("scheme", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+# fjmi_jetdef as a numpy record (per-event definitions in device memory)
+JETDEF_DTYPE = np.dtype([("p", np.float64), ("scheme", np.int32), ("pad", np.int32)])
 
 
 class BucketStats(ctypes.Structure):
@@ -118,6 +133,10 @@ def _declare(lib):
         "fjmi_cluster_event_device_noalloc": ([_vp, _vp, i, i, d, _vp, _vp, _vp, _vp, _vp, _vp], i),
         "fjmi_batch_create": ([_vp, i, _vp, _vp, _vp, _vp, _vp, i, d, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)],
                               i),
+        "fjmi_cluster_event_def": ([_vp, _vp, i, _vp, d, _vp, _vp, _vp, _vp, _vp], i),
+        "fjmi_cluster_event_device_def": ([_vp, _vp, i, _vp, d, _vp, _vp, _vp, _vp, _vp, _vp], i),
+        "fjmi_batch_create_def": ([_vp, i, _vp, _vp, _vp, _vp, _vp, _vp, d, _vp, _vp, _vp, _vp, _vp,
+                                   ctypes.POINTER(_vp)], i),
         "fjmi_batch_launch": ([_vp, _vp], i),
         "fjmi_batch_sync": ([_vp, _vp], i),
         "fjmi_batch_destroy": ([_vp], i),
@@ -198,6 +217,22 @@ class Engine:
                                                      _ptr(h2), _ptr(hd), _ptr(jets), _ptr(nj)), "fjmi_cluster_event")
         return h1, h2, hd, jets[:nj[0]].copy()
 
+    def cluster_event_def(self, p4, p, scheme, R):
+        """Clusters one event under a general jet definition: genkt exponent p (any finite real) and
+        recombination scheme SCHEME_*. Returns (hist_p1, hist_p2, hist_dij, jets)."""
+        p4 = np.ascontiguousarray(p4, dtype=np.float64).reshape(-1, 4)
+        n = len(p4)
+        h1 = np.zeros(n, np.int32)
+        h2 = np.zeros(n, np.int32)
+        hd = np.zeros(n, np.float64)
+        jets = np.zeros((max(n, 1), 4), np.float64)
+        nj = np.zeros(1, np.int32)
+        jd = JetDef(float(p), int(scheme), 0)
+        _check(self.lib, self.lib.fjmi_cluster_event_def(self.ctx, _ptr(p4), n, ctypes.byref(jd), float(R), _ptr(h1),
+                                                         _ptr(h2), _ptr(hd), _ptr(jets), _ptr(nj)),
+               "fjmi_cluster_event_def")
+        return h1, h2, hd, jets[:nj[0]].copy()
+
     def cluster_batch(self, p4, offsets, p, R):
         """Clusters many events in one graph launch. p and R are scalars or per-event arrays.
         Returns (hist_p1, hist_p2, hist_dij, jets, njets) with per-particle arrays."""
@@ -223,6 +258,12 @@ class Engine:
         _check(self.lib, self.lib.fjmi_cluster_event_device(self.ctx, d_p4, n, int(p), float(R), engine, d_h1, d_h2,
                                                             d_hd, d_jets, d_nj, stream or None),
                "fjmi_cluster_event_device")
+
+    def cluster_event_device_def(self, d_p4, n, p, scheme, R, d_h1, d_h2, d_hd, d_jets, d_nj, stream=0):
+        jd = JetDef(float(p), int(scheme), 0)
+        _check(self.lib, self.lib.fjmi_cluster_event_device_def(self.ctx, d_p4, n, ctypes.byref(jd), float(R), d_h1,
+                                                                d_h2, d_hd, d_jets, d_nj, stream or None),
+               "fjmi_cluster_event_device_def")
 
     def cluster_pseudojets(self, d_particles, n, p, R, engine=ENGINE_AUTO, stream=0):
         _check(self.lib, self.lib.fjmi_cluster_pseudojets(self.ctx, d_particles, n, int(p), float(R), engine,
@@ -269,6 +310,20 @@ class Engine:
         _check(self.lib, self.lib.fjmi_batch_create(self.ctx, len(h_offsets) - 1, _ptr(h_offsets), d_p4, d_offsets,
                                                     d_p, d_R, int(p_uni), float(R_uni), d_h1, d_h2, d_hd, d_jets,
                                                     d_nj, ctypes.byref(b)), "fjmi_batch_create")
+        return Batch(self, b)
+
+
+    def batch_create_def(self, h_offsets, d_p4, d_offsets, d_defs, p, scheme, d_R, R_uni, d_h1, d_h2, d_hd, d_jets,
+                         d_nj):
+        """A batch plan under general jet definitions: per event (d_defs, device records of JETDEF_DTYPE)
+        or (p, scheme) for all when d_defs is 0."""
+        h_offsets = np.ascontiguousarray(h_offsets, dtype=np.int64)
+        b = _vp()
+        jd = JetDef(float(p), int(scheme), 0)
+        _check(self.lib, self.lib.fjmi_batch_create_def(self.ctx, len(h_offsets) - 1, _ptr(h_offsets), d_p4,
+                                                        d_offsets, d_defs or None, ctypes.byref(jd), d_R or None,
+                                                        float(R_uni), d_h1, d_h2, d_hd, d_jets, d_nj,
+                                                        ctypes.byref(b)), "fjmi_batch_create_def")
         return Batch(self, b)
 
 

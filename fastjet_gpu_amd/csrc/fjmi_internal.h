@@ -72,6 +72,15 @@ struct ServeArgs {
   unsigned long long epoch;      /* this launch's number (1, 2, ...) */
 };
 
+/* A general jet definition (the tiled_gen_kernel variants): genkt exponent p
+ * (any finite real) and recombination scheme FJMI_SCHEME_*; the layout of the
+ * public fjmi_jetdef. */
+struct JetDef {
+  double p;
+  int32_t scheme;
+  int32_t pad;
+};
+
 struct LaunchArgs {
   /* input particles: event e occupies particles [offsets[e], offsets[e+1]);
    * particle i's (px,py,pz,E) are in[i*in_stride + in_off + 0..3] */
@@ -102,7 +111,8 @@ struct LaunchArgs {
   double *p4_scratch;
   /* global scratch for (y, phi, w) when the event does not fit the LDS stage: 4 doubles per particle */
   double *coord_scratch;
-  /* error word: bit 0 = event larger than the kernel's capacity, bit 1 = internal */
+  /* error word: bit 0 = event larger than the kernel's capacity, bit 1 = internal,
+   * bit 2 = a per-event jet definition the kernel cannot run */
   unsigned int *status;
   /* optional: each workgroup leaving the queue raises this to its wall-clock
    * time (the constant-rate counter of wall_clock64(), 100 MHz on gfx950) */
@@ -110,9 +120,12 @@ struct LaunchArgs {
   int stage_chunk; /* number of staging entries in LDS when four-momenta live in global memory */
   int tile_q;      /* tiled engine: tile side = max(tile_q, 4)/4 * R (rounded up) */
   ServeArgs serve; /* serving launches of the tiled engine only */
+  /* general jet definitions (tiled_gen_kernel only): per event, or nullptr -> def_uni */
+  const JetDef *def_arr;
+  JetDef def_uni;
 };
 
-enum { STATUS_CAPACITY = 1, STATUS_INTERNAL = 2 };
+enum { STATUS_CAPACITY = 1, STATUS_INTERNAL = 2, STATUS_DEFINITION = 4 };
 
 /* The tiled engine's per-jet record: one 64-byte block (half a cache line), so
  * that visiting a jet in a gather or a rescan costs one line, not four. */

@@ -29,6 +29,7 @@ constexpr int kNumTiledKernels = 4;
 constexpr int kTiledMaxN = 65535;
 extern const void *const kTiledKernels[kNumTiledKernels];
 extern const void *const kTiledServeKernels[kNumTiledKernels]; /* same, serving mode (LaunchArgs::serve) */
+extern const void *const kTiledGenKernels[kNumTiledKernels];   /* same, general jet definitions (LaunchArgs::def_*) */
 extern const int kTiledMaxTiles[kNumTiledKernels];
 
 /* post-processing (postproc.hip), 256 threads, one workgroup per event */

@@ -313,12 +313,14 @@ int zero_words(void *p, int n, hipStream_t stream) {
  * of `st`, the state of `stream`. */
 int cluster_one_device(fjmi_context *ctx, fjmi::rt::StreamState &st, int engine, const double *d_in, int in_stride,
                        int in_off, int n, int p, double R, int32_t *h1, int32_t *h2, double *hd, double *jets,
-                       int32_t *nj, unsigned char *pj, hipStream_t stream) {
+                       int32_t *nj, unsigned char *pj, hipStream_t stream, const fjmi::JetDef *def = nullptr) {
   /* under the default table: FJMI_ENGINE_AUTO follows the latency rule (the
    * dense shape with the fewest slots per thread, or the latency tiled row), and
    * the tiled engine takes the latency tiled row; under a caller's table, its
    * row for n, else (engine forced) tile side R with the 512-tile variant or the
-   * dense shape with the fewest slots per thread */
+   * dense shape with the fewest slots per thread.  A general jet definition
+   * (def) runs on the tiled engine. */
+  if (def) engine = FJMI_ENGINE_TILED;
   fjmi_bucket forced;
   const fjmi_bucket *b = nullptr;
   if (engine == FJMI_ENGINE_AUTO && !ctx->caller_table) {
@@ -367,9 +369,11 @@ int cluster_one_device(fjmi_context *ctx, fjmi::rt::StreamState &st, int engine,
   a.coord_scratch = st.coords.as<double>();
   a.status = st.status.as<unsigned int>();
   a.tile_q = tiled ? b->slots : 0;
+  if (def) a.def_uni = *def;
   if (tiled) {
     void *params[] = {&a, &ts};
-    HIPCHK(hipLaunchKernel(fjmi::kTiledKernels[b->p4_global], dim3(1), dim3(64), params, 0, stream));
+    const void *fn = def ? fjmi::kTiledGenKernels[b->p4_global] : fjmi::kTiledKernels[b->p4_global];
+    HIPCHK(hipLaunchKernel(fn, dim3(1), dim3(64), params, 0, stream));
   } else if ((rc = launch_dense(*kd, 1, dyn_lds_bytes(*kd), a, stream))) {
     return rc;
   }
@@ -380,6 +384,9 @@ int cluster_one_device(fjmi_context *ctx, fjmi::rt::StreamState &st, int engine,
 int status_error(uint32_t status) {
   if (status & fjmi::STATUS_INTERNAL)
     return fail(FJMI_EINTERNAL, "a clustering kernel found an inconsistent state (status 0x%x)", status);
+  if (status & fjmi::STATUS_DEFINITION)
+    return fail(FJMI_EINVAL, "an event's jet definition has a non-finite exponent or an unknown scheme (status 0x%x)",
+                status);
   return fail(FJMI_ECAPACITY, "a clustering kernel reported an event beyond its capacity (status 0x%x)", status);
 }
 
@@ -395,8 +402,16 @@ int check_engine(int engine) {
  * queues are left in pl.flat for the caller to upload to pl.queue. */
 int make_plan(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const double *d_p4, const int64_t *d_offsets,
               const int32_t *d_p, const double *d_R, int p_uni, double R_uni, int32_t *d_h1, int32_t *d_h2,
-              double *d_hd, double *d_jets, int32_t *d_nj, Plan &pl) {
+              double *d_hd, double *d_jets, int32_t *d_nj, Plan &pl, const fjmi::JetDef *d_defs = nullptr,
+              const fjmi::JetDef *def_uni = nullptr) {
   const int nb = (int)ctx->buckets.size();
+  /* general jet definitions run on the tiled engine: a dense row of the table
+   * becomes the one-event tiled row (384 tiles, side R) for its sizes */
+  const bool gen = d_defs || def_uni;
+  std::vector<fjmi_bucket> rows(ctx->buckets.begin(), ctx->buckets.end());
+  if (gen)
+    for (fjmi_bucket &r : rows)
+      if (r.engine != FJMI_ENGINE_TILED) r = fjmi_bucket{r.max_n, FJMI_ENGINE_TILED, 64, 0, 1};
   pl.nb = nb;
   pl.runs.clear();
   pl.flat.clear();
@@ -407,15 +422,15 @@ int make_plan(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const d
     const int64_t n = h_offsets[e + 1] - h_offsets[e];
     if (n < 0 || h_offsets[e] < 0) return fail(FJMI_EINVAL, "offsets not monotonic at event %d", e);
     int b = 0;
-    while (b < nb && n > ctx->buckets[b].max_n) ++b;
+    while (b < nb && n > rows[b].max_n) ++b;
     if (b == nb) return fail(FJMI_ECAPACITY, "event %d has %lld particles, above the largest bucket", e, (long long)n);
-    if (ctx->buckets[b].engine == FJMI_ENGINE_TILED && n > fjmi::kTiledMaxN)
+    if (rows[b].engine == FJMI_ENGINE_TILED && n > fjmi::kTiledMaxN)
       return fail(FJMI_ECAPACITY, "event %d has %lld particles: the tiled engine takes at most %d", e, (long long)n,
                   fjmi::kTiledMaxN);
     lists[b].push_back(e);
-    if (ctx->buckets[b].engine == FJMI_ENGINE_TILED)
+    if (rows[b].engine == FJMI_ENGINE_TILED)
       any_tiled = true;
-    else if (ctx->buckets[b].p4_global)
+    else if (rows[b].p4_global)
       max_p4g_total = h_offsets[n_events];
   }
   /* cost-ordered queues (longest first): the last events to start are the cheapest */
@@ -443,11 +458,11 @@ int make_plan(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const d
   pl.d_status = heads + nb;
   for (int b = nb - 1; b >= 0; --b) { /* largest multiplicities first */
     if (lists[b].empty()) continue;
-    const fjmi_bucket &bk = ctx->buckets[b];
+    const fjmi_bucket &bk = rows[b];
     BucketRun r;
     r.tiled = bk.engine == FJMI_ENGINE_TILED;
     r.kd = r.tiled ? nullptr : find_kernel(bk.threads, bk.slots, bk.p4_global);
-    r.fn = r.tiled ? fjmi::kTiledKernels[bk.p4_global] : r.kd->fn;
+    r.fn = r.tiled ? (gen ? fjmi::kTiledGenKernels[bk.p4_global] : fjmi::kTiledKernels[bk.p4_global]) : r.kd->fn;
     r.threads = r.tiled ? 64 : r.kd->T;
     r.lds = r.tiled ? 0 : dyn_lds_bytes(*r.kd);
     r.ts = r.tiled ? fjmi::tiled_scratch_carve(pl.tscratch.ptr, h_offsets[n_events], fjmi::kTiledPoolPerEvent)
@@ -494,6 +509,8 @@ int make_plan(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const d
     a.status = pl.d_status;
     a.t_end = pl.times.as<unsigned long long>() + 1 + pl.runs.size();
     a.tile_q = r.tile_q;
+    a.def_arr = d_defs;
+    if (def_uni) a.def_uni = *def_uni;
     r.args = a;
     pl.runs.push_back(r);
   }
@@ -974,8 +991,14 @@ int fjmi_stream_check(fjmi_context *ctx, void *stream) {
   return status_error(status);
 }
 
-int fjmi_cluster_event(fjmi_context *ctx, const double *p4, int n, int p, double R, int engine, int32_t *h1,
-                       int32_t *h2, double *hd, double *jets, int32_t *nj) {
+}  // extern "C"
+
+namespace {
+
+/* One event from host memory, synchronously: the standard algorithms (def
+ * null) or a general jet definition. */
+int cluster_event_host(fjmi_context *ctx, const double *p4, int n, int p, double R, int engine,
+                       const fjmi::JetDef *def, int32_t *h1, int32_t *h2, double *hd, double *jets, int32_t *nj) {
   if (!ctx) return fail(FJMI_EINVAL, "null context");
   int rc = check_args(n, p, R);
   if (rc) return rc;
@@ -997,7 +1020,7 @@ int fjmi_cluster_event(fjmi_context *ctx, const double *p4, int n, int p, double
   HIPCHK(hipMemcpy(st->p4in.ptr, p4, 32 * nn, hipMemcpyHostToDevice));
   if ((rc = cluster_one_device(ctx, *st, engine, st->p4in.as<double>(), 4, 0, n, p, R, st->hist1.as<int32_t>(),
                                st->hist2.as<int32_t>(), st->hdij.as<double>(), st->jets.as<double>(),
-                               st->njets.as<int32_t>(), nullptr, nullptr)))
+                               st->njets.as<int32_t>(), nullptr, nullptr, def)))
     return rc;
   HIPCHK(hipStreamSynchronize(nullptr));
   uint32_t status = 0;
@@ -1016,15 +1039,70 @@ int fjmi_cluster_event(fjmi_context *ctx, const double *p4, int n, int p, double
   return FJMI_OK;
 }
 
+/* A general jet definition the kernels can run, as the kernels' JetDef. */
+int check_def(const fjmi_jetdef *def, fjmi::JetDef *out) {
+  if (!def) return fail(FJMI_EINVAL, "null jet definition");
+  if (!(def->p == def->p && def->p > -1e300 && def->p < 1e300))
+    return fail(FJMI_EINVAL, "exponent p must be finite (|p| < 1e300)");
+  if (def->scheme < 0 || def->scheme >= FJMI_NUM_SCHEMES)
+    return fail(FJMI_EINVAL, "unknown recombination scheme %d", def->scheme);
+  *out = fjmi::JetDef{def->p, def->scheme, 0};
+  return FJMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fjmi_cluster_event(fjmi_context *ctx, const double *p4, int n, int p, double R, int engine, int32_t *h1,
+                       int32_t *h2, double *hd, double *jets, int32_t *nj) {
+  return cluster_event_host(ctx, p4, n, p, R, engine, nullptr, h1, h2, hd, jets, nj);
+}
+
+int fjmi_cluster_event_def(fjmi_context *ctx, const double *p4, int n, const fjmi_jetdef *def, double R,
+                           int32_t *h1, int32_t *h2, double *hd, double *jets, int32_t *nj) {
+  fjmi::JetDef d;
+  const int rc = check_def(def, &d);
+  if (rc) return rc;
+  return cluster_event_host(ctx, p4, n, 0, R, FJMI_ENGINE_TILED, &d, h1, h2, hd, jets, nj);
+}
+
+int fjmi_cluster_event_device_def(fjmi_context *ctx, const double *d_p4, int n, const fjmi_jetdef *def, double R,
+                                  int32_t *d_h1, int32_t *d_h2, double *d_hd, double *d_jets, int32_t *d_nj,
+                                  void *stream) {
+  if (!ctx) return fail(FJMI_EINVAL, "null context");
+  fjmi::JetDef d;
+  int rc = check_def(def, &d);
+  if (rc) return rc;
+  if ((rc = check_args(n, 0, R))) return rc;
+  if (n > 0 && (!d_p4 || !d_h1 || !d_h2 || !d_hd || !d_jets || !d_nj)) return fail(FJMI_EINVAL, "null buffer");
+  if (n == 0 && !d_nj) return fail(FJMI_EINVAL, "null njets");
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  if ((rc = set_device(ctx))) return rc;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (n == 0) return zero_words(d_nj, 1, s);
+  fjmi::rt::StreamState *st = nullptr;
+  if ((rc = stream_state(ctx, s, &st))) return rc;
+  return cluster_one_device(ctx, *st, FJMI_ENGINE_TILED, d_p4, 4, 0, n, 0, R, d_h1, d_h2, d_hd, d_jets, d_nj, nullptr,
+                            s, &d);
+}
+
 /* ---- batched path ---- */
 
-int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const double *d_p4,
-                      const int64_t *d_offsets, const int32_t *d_p, const double *d_R, int p_uni, double R_uni,
-                      int32_t *d_h1, int32_t *d_h2, double *d_hd, double *d_jets, int32_t *d_nj, fjmi_batch **out) {
+}  // extern "C"
+
+namespace {
+
+/* A batch plan and its captured graph: the standard algorithms (d_defs and
+ * def_uni null) or general jet definitions. */
+int batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const double *d_p4,
+                 const int64_t *d_offsets, const int32_t *d_p, const double *d_R, int p_uni, double R_uni,
+                 const fjmi::JetDef *d_defs, const fjmi::JetDef *def_uni, int32_t *d_h1, int32_t *d_h2, double *d_hd,
+                 double *d_jets, int32_t *d_nj, fjmi_batch **out) {
   if (!ctx || !out || !h_offsets || n_events < 0) return fail(FJMI_EINVAL, "bad argument");
   *out = nullptr;
   if (!d_p && (p_uni < -1 || p_uni > 1)) return fail(FJMI_EINVAL, "exponent p must be -1, 0 or 1");
-  if (!d_R && !(R_uni > 0.0)) return fail(FJMI_EINVAL, "radius must be positive");
+  if (!d_R && !(R_uni > 0.0 && R_uni < 1e150)) return fail(FJMI_EINVAL, "radius must be positive and finite");
   std::lock_guard<std::mutex> lk(ctx->mtx);
   int rc;
   if ((rc = set_device(ctx))) return rc;
@@ -1037,7 +1115,7 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
   };
   Plan &pl = bt->plan;
   if ((rc = make_plan(ctx, n_events, h_offsets, d_p4, d_offsets, d_p, d_R, p_uni, R_uni, d_h1, d_h2, d_hd, d_jets,
-                      d_nj, pl)))
+                      d_nj, pl, d_defs, def_uni)))
     return cleanup(rc);
   if (!pl.flat.empty()) {
     hipError_t e = hipMemcpy(pl.queue.ptr, pl.flat.data(), sizeof(int32_t) * pl.flat.size(), hipMemcpyHostToDevice);
@@ -1078,6 +1156,33 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
   if (e != hipSuccess) return cleanup(fail(FJMI_EHIP, "graph capture: %s", hipGetErrorString(e)));
   *out = bt;
   return FJMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const double *d_p4,
+                      const int64_t *d_offsets, const int32_t *d_p, const double *d_R, int p_uni, double R_uni,
+                      int32_t *d_h1, int32_t *d_h2, double *d_hd, double *d_jets, int32_t *d_nj, fjmi_batch **out) {
+  return batch_create(ctx, n_events, h_offsets, d_p4, d_offsets, d_p, d_R, p_uni, R_uni, nullptr, nullptr, d_h1, d_h2,
+                      d_hd, d_jets, d_nj, out);
+}
+
+int fjmi_batch_create_def(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const double *d_p4,
+                          const int64_t *d_offsets, const fjmi_jetdef *d_defs, const fjmi_jetdef *def,
+                          const double *d_R, double R_uni, int32_t *d_h1, int32_t *d_h2, double *d_hd, double *d_jets,
+                          int32_t *d_nj, fjmi_batch **out) {
+  fjmi::JetDef d{};
+  if (!d_defs) {
+    const int rc = check_def(def, &d);
+    if (rc) return rc;
+  }
+  static_assert(sizeof(fjmi_jetdef) == sizeof(fjmi::JetDef) && offsetof(fjmi_jetdef, scheme) == offsetof(fjmi::JetDef, scheme),
+                "fjmi_jetdef is the kernels' JetDef");
+  return batch_create(ctx, n_events, h_offsets, d_p4, d_offsets, nullptr, d_R, 0, R_uni,
+                      reinterpret_cast<const fjmi::JetDef *>(d_defs), d_defs ? nullptr : &d, d_h1, d_h2, d_hd, d_jets,
+                      d_nj, out);
 }
 
 int fjmi_batch_launch(fjmi_batch *b, void *stream) {

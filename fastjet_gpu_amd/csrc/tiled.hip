@@ -260,9 +260,11 @@ struct Ev {
  * fjmi_phi and fjmi_weight.  In the common case (finite nonzero px and py, a
  * normal positive log argument) both go through the branch-free cores in one
  * straight-line block, so that the two dependent chains interleave. */
-__device__ __forceinline__ double4 kin_coords_t(double px, double py, double pz, double E, int p) {
+template <bool GEN = false>
+__device__ __forceinline__ double4 kin_coords_t(double px, double py, double pz, double E, int p, double pd = 0.0) {
   const double kt2 = px * px + py * py;
-  const double w = fjmi_weight(kt2, p);
+  /* GEN: the genkt weight for a real exponent pd (fjmi_weight_gen) */
+  const double w = GEN ? fjmi_weight_gen(kt2, pd) : fjmi_weight(kt2, p);
   const double apz = __builtin_fabs(pz);
   const double m2 = (E + pz) * (E - pz) - kt2;
   const double em2 = m2 > 0.0 ? m2 : 0.0;
@@ -291,6 +293,37 @@ __device__ __forceinline__ double4 load_in(const LaunchArgs &a, int64_t base, in
 
 __device__ __forceinline__ double diJ_of(double nnd, double w, int nn, double wnn) {
   return nnd * ((nn >= 0 && wnn < w) ? wnn : w);
+}
+
+/* ---- general jet definitions (the tiled_gen_kernel variants): any real
+ * genkt exponent and FastJet's recombination schemes ---- */
+
+/* an input particle as the recombination scheme takes it (fjmi_preprocess) */
+__device__ __forceinline__ double4 preprocess4(int scheme, double4 q) {
+  fjmi_preprocess(scheme, &q.x, &q.y, &q.z, &q.w);
+  return q;
+}
+
+/* The merged jet of the parents with the higher label (qh, its rapidity and
+ * azimuth rh) and the lower label (ql, rl): FastJet's N2Plain recombines jetA,
+ * the higher slot, with jetB.  Its (rapidity, azimuth, weight) go to *kin, as
+ * the scheme fixes them or from the merged momentum. */
+__device__ __forceinline__ double4 merge_gen(int scheme, double pd, double4 qh, double2 rh, double4 ql, double2 rl,
+                                             double4 *kin) {
+  const double qa[4] = {qh.x, qh.y, qh.z, qh.w}, qb[4] = {ql.x, ql.y, ql.z, ql.w};
+  double qo[4], y = 0.0, phi = 0.0;
+  const int fixed = fjmi_recombine(scheme, qa, rh.x, rh.y, qb, rl.x, rl.y, qo, &y, &phi);
+  const double4 q = make_double4(qo[0], qo[1], qo[2], qo[3]);
+  if (fixed)
+    *kin = make_double4(y, phi, fjmi_weight_gen(q.x * q.x + q.y * q.y, pd), 0.0);
+  else
+    *kin = kin_coords_t<true>(q.x, q.y, q.z, q.w, 0, pd);
+  return q;
+}
+
+/* a jet definition the kernel can run: finite exponent, known scheme */
+__device__ __forceinline__ bool jetdef_ok(double pd, int scheme) {
+  return pd == pd && pd > -1e300 && pd < 1e300 && scheme >= 0 && scheme < FJMI_NUM_SCHEMES;
 }
 
 /* Segment capacity = count + (count >> FJMI_TILED_SLACK_SHIFT) +
@@ -777,788 +810,28 @@ __device__ __forceinline__ void serve_finish(const LaunchArgs &a, int slot, int6
   }
 }
 
-/* MAXT tiles per event; at least WPE waves per SIMD (the register budget:
- * 512 / WPE VGPRs per lane).  SERVE: events come from the host-published
- * descriptor ring of LaunchArgs::serve instead of a fixed queue, and each
- * event's results are copied to host memory and announced on completion. */
+/* The tiled kernels, whose body is tiled_body.inc: MAXT tiles per event; at least WPE waves per
+ * SIMD (the register budget: 512 / WPE VGPRs per lane).  SERVE: events come
+ * from the host-published descriptor ring of LaunchArgs::serve instead of a
+ * fixed queue, and each event's results are copied to host memory and
+ * announced on completion.  GEN: the jet definition is LaunchArgs::def_arr[e]
+ * (or def_uni), any real genkt exponent with a recombination scheme, instead
+ * of p in {1, 0, -1} with the E-scheme.
+ * The body is included into each kernel rather than called as a forced-inline
+ * function: inlined from a function, the same code came out with more spilled
+ * registers (18 -> 23 VGPRs on the 256-tile kernel). */
 template <int MAXT, int WPE, bool SERVE>
 __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScratch ts) {
-  static_assert(MAXT % W == 0, "the argmin loads MAXT / W tile minima per lane");
-  __shared__ Shared<MAXT> sh;
-  const int lane = threadIdx.x;
-  unsigned stampv = 0; /* gather counter; the stamps are cleared below it at each event */
-  int pass = 0;        /* a launch without a queue head (one event) takes queue entry 0 and leaves */
-  for (;;) {
-    wave_sync();
-    if (lane == 0) {
-      if constexpr (SERVE) {
-        const long long h = serve_claim(a.serve);
-        sh.ev = h < 0 ? -1 : 0;
-        if (h >= 0) {
-          ServeDesc *d = const_cast<ServeDesc *>(a.serve.ring) + (unsigned long long)h % a.serve.ring_cap;
-          const unsigned want = serve_seq((unsigned long long)h);
-          const unsigned got = __hip_atomic_load(&d->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-          sh.sd.base = d->base;
-          sh.sd.pool_off = d->pool_off;
-          sh.sd.n = d->n;
-          sh.sd.p = d->p;
-          sh.sd.R = d->R;
-          sh.sd.slot = d->slot;
-          if (got != want) { /* the host broke the ring protocol: fail the launch's status, skip nothing */
-            atomicOr(a.status, (unsigned)STATUS_INTERNAL);
-          }
-          /* the cell may be written again once the copy above is complete */
-          __hip_atomic_store(&d->seq, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-        }
-      } else {
-        sh.ev = a.head ? (int)(atomicAdd(a.head, 1u) - a.head_base) : pass;
-      }
-    }
-    ++pass;
-    wave_sync();
-    const int qi = uni(sh.ev);
-    if (SERVE ? qi < 0 : (qi < 0 || qi >= a.n_queue)) {
-      if (a.t_end && threadIdx.x == 0) atomicMax(a.t_end, (unsigned long long)wall_clock64());
-      break;
-    }
-    int e, N, p;
-    int64_t base, pool_off;
-    double R;
-    if constexpr (SERVE) {
-      e = uni(sh.sd.slot);
-      N = uni(sh.sd.n);
-      p = uni(sh.sd.p);
-      R = uni_d(sh.sd.R);
-      base = ((int64_t)(unsigned)uni((int)(sh.sd.base >> 32)) << 32) | (unsigned)uni((int)sh.sd.base);
-      pool_off = ((int64_t)(unsigned)uni((int)(sh.sd.pool_off >> 32)) << 32) | (unsigned)uni((int)sh.sd.pool_off);
-    } else {
-      e = a.queue ? uni(a.queue[qi]) : qi;
-      base = a.offsets ? a.offsets[e] : 0;
-      N = a.offsets ? uni((int)(a.offsets[e + 1] - base)) : a.n_single;
-      p = a.p_arr ? a.p_arr[e] : a.p_uni;
-      R = a.R_arr ? a.R_arr[e] : a.R_uni;
-      pool_off = 3 * base + (int64_t)e * ts.pool_per_event;
-    }
-    if (N > kTiledMaxN) { /* the host rejects such events; never clustered here */
-      if (lane == 0) atomicOr(a.status, (unsigned)STATUS_CAPACITY);
-      if constexpr (SERVE) {
-        serve_finish(a, e, base, 0, -1, lane);
-      } else if (lane == 0) {
-        a.njets[e] = -1; /* failed */
-      }
-      continue;
-    }
-    if (N <= 0) {
-      if constexpr (SERVE) {
-        serve_finish(a, e, base, 0, 0, lane);
-      } else if (lane == 0) {
-        a.njets[e] = 0;
-      }
-      continue;
-    }
-    const double R2 = R * R;
-    const double invR2 = 1.0 / R2;
-    for (int t = lane; t < MAXT; t += W) sh.stamp[t] = 0;
-    stampv = 0;
-    Ev ev;
-    ev.rec = ts.rec + base;
-    ev.p4 = reinterpret_cast<double4 *>(ts.p4) + base;
-    ev.hist = ts.hist + base;
-    ev.lab2 = ts.lab2 + base;
-    ev.pool = ts.pool + pool_off;
-    ev.pool_size = 3 * N + ts.pool_per_event;
+  constexpr bool GEN = false;
+#include "tiled_body.inc"
+}
 
-    /* ---- prologue.  Jets are laid out in tile order: particle s (label s)
-     * gets the physical index i = (members of lower tiles) + (its rank in
-     * its tile), so that the members of a tile, and of a row of adjacent
-     * tiles, are contiguous records.  Labels, not physical indices, decide
-     * ties, so the layout does not change any result. ----
-     * 1. kinematics into a staging copy indexed by s (the p4 array, which is
-     *    written in tile order only in step 4), and the rapidity range */
-    double4 *stage = ev.p4;
-    /* loads in flight per lane in the prologue's passes (PB particles) and
-     * scans (NB candidates); fewer where more waves share the registers */
-    constexpr int PB = WPE <= 2 ? 4 : (WPE == 3 ? 2 : 1);
-    constexpr int kScanChunks = WPE <= 2 ? 4 : 1; /* chunks per batch in the orphan rescans */
-    constexpr int NB = WPE <= 2 ? 8 : (WPE == 3 ? 4 : 2);
-    double ylo = 1e300, yhi = -1e300;
-    /* the next particle's momentum is loaded before this one's kinematics */
-    double4 qn = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (lane < N) qn = load_in(a, base, lane);
-    for (int s = lane; s < N; s += W) {
-      const double4 q4 = qn;
-      if (s + W < N) qn = load_in(a, base, s + W);
-      const double4 c = kin_coords_t(q4.x, q4.y, q4.z, q4.w, p);
-      stage[s] = make_double4(c.x, c.y, c.z, 0.0);
-      if (c.x > -1e4 && c.x < 1e4) {
-        ylo = c.x < ylo ? c.x : ylo;
-        yhi = c.x > yhi ? c.x : yhi;
-      }
-    }
-    {
-      const double mn = wave_min_d(ylo);
-      const double mx = -wave_min_d(-yhi);
-      if (lane == 0) {
-        /* side >= R: the 3x3 neighbourhood of a tile holds every jet within R of its members */
-        const double side = R * ((a.tile_q > 4 ? a.tile_q : 4) * 0.25) * (1.0 + 0x1p-20);
-        double lo = mn, hi = mx;
-        if (!(lo <= hi)) {
-          lo = 0.0;
-          hi = 0.0;
-        }
-        int np = (int)(FJMI_TWOPI / side);
-        np = np < 1 ? 1 : (np > MAXT ? MAXT : np);
-        int nyc = (int)((hi - lo) / side) + 1;
-        const int nymax = MAXT / np;
-        nyc = nyc > nymax ? nymax : (nyc < 1 ? 1 : nyc);
-        sh.ny = nyc;
-        sh.nphi = np;
-        sh.y0 = lo;
-        sh.sy = (hi - lo) / nyc > side ? (hi - lo) / nyc : side;
-        sh.sphi = FJMI_TWOPI / np;
-      }
-    }
-    wave_sync();
-    Geo geo;
-    geo.ny = uni(sh.ny);
-    geo.nphi = uni(sh.nphi);
-    geo.y0 = uni_d(sh.y0);
-    geo.sy = uni_d(sh.sy);
-    geo.sphi = uni_d(sh.sphi);
-    geo.rsy = 1.0 / geo.sy;
-    geo.rsphi = 1.0 / geo.sphi;
-    geo.rnphi = 1.0f / (float)geo.nphi;
-    const int ntiles = geo.ny * geo.nphi;
-    for (int t = lane; t < ntiles; t += W) sh.seg_count[t] = 0;
-    wave_sync();
-    /* 2. each particle's tile and its rank among the tile's members (the
-     *    order of equal-tile particles is immaterial: ties go by label) */
-    for (int s0 = lane; s0 < N; s0 += PB * W) { /* PB particles' loads at once */
-      double2 c[PB];
-#pragma unroll
-      for (int k = 0; k < PB; ++k)
-        if (s0 + k * W < N) c[k] = make_double2(stage[s0 + k * W].x, stage[s0 + k * W].y);
-#pragma unroll
-      for (int k = 0; k < PB; ++k)
-        if (s0 + k * W < N) {
-          const int t = geo.tile(c[k].x, c[k].y);
-          const int rank = atomicAdd(&sh.seg_count[t], 1);
-          stage[s0 + k * W].w = __hiloint2double(t, rank);
-        }
-    }
-    wave_sync();
-    /* 3. segments with 50% slack in the pool, and each tile's first physical
-     *    index (kept in tmin[t].p until the tile minima are set) */
-    {
-      int carry = 0, pcarry = 0;
-      for (int t0 = 0; t0 < ntiles; t0 += W) {
-        const int t = t0 + lane;
-        const int c = t < ntiles ? sh.seg_count[t] : 0;
-        const int cap = c + (c >> FJMI_TILED_SLACK_SHIFT) + FJMI_TILED_SLACK_ADD;
-        const int incl = wave_incl_scan(cap, lane);
-        const int pincl = wave_incl_scan(c, lane);
-        if (t < ntiles) {
-          sh.seg_start[t] = carry + incl - cap;
-          sh.seg_cap[t] = (unsigned short)min(cap, 0xFFFF);
-          sh.tmin[t].p = pcarry + pincl - c;
-        }
-        carry += __builtin_amdgcn_readlane(incl, 63);
-        pcarry += __builtin_amdgcn_readlane(pincl, 63);
-      }
-      if (lane == 0) {
-        sh.pool_top = carry;
-        sh.rebuild = 0;
-      }
-    }
-    wave_sync();
-    /* 4. the records in tile order */
-    for (int s0 = lane; s0 < N; s0 += PB * W) {
-      double4 c[PB];
-#pragma unroll
-      for (int k = 0; k < PB; ++k)
-        if (s0 + k * W < N) c[k] = stage[s0 + k * W];
-#pragma unroll
-      for (int k = 0; k < PB; ++k)
-        if (s0 + k * W < N) {
-          const int s = s0 + k * W;
-          const int t = __double2hiint(c[k].w), rank = __double2loint(c[k].w);
-          const int i = sh.tmin[t].p + rank;
-          const int pos = sh.seg_start[t] + rank;
-          ev.rec[i].rp = make_double2(c[k].x, c[k].y);
-          ev.rec[i].wn = make_double2(c[k].z, R2);
-          ev.rec[i].meta = make_int4(-1, s, t, pos);
-          ev.pool[pos] = i;
-          ev.lab2[s] = i;
-          ev.hist[i] = s;
-        }
-    }
-    wave_sync(); /* the staging copy is read in full before p4 is overwritten */
-    for (int s = lane; s < N; s += W) ev.p4[ev.lab2[s]] = load_in(a, base, s);
-
-    /* 5. initial nearest neighbours, one jet per lane in tile order.  The 3x3
-     *    neighbourhood of a tile is, per tile row, one contiguous range of
-     *    physical indices (two when the row wraps around in phi, the whole
-     *    row when it has at most 3 columns). */
-    for (int i = lane; i < N; i += W) {
-      const double2 rp = ev.rec[i].rp;
-      const double wi = ev.rec[i].wn.x;
-      const int ti = ev.rec[i].meta.z;
-      const int ty0 = geo.row(ti), tp0 = ti - ty0 * geo.nphi;
-      double bd = R2;
-      int bl = BIG, bp = -1;
-      for (int r = 0; r < 6; ++r) {
-        const int y = ty0 + r / 2 - 1;
-        if (y < 0 || y >= geo.ny) continue;
-        const int row = y * geo.nphi;
-        int c0, c1; /* columns [c0, c1] */
-        if (geo.nphi <= 3) {
-          if (r & 1) continue;
-          c0 = 0;
-          c1 = geo.nphi - 1;
-        } else if (tp0 == 0) {
-          c0 = (r & 1) ? geo.nphi - 1 : 0;
-          c1 = (r & 1) ? geo.nphi - 1 : 1;
-        } else if (tp0 == geo.nphi - 1) {
-          c0 = (r & 1) ? 0 : tp0 - 1;
-          c1 = (r & 1) ? 0 : tp0;
-        } else {
-          if (r & 1) continue;
-          c0 = tp0 - 1;
-          c1 = tp0 + 1;
-        }
-        const int j0 = sh.tmin[row + c0].p, j1 = sh.tmin[row + c1].p + sh.seg_count[row + c1];
-        /* candidates in blocks of NB, all of a block's loads issued together
-         * (indices past the range repeat its last member and are masked) */
-        for (int j = j0; j < j1; j += NB) {
-          double2 rq[NB];
-          int lq[NB];
-#pragma unroll
-          for (int k = 0; k < NB; ++k) {
-            const int jk = min(j + k, j1 - 1);
-            rq[k] = ev.rec[jk].rp;
-            lq[k] = ev.rec[jk].meta.y;
-          }
-#pragma unroll
-          for (int k = 0; k < NB; ++k) {
-            const double d = fjmi_dr2(rp.x, rp.y, rq[k].x, rq[k].y);
-            const bool take = (j + k < j1) & (j + k != i) & ((d < bd) | ((d == bd) & (d < R2) & (lq[k] < bl)));
-            bd = take ? d : bd;
-            bl = take ? lq[k] : bl;
-            bp = take ? j + k : bp;
-          }
-        }
-      }
-      const double bw = bp >= 0 ? ev.rec[bp].wn.x : 0.0;
-      ev.rec[i].meta.x = bp;
-      ev.rec[i].wn = make_double2(wi, bd);
-      ev.rec[i].dij = diJ_of(bd, wi, bp, bw);
-    }
-    wave_sync();
-    /* 6. tile minima, one tile per lane over its contiguous members */
-    for (int t = lane; t < ntiles; t += W) {
-      const int j0 = sh.tmin[t].p, j1 = j0 + sh.seg_count[t];
-      unsigned long long bd = ~0ull;
-      int bl = BIG, bp = -1;
-      for (int j = j0; j < j1; j += NB / 2) {
-        unsigned long long dq[NB / 2];
-        int lq[NB];
-#pragma unroll
-        for (int k = 0; k < NB / 2; ++k) {
-          const int jk = min(j + k, j1 - 1);
-          dq[k] = (unsigned long long)__double_as_longlong(ev.rec[jk].dij);
-          lq[k] = tkey(ev.rec[jk].meta.y, ev.rec[jk].meta.x);
-        }
-#pragma unroll
-        for (int k = 0; k < NB / 2; ++k) {
-          const bool take = (j + k < j1) & ((dq[k] < bd) | ((dq[k] == bd) & (lq[k] < bl)));
-          bd = take ? dq[k] : bd;
-          bl = take ? lq[k] : bl;
-          bp = take ? j + k : bp;
-        }
-      }
-      sh.tmin[t].d = bd;
-      sh.tmin[t].l = bl;
-      sh.tmin[t].p = bp;
-    }
-    wave_sync();
-    Gather &G = sh.g[0];
-    Gather &GO = sh.g[1];
-
-    /* ---- the N recombination steps ---- */
-    int nj = 0;
-    bool bad = false;
-    int step = 0;
-    for (; step < N; ++step) {
-      if ((step & (W - 1)) == 0 && step > 0) { /* the previous W history entries, one coalesced store each */
-        wave_sync();
-        a.hist_p1[base + step - W + lane] = sh.hb_p1[lane];
-        a.hist_p2[base + step - W + lane] = sh.hb_p2[lane];
-        a.hist_dij[base + step - W + lane] = sh.hb_dij[lane];
-      }
-      const int n_old = N - step;
-      /* the jet with the last label moves in this step (N2Plain compaction); its
-       * index is loaded first so the load overlaps the argmin */
-      const int tail = n_old - 1;
-      const int pt_raw = ev.lab2[tail];
-      /* 1. argmin over the tile minima: every lane loads all its MAXT/W entries
-       * at once (always in bounds; entries past ntiles are masked), then a
-       * pairwise tree of lexicographic (diJ, label) compares */
-      constexpr int KT = MAXT / W;
-      double kd[KT];
-      int kl[KT], kt[KT], kp[KT];
-      uint4 raw[KT];
-#pragma unroll
-      for (int k = 0; k < KT; ++k) raw[k] = *reinterpret_cast<const uint4 *>(&sh.tmin[k * W + lane]);
-#pragma unroll
-      for (int k = 0; k < KT; ++k) {
-        const int l = (int)raw[k].z;
-        const bool ok = (k * W + lane < ntiles) & (l != BIG);
-        kd[k] = ok ? __hiloint2double((int)raw[k].y, (int)raw[k].x) : __builtin_inf();
-        kl[k] = ok ? l : BIG;
-        kt[k] = k * W + lane;
-        kp[k] = (int)raw[k].w;
-      }
-#pragma unroll
-      for (int w = 1; w < KT; w *= 2) {
-#pragma unroll
-        for (int k = 0; k + w < KT; k += 2 * w) {
-          const bool c = (kd[k + w] < kd[k]) | ((kd[k + w] == kd[k]) & (kl[k + w] < kl[k]));
-          kd[k] = c ? kd[k + w] : kd[k];
-          kl[k] = c ? kl[k + w] : kl[k];
-          kt[k] = c ? kt[k + w] : kt[k];
-          kp[k] = c ? kp[k + w] : kp[k];
-        }
-      }
-      const double bd = kd[0];
-      const int bl = kl[0], bt = kt[0];
-      double dmin;
-      int lA;
-      const int wl = wave_argmin(bd, bl, &dmin, &lA);
-      const int tA = wl < 0 ? -1 : uni(__builtin_amdgcn_readlane(bt, wl));
-      const int A = tA < 0 ? -1 : uni(__builtin_amdgcn_readlane(kp[0], wl));
-      const int B = tA < 0 ? -1 : uni(tkey_nn(lA));
-      if (A < 0) { /* no live jet found: inconsistent state, stop this event */
-        if (lane == 0) atomicOr(a.status, 2u);
-        bad = true;
-        break;
-      }
-      /* 2. the merge.  The partner B comes with the tile minimum, so both
-       * jets' records load at once.  Every value loaded for the merge is taken
-       * into registers before the merge's first store: a wait for a load also
-       * waits for every store issued before it. */
-      const int4 mA = ev.rec[A].meta;
-      const double4 qA = ev.p4[A];
-      const int hA = ev.hist[A];
-      int4 mB = make_int4(-1, -1, -1, -1);
-      double4 qB = make_double4(0.0, 0.0, 0.0, 0.0);
-      int hB = -1;
-      if (B >= 0) {
-        mB = ev.rec[B].meta;
-        qB = ev.p4[B];
-        hB = ev.hist[B];
-      }
-      const int pt = uni(pt_raw);
-      const int tile_pt_raw = ev.rec[pt].meta.z;
-      double4 qt_raw = make_double4(0.0, 0.0, 0.0, 0.0);
-      if (a.pj_out) qt_raw = ev.p4[pt];
-      const bool a_hi = B < 0 || uni(mA.y) > uni(mB.y);
-      const int phys_hi = a_hi ? A : B;
-      const int phys_lo = B < 0 ? -1 : (a_hi ? B : A);
-      const int lab_hi = uni(a_hi ? mA.y : mB.y);
-      const int tile_hi = uni(a_hi ? mA.z : mB.z);
-      const int tile_lo_old = phys_lo < 0 ? -1 : uni(a_hi ? mB.z : mA.z);
-      /* Segment removals swap the removed jet with its segment's last entry.  The
-       * pool entries that may be needed are loaded now, by every lane, so that
-       * their latency overlaps the new jet's kinematics:
-       *   removal 1: phys_hi leaves tile_hi (last entry at last_hi);
-       *   removal 2: phys_lo leaves tile_lo_old if the merged jet changes tile;
-       *   when both are in one tile, removal 2 sees the segment as removal 1
-       *   left it (one shorter, phys_lo possibly moved into phys_hi's place). */
-      const int pos_hi = uni(a_hi ? mA.w : mB.w);
-      const int pos_lo = phys_lo < 0 ? -1 : uni(a_hi ? mB.w : mA.w);
-      const int cnt_hi = uni(sh.seg_count[tile_hi]);
-      const int last_hi = uni(sh.seg_start[tile_hi]) + cnt_hi - 1;
-      const int q_last_hi_raw = ev.pool[last_hi];
-      const bool same_tile = phys_lo >= 0 && tile_lo_old == tile_hi;
-      int last2 = -1, q_last2_raw = -1;
-      if (phys_lo >= 0) {
-        last2 = uni(sh.seg_start[tile_lo_old]) + uni(sh.seg_count[tile_lo_old]) - 1 - (same_tile ? 1 : 0);
-        q_last2_raw = ev.pool[last2];
-      }
-
-      double nrap = 0.0, nphi = 0.0, nw = 0.0;
-      int tile_new = -1;
-      const double4 qhi = a_hi ? qA : qB;
-      const int hhi = a_hi ? hA : hB;
-      const double4 qlo = a_hi ? qB : qA;
-      const int hlo = a_hi ? hB : hA;
-      const double4 qn = make_double4(qhi.x + qlo.x, qhi.y + qlo.y, qhi.z + qlo.z, qhi.w + qlo.w);
-      if (phys_lo >= 0) {
-        const double4 c = kin_coords_t(qn.x, qn.y, qn.z, qn.w, p);
-        nrap = c.x;
-        nphi = c.y;
-        nw = c.z;
-        tile_new = uni(geo.tile(nrap, nphi));
-      }
-      const int tile_pt = uni(tile_pt_raw);
-      const int q_last_hi = uni(q_last_hi_raw);
-      const int q_last2_v = uni(q_last2_raw);
-      const double4 qt = qt_raw;
-
-      /* the stores: the history entry (to the LDS block) and the new jet's
-       * record, then removal 1 and the N2Plain compaction (the last label
-       * moves), then, if the merged jet changes tile, removal 2 (from the
-       * segment as removal 1 left it) and its insertion.  The stores nothing
-       * later in this step reads (the new jet's four-momentum and history
-       * index, the beam jet, the removed jet's record, the label map, the
-       * drop-in record) wait until the members' loads have been issued, so
-       * that those loads do not wait for them (loads and stores share one
-       * counter). */
-      bool need_rebuild = false; /* lane 0's segment insertion exhausted the pool */
-      const int jet_slot = nj;
-      if (lane == 0) {
-        if (phys_lo >= 0) {
-          sh.hb_p1[step & (W - 1)] = min(hhi, hlo);
-          sh.hb_p2[step & (W - 1)] = max(hhi, hlo);
-          sh.hb_dij[step & (W - 1)] = dmin * invR2;
-          ev.rec[phys_lo].rp = make_double2(nrap, nphi);
-          ev.rec[phys_lo].wn = make_double2(nw, R2);
-          ev.rec[phys_lo].meta.x = -1;
-        } else {
-          sh.hb_p1[step & (W - 1)] = hhi;
-          sh.hb_p2[step & (W - 1)] = -1;
-          sh.hb_dij[step & (W - 1)] = dmin * invR2;
-        }
-        if (pos_hi != last_hi) {
-          ev.pool[pos_hi] = q_last_hi;
-          ev.rec[q_last_hi].meta.w = pos_hi;
-        }
-        sh.seg_count[tile_hi] = cnt_hi - 1;
-        if (pt != phys_hi) ev.rec[pt].meta.y = lab_hi;
-        if (phys_lo >= 0 && tile_new != tile_lo_old) {
-          const bool moved = same_tile && pos_lo == last_hi && pos_hi != last_hi;
-          const int pos2 = moved ? pos_hi : pos_lo;
-          const int q_last2 = (same_tile && last2 == pos_hi) ? q_last_hi : q_last2_v;
-          if (pos2 != last2) {
-            ev.pool[pos2] = q_last2;
-            ev.rec[q_last2].meta.w = pos2;
-          }
-          sh.seg_count[tile_lo_old] -= 1;
-          ev.rec[phys_lo].meta.z = tile_new;
-          need_rebuild = seg_insert(sh, ev, tile_new, phys_lo);
-        }
-      }
-      auto deferred_stores = [&]() {
-        if (lane != 0) return;
-        if (phys_lo >= 0) {
-          ev.p4[phys_lo] = qn;
-          ev.hist[phys_lo] = N + step;
-        } else {
-          double *jo = a.jets + 4 * (base + jet_slot);
-          jo[0] = qhi.x;
-          jo[1] = qhi.y;
-          jo[2] = qhi.z;
-          jo[3] = qhi.w;
-        }
-        ev.rec[phys_hi].meta = make_int4(-1, -1, -1, -1);
-        if (pt != phys_hi) ev.lab2[lab_hi] = pt;
-        if (a.pj_out) {
-          unsigned char *rec = a.pj_out + 40 * (size_t)(base + tail);
-          reinterpret_cast<int *>(rec)[0] = tail;
-          reinterpret_cast<int *>(rec)[1] = phys_lo < 0 ? 1 : 0;
-          double *dp = reinterpret_cast<double *>(rec + 8);
-          const double4 o = phys_lo < 0 ? qhi : qt;
-          dp[0] = o.x;
-          dp[1] = o.y;
-          dp[2] = o.z;
-          dp[3] = o.w;
-        }
-      };
-      if (phys_lo < 0) ++nj;
-      wave_sync();
-      const int n = n_old - 1;
-      if (n == 0) {
-        deferred_stores();
-        ++step; /* this step's entry is recorded */
-        break;
-      }
-      bool deferred_done = false;
-      if (__builtin_amdgcn_readlane((int)need_rebuild, 0)) { /* pool exhausted by a relocation: lay all segments out again */
-        deferred_stores(); /* the rebuild reads the label map */
-        deferred_done = true;
-        wave_sync();
-        rebuild_segments(sh, ev, n, ntiles, lane);
-      }
-
-      /* 3. gather the 3x3 tiles around the old and new positions and the relabelled jet's tile */
-      int total;
-      const int ng = build_gather(sh, G, geo, tile_hi, tile_lo_old, tile_new, tile_pt, lane, stampv, &total);
-      const unsigned main_stamp = stampv;
-      if (lane == 0) {
-        sh.norph = 0;
-        sh.nchg = 0;
-      }
-      wave_sync();
-      double cbd = R2, cbw = 0.0;
-      int cbl = BIG, cbp = -1;
-      /* a gather of at most 64 members keeps each member's (index, label, diJ)
-       * in its lane for the tile-minimum refresh */
-      const bool single = total <= W;
-      int myq = -1, mylab = BIG, mytile = -1, mynn = -1;
-      unsigned long long mybits = ~0ull;
-      double2 myrp = make_double2(0.0, 0.0);
-      double myw = 0.0;
-      unsigned long long orph_lanes = 0; /* single-chunk gathers: lanes holding orphans */
-      int q_next = gathered(sh, G, ev, ng, lane);
-      for (int j0 = 0; j0 < total; j0 += W) {
-        const int q = q_next;
-        if (j0 + W < total) q_next = gathered(sh, G, ev, ng, j0 + W + lane); /* the next chunk's pool entries load meanwhile */
-        bool orph = false;
-        if (q >= 0) { /* every member's record, kept in its lane for the orphan rescans */
-          const JetRec &r = ev.rec[q];
-          const int4 mq0 = r.meta;
-          myq = q;
-          mylab = mq0.y;
-          mytile = mq0.z;
-          mynn = mq0.x;
-          myrp = r.rp;
-          myw = r.wn.x;
-        }
-        bool chg = false; /* multi-chunk gathers: the member's tile-minimum entry changed */
-        int chg_key = 0, chg_tile = -1;
-        if (q >= 0 && q != phys_lo) {
-          int4 mq = ev.rec[q].meta;
-          double dq = ev.rec[q].dij;
-          orph = (mq.x == phys_hi) || (phys_lo >= 0 && mq.x == phys_lo);
-          chg = q == pt && !orph; /* relabelled */
-          if (phys_lo >= 0) {
-            const double2 rq = ev.rec[q].rp;
-            const double2 wq = ev.rec[q].wn;
-            const double d = fjmi_dr2(rq.x, rq.y, nrap, nphi);
-            if (!orph && d < wq.y) {
-              dq = d * (nw < wq.x ? nw : wq.x);
-              ev.rec[q].meta.x = phys_lo;
-              ev.rec[q].wn = make_double2(wq.x, d);
-              ev.rec[q].dij = dq;
-              mynn = phys_lo;
-              chg = true;
-            }
-            if (d < R2 && (d < cbd || (d == cbd && mq.y < cbl))) {
-              cbd = d;
-              cbl = mq.y;
-              cbp = q;
-              cbw = wq.x;
-            }
-          }
-          mybits = (unsigned long long)__double_as_longlong(dq);
-          chg_key = tkey(mq.y, mynn);
-          chg_tile = mq.z;
-        }
-        if (!single) { /* list the changed entries for the incremental refresh */
-          const unsigned long long cb = __ballot(chg);
-          if (cb) {
-            int cbase = 0;
-            if (lane == 0) cbase = atomicAdd(&sh.nchg, (int)__popcll(cb));
-            cbase = uni(cbase);
-            const int idx = cbase + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(cb >> 32),
-                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)cb, 0u));
-            if (chg && idx < W) {
-              sh.mt_d[idx] = mybits;
-              sh.mt_l[idx] = chg_key;
-              sh.mt_p[idx] = q;
-              sh.mt_t[idx] = chg_tile;
-            }
-          }
-        }
-        const unsigned long long ob = __ballot(orph);
-        orph_lanes = ob;
-        if (!single && ob) { /* single-chunk gathers keep their orphans in orph_lanes */
-          int obase = 0;
-          if (lane == 0) obase = atomicAdd(&sh.norph, (int)__popcll(ob));
-          obase = uni(obase);
-          const int idx = obase + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(ob >> 32),
-                                                                 __builtin_amdgcn_mbcnt_lo((unsigned)ob, 0u));
-          if (orph && idx < MAXO) sh.orph[idx] = q;
-        }
-      }
-      if (!deferred_done) deferred_stores();
-      if (phys_lo >= 0) { /* the new jet's own nearest neighbour */
-        double md;
-        int ml;
-        const int ln = wave_argmin(cbd, cbl, &md, &ml);
-        const int nnB = ln < 0 ? -1 : __builtin_amdgcn_readlane(cbp, ln);
-        const double wB = ln < 0 ? 0.0 : readlane_d(cbw, ln);
-        const double dB = diJ_of(md, nw, nnB, wB);
-        if (lane == 0) {
-          ev.rec[phys_lo].meta.x = nnB;
-          ev.rec[phys_lo].wn = make_double2(nw, md);
-          ev.rec[phys_lo].dij = dB;
-        }
-        if (myq == phys_lo) {
-          mybits = (unsigned long long)__double_as_longlong(dB);
-          mynn = nnB;
-        }
-        const int lab_lo = phys_lo == pt ? lab_hi : uni(a_hi ? mB.y : mA.y); /* after the relabelling */
-        if (!single && lane == 0) {
-          const int i = sh.nchg;
-          sh.nchg = i + 1;
-          if (i < W) {
-            sh.mt_d[i] = (unsigned long long)__double_as_longlong(dB);
-            sh.mt_l[i] = tkey(lab_lo, nnB);
-            sh.mt_p[i] = phys_lo;
-            sh.mt_t[i] = tile_new;
-          }
-        }
-      }
-      wave_sync();
-
-      /* 4. orphan rescans over their own 3x3 tiles.  With a single-chunk gather
-       * whose tiles hold the orphan's whole 3x3 neighbourhood, the candidates
-       * are already in the lanes' registers: one distance per lane and one
-       * wave argmin (gathered jets outside the neighbourhood are farther than
-       * R and drop out).  Otherwise the neighbourhood is gathered afresh. */
-      const int norph = single ? (int)__popcll(orph_lanes) : uni(sh.norph);
-      if (norph > 0 && single) {
-        unsigned long long pend = orph_lanes;
-        while (pend) {
-          const int lo = __ffsll(pend) - 1;
-          pend &= pend - 1;
-          const int o = __builtin_amdgcn_readlane(myq, lo);
-          const int otile = __builtin_amdgcn_readlane(mytile, lo);
-          const double oy = readlane_d(myrp.x, lo), ophi = readlane_d(myrp.y, lo), ow = readlane_d(myw, lo);
-          /* the nearest gathered candidate, from the lanes' registers */
-          const bool cand = myq >= 0 && myq != o;
-          const double dc = cand ? fjmi_dr2(oy, ophi, myrp.x, myrp.y) : R2;
-          const bool in = cand && dc < R2;
-          double md;
-          int ml;
-          const int ln = wave_argmin(in ? dc : R2, in ? mylab : BIG, &md, &ml);
-          /* It is the orphan's nearest neighbour unless a tile of its 3x3
-           * neighbourhood that the gather missed could hold a jet at distance
-           * <= md: lane i < 9 bounds from below the distance to neighbour tile
-           * i (the orphan's distance to its own tile's edges on that side, less
-           * a margin far above the rounding of the tile boundaries). */
-          bool open = false;
-          if (lane < 9) {
-            const int ty = geo.row(otile), tp = otile - ty * geo.nphi, dy = lane / 3 - 1, dp = lane % 3 - 1;
-            const int y = ty + dy;
-            if (y >= 0 && y < geo.ny) {
-              const int t = y * geo.nphi + geo.wrap(tp + dp);
-              if (sh.stamp[t] != main_stamp) {
-                double by = dy < 0 ? oy - (geo.y0 + ty * geo.sy) : (dy > 0 ? geo.y0 + (ty + 1) * geo.sy - oy : 0.0);
-                double bp = dp < 0 ? ophi - tp * geo.sphi : (dp > 0 ? (tp + 1) * geo.sphi - ophi : 0.0);
-                by = dy != 0 ? (by - 1e-10 > 0.0 ? by - 1e-10 : 0.0) : 0.0;
-                bp = dp != 0 ? (bp - 1e-10 > 0.0 ? bp - 1e-10 : 0.0) : 0.0;
-                open = !(by * by + bp * bp > md);
-              }
-            }
-          }
-          double od, owb;
-          int nno;
-          if (__ballot(open) == 0) {
-            od = md;
-            nno = ln < 0 ? -1 : __builtin_amdgcn_readlane(myq, ln);
-            owb = ln < 0 ? 0.0 : readlane_d(myw, ln);
-          } else {
-            const int ngo = build_gather(sh, GO, geo, otile, -1, -1, -1, lane, stampv);
-            nno = nn_scan<kScanChunks>(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &od, &owb);
-          }
-          const double dO = diJ_of(od, ow, nno, owb);
-          if (lane == 0) {
-            ev.rec[o].meta.x = nno;
-            ev.rec[o].wn = make_double2(ow, od);
-            ev.rec[o].dij = dO;
-          }
-          if (lane == lo) {
-            mybits = (unsigned long long)__double_as_longlong(dO);
-            mynn = nno;
-          }
-        }
-        wave_sync();
-      } else if (norph > 0) {
-        const int nlist = min(norph, MAXO);
-        for (int i = 0; i < nlist; ++i) {
-          const int o = uni(sh.orph[i]);
-          const int4 mo = ev.rec[o].meta;
-          const double2 ro = ev.rec[o].rp;
-          const double wo = ev.rec[o].wn.x;
-          double od, owb;
-          const int nno = nn_rescan<kScanChunks>(sh, GO, geo, ev, o, mo.z, ro.x, ro.y, R2, lane, stampv, &od, &owb);
-          const double dO = diJ_of(od, wo, nno, owb);
-          if (lane == 0) {
-            ev.rec[o].meta.x = nno;
-            ev.rec[o].wn = make_double2(wo, od);
-            ev.rec[o].dij = dO;
-            const int c = sh.nchg;
-            sh.nchg = c + 1;
-            if (c < W) {
-              sh.mt_d[c] = (unsigned long long)__double_as_longlong(dO);
-              sh.mt_l[c] = tkey(mo.y, nno);
-              sh.mt_p[c] = o;
-              sh.mt_t[c] = mo.z;
-            }
-          }
-          if (myq == o) {
-            mybits = (unsigned long long)__double_as_longlong(dO);
-            mynn = nno;
-          }
-          wave_sync();
-        }
-        if (norph > MAXO) { /* more orphans than the list holds: rescan the rest from the gather */
-          const int ng2 = ng;
-          const int tot2 = G.pref[ng2];
-          for (int j0 = 0; j0 < tot2; j0 += W) {
-            const int q = gathered(sh, G, ev, ng2, j0 + lane);
-            const bool orph = q >= 0 && q != phys_lo &&
-                              (ev.rec[q].meta.x == phys_hi || (phys_lo >= 0 && ev.rec[q].meta.x == phys_lo));
-            unsigned long long ob = __ballot(orph);
-            while (ob) {
-              const int ln = __ffsll(ob) - 1;
-              ob &= ob - 1;
-              const int o = __builtin_amdgcn_readlane(q, ln);
-              const int4 mo = ev.rec[o].meta;
-              const double2 ro = ev.rec[o].rp;
-              const double wo = ev.rec[o].wn.x;
-              const int ngo = build_gather(sh, GO, geo, mo.z, -1, -1, -1, lane, stampv);
-              double od, owb;
-              const int nno = nn_scan<kScanChunks>(sh, GO, ev, ngo, o, ro.x, ro.y, R2, lane, &od, &owb);
-              if (lane == 0) {
-                ev.rec[o].meta.x = nno;
-                ev.rec[o].wn = make_double2(wo, od);
-                ev.rec[o].dij = diJ_of(od, wo, nno, owb);
-              }
-              wave_sync();
-            }
-          }
-        }
-      }
-
-      /* 5. tile minima of the gathered tiles: from the member table (single
-       * chunk), incrementally from the changed entries (several chunks), or
-       * from all members */
-      if (single && norph <= MAXO)
-        refresh_from_regs(sh, G, ng, total, lane, myq, mytile, tkey(mylab, mynn), mybits);
-      else if (norph <= MAXO && uni(sh.nchg) <= W)
-        refresh_changed(sh, G, GO, ev, ng, phys_hi, lane, stampv);
-      else
-        refresh_minima(sh, G, ev, ng, lane);
-    }
-    { /* the history entries recorded since the last full block */
-      wave_sync();
-      const int done = step;
-      const int first = (done - 1) & ~(W - 1);
-      if (done > 0 && first + lane < done) {
-        a.hist_p1[base + first + lane] = sh.hb_p1[lane];
-        a.hist_p2[base + first + lane] = sh.hb_p2[lane];
-        a.hist_dij[base + first + lane] = sh.hb_dij[lane];
-      }
-    }
-    if constexpr (SERVE) {
-      serve_finish(a, e, base, N, bad ? -1 : nj, lane);
-    } else if (lane == 0) {
-      a.njets[e] = bad ? -1 : nj; /* -1: the clustering failed */
-    }
-  }
+/* general jet definitions: the same engine with a real genkt exponent and a
+ * recombination scheme per event (LaunchArgs::def_arr / def_uni) */
+template <int MAXT, int WPE>
+__global__ __launch_bounds__(64, WPE) void tiled_gen_kernel(LaunchArgs a, TiledScratch ts) {
+  constexpr bool SERVE = false, GEN = true;
+#include "tiled_body.inc"
 }
 
 /* Workgroups per CU, from 160 KiB of LDS and the register budget of the
@@ -1574,6 +847,10 @@ const void *const kTiledKernels[kNumTiledKernels] = {
 const void *const kTiledServeKernels[kNumTiledKernels] = {
     reinterpret_cast<const void *>(&tiled_kernel<512, 2, true>), reinterpret_cast<const void *>(&tiled_kernel<384, 3, true>),
     reinterpret_cast<const void *>(&tiled_kernel<320, 3, true>), reinterpret_cast<const void *>(&tiled_kernel<256, 3, true>)};
+/* general jet definitions: the same variants */
+const void *const kTiledGenKernels[kNumTiledKernels] = {
+    reinterpret_cast<const void *>(&tiled_gen_kernel<512, 2>), reinterpret_cast<const void *>(&tiled_gen_kernel<384, 3>),
+    reinterpret_cast<const void *>(&tiled_gen_kernel<320, 3>), reinterpret_cast<const void *>(&tiled_gen_kernel<256, 3>)};
 const int kTiledMaxTiles[kNumTiledKernels] = {512, 384, 320, 256};
 
 }  // namespace fjmi

@@ -223,6 +223,52 @@ typedef struct {
 } fjmi_bucket_stats;
 int fjmi_batch_stats(fjmi_batch *b, fjmi_bucket_stats *out, int cap, int *n_buckets);
 
+/* ---- General jet definitions --------------------------------------------------
+ * FastJet's genkt algorithm for any finite real exponent p (weight kt^(2p), with
+ * FastJet's kt^2 >= 1e-300 guard for p <= 0; p = 1, 0, -1 give kt, C/A and
+ * anti-kt bit for bit) under one of FastJet's recombination schemes (its
+ * JetDefinition::DefaultRecombiner: inputs preprocessed as the scheme requires,
+ * merged jets built by the scheme).  Not in the reference, which has p in
+ * {1, 0, -1} (cluster.h:4) and the E-scheme (tri_matrix.cu:247-250).  These
+ * calls run on the tiled engine (a dense row of a caller's table becomes the
+ * 384-tile row with side R); outputs as above, with the jets' momenta as the
+ * scheme builds them.  The scheme numbers are FastJet's RecombinationScheme
+ * values. */
+#define FJMI_SCHEME_E 0        /* four-momentum sum */
+#define FJMI_SCHEME_PT 1       /* massless (pt, y, phi), pt-weighted y and phi; inputs made massless, E = |p| */
+#define FJMI_SCHEME_PT2 2      /* as pt, pt^2-weighted */
+#define FJMI_SCHEME_ET 3       /* as pt; inputs made massless by rescaling |p| to E */
+#define FJMI_SCHEME_ET2 4      /* as Et, pt^2-weighted */
+#define FJMI_SCHEME_BIPT 5     /* as pt, inputs unchanged */
+#define FJMI_SCHEME_BIPT2 6    /* as pt2, inputs unchanged */
+#define FJMI_SCHEME_WTA_PT 7   /* winner takes all: the harder jet's y, phi and mass, pt = pta + ptb */
+#define FJMI_SCHEME_WTA_MODP 8 /* winner takes all: the harder jet's direction, |p| = |pa| + |pb| */
+#define FJMI_NUM_SCHEMES 9
+
+typedef struct {
+  double p;       /* genkt exponent, finite */
+  int32_t scheme; /* FJMI_SCHEME_* */
+  int32_t pad;    /* 0 */
+} fjmi_jetdef;
+
+/* One event, host memory in and out, synchronous (as fjmi_cluster_event). */
+int fjmi_cluster_event_def(fjmi_context *ctx, const double *p4, int n, const fjmi_jetdef *def, double R,
+                           int32_t *hist_p1, int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets);
+/* One event, device memory, asynchronous on `stream` (as fjmi_cluster_event_device;
+ * failures are reported by fjmi_stream_check). */
+int fjmi_cluster_event_device_def(fjmi_context *ctx, const double *d_p4, int n, const fjmi_jetdef *def, double R,
+                                  int32_t *d_hist_p1, int32_t *d_hist_p2, double *d_hist_dij, double *d_jets,
+                                  int32_t *d_njets, void *stream);
+/* A batch plan (as fjmi_batch_create) with one definition per event (d_defs,
+ * device memory, n_events entries) or one for all (def, host memory; d_defs
+ * NULL), and per-event radii d_R or R_uni.  An event whose device-side
+ * definition is invalid gets njets = -1 and fails fjmi_batch_sync with
+ * FJMI_EINVAL. */
+int fjmi_batch_create_def(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const double *d_p4,
+                          const int64_t *d_offsets, const fjmi_jetdef *d_defs, const fjmi_jetdef *def,
+                          const double *d_R, double R_uni, int32_t *d_hist_p1, int32_t *d_hist_p2,
+                          double *d_hist_dij, double *d_jets, int32_t *d_njets, fjmi_batch **out);
+
 /* Host-memory batch: copies in, clusters, copies out, synchronises.
  * Per-particle outputs are sized like p4/4 (total particles).  The events are
  * taken longest first in up to four chunks, each on its own stream with pinned

@@ -4,6 +4,7 @@
 //   tiled_emu WORKLOAD E0 E1 [VARIANT P R TILE_Q NMAX POOL_EXTRA POOL_PER_N]   generated events
 //   tiled_emu file PATH VARIANT P R                       raw little-endian px,py,pz,E doubles
 //   tiled_emu qbound                                      queue indices outside [0, n_queue) leave at once
+//   tiled_emu gen WORKLOAD E0 E1 VARIANT P SCHEME R       general jet definition (real p, scheme)
 // FJMI_EMU_HEADLESS=1 launches without a queue head (the one-event form).
 // The tile pool of an event holds POOL_PER_N * N + POOL_EXTRA entries (default
 // 3 N + kTiledPoolPerEvent, as the library allocates); a small pool, with a
@@ -29,8 +30,11 @@ void run_wave(const std::function<void()> &fn);
 static int g_pool_extra = fjmi::kTiledPoolPerEvent, g_pool_per_n = 3;
 static bool g_headless = false;
 
+/* def: a general jet definition (tiled_gen_kernel, checked against the
+ * oracle's *_def restatements), or null for p in {1, 0, -1} with the E-scheme */
 template <int MAXT, int WPE>
-static int run_event(const double *p4, int N, int p, double R, int tile_q, bool verbose) {
+static int run_event(const double *p4, int N, int p, double R, int tile_q, bool verbose,
+                     const fjmi::JetDef *def = nullptr) {
   using namespace fjmi;
   std::vector<JetRec> rec(N > 0 ? N : 1);
   std::vector<double> p4s(4 * (size_t)(N > 0 ? N : 1));
@@ -63,11 +67,20 @@ static int run_event(const double *p4, int N, int p, double R, int tile_q, bool 
   a.status = &status;
   a.stage_chunk = kStageChunk;
   a.tile_q = tile_q;
-  emu::run_wave([&] { tiled_kernel<MAXT, WPE, false>(a, ts); });
+  if (def) {
+    a.def_uni = *def;
+    emu::run_wave([&] { tiled_gen_kernel<MAXT, WPE>(a, ts); });
+  } else {
+    emu::run_wave([&] { tiled_kernel<MAXT, WPE, false>(a, ts); });
+  }
   std::vector<int32_t> w1(N > 0 ? N : 1), w2(N > 0 ? N : 1);
   std::vector<double> wd(N > 0 ? N : 1), wj(4 * (size_t)(N > 0 ? N : 1));
   int32_t wn = 0;
-  if (N <= 3000)
+  if (def && N <= 3000)
+    fjo_cluster_n2plain_def(p4, N, def->p, def->scheme, R, 0, w1.data(), w2.data(), wd.data(), wj.data(), &wn);
+  else if (def)
+    fjo_cluster_tiled_def(p4, N, def->p, def->scheme, R, 0, w1.data(), w2.data(), wd.data(), wj.data(), &wn);
+  else if (N <= 3000)
     fjo_cluster_n2plain(p4, N, p, R, 0, w1.data(), w2.data(), wd.data(), wj.data(), &wn);
   else
     fjo_cluster_tiled(p4, N, p, R, 0, w1.data(), w2.data(), wd.data(), wj.data(), &wn);
@@ -80,8 +93,8 @@ static int run_event(const double *p4, int N, int p, double R, int tile_q, bool 
     }
   if (!bad && std::memcmp(jets.data(), wj.data(), 32 * (size_t)wn) != 0) bad = 1;
   if (bad || verbose)
-    std::printf("N=%d p=%d R=%g status=%u nj=%d/%d first_diff_step=%d %s\n", N, p, R, status, nj, wn, first,
-                bad ? "MISMATCH" : "ok");
+    std::printf("N=%d p=%g scheme=%d R=%g status=%u nj=%d/%d first_diff_step=%d %s\n", N, def ? def->p : (double)p,
+                def ? def->scheme : 0, R, status, nj, wn, first, bad ? "MISMATCH" : "ok");
   return bad;
 }
 
@@ -187,8 +200,33 @@ static int run_qbound() {
   return fails ? 1 : 0;
 }
 
+/* Generated events under a general jet definition: tiled_gen_kernel against
+ * the oracle's *_def restatements. */
+static int run_gen(int workload, long e0, long e1, int variant, double pd, int scheme, double R) {
+  const fjmi::JetDef def{pd, scheme, 0};
+  int fails = 0;
+  for (long e = e0; e < e1; ++e) {
+    const int n = fjmi_gen_event_size(workload, e);
+    std::vector<double> p4(4 * (size_t)(n > 0 ? n : 1));
+    fjmi_gen_event(workload, e, p4.data(), n);
+    std::printf("[event %ld N=%d]\n", e, n);
+    std::fflush(stdout);
+    switch (variant) {
+      case 0: fails += run_event<512, 2>(p4.data(), n, 0, R, 0, true, &def); break;
+      case 1: fails += run_event<384, 3>(p4.data(), n, 0, R, 0, true, &def); break;
+      case 2: fails += run_event<320, 3>(p4.data(), n, 0, R, 0, true, &def); break;
+      default: fails += run_event<256, 3>(p4.data(), n, 0, R, 0, true, &def); break;
+    }
+  }
+  std::printf("done: %d mismatches\n", fails);
+  return fails ? 1 : 0;
+}
+
 int main(int argc, char **argv) {
   if (const char *h = std::getenv("FJMI_EMU_HEADLESS")) g_headless = std::atoi(h) != 0;
+  if (argc >= 9 && std::strcmp(argv[1], "gen") == 0)
+    return run_gen(std::atoi(argv[2]), std::atol(argv[3]), std::atol(argv[4]), std::atoi(argv[5]), std::atof(argv[6]),
+                   std::atoi(argv[7]), std::atof(argv[8]));
   if (argc >= 2 && std::strcmp(argv[1], "qbound") == 0) return run_qbound();
   if (argc >= 2 && std::strcmp(argv[1], "kin") == 0) return run_kin(std::atol(argv[2]), (unsigned)std::atol(argv[3]));
   if (argc >= 2 && std::strcmp(argv[1], "file") == 0)

@@ -34,9 +34,27 @@ def test_header_parse_finds_every_entry_point():
                  "fjmi_batch_launch", "fjmi_batch_stats", "fjmi_cluster_batch", "fjmi_server_create",
                  "fjmi_server_submit", "fjmi_server_poll", "fjmi_server_release", "fjmi_server_destroy",
                  "fjmi_stream_check", "fjmi_cluster_event_device_noalloc", "fjmi_inclusive_jets",
-                 "fjmi_inclusive_pseudojets", "fjmi_history_view", "fjmi_exclusive_jets"):
+                 "fjmi_inclusive_pseudojets", "fjmi_history_view", "fjmi_exclusive_jets", "fjmi_cluster_event_def",
+                 "fjmi_cluster_event_device_def", "fjmi_batch_create_def"):
         assert must in names
-    assert len(names) == 31
+    assert len(names) == 34
+
+
+def test_scheme_numbers_agree_with_the_kernels_header(tmp_path):
+    """The public FJMI_SCHEME_* numbers and the kernels' (fjmi_math.h) are the same
+    macros: a translation unit that includes both compiles without a redefinition
+    warning, and the values are FastJet's RecombinationScheme order."""
+    src = tmp_path / "schemes.c"
+    src.write_text('#include "fjmi.h"\n#include "fjmi_math.h"\n#include <stdio.h>\n'
+                   'int main(void) { printf("%d %d %d %d %d %d %d %d %d %d\\n", FJMI_SCHEME_E, FJMI_SCHEME_PT, '
+                   'FJMI_SCHEME_PT2, FJMI_SCHEME_ET, FJMI_SCHEME_ET2, FJMI_SCHEME_BIPT, FJMI_SCHEME_BIPT2, '
+                   'FJMI_SCHEME_WTA_PT, FJMI_SCHEME_WTA_MODP, FJMI_NUM_SCHEMES); '
+                   'return (int)sizeof(fjmi_jetdef) - 16; }\n')
+    exe = tmp_path / "schemes"
+    subprocess.run(["gcc", "-std=c11", "-Werror", f"-I{INC}", f"-I{os.path.join(ROOT, 'fastjet_gpu_amd', 'csrc')}",
+                    str(src), "-o", str(exe), "-lm"], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert out == [str(k) for k in range(10)]
 
 
 @pytest.mark.parametrize("header,lib", [("fjmi.h", "libfjmi.so"), ("fjmi_gen.h", "libfjmi_gen.so")])

@@ -139,3 +139,17 @@ def test_gather_counter_restarts(emu_stamp_bin, workload, e0, e1, variant, p, R)
     result bit-exact, single-chunk and several-chunk gathers alike."""
     out = run(emu_stamp_bin, workload, e0, e1, variant, p, R, 0)
     assert "done: 0 mismatches" in out
+
+
+@pytest.mark.parametrize("workload,e0,e1,variant,p,scheme,R", [
+    (1, 0, 1, 0, -1.0, 0, 0.4), (1, 0, 1, 3, 1.0, 0, 0.4),          # standard definitions on the general kernel
+    (1, 0, 1, 1, 0.5, 1, 0.7), (2, 0, 1, 0, 2.0, 2, 1.0), (3, 0, 3, 1, -1.0, 3, 0.4),
+    (3, 3, 6, 2, 0.3, 4, 0.6), (1, 0, 1, 3, -0.5, 5, 0.4), (1, 0, 1, 1, 1.0, 6, 0.4),
+    (3, 6, 9, 3, -2.5, 7, 0.5), (3, 9, 12, 0, 0.0, 8, 0.8),
+    (4, 1, 2, 0, 0.5, 1, 0.4),                                      # several-chunk gathers (N = 15716)
+])
+def test_general_definitions(emu_bin, workload, e0, e1, variant, p, scheme, R):
+    """tiled_gen_kernel (any real genkt exponent, FastJet's recombination schemes)
+    bit-exact against the oracle's restatement of the same definition."""
+    out = run(emu_bin, "gen", workload, e0, e1, variant, p, scheme, R)
+    assert "done: 0 mismatches" in out and "MISMATCH" not in out
