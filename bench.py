@@ -62,7 +62,7 @@ COUNTER_PROFILE = os.path.join(ROOT, "profiles", "tiled_kernel_counters_workload
 TILED_VARIANTS = {0: "tiled_kernel<512, 2, false>", 1: "tiled_kernel<384, 3, false>", 2: "tiled_kernel<320, 3, false>",
                   3: "tiled_kernel<256, 3, false>"}
 PROFILED_KERNEL = TILED_VARIANTS[3]  # the default table's kernel for the 4096-event batch
-KERNEL_SOURCES = ["fastjet_gpu_amd/csrc/tiled.hip", "fastjet_gpu_amd/csrc/fjmi_math.h",
+KERNEL_SOURCES = ["fastjet_gpu_amd/csrc/tiled.hip", "fastjet_gpu_amd/csrc/tiled_body.inc", "fastjet_gpu_amd/csrc/fjmi_math.h",
                   "fastjet_gpu_amd/csrc/fjmi_math_tables.h", "fastjet_gpu_amd/csrc/fjmi_internal.h",
                   "fastjet_gpu_amd/csrc/tune_table.inc", "fastjet_gpu_amd/csrc/runtime.hip"]
 

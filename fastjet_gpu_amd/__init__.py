@@ -133,10 +133,11 @@ def _declare(lib):
         "fjmi_cluster_event_device_noalloc": ([_vp, _vp, i, i, d, _vp, _vp, _vp, _vp, _vp, _vp], i),
         "fjmi_batch_create": ([_vp, i, _vp, _vp, _vp, _vp, _vp, i, d, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)],
                               i),
-        "fjmi_cluster_event_def": ([_vp, _vp, i, _vp, d, _vp, _vp, _vp, _vp, _vp], i),
-        "fjmi_cluster_event_device_def": ([_vp, _vp, i, _vp, d, _vp, _vp, _vp, _vp, _vp, _vp], i),
-        "fjmi_batch_create_def": ([_vp, i, _vp, _vp, _vp, _vp, _vp, _vp, d, _vp, _vp, _vp, _vp, _vp,
+        "fjmi_cluster_event_def": ([_vp, _vp, i, _vp, d, _vp, _vp, _vp, _vp, _vp, _vp], i),
+        "fjmi_cluster_event_device_def": ([_vp, _vp, i, _vp, d, _vp, _vp, _vp, _vp, _vp, _vp, _vp], i),
+        "fjmi_batch_create_def": ([_vp, i, _vp, _vp, _vp, _vp, _vp, _vp, d, _vp, _vp, _vp, _vp, _vp, _vp,
                                    ctypes.POINTER(_vp)], i),
+        "fjmi_exclusive_jets_def": ([_vp, i, _vp, i, _vp, _vp, _vp, _vp, _vp, i, i, d, _vp, _vp, _vp], i),
         "fjmi_batch_launch": ([_vp, _vp], i),
         "fjmi_batch_sync": ([_vp, _vp], i),
         "fjmi_batch_destroy": ([_vp], i),
@@ -217,9 +218,10 @@ class Engine:
                                                      _ptr(h2), _ptr(hd), _ptr(jets), _ptr(nj)), "fjmi_cluster_event")
         return h1, h2, hd, jets[:nj[0]].copy()
 
-    def cluster_event_def(self, p4, p, scheme, R):
+    def cluster_event_def(self, p4, p, scheme, R, hist_jets=False):
         """Clusters one event under a general jet definition: genkt exponent p (any finite real) and
-        recombination scheme SCHEME_*. Returns (hist_p1, hist_p2, hist_dij, jets)."""
+        recombination scheme SCHEME_*. Returns (hist_p1, hist_p2, hist_dij, jets), and with hist_jets=True
+        also the N x 4 momenta each merge step stores (rows of beam steps NaN)."""
         p4 = np.ascontiguousarray(p4, dtype=np.float64).reshape(-1, 4)
         n = len(p4)
         h1 = np.zeros(n, np.int32)
@@ -227,11 +229,13 @@ class Engine:
         hd = np.zeros(n, np.float64)
         jets = np.zeros((max(n, 1), 4), np.float64)
         nj = np.zeros(1, np.int32)
+        hj = np.full((max(n, 1), 4), np.nan) if hist_jets else None
         jd = JetDef(float(p), int(scheme), 0)
         _check(self.lib, self.lib.fjmi_cluster_event_def(self.ctx, _ptr(p4), n, ctypes.byref(jd), float(R), _ptr(h1),
-                                                         _ptr(h2), _ptr(hd), _ptr(jets), _ptr(nj)),
+                                                         _ptr(h2), _ptr(hd), _ptr(jets), _ptr(nj), _ptr(hj)),
                "fjmi_cluster_event_def")
-        return h1, h2, hd, jets[:nj[0]].copy()
+        out = (h1, h2, hd, jets[:nj[0]].copy())
+        return out + (hj[:n],) if hist_jets else out
 
     def cluster_batch(self, p4, offsets, p, R):
         """Clusters many events in one graph launch. p and R are scalars or per-event arrays.
@@ -259,10 +263,12 @@ class Engine:
                                                             d_hd, d_jets, d_nj, stream or None),
                "fjmi_cluster_event_device")
 
-    def cluster_event_device_def(self, d_p4, n, p, scheme, R, d_h1, d_h2, d_hd, d_jets, d_nj, stream=0):
+    def cluster_event_device_def(self, d_p4, n, p, scheme, R, d_h1, d_h2, d_hd, d_jets, d_nj, d_hist_jets=0,
+                                 stream=0):
         jd = JetDef(float(p), int(scheme), 0)
         _check(self.lib, self.lib.fjmi_cluster_event_device_def(self.ctx, d_p4, n, ctypes.byref(jd), float(R), d_h1,
-                                                                d_h2, d_hd, d_jets, d_nj, stream or None),
+                                                                d_h2, d_hd, d_jets, d_nj, d_hist_jets or None,
+                                                                stream or None),
                "fjmi_cluster_event_device_def")
 
     def cluster_pseudojets(self, d_particles, n, p, R, engine=ENGINE_AUTO, stream=0):
@@ -296,6 +302,13 @@ class Engine:
                                                       d_h2, d_hd, int(mode), float(param), d_work, d_out, d_out_n,
                                                       stream or None), "fjmi_exclusive_jets")
 
+    def exclusive_jets_def(self, n_events, d_offsets, n_single, d_p4, d_h1, d_h2, d_hd, d_hist_jets, scheme, mode,
+                           param, d_out, d_out_n, stream=0):
+        _check(self.lib, self.lib.fjmi_exclusive_jets_def(self.ctx, n_events, d_offsets or None, n_single, d_p4, d_h1,
+                                                          d_h2, d_hd, d_hist_jets, int(scheme), int(mode),
+                                                          float(param), d_out, d_out_n, stream or None),
+               "fjmi_exclusive_jets_def")
+
     def stream_check(self, stream=0):
         """Synchronises `stream`; raises if a one-event device call on it reported a failure."""
         _check(self.lib, self.lib.fjmi_stream_check(self.ctx, stream or None), "fjmi_stream_check")
@@ -314,16 +327,17 @@ class Engine:
 
 
     def batch_create_def(self, h_offsets, d_p4, d_offsets, d_defs, p, scheme, d_R, R_uni, d_h1, d_h2, d_hd, d_jets,
-                         d_nj):
+                         d_nj, d_hist_jets=0):
         """A batch plan under general jet definitions: per event (d_defs, device records of JETDEF_DTYPE)
-        or (p, scheme) for all when d_defs is 0."""
+        or (p, scheme) for all when d_defs is 0; d_hist_jets (optional) gets each merge step's momentum."""
         h_offsets = np.ascontiguousarray(h_offsets, dtype=np.int64)
         b = _vp()
         jd = JetDef(float(p), int(scheme), 0)
         _check(self.lib, self.lib.fjmi_batch_create_def(self.ctx, len(h_offsets) - 1, _ptr(h_offsets), d_p4,
                                                         d_offsets, d_defs or None, ctypes.byref(jd), d_R or None,
                                                         float(R_uni), d_h1, d_h2, d_hd, d_jets, d_nj,
-                                                        ctypes.byref(b)), "fjmi_batch_create_def")
+                                                        d_hist_jets or None, ctypes.byref(b)),
+               "fjmi_batch_create_def")
         return Batch(self, b)
 
 

@@ -123,6 +123,9 @@ struct LaunchArgs {
   /* general jet definitions (tiled_gen_kernel only): per event, or nullptr -> def_uni */
   const JetDef *def_arr;
   JetDef def_uni;
+  /* tiled_gen_kernel, optional: 4 doubles per particle, the momentum of the
+   * history entry each merge step creates (beam steps leave theirs unwritten) */
+  double *hist_jets;
 };
 
 enum { STATUS_CAPACITY = 1, STATUS_INTERNAL = 2, STATUS_DEFINITION = 4 };

@@ -68,9 +68,10 @@ namespace rt {
  * get their own. */
 struct StreamState {
   DevBuf status; /* 4 bytes, zeroed at creation and by fjmi_stream_check */
-  DevBuf hist1, hist2, hdij, jets, njets, p4s, coords, p4in, tscratch;
+  DevBuf hist1, hist2, hdij, jets, njets, p4s, coords, p4in, tscratch, hjets;
   void release_all() {
-    for (DevBuf *b : {&status, &hist1, &hist2, &hdij, &jets, &njets, &p4s, &coords, &p4in, &tscratch}) b->release();
+    for (DevBuf *b : {&status, &hist1, &hist2, &hdij, &jets, &njets, &p4s, &coords, &p4in, &tscratch, &hjets})
+      b->release();
   }
 };
 

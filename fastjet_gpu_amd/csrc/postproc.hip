@@ -10,7 +10,11 @@
  *                    each history entry's child and max_dij_so_far;
  *   exclusive jets   FastJet's exclusive_jets(njets) / exclusive_jets(dcut): the
  *                    entries alive at the history's stop point, their momenta the
- *                    E-scheme sums of the history replayed in step order.
+ *                    E-scheme sums of the history replayed in step order, or, for
+ *                    a general jet definition, the momenta its clustering stored
+ *                    per merge step (FastJet's jets of the ClusterSequence; a
+ *                    replay cannot rebuild them: a scheme's merge depends on the
+ *                    order of its two operands, which the history does not keep).
  *
  * Every comparison and sum follows the oracle's restatements
  * (oracle/fj_oracle.c: fjo_inclusive_jets, fjo_history_children,
@@ -21,6 +25,7 @@
 
 #include "fjmi_internal.h"
 #include "fjmi_kernels.h"
+#include "fjmi_math.h"
 
 namespace fjmi {
 
@@ -183,17 +188,19 @@ __global__ __launch_bounds__(256) void history_kernel(const int64_t *offsets, in
 }
 
 /* Exclusive jets of event e (mode 0: njets = param; mode 1: dcut = param >= 0),
- * written to 4*base of `out`, their count to n_out[e] (-1 if njets > N).  work
- * holds 8 doubles per particle (the 2N history momenta).  The replay and the
- * selection are sequential in history order, in one thread. */
+ * written to 4*base of `out`, their count to n_out[e] (-1 if njets > N).
+ * E-scheme history (hist_jets null): work holds 8 doubles per particle (the 2N
+ * history momenta), replayed in step order.  General jet definition: the
+ * particles as `scheme` preprocesses them and the merge steps' momenta from
+ * hist_jets (4 doubles per particle, as the clustering wrote them).  The replay
+ * and the selection are sequential in history order, in one thread. */
 __global__ __launch_bounds__(256) void exclusive_kernel(const int64_t *offsets, int n_single, const double *p4,
                                                         const int32_t *p1, const int32_t *p2, const double *dij,
                                                         int mode, double param, double *work, double *out,
-                                                        int32_t *n_out) {
+                                                        int32_t *n_out, const double *hist_jets, int scheme) {
   const int e = blockIdx.x;
   const int64_t base = ev_base(offsets, e);
   const int N = ev_size(offsets, e, n_single);
-  double4 *h = reinterpret_cast<double4 *>(work + 8 * base);
   const double4 *in = reinterpret_cast<const double4 *>(p4 + 4 * base);
   /* the stop point */
   int stop;
@@ -219,6 +226,27 @@ __global__ __launch_bounds__(256) void exclusive_kernel(const int64_t *offsets, 
         }
       }
   }
+  if (hist_jets) { /* a general jet definition: the stored momenta, no replay */
+    if (threadIdx.x != 0) return;
+    const double4 *hj = reinterpret_cast<const double4 *>(hist_jets + 4 * base);
+    auto entry = [&](int x) {
+      if (x >= N) return hj[x - N];
+      double4 q = in[x];
+      fjmi_preprocess(scheme, &q.x, &q.y, &q.z, &q.w);
+      return q;
+    };
+    int k = 0;
+    double4 *o = reinterpret_cast<double4 *>(out + 4 * base);
+    for (int en = stop; en < 2 * N; ++en) {
+      const int s = en - N;
+      const int a = p1[base + s], b = p2[base + s];
+      if (a >= 0 && a < stop && k < N) o[k++] = entry(a);
+      if (b < stop && b > 0 && k < N) o[k++] = entry(b);
+    }
+    n_out[e] = k;
+    return;
+  }
+  double4 *h = reinterpret_cast<double4 *>(work + 8 * base);
   for (int k = threadIdx.x; k < N; k += T) h[k] = in[k];
   __syncthreads();
   if (threadIdx.x != 0) return;

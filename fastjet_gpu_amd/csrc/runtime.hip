@@ -313,7 +313,8 @@ int zero_words(void *p, int n, hipStream_t stream) {
  * of `st`, the state of `stream`. */
 int cluster_one_device(fjmi_context *ctx, fjmi::rt::StreamState &st, int engine, const double *d_in, int in_stride,
                        int in_off, int n, int p, double R, int32_t *h1, int32_t *h2, double *hd, double *jets,
-                       int32_t *nj, unsigned char *pj, hipStream_t stream, const fjmi::JetDef *def = nullptr) {
+                       int32_t *nj, unsigned char *pj, hipStream_t stream, const fjmi::JetDef *def = nullptr,
+                       double *hist_jets = nullptr) {
   /* under the default table: FJMI_ENGINE_AUTO follows the latency rule (the
    * dense shape with the fewest slots per thread, or the latency tiled row), and
    * the tiled engine takes the latency tiled row; under a caller's table, its
@@ -369,7 +370,10 @@ int cluster_one_device(fjmi_context *ctx, fjmi::rt::StreamState &st, int engine,
   a.coord_scratch = st.coords.as<double>();
   a.status = st.status.as<unsigned int>();
   a.tile_q = tiled ? b->slots : 0;
-  if (def) a.def_uni = *def;
+  if (def) {
+    a.def_uni = *def;
+    a.hist_jets = hist_jets;
+  }
   if (tiled) {
     void *params[] = {&a, &ts};
     const void *fn = def ? fjmi::kTiledGenKernels[b->p4_global] : fjmi::kTiledKernels[b->p4_global];
@@ -403,7 +407,7 @@ int check_engine(int engine) {
 int make_plan(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const double *d_p4, const int64_t *d_offsets,
               const int32_t *d_p, const double *d_R, int p_uni, double R_uni, int32_t *d_h1, int32_t *d_h2,
               double *d_hd, double *d_jets, int32_t *d_nj, Plan &pl, const fjmi::JetDef *d_defs = nullptr,
-              const fjmi::JetDef *def_uni = nullptr) {
+              const fjmi::JetDef *def_uni = nullptr, double *d_hist_jets = nullptr) {
   const int nb = (int)ctx->buckets.size();
   /* general jet definitions run on the tiled engine: a dense row of the table
    * becomes the one-event tiled row (384 tiles, side R) for its sizes */
@@ -511,6 +515,7 @@ int make_plan(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const d
     a.tile_q = r.tile_q;
     a.def_arr = d_defs;
     if (def_uni) a.def_uni = *def_uni;
+    if (gen) a.hist_jets = d_hist_jets;
     r.args = a;
     pl.runs.push_back(r);
   }
@@ -968,8 +973,37 @@ int fjmi_exclusive_jets(fjmi_context *ctx, int n_events, const int64_t *d_offset
   std::lock_guard<std::mutex> lk(ctx->mtx);
   int rc;
   if ((rc = set_device(ctx))) return rc;
+  const double *no_hist = nullptr;
+  int scheme = FJMI_SCHEME_E;
   void *params[] = {&d_offsets, &n_single, &d_p4, &d_hist_p1, &d_hist_p2, &d_hist_dij, &mode, &param, &d_work,
-                    &d_out_jets, &d_out_njets};
+                    &d_out_jets, &d_out_njets, &no_hist, &scheme};
+  HIPCHK(hipLaunchKernel(fjmi::kExclusiveKernel, dim3(n_events), dim3(256), params, 0, (hipStream_t)stream));
+  return FJMI_OK;
+}
+
+int fjmi_exclusive_jets_def(fjmi_context *ctx, int n_events, const int64_t *d_offsets, int n_single,
+                            const double *d_p4, const int32_t *d_hist_p1, const int32_t *d_hist_p2,
+                            const double *d_hist_dij, const double *d_hist_jets, int scheme, int mode, double param,
+                            double *d_out_jets, int32_t *d_out_njets, void *stream) {
+  if (!ctx) return fail(FJMI_EINVAL, "null context");
+  if (n_events < 0 || (!d_offsets && (n_events > 1 || n_single < 0))) return fail(FJMI_EINVAL, "bad event layout");
+  if (scheme < 0 || scheme >= FJMI_NUM_SCHEMES) return fail(FJMI_EINVAL, "unknown recombination scheme %d", scheme);
+  if (mode == FJMI_EXCLUSIVE_NJETS) {
+    if (!(param >= 0.0) || param != (double)(int)param) return fail(FJMI_EINVAL, "njets must be an integer >= 0");
+  } else if (mode == FJMI_EXCLUSIVE_DCUT) {
+    if (!(param >= 0.0)) return fail(FJMI_EINVAL, "dcut must be >= 0");
+  } else {
+    return fail(FJMI_EINVAL, "mode must be FJMI_EXCLUSIVE_NJETS or FJMI_EXCLUSIVE_DCUT");
+  }
+  if (n_events == 0) return FJMI_OK;
+  if (!d_p4 || !d_hist_p1 || !d_hist_p2 || !d_hist_dij || !d_hist_jets || !d_out_jets || !d_out_njets)
+    return fail(FJMI_EINVAL, "null buffer");
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  double *no_work = nullptr;
+  void *params[] = {&d_offsets, &n_single, &d_p4, &d_hist_p1, &d_hist_p2, &d_hist_dij, &mode, &param, &no_work,
+                    &d_out_jets, &d_out_njets, &d_hist_jets, &scheme};
   HIPCHK(hipLaunchKernel(fjmi::kExclusiveKernel, dim3(n_events), dim3(256), params, 0, (hipStream_t)stream));
   return FJMI_OK;
 }
@@ -998,7 +1032,8 @@ namespace {
 /* One event from host memory, synchronously: the standard algorithms (def
  * null) or a general jet definition. */
 int cluster_event_host(fjmi_context *ctx, const double *p4, int n, int p, double R, int engine,
-                       const fjmi::JetDef *def, int32_t *h1, int32_t *h2, double *hd, double *jets, int32_t *nj) {
+                       const fjmi::JetDef *def, int32_t *h1, int32_t *h2, double *hd, double *jets, int32_t *nj,
+                       double *hist_jets = nullptr) {
   if (!ctx) return fail(FJMI_EINVAL, "null context");
   int rc = check_args(n, p, R);
   if (rc) return rc;
@@ -1015,12 +1050,14 @@ int cluster_event_host(fjmi_context *ctx, const double *p4, int n, int p, double
   if ((rc = stream_state(ctx, nullptr, &st))) return rc;
   const size_t nn = (size_t)n;
   if ((rc = st->p4in.ensure(32 * nn)) || (rc = st->hist1.ensure(4 * nn)) || (rc = st->hist2.ensure(4 * nn)) ||
-      (rc = st->hdij.ensure(8 * nn)) || (rc = st->jets.ensure(32 * nn)) || (rc = st->njets.ensure(4)))
+      (rc = st->hdij.ensure(8 * nn)) || (rc = st->jets.ensure(32 * nn)) || (rc = st->njets.ensure(4)) ||
+      (hist_jets && (rc = st->hjets.ensure(32 * nn))))
     return rc;
   HIPCHK(hipMemcpy(st->p4in.ptr, p4, 32 * nn, hipMemcpyHostToDevice));
   if ((rc = cluster_one_device(ctx, *st, engine, st->p4in.as<double>(), 4, 0, n, p, R, st->hist1.as<int32_t>(),
                                st->hist2.as<int32_t>(), st->hdij.as<double>(), st->jets.as<double>(),
-                               st->njets.as<int32_t>(), nullptr, nullptr, def)))
+                               st->njets.as<int32_t>(), nullptr, nullptr, def,
+                               hist_jets ? st->hjets.as<double>() : nullptr)))
     return rc;
   HIPCHK(hipStreamSynchronize(nullptr));
   uint32_t status = 0;
@@ -1036,6 +1073,7 @@ int cluster_event_host(fjmi_context *ctx, const double *p4, int n, int p, double
   HIPCHK(hipMemcpy(nj, st->njets.ptr, 4, hipMemcpyDeviceToHost));
   if (*nj < 0 || *nj > n) return fail(FJMI_EHIP, "corrupt jet count %d", *nj);
   HIPCHK(hipMemcpy(jets, st->jets.ptr, 32 * (size_t)(*nj), hipMemcpyDeviceToHost));
+  if (hist_jets) HIPCHK(hipMemcpy(hist_jets, st->hjets.ptr, 32 * nn, hipMemcpyDeviceToHost));
   return FJMI_OK;
 }
 
@@ -1060,16 +1098,16 @@ int fjmi_cluster_event(fjmi_context *ctx, const double *p4, int n, int p, double
 }
 
 int fjmi_cluster_event_def(fjmi_context *ctx, const double *p4, int n, const fjmi_jetdef *def, double R,
-                           int32_t *h1, int32_t *h2, double *hd, double *jets, int32_t *nj) {
+                           int32_t *h1, int32_t *h2, double *hd, double *jets, int32_t *nj, double *hist_jets) {
   fjmi::JetDef d;
   const int rc = check_def(def, &d);
   if (rc) return rc;
-  return cluster_event_host(ctx, p4, n, 0, R, FJMI_ENGINE_TILED, &d, h1, h2, hd, jets, nj);
+  return cluster_event_host(ctx, p4, n, 0, R, FJMI_ENGINE_TILED, &d, h1, h2, hd, jets, nj, hist_jets);
 }
 
 int fjmi_cluster_event_device_def(fjmi_context *ctx, const double *d_p4, int n, const fjmi_jetdef *def, double R,
                                   int32_t *d_h1, int32_t *d_h2, double *d_hd, double *d_jets, int32_t *d_nj,
-                                  void *stream) {
+                                  double *d_hist_jets, void *stream) {
   if (!ctx) return fail(FJMI_EINVAL, "null context");
   fjmi::JetDef d;
   int rc = check_def(def, &d);
@@ -1084,7 +1122,7 @@ int fjmi_cluster_event_device_def(fjmi_context *ctx, const double *d_p4, int n, 
   fjmi::rt::StreamState *st = nullptr;
   if ((rc = stream_state(ctx, s, &st))) return rc;
   return cluster_one_device(ctx, *st, FJMI_ENGINE_TILED, d_p4, 4, 0, n, 0, R, d_h1, d_h2, d_hd, d_jets, d_nj, nullptr,
-                            s, &d);
+                            s, &d, d_hist_jets);
 }
 
 /* ---- batched path ---- */
@@ -1098,7 +1136,7 @@ namespace {
 int batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const double *d_p4,
                  const int64_t *d_offsets, const int32_t *d_p, const double *d_R, int p_uni, double R_uni,
                  const fjmi::JetDef *d_defs, const fjmi::JetDef *def_uni, int32_t *d_h1, int32_t *d_h2, double *d_hd,
-                 double *d_jets, int32_t *d_nj, fjmi_batch **out) {
+                 double *d_jets, int32_t *d_nj, double *d_hist_jets, fjmi_batch **out) {
   if (!ctx || !out || !h_offsets || n_events < 0) return fail(FJMI_EINVAL, "bad argument");
   *out = nullptr;
   if (!d_p && (p_uni < -1 || p_uni > 1)) return fail(FJMI_EINVAL, "exponent p must be -1, 0 or 1");
@@ -1115,7 +1153,7 @@ int batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets, cons
   };
   Plan &pl = bt->plan;
   if ((rc = make_plan(ctx, n_events, h_offsets, d_p4, d_offsets, d_p, d_R, p_uni, R_uni, d_h1, d_h2, d_hd, d_jets,
-                      d_nj, pl, d_defs, def_uni)))
+                      d_nj, pl, d_defs, def_uni, d_hist_jets)))
     return cleanup(rc);
   if (!pl.flat.empty()) {
     hipError_t e = hipMemcpy(pl.queue.ptr, pl.flat.data(), sizeof(int32_t) * pl.flat.size(), hipMemcpyHostToDevice);
@@ -1166,13 +1204,13 @@ int fjmi_batch_create(fjmi_context *ctx, int n_events, const int64_t *h_offsets,
                       const int64_t *d_offsets, const int32_t *d_p, const double *d_R, int p_uni, double R_uni,
                       int32_t *d_h1, int32_t *d_h2, double *d_hd, double *d_jets, int32_t *d_nj, fjmi_batch **out) {
   return batch_create(ctx, n_events, h_offsets, d_p4, d_offsets, d_p, d_R, p_uni, R_uni, nullptr, nullptr, d_h1, d_h2,
-                      d_hd, d_jets, d_nj, out);
+                      d_hd, d_jets, d_nj, nullptr, out);
 }
 
 int fjmi_batch_create_def(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const double *d_p4,
                           const int64_t *d_offsets, const fjmi_jetdef *d_defs, const fjmi_jetdef *def,
                           const double *d_R, double R_uni, int32_t *d_h1, int32_t *d_h2, double *d_hd, double *d_jets,
-                          int32_t *d_nj, fjmi_batch **out) {
+                          int32_t *d_nj, double *d_hist_jets, fjmi_batch **out) {
   fjmi::JetDef d{};
   if (!d_defs) {
     const int rc = check_def(def, &d);
@@ -1182,7 +1220,7 @@ int fjmi_batch_create_def(fjmi_context *ctx, int n_events, const int64_t *h_offs
                 "fjmi_jetdef is the kernels' JetDef");
   return batch_create(ctx, n_events, h_offsets, d_p4, d_offsets, nullptr, d_R, 0, R_uni,
                       reinterpret_cast<const fjmi::JetDef *>(d_defs), d_defs ? nullptr : &d, d_h1, d_h2, d_hd, d_jets,
-                      d_nj, out);
+                      d_nj, d_hist_jets, out);
 }
 
 int fjmi_batch_launch(fjmi_batch *b, void *stream) {

@@ -251,14 +251,21 @@ typedef struct {
   int32_t pad;    /* 0 */
 } fjmi_jetdef;
 
+/* The calls below take an optional hist_jets output (NULL: none), indexed like
+ * the particles, 4 doubles each: step s of a merge writes the momentum of the
+ * history entry N + s it creates (beam steps leave theirs unwritten).  These
+ * are FastJet's ClusterSequence jets beyond the inputs; fjmi_exclusive_jets_def
+ * reads them. */
+
 /* One event, host memory in and out, synchronous (as fjmi_cluster_event). */
 int fjmi_cluster_event_def(fjmi_context *ctx, const double *p4, int n, const fjmi_jetdef *def, double R,
-                           int32_t *hist_p1, int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets);
+                           int32_t *hist_p1, int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets,
+                           double *hist_jets);
 /* One event, device memory, asynchronous on `stream` (as fjmi_cluster_event_device;
  * failures are reported by fjmi_stream_check). */
 int fjmi_cluster_event_device_def(fjmi_context *ctx, const double *d_p4, int n, const fjmi_jetdef *def, double R,
                                   int32_t *d_hist_p1, int32_t *d_hist_p2, double *d_hist_dij, double *d_jets,
-                                  int32_t *d_njets, void *stream);
+                                  int32_t *d_njets, double *d_hist_jets, void *stream);
 /* A batch plan (as fjmi_batch_create) with one definition per event (d_defs,
  * device memory, n_events entries) or one for all (def, host memory; d_defs
  * NULL), and per-event radii d_R or R_uni.  An event whose device-side
@@ -267,7 +274,19 @@ int fjmi_cluster_event_device_def(fjmi_context *ctx, const double *d_p4, int n, 
 int fjmi_batch_create_def(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const double *d_p4,
                           const int64_t *d_offsets, const fjmi_jetdef *d_defs, const fjmi_jetdef *def,
                           const double *d_R, double R_uni, int32_t *d_hist_p1, int32_t *d_hist_p2,
-                          double *d_hist_dij, double *d_jets, int32_t *d_njets, fjmi_batch **out);
+                          double *d_hist_dij, double *d_jets, int32_t *d_njets, double *d_hist_jets,
+                          fjmi_batch **out);
+/* Exclusive jets (as fjmi_exclusive_jets) of a clustering under a general jet
+ * definition with recombination scheme `scheme`: the entries alive at the stop
+ * point, particles as the scheme preprocesses them and merged entries from
+ * d_hist_jets.  A replay of the history cannot rebuild those for schemes other
+ * than E: their merge depends on the order of its operands, which the history
+ * (parents as min, max, as FastJet stores them) does not keep.  Not in the
+ * reference. */
+int fjmi_exclusive_jets_def(fjmi_context *ctx, int n_events, const int64_t *d_offsets, int n_single,
+                            const double *d_p4, const int32_t *d_hist_p1, const int32_t *d_hist_p2,
+                            const double *d_hist_dij, const double *d_hist_jets, int scheme, int mode, double param,
+                            double *d_out_jets, int32_t *d_out_njets, void *stream);
 
 /* Host-memory batch: copies in, clusters, copies out, synchronises.
  * Per-particle outputs are sized like p4/4 (total particles).  The events are

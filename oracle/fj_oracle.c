@@ -97,7 +97,11 @@ typedef struct {
   int32_t nout;
   jdef_t def;
   int mm;
+  double *hist_jets; /* optional: step s's new entry's momentum at 4 s */
 } book_t;
+
+/* the hist_jets output of the *_defh entry points for the calling thread's clustering */
+static _Thread_local double *t_hist_jets;
 
 /* the inputs, as the recombination scheme takes them, with their rap/phi */
 static int book_init(book_t *b, const double *p4, int n, const jdef_t *def, int mm, int32_t *hp1, int32_t *hp2,
@@ -130,6 +134,7 @@ static int book_init(book_t *b, const double *p4, int n, const jdef_t *def, int 
   b->nout = 0;
   b->def = *def;
   b->mm = mm;
+  b->hist_jets = t_hist_jets;
   return 0;
 }
 
@@ -158,6 +163,7 @@ static int book_merge(book_t *b, int ja, int jb, double dij) {
   } else {
     rap_phi(b->p4 + 4 * (size_t)k, b->mm, &b->rap[k], &b->phi[k]);
   }
+  if (b->hist_jets) memcpy(b->hist_jets + 4 * (size_t)b->step, b->p4 + 4 * (size_t)k, sizeof(double) * 4);
   int ha = b->hist[ja], hb = b->hist[jb];
   b->hist[k] = b->N + b->step;
   b->hp1[b->step] = ha < hb ? ha : hb;
@@ -824,6 +830,56 @@ int fjo_exclusive_jets(const double *p4, const int32_t *hp1, const int32_t *hp2,
   return 0;
 }
 
+/* Exclusive jets of a clustering under recombination scheme `scheme`
+ * (ClusterSequence::exclusive_jets reads its stored jets): the stop point as in
+ * fjo_exclusive_jets, the particles as the scheme preprocesses them, merged
+ * entries N + s from hist_jets[4 s]. */
+int fjo_exclusive_jets_def(const double *p4, const int32_t *hp1, const int32_t *hp2, const double *hdij,
+                           const double *hist_jets, int n, int scheme, int mode, double param, double *out,
+                           int32_t *n_out) {
+  if (n < 0 || !n_out || scheme < 0 || scheme >= FJMI_NUM_SCHEMES) return -1;
+  int njets;
+  if (mode == 0) {
+    njets = (int)param;
+    if (!(param >= 0.0) || njets > n || (double)njets != param) return -1;
+  } else {
+    int i = 2 * n - 1;
+    double m = 0.0;
+    double *mx = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    if (!mx) return -2;
+    for (int s = 0; s < n; ++s) {
+      m = hdij[s] > m ? hdij[s] : m;
+      mx[s] = m;
+    }
+    while (i >= 0) {
+      const double ms = i >= n ? mx[i - n] : 0.0;
+      if (ms <= param) break;
+      --i;
+    }
+    free(mx);
+    njets = 2 * n - (i + 1);
+  }
+  const int stop = 2 * n - njets;
+  int k = 0;
+  for (int e = stop; e < 2 * n; ++e) {
+    const int s = e - n;
+    const int sel[2] = {hp1[s], hp2[s]};
+    for (int c = 0; c < 2; ++c) {
+      const int x = sel[c];
+      if (x < 0 || x >= stop || (c == 1 && x == 0)) continue;
+      double *o = out + 4 * (size_t)k++;
+      if (x >= n) {
+        memcpy(o, hist_jets + 4 * (size_t)(x - n), sizeof(double) * 4);
+      } else {
+        memcpy(o, p4 + 4 * (size_t)x, sizeof(double) * 4);
+        fjmi_preprocess(scheme, &o[0], &o[1], &o[2], &o[3]);
+      }
+    }
+  }
+  *n_out = k;
+  return 0;
+}
+
 /* Public entry points: kt / C/A / anti-kt with the E-scheme, and any jet definition. */
 #define FJO_WRAP(name)                                                                                              \
   int fjo_cluster_##name(const double *p4, int n, int p, double R, int mm, int32_t *hp1, int32_t *hp2, double *hdij, \
@@ -836,6 +892,14 @@ int fjo_exclusive_jets(const double *p4, const int32_t *hp1, const int32_t *hp2,
                                int32_t *hp2, double *hdij, double *jets, int32_t *njets) {                          \
     const jdef_t d = {p, scheme};                                                                                    \
     return cluster_##name(p4, n, &d, R, mm, hp1, hp2, hdij, jets, njets);                                            \
+  }                                                                                                                  \
+  int fjo_cluster_##name##_defh(const double *p4, int n, double p, int scheme, double R, int mm, int32_t *hp1,     \
+                                int32_t *hp2, double *hdij, double *jets, int32_t *njets, double *hist_jets) {      \
+    const jdef_t d = {p, scheme};                                                                                    \
+    t_hist_jets = hist_jets;                                                                                         \
+    const int rc = cluster_##name(p4, n, &d, R, mm, hp1, hp2, hdij, jets, njets);                                    \
+    t_hist_jets = NULL;                                                                                              \
+    return rc;                                                                                                       \
   }
 FJO_WRAP(n2plain)
 FJO_WRAP(brute)

@@ -72,6 +72,18 @@ int fjo_cluster_pairwise_def(const double *p4, int n, double p, int scheme, doub
                              int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets);
 int fjo_cluster_tiled_def(const double *p4, int n, double p, int scheme, double R, int math_mode, int32_t *hist_p1,
                           int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets);
+/* The same, and the momentum of the history entry each merge step creates into
+ * hist_jets[4 s .. 4 s + 3] (beam steps leave theirs untouched): FastJet's
+ * ClusterSequence jets beyond the inputs. */
+int fjo_cluster_n2plain_defh(const double *p4, int n, double p, int scheme, double R, int math_mode, int32_t *hist_p1,
+                             int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets, double *hist_jets);
+int fjo_cluster_brute_defh(const double *p4, int n, double p, int scheme, double R, int math_mode, int32_t *hist_p1,
+                           int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets, double *hist_jets);
+int fjo_cluster_pairwise_defh(const double *p4, int n, double p, int scheme, double R, int math_mode,
+                              int32_t *hist_p1, int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets,
+                              double *hist_jets);
+int fjo_cluster_tiled_defh(const double *p4, int n, double p, int scheme, double R, int math_mode, int32_t *hist_p1,
+                           int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets, double *hist_jets);
 /* A batch with one jet definition (and optional per-event R) on nthreads host threads. */
 double fjo_cluster_batch_def(const double *p4, const int64_t *offsets, int n_events, double p, int scheme, double R,
                              const double *R_arr, int nthreads, int method, int32_t *hist_p1, int32_t *hist_p2,
@@ -118,6 +130,13 @@ int fjo_history_children(const int32_t *hp1, const int32_t *hp2, const double *h
  * mode 1 dcut = param.  out holds up to n jets. */
 int fjo_exclusive_jets(const double *p4, const int32_t *hp1, const int32_t *hp2, const double *hdij, int n, int mode,
                        double param, double *out, int32_t *n_out);
+
+/* FastJet's exclusive jets of a clustering under a general jet definition
+ * (recombination scheme `scheme`), from its stored history momenta hist_jets
+ * (fjo_cluster_*_defh): mode and param as fjo_exclusive_jets. */
+int fjo_exclusive_jets_def(const double *p4, const int32_t *hp1, const int32_t *hp2, const double *hdij,
+                           const double *hist_jets, int n, int scheme, int mode, double param, double *out,
+                           int32_t *n_out);
 
 /* Pair distances the tiled restatement evaluates clustering one event (its
  * initial nearest neighbours, orphan rescans and merged-jet updates): the

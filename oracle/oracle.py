@@ -33,6 +33,15 @@ def lib():
             f.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp,
                           _vp, _vp, _vp]
             f.restype = ctypes.c_int
+        for name in ("fjo_cluster_n2plain_defh", "fjo_cluster_brute_defh", "fjo_cluster_pairwise_defh",
+                     "fjo_cluster_tiled_defh"):
+            f = getattr(L, name)
+            f.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp,
+                          _vp, _vp, _vp, _vp]
+            f.restype = ctypes.c_int
+        L.fjo_exclusive_jets_def.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_double, _vp, _vp]
+        L.fjo_exclusive_jets_def.restype = ctypes.c_int
         L.fjo_cluster_batch_def.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                             _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
         L.fjo_cluster_batch_def.restype = ctypes.c_double
@@ -99,6 +108,39 @@ def cluster_def(p4, p, scheme, R, method="n2plain"):
     """One event under a general jet definition: genkt exponent p (any finite real) and recombination scheme
     (0 E, 1 pt, 2 pt2, 3 Et, 4 Et2, 5 BIpt, 6 BIpt2, 7 WTA_pt, 8 WTA_modp).  Returns (h1, h2, dij, jets)."""
     return _run_def(f"fjo_cluster_{method}_def", p4, p, scheme, R)
+
+
+def cluster_def_hist(p4, p, scheme, R, method="n2plain"):
+    """cluster_def and the momentum of the history entry each merge step creates (N x 4; rows of
+    beam steps are NaN).  Returns (h1, h2, dij, jets, hist_jets)."""
+    p4 = np.ascontiguousarray(p4, dtype=np.float64).reshape(-1, 4)
+    n = len(p4)
+    h1 = np.zeros(max(n, 1), np.int32)
+    h2 = np.zeros(max(n, 1), np.int32)
+    hd = np.zeros(max(n, 1), np.float64)
+    jets = np.zeros((max(n, 1), 4), np.float64)
+    hj = np.full((max(n, 1), 4), np.nan)
+    nj = np.zeros(1, np.int32)
+    rc = getattr(lib(), f"fjo_cluster_{method}_defh")(_P(p4), n, float(p), int(scheme), float(R), MATH_SHARED, _P(h1),
+                                                       _P(h2), _P(hd), _P(jets), _P(nj), _P(hj))
+    if rc != 0:
+        raise ValueError(f"fjo_cluster_{method}_defh failed ({rc})")
+    return h1[:n], h2[:n], hd[:n], jets[:nj[0]].copy(), hj[:n]
+
+
+def exclusive_jets_def(p4, h1, h2, hd, hist_jets, scheme, mode, param):
+    """Exclusive jets of a clustering under recombination scheme `scheme`, from its stored history
+    momenta (cluster_def_hist)."""
+    p4 = np.ascontiguousarray(p4, np.float64).reshape(-1, 4)
+    h1, h2 = np.ascontiguousarray(h1, np.int32), np.ascontiguousarray(h2, np.int32)
+    hd = np.ascontiguousarray(hd, np.float64)
+    hj = np.ascontiguousarray(hist_jets, np.float64).reshape(-1, 4)
+    out = np.zeros((max(len(p4), 1), 4))
+    k = np.zeros(1, np.int32)
+    if lib().fjo_exclusive_jets_def(_P(p4), _P(h1), _P(h2), _P(hd), _P(hj), len(p4), int(scheme), int(mode),
+                                    float(param), _P(out), _P(k)) != 0:
+        raise ValueError("fjo_exclusive_jets_def failed")
+    return out[:k[0]].copy()
 
 
 def cluster_batch_def(p4, offsets, p, scheme, R, nthreads, method="tiled"):

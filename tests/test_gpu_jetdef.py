@@ -224,3 +224,76 @@ def test_device_entry_point(engine):
            out[3].cpu().numpy().reshape(-1, 4)[:nj])
     want = orc.cluster_def(ev, -0.5, fj.SCHEME_WTA_PT, 0.8)
     assert same(want, got), first_diff(want, got)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", [0, 1, 4, 7, 8])
+def test_history_momenta(engine, scheme):
+    """The momentum each merge step stores (FastJet's ClusterSequence jets), bit for bit."""
+    ev = orc.read_events(os.path.join(GOLDEN, "354_fastjet.dat"))[0]
+    print(f"[gpu jetdef] history momenta 354_fastjet.dat p=0.5 scheme={scheme} R=0.8", flush=True)
+    got = engine.cluster_event_def(ev, 0.5, scheme, 0.8, hist_jets=True)
+    want = orc.cluster_def_hist(ev, 0.5, scheme, 0.8)
+    assert same(want[:4], got[:4]), first_diff(want, got)
+    merge = want[1] >= 0
+    assert want[4][merge].tobytes() == got[4][merge].tobytes()
+
+
+@pytest.mark.gpu
+def test_exclusive_jets_under_schemes(engine):
+    """A batch with per-event definitions writes each merge step's momentum;
+    fjmi_exclusive_jets_def then gives the oracle's exclusive jets by count and by dcut."""
+    import torch
+    import fastjet_gpu_amd as fj
+    p4, off = fj.gen_workload(3, 18, first=1200)
+    ne, tot = len(off) - 1, len(p4)
+    ps = np.array([(1.0, 0.5, -1.0, 0.0, 2.0, -0.5)[e % 6] for e in range(ne)])
+    schemes = np.array([(0, 1, 3, 5, 7, 8, 2, 4, 6)[e % 9] for e in range(ne)], np.int32)
+    Rs = np.full(ne, 1.0)
+    print(f"[gpu jetdef] exclusive jets under schemes: batch x{ne}", flush=True)
+    d = _device_batch(engine, p4, off, (ps, schemes), Rs)
+    hj = torch.zeros(4 * tot, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    b = engine.batch_create_def(off, d["p4"].data_ptr(), d["off"].data_ptr(), d["defs"].data_ptr(), 0.0, 0,
+                                d["R"].data_ptr(), 0.0, d["h1"].data_ptr(), d["h2"].data_ptr(), d["hd"].data_ptr(),
+                                d["jets"].data_ptr(), d["nj"].data_ptr(), hj.data_ptr())
+    try:
+        b.launch()
+        b.sync()
+    finally:
+        b.close()
+    h1, h2, hd = d["h1"].cpu().numpy(), d["h2"].cpu().numpy(), d["hd"].cpu().numpy()
+    hjn = hj.cpu().numpy().reshape(-1, 4)
+    ref = []
+    for e in range(ne):
+        a, c = off[e], off[e + 1]
+        want = orc.cluster_def_hist(p4[a:c], ps[e], int(schemes[e]), 1.0)
+        assert (np.array_equal(want[0], h1[a:c]) and np.array_equal(want[1], h2[a:c])
+                and want[2].tobytes() == hd[a:c].tobytes()), f"event {e}"
+        merge = want[1] >= 0
+        assert want[4][merge].tobytes() == hjn[a:c][merge].tobytes(), f"event {e} history momenta"
+        ref.append(want)
+    off_d = d["off"]
+    for mode, params in ((0, (0, 1, 4, 30)), (1, (0.0, 0.05, 1.0, 1e9))):
+        for param in params:
+            for sch in range(9):  # each call takes one scheme: check the events clustered under it
+                evs = [e for e in range(ne) if schemes[e] == sch]
+                if not evs:
+                    continue
+                out = torch.zeros(4 * tot, dtype=torch.float64, device="cuda")
+                on = torch.full((ne,), -2, dtype=torch.int32, device="cuda")
+                torch.cuda.synchronize()
+                engine.exclusive_jets_def(ne, off_d.data_ptr(), 0, d["p4"].data_ptr(), d["h1"].data_ptr(),
+                                          d["h2"].data_ptr(), d["hd"].data_ptr(), hj.data_ptr(), sch, mode, param,
+                                          out.data_ptr(), on.data_ptr())
+                torch.cuda.synchronize()
+                o, n = out.cpu().numpy().reshape(-1, 4), on.cpu().numpy()
+                for e in evs:
+                    a, c = off[e], off[e + 1]
+                    if mode == 0 and param > c - a:
+                        assert n[e] == -1
+                        continue
+                    w = ref[e]
+                    want = orc.exclusive_jets_def(p4[a:c], w[0], w[1], w[2], w[4], sch, mode, param)
+                    assert n[e] == len(want) and o[a:a + n[e]].tobytes() == want.tobytes(), \
+                        f"event {e} scheme {sch} mode {mode} param {param}"

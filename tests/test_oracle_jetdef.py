@@ -203,3 +203,88 @@ def test_invalid_definitions_are_refused():
     for p, scheme in ((math.nan, E), (math.inf, E), (0.5, 9), (0.5, -1)):
         with pytest.raises(ValueError):
             orc.cluster_def(ev, p, scheme, 0.4)
+
+
+def _entry(p4, scheme, hist_jets, x):
+    """History entry x's momentum: a particle as the scheme preprocesses it, or a merge step's result."""
+    n = len(p4)
+    return hist_jets[x - n] if x >= n else orc.cluster_def(p4[x:x + 1], 1.0, scheme, 1.0)[3][0]
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("seed", [3, 4])
+def test_history_momenta_restatements_and_jets(scheme, seed):
+    """The momenta each merge step stores (FastJet's ClusterSequence jets): equal
+    across the N2Plain, brute-force and tiled restatements, and every beam step
+    promotes exactly the stored momentum of its parent entry (the jets output,
+    pinned by the restatements above)."""
+    rng = np.random.default_rng(seed)
+    ev = random_event(rng, 90)
+    p = PS[(scheme + seed) % len(PS)]
+    res = orc.cluster_def_hist(ev, p, scheme, 0.7)
+    for method in ("brute", "tiled"):
+        other = orc.cluster_def_hist(ev, p, scheme, 0.7, method)
+        assert same(res, other), method
+    h1, h2, hd, jets, hj = res
+    n = len(ev)
+    k = 0
+    for s in range(n):
+        if h2[s] >= 0:
+            assert np.all(np.isfinite(hj[s]))
+        else:
+            assert np.isnan(hj[s]).all()  # beam steps store nothing
+            assert _entry(ev, scheme, hj, h1[s]).tobytes() == jets[k].tobytes()
+            k += 1
+    assert k == len(jets)
+
+
+def test_history_momenta_e_scheme_are_the_parent_sums():
+    rng = np.random.default_rng(11)
+    ev = random_event(rng, 120)
+    h1, h2, hd, jets, hj = orc.cluster_def_hist(ev, -0.4, E, 0.6)
+    for s in range(len(ev)):
+        if h2[s] >= 0:
+            want = _entry(ev, E, hj, h1[s]) + _entry(ev, E, hj, h2[s])
+            assert want.tobytes() == hj[s].tobytes()
+
+
+@pytest.mark.parametrize("p", [1.0, 0.0, -1.0])
+def test_exclusive_jets_def_e_scheme_is_the_replay(p):
+    """Under the E-scheme the stored momenta and a replay of the history sums agree bit for bit."""
+    rng = np.random.default_rng(21)
+    ev = random_event(rng, 80)
+    h1, h2, hd, jets, hj = orc.cluster_def_hist(ev, p, E, 0.8)
+    for nj in (0, 1, 3, 10, 80):
+        a = orc.exclusive_jets(ev, h1, h2, hd, 0, nj)
+        b = orc.exclusive_jets_def(ev, h1, h2, hd, hj, E, 0, nj)
+        assert a.tobytes() == b.tobytes()
+    for dcut in (0.0, 1e-3, 0.5, 10.0, 1e9):
+        a = orc.exclusive_jets(ev, h1, h2, hd, 1, dcut)
+        b = orc.exclusive_jets_def(ev, h1, h2, hd, hj, E, 1, dcut)
+        assert a.tobytes() == b.tobytes()
+
+
+@pytest.mark.parametrize("scheme", [PT, ET2, WTA_PT, WTA_MODP])
+def test_exclusive_jets_def_counts_and_steps(scheme):
+    """n_jets = N gives the preprocessed particles; each n -> n - 1 replaces two of
+    the n jets (or one, sent to the beam) by at most one stored entry; counts are n."""
+    rng = np.random.default_rng(30 + scheme)
+    ev = random_event(rng, 60)
+    n = len(ev)
+    h1, h2, hd, jets, hj = orc.cluster_def_hist(ev, 1.0, scheme, 1.0)
+    parts = orc.exclusive_jets_def(ev, h1, h2, hd, hj, scheme, 0, n)
+    pre = np.array([_entry(ev, scheme, hj, i) for i in range(n)])
+    assert sorted(map(bytes, parts)) == sorted(map(bytes, pre))
+    prev = {bytes(r) for r in parts}
+    for nj in range(n - 1, -1, -1):
+        cur = orc.exclusive_jets_def(ev, h1, h2, hd, hj, scheme, 0, nj)
+        assert len(cur) == nj
+        cur_set = {bytes(r) for r in cur}
+        s = n - nj - 1  # the step between the two stop points
+        gone = prev - cur_set
+        if h2[s] >= 0:
+            assert cur_set - prev == {bytes(hj[s])}
+            assert gone == {bytes(_entry(ev, scheme, hj, h1[s])), bytes(_entry(ev, scheme, hj, h2[s]))}
+        else:
+            assert not (cur_set - prev) and gone == {bytes(_entry(ev, scheme, hj, h1[s]))}
+        prev = cur_set
