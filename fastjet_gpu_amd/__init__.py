@@ -137,6 +137,7 @@ def _declare(lib):
         "fjmi_cluster_event_device_def": ([_vp, _vp, i, _vp, d, _vp, _vp, _vp, _vp, _vp, _vp, _vp], i),
         "fjmi_batch_create_def": ([_vp, i, _vp, _vp, _vp, _vp, _vp, _vp, d, _vp, _vp, _vp, _vp, _vp, _vp,
                                    ctypes.POINTER(_vp)], i),
+        "fjmi_cluster_batch_def": ([_vp, i, _vp, _vp, _vp, _vp, _vp, d, _vp, _vp, _vp, _vp, _vp, _vp], i),
         "fjmi_exclusive_jets_def": ([_vp, i, _vp, i, _vp, _vp, _vp, _vp, _vp, i, i, d, _vp, _vp, _vp], i),
         "fjmi_batch_launch": ([_vp, _vp], i),
         "fjmi_batch_sync": ([_vp, _vp], i),
@@ -256,6 +257,34 @@ class Engine:
             float(R) if Ra is None else 0.0, _ptr(h1), _ptr(h2), _ptr(hd), _ptr(jets), _ptr(nj)),
             "fjmi_cluster_batch")
         return h1, h2, hd, jets, nj[:ne]
+
+    def cluster_batch_def(self, p4, offsets, p, scheme, R, hist_jets=False):
+        """Clusters many events from host memory under general jet definitions: p and scheme scalars
+        (one definition) or per-event arrays; R scalar or per event.  Returns (hist_p1, hist_p2, hist_dij,
+        jets, njets) and, with hist_jets=True, the per-particle history momenta."""
+        p4 = np.ascontiguousarray(p4, dtype=np.float64).reshape(-1, 4)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        ne, tot = len(offsets) - 1, len(p4)
+        h1 = np.zeros(tot, np.int32)
+        h2 = np.zeros(tot, np.int32)
+        hd = np.zeros(tot, np.float64)
+        jets = np.zeros((max(tot, 1), 4), np.float64)
+        nj = np.zeros(max(ne, 1), np.int32)
+        hj = np.full((max(tot, 1), 4), np.nan) if hist_jets else None
+        defs = None
+        jd = JetDef(0.0, 0, 0)
+        if np.ndim(p) or np.ndim(scheme):
+            defs = np.zeros(ne, JETDEF_DTYPE)
+            defs["p"], defs["scheme"] = np.broadcast_to(p, ne), np.broadcast_to(scheme, ne)
+        else:
+            jd = JetDef(float(p), int(scheme), 0)
+        Ra = np.ascontiguousarray(R, dtype=np.float64) if np.ndim(R) else None
+        _check(self.lib, self.lib.fjmi_cluster_batch_def(
+            self.ctx, ne, _ptr(offsets), _ptr(p4), _ptr(defs), ctypes.byref(jd), _ptr(Ra),
+            float(R) if Ra is None else 0.0, _ptr(h1), _ptr(h2), _ptr(hd), _ptr(jets), _ptr(nj), _ptr(hj)),
+            "fjmi_cluster_batch_def")
+        out = (h1, h2, hd, jets, nj[:ne])
+        return out + (hj[:tot],) if hist_jets else out
 
     # -- device-memory entry points (pointers as ints, e.g. torch tensor data_ptr()) --
     def cluster_event_device(self, d_p4, n, p, R, d_h1, d_h2, d_hd, d_jets, d_nj, stream=0, engine=ENGINE_AUTO):

@@ -124,11 +124,11 @@ constexpr int64_t kHostChunkParticles = 1 << 18;  /* and at least this many part
 
 struct HostSlot {
   Plan plan;
-  fjmi::rt::DevBuf d_in, d_off, d_p, d_R, d_h1, d_h2, d_hd, d_jets, d_nj, d_joff, d_jc;
+  fjmi::rt::DevBuf d_in, d_off, d_p, d_R, d_h1, d_h2, d_hd, d_jets, d_nj, d_joff, d_jc, d_defs, d_hj;
   char *pin = nullptr; /* pinned staging */
   size_t pin_cap = 0;
   int32_t *h_h1 = nullptr, *h_h2 = nullptr, *h_nj = nullptr, *h_joff = nullptr;
-  double *h_hd = nullptr, *h_jc = nullptr;
+  double *h_hd = nullptr, *h_jc = nullptr, *h_hj = nullptr; /* h_hj: history momenta, or null */
   hipStream_t s = nullptr;
   std::vector<hipStream_t> fork; /* further buckets' kernels */
   std::vector<hipEvent_t> ev_join;
@@ -142,7 +142,8 @@ struct HostSlot {
   void release() {
     if (s) (void)hipStreamSynchronize(s);
     plan.release();
-    for (fjmi::rt::DevBuf *b : {&d_in, &d_off, &d_p, &d_R, &d_h1, &d_h2, &d_hd, &d_jets, &d_nj, &d_joff, &d_jc})
+    for (fjmi::rt::DevBuf *b :
+         {&d_in, &d_off, &d_p, &d_R, &d_h1, &d_h2, &d_hd, &d_jets, &d_nj, &d_joff, &d_jc, &d_defs, &d_hj})
       b->release();
     if (pin) (void)hipHostFree(pin);
     pin = nullptr;
@@ -559,9 +560,12 @@ int host_alloc_pinned(HostSlot &sl, size_t bytes) {
 }
 
 /* Stages the chunk's events (original indices ev[0..n)) and enqueues its
- * copies in, its kernels and its copies out on the slot's stream. */
+ * copies in, its kernels and its copies out on the slot's stream.  A general
+ * jet definition comes per event (defs, host memory) or for all (def_uni);
+ * want_hj brings the history momenta back too. */
 int host_start(fjmi_context *ctx, HostSlot &sl, const int32_t *ev, int n, const int64_t *offsets, const double *p4,
-               const int32_t *p, const double *R, int p_uni, double R_uni) {
+               const int32_t *p, const double *R, int p_uni, double R_uni, const fjmi::JetDef *defs = nullptr,
+               const fjmi::JetDef *def_uni = nullptr, bool want_hj = false) {
   int rc;
   if (!sl.s) {
     HIPCHK(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking));
@@ -577,22 +581,27 @@ int host_start(fjmi_context *ctx, HostSlot &sl, const int32_t *ev, int n, const 
   sl.particles = P;
   const size_t Pn = (size_t)std::max<int64_t>(P, 1), En = (size_t)n;
   /* pinned layout: in (32 P) | h1 (4 P) | h2 (4 P) | hd (8 P) | jc (32 P) | offsets (8 (E+1)) | p (4 E) |
-   * R (8 E) | queue (4 E) | nj (4 E) | joff (4 (E+1)) | status (4) */
+   * R (8 E) | queue (4 E) | nj (4 E) | joff (4 (E+1)) | status (4, padded to 8) | defs (16 E) | hj (32 P) */
   const size_t o_in = 0, o_h1 = o_in + 32 * Pn, o_h2 = o_h1 + 4 * Pn, o_hd = o_h2 + 4 * Pn, o_jc = o_hd + 8 * Pn,
                o_off = o_jc + 32 * Pn, o_p = o_off + 8 * (En + 1), o_R = o_p + 4 * En, o_q = o_R + 8 * En,
-               o_nj = o_q + 4 * En, o_jo = o_nj + 4 * En, o_st = o_jo + 4 * (En + 1), o_end = o_st + 4;
+               o_nj = o_q + 4 * En, o_jo = o_nj + 4 * En, o_st = o_jo + 4 * (En + 1),
+               o_df = (o_st + 4 + 7) & ~(size_t)7, o_hj = o_df + (defs ? 16 * En : 0),
+               o_end = o_hj + (want_hj ? 32 * Pn : 0);
   if ((rc = host_alloc_pinned(sl, o_end))) return rc;
   double *in = reinterpret_cast<double *>(sl.pin + o_in);
   int64_t *off = reinterpret_cast<int64_t *>(sl.pin + o_off);
   int32_t *pp = reinterpret_cast<int32_t *>(sl.pin + o_p);
   double *RR = reinterpret_cast<double *>(sl.pin + o_R);
   std::memcpy(off, sl.off.data(), 8 * (En + 1));
+  fjmi::JetDef *dd = reinterpret_cast<fjmi::JetDef *>(sl.pin + o_df);
   parallel_for(n, [&](int i) {
     const int e = ev[i];
     std::memcpy(in + 4 * off[i], p4 + 4 * offsets[e], 32 * (size_t)(offsets[e + 1] - offsets[e]));
     if (p) pp[i] = p[e];
     if (R) RR[i] = R[e];
+    if (defs) dd[i] = defs[e];
   });
+  sl.h_hj = want_hj ? reinterpret_cast<double *>(sl.pin + o_hj) : nullptr;
   sl.h_h1 = reinterpret_cast<int32_t *>(sl.pin + o_h1);
   sl.h_h2 = reinterpret_cast<int32_t *>(sl.pin + o_h2);
   sl.h_hd = reinterpret_cast<double *>(sl.pin + o_hd);
@@ -604,12 +613,14 @@ int host_start(fjmi_context *ctx, HostSlot &sl, const int32_t *ev, int n, const 
   if ((rc = sl.d_in.ensure(32 * Pn)) || (rc = sl.d_off.ensure(8 * (En + 1))) || (rc = sl.d_p.ensure(4 * En)) ||
       (rc = sl.d_R.ensure(8 * En)) || (rc = sl.d_h1.ensure(4 * Pn)) || (rc = sl.d_h2.ensure(4 * Pn)) ||
       (rc = sl.d_hd.ensure(8 * Pn)) || (rc = sl.d_jets.ensure(32 * Pn)) || (rc = sl.d_nj.ensure(4 * En)) ||
-      (rc = sl.d_joff.ensure(4 * (En + 1))) || (rc = sl.d_jc.ensure(32 * Pn)))
+      (rc = sl.d_joff.ensure(4 * (En + 1))) || (rc = sl.d_jc.ensure(32 * Pn)) ||
+      (defs && (rc = sl.d_defs.ensure(16 * En))) || (want_hj && (rc = sl.d_hj.ensure(32 * Pn))))
     return rc;
   if ((rc = make_plan(ctx, n, sl.off.data(), sl.d_in.as<double>(), sl.d_off.as<int64_t>(),
                       p ? sl.d_p.as<int32_t>() : nullptr, R ? sl.d_R.as<double>() : nullptr, p_uni, R_uni,
                       sl.d_h1.as<int32_t>(), sl.d_h2.as<int32_t>(), sl.d_hd.as<double>(), sl.d_jets.as<double>(),
-                      sl.d_nj.as<int32_t>(), sl.plan)))
+                      sl.d_nj.as<int32_t>(), sl.plan, defs ? sl.d_defs.as<fjmi::JetDef>() : nullptr, def_uni,
+                      want_hj ? sl.d_hj.as<double>() : nullptr)))
     return rc;
   int32_t *q = reinterpret_cast<int32_t *>(sl.pin + o_q);
   std::memcpy(q, sl.plan.flat.data(), 4 * sl.plan.flat.size());
@@ -618,6 +629,7 @@ int host_start(fjmi_context *ctx, HostSlot &sl, const int32_t *ev, int n, const 
   HIPCHK(hipMemcpyAsync(sl.d_off.ptr, off, 8 * (En + 1), hipMemcpyHostToDevice, s));
   if (p) HIPCHK(hipMemcpyAsync(sl.d_p.ptr, pp, 4 * En, hipMemcpyHostToDevice, s));
   if (R) HIPCHK(hipMemcpyAsync(sl.d_R.ptr, RR, 8 * En, hipMemcpyHostToDevice, s));
+  if (defs) HIPCHK(hipMemcpyAsync(sl.d_defs.ptr, dd, 16 * En, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(sl.plan.queue.ptr, q, 4 * sl.plan.flat.size(), hipMemcpyHostToDevice, s));
   /* the kernels: queue heads and status cleared, then each bucket's persistent
    * kernel (buckets after the first on streams of their own, joined again) */
@@ -671,6 +683,7 @@ int host_start(fjmi_context *ctx, HostSlot &sl, const int32_t *ev, int n, const 
   HIPCHK(hipMemcpyAsync(sl.h_nj, sl.d_nj.ptr, 4 * En, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(sl.h_joff, sl.d_joff.ptr, 4 * (En + 1), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(sl.h_status, sl.plan.d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (want_hj) HIPCHK(hipMemcpyAsync(sl.h_hj, sl.d_hj.ptr, 32 * (size_t)P, hipMemcpyDeviceToHost, s));
   HIPCHK(hipEventRecord(sl.ev_out, s));
   sl.busy = true;
   return FJMI_OK;
@@ -679,7 +692,7 @@ int host_start(fjmi_context *ctx, HostSlot &sl, const int32_t *ev, int n, const 
 /* Waits for a slot's chunk, copies its packed jets out, and writes its results
  * into the caller's arrays at the events' own positions. */
 int host_finish(HostSlot &sl, const int64_t *offsets, int32_t *h1, int32_t *h2, double *hd, double *jets,
-                int32_t *nj) {
+                int32_t *nj, double *hist_jets = nullptr) {
   sl.busy = false;
   HIPCHK(hipEventSynchronize(sl.ev_out));
   const int n = sl.n_events;
@@ -697,6 +710,7 @@ int host_finish(HostSlot &sl, const int64_t *offsets, int32_t *h1, int32_t *h2, 
     const int k = sl.h_nj[i];
     nj[e] = k;
     if (k > 0) std::memcpy(jets + 4 * dst, sl.h_jc + 4 * (size_t)sl.h_joff[i], 32 * (size_t)k);
+    if (hist_jets && sl.h_hj) std::memcpy(hist_jets + 4 * dst, sl.h_hj + 4 * lo, 32 * (size_t)m);
   });
   return FJMI_OK;
 }
@@ -1279,9 +1293,18 @@ int fjmi_batch_destroy(fjmi_batch *b) {
   return FJMI_OK;
 }
 
-int fjmi_cluster_batch(fjmi_context *ctx, int n_events, const int64_t *offsets, const double *p4, const int32_t *p,
-                       const double *R, int p_uni, double R_uni, int32_t *h1, int32_t *h2, double *hd, double *jets,
-                       int32_t *nj) {
+}  // extern "C"
+
+namespace {
+
+/* A host-memory batch: the standard algorithms (defs and def_uni null) or
+ * general jet definitions (per event in host memory, or one for all), with the
+ * history momenta back into hist_jets when it is given. */
+int cluster_batch_host(fjmi_context *ctx, int n_events, const int64_t *offsets, const double *p4, const int32_t *p,
+                       const double *R, int p_uni, double R_uni, const fjmi::JetDef *defs,
+                       const fjmi::JetDef *def_uni, int32_t *h1, int32_t *h2, double *hd, double *jets, int32_t *nj,
+                       double *hist_jets) {
+  const bool gen = defs || def_uni;
   if (!ctx || !offsets || n_events < 0) return fail(FJMI_EINVAL, "bad argument");
   if (n_events == 0) return FJMI_OK;
   const int64_t total = offsets[n_events];
@@ -1296,7 +1319,7 @@ int fjmi_cluster_batch(fjmi_context *ctx, int n_events, const int64_t *offsets, 
     if (n < 0 || offsets[e] < 0) return fail(FJMI_EINVAL, "offsets not monotonic at event %d", e);
     const fjmi_bucket *b = bucket_for(ctx, (int)std::min<int64_t>(n, INT32_MAX), FJMI_ENGINE_AUTO);
     if (!b) return fail(FJMI_ECAPACITY, "event %d has %lld particles, above the largest bucket", e, (long long)n);
-    if (b->engine == FJMI_ENGINE_TILED && n > fjmi::kTiledMaxN)
+    if ((gen || b->engine == FJMI_ENGINE_TILED) && n > fjmi::kTiledMaxN)
       return fail(FJMI_ECAPACITY, "event %d has %lld particles: the tiled engine takes at most %d", e, (long long)n,
                   fjmi::kTiledMaxN);
   }
@@ -1337,8 +1360,9 @@ int fjmi_cluster_batch(fjmi_context *ctx, int n_events, const int64_t *offsets, 
   for (int c = 0; c < n_chunks; ++c) { /* a slot's earlier chunk must be finished before it is reused */
     if (cut[c + 1] == cut[c]) continue;
     HostSlot &sl = hp.slot[c % kHostSlots];
-    if (sl.busy && (rc = host_finish(sl, offsets, h1, h2, hd, jets, nj))) return abort_all(rc);
-    if ((rc = host_start(ctx, sl, order.data() + cut[c], cut[c + 1] - cut[c], offsets, p4, p, R, p_uni, R_uni)))
+    if (sl.busy && (rc = host_finish(sl, offsets, h1, h2, hd, jets, nj, hist_jets))) return abort_all(rc);
+    if ((rc = host_start(ctx, sl, order.data() + cut[c], cut[c + 1] - cut[c], offsets, p4, p, R, p_uni, R_uni, defs,
+                         def_uni, hist_jets != nullptr)))
       return abort_all(rc);
   }
   /* finished in completion order: the first chunk holds the largest events and
@@ -1361,9 +1385,38 @@ int fjmi_cluster_batch(fjmi_context *ctx, int n_events, const int64_t *offsets, 
       std::this_thread::sleep_for(std::chrono::microseconds(50));
       continue;
     }
-    if ((rc = host_finish(*ready, offsets, h1, h2, hd, jets, nj))) return abort_all(rc);
+    if ((rc = host_finish(*ready, offsets, h1, h2, hd, jets, nj, hist_jets))) return abort_all(rc);
   }
   return FJMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fjmi_cluster_batch(fjmi_context *ctx, int n_events, const int64_t *offsets, const double *p4, const int32_t *p,
+                       const double *R, int p_uni, double R_uni, int32_t *h1, int32_t *h2, double *hd, double *jets,
+                       int32_t *nj) {
+  return cluster_batch_host(ctx, n_events, offsets, p4, p, R, p_uni, R_uni, nullptr, nullptr, h1, h2, hd, jets, nj,
+                            nullptr);
+}
+
+int fjmi_cluster_batch_def(fjmi_context *ctx, int n_events, const int64_t *offsets, const double *p4,
+                           const fjmi_jetdef *defs, const fjmi_jetdef *def, const double *R, double R_uni,
+                           int32_t *h1, int32_t *h2, double *hd, double *jets, int32_t *nj, double *hist_jets) {
+  fjmi::JetDef d{};
+  int rc;
+  if (defs) {
+    for (int e = 0; e < n_events; ++e) {
+      fjmi::JetDef x;
+      if ((rc = check_def(defs + e, &x))) return fail(rc, "event %d: %s", e, fjmi_last_error());
+    }
+  } else if ((rc = check_def(def, &d))) {
+    return rc;
+  }
+  return cluster_batch_host(ctx, n_events, offsets, p4, nullptr, R, 0, R_uni,
+                            reinterpret_cast<const fjmi::JetDef *>(defs), defs ? nullptr : &d, h1, h2, hd, jets, nj,
+                            hist_jets);
 }
 
 }  // extern "C"

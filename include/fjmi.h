@@ -276,6 +276,14 @@ int fjmi_batch_create_def(fjmi_context *ctx, int n_events, const int64_t *h_offs
                           const double *d_R, double R_uni, int32_t *d_hist_p1, int32_t *d_hist_p2,
                           double *d_hist_dij, double *d_jets, int32_t *d_njets, double *d_hist_jets,
                           fjmi_batch **out);
+/* A host-memory batch (as fjmi_cluster_batch: pinned chunks, copies overlapped
+ * with the kernels) with one definition per event (defs, host memory) or one
+ * for all (def; defs NULL), per-event radii R or R_uni, and optionally the
+ * history momenta into hist_jets (4 doubles per particle). */
+int fjmi_cluster_batch_def(fjmi_context *ctx, int n_events, const int64_t *offsets, const double *p4,
+                           const fjmi_jetdef *defs, const fjmi_jetdef *def, const double *R, double R_uni,
+                           int32_t *hist_p1, int32_t *hist_p2, double *hist_dij, double *jets, int32_t *njets,
+                           double *hist_jets);
 /* Exclusive jets (as fjmi_exclusive_jets) of a clustering under a general jet
  * definition with recombination scheme `scheme`: the entries alive at the stop
  * point, particles as the scheme preprocesses them and merged entries from

@@ -35,9 +35,10 @@ def test_header_parse_finds_every_entry_point():
                  "fjmi_server_submit", "fjmi_server_poll", "fjmi_server_release", "fjmi_server_destroy",
                  "fjmi_stream_check", "fjmi_cluster_event_device_noalloc", "fjmi_inclusive_jets",
                  "fjmi_inclusive_pseudojets", "fjmi_history_view", "fjmi_exclusive_jets", "fjmi_cluster_event_def",
-                 "fjmi_cluster_event_device_def", "fjmi_batch_create_def", "fjmi_exclusive_jets_def"):
+                 "fjmi_cluster_event_device_def", "fjmi_batch_create_def", "fjmi_exclusive_jets_def",
+                 "fjmi_cluster_batch_def"):
         assert must in names
-    assert len(names) == 35
+    assert len(names) == 36
 
 
 def test_scheme_numbers_agree_with_the_kernels_header(tmp_path):

@@ -297,3 +297,32 @@ def test_exclusive_jets_under_schemes(engine):
                     want = orc.exclusive_jets_def(p4[a:c], w[0], w[1], w[2], w[4], sch, mode, param)
                     assert n[e] == len(want) and o[a:a + n[e]].tobytes() == want.tobytes(), \
                         f"event {e} scheme {sch} mode {mode} param {param}"
+
+
+@pytest.mark.gpu
+def test_host_memory_batch(engine):
+    """fjmi_cluster_batch_def from host arrays (pinned chunks): per-event
+    definitions and radii with the history momenta, then one definition for all."""
+    import fastjet_gpu_amd as fj
+    p4, off = fj.gen_workload(3, 96, first=2000)
+    ne = len(off) - 1
+    ps = np.array([(0.5, -1.0, 1.0, -0.3, 0.0, 1.7)[e % 6] for e in range(ne)])
+    schemes = np.array([e % 9 for e in range(ne)], np.int32)
+    Rs = np.array([(0.4, 0.7, 1.0)[e % 3] for e in range(ne)])
+    print(f"[gpu jetdef] host-memory batch x{ne} per-event definitions", flush=True)
+    h1, h2, hd, jets, nj, hj = engine.cluster_batch_def(p4, off, ps, schemes, Rs, hist_jets=True)
+    for e in range(ne):
+        a, c = off[e], off[e + 1]
+        w = orc.cluster_def_hist(p4[a:c], ps[e], int(schemes[e]), Rs[e])
+        assert same(w[:4], (h1[a:c], h2[a:c], hd[a:c], jets[a:a + nj[e]])), f"event {e}"
+        merge = w[1] >= 0
+        assert w[4][merge].tobytes() == hj[a:c][merge].tobytes(), f"event {e} history momenta"
+    print(f"[gpu jetdef] host-memory batch x{ne} one definition p=-1 WTA_modp R=0.5", flush=True)
+    h1, h2, hd, jets, nj = engine.cluster_batch_def(p4, off, -1.0, fj.SCHEME_WTA_MODP, 0.5)
+    for e in range(0, ne, 7):
+        a, c = off[e], off[e + 1]
+        w = orc.cluster_def(p4[a:c], -1.0, fj.SCHEME_WTA_MODP, 0.5)
+        assert same(w, (h1[a:c], h2[a:c], hd[a:c], jets[a:a + nj[e]])), f"event {e}"
+    bad = np.array([0.5] * ne)
+    with pytest.raises(fj.FjmiError):
+        engine.cluster_batch_def(p4, off, bad, np.array([0] * (ne - 1) + [12], np.int32), 0.4)
