@@ -30,6 +30,10 @@ constexpr int kTiledMaxN = 65535;
 extern const void *const kTiledKernels[kNumTiledKernels];
 extern const void *const kTiledServeKernels[kNumTiledKernels]; /* same, serving mode (LaunchArgs::serve) */
 extern const void *const kTiledGenKernels[kNumTiledKernels];   /* same, general jet definitions (LaunchArgs::def_*) */
+/* large events: 8 waves per event (512 threads; the table's `threads` field of a
+ * tiled row), one event per CU */
+constexpr int kTiledCoopThreads = 512;
+extern const void *const kTiledCoopKernels[kNumTiledKernels];
 extern const int kTiledMaxTiles[kNumTiledKernels];
 
 /* post-processing (postproc.hip), 256 threads, one workgroup per event */

@@ -187,7 +187,9 @@ int check_table(const fjmi_bucket *rows, int n) {
     const fjmi_bucket &b = rows[i];
     if (i > 0 && b.max_n <= rows[i - 1].max_n) return fail(FJMI_EINVAL, "bucket table not sorted by max_n");
     if (b.engine == FJMI_ENGINE_TILED) {
-      if (b.threads != 64) return fail(FJMI_EINVAL, "tiled rows run 64 threads");
+      if (b.threads != 64 && b.threads != fjmi::kTiledCoopThreads)
+        return fail(FJMI_EINVAL, "tiled rows run 64 threads (one wave per event) or %d (%d waves per event)",
+                    fjmi::kTiledCoopThreads, fjmi::kTiledCoopThreads / 64);
       if (b.slots != 0 && (b.slots < 4 || b.slots > 16))
         return fail(FJMI_EINVAL, "tiled rows: slots is the tile side in quarters of R, 4..16 (0 = 4)");
       if (b.p4_global < 0 || b.p4_global >= fjmi::kNumTiledKernels)
@@ -251,6 +253,13 @@ bool latency_prefers_dense(int n, double R) {
   return (double)n <= nc;
 }
 
+/* Whether one event alone runs faster on the cooperative kernel (8 waves share
+ * gathers of several chunks) than on one wave: its 3x3 neighbourhoods hold
+ * about 0.14 N R^2 jets.  Measured on MI355X, anti-kt, 384 tiles: at R = 0.4,
+ * 0.96x at N = 2000, 1.07x at 9278, 1.17x at 20000; at R = 0.8, 0.98x at
+ * N = 2000, 1.33x at 9278, 1.67x at 20000.  So: N R^2 > 1200. */
+bool latency_prefers_coop(int n, double R) { return (double)n * R * R > 1200.0; }
+
 /* The tiled row for one event alone (tile side R): the 384-tile variant, the
  * shortest step at 3 waves' registers, except for small R and large N, where
  * the 512-tile variant's finer grid and full register budget win (measured at
@@ -259,7 +268,8 @@ bool latency_prefers_dense(int n, double R) {
  * default table's variants hold more events per CU, which a batch needs and a
  * single event does not. */
 fjmi_bucket latency_tiled_row(int n, double R) {
-  return fjmi_bucket{0x7fffffff, FJMI_ENGINE_TILED, 64, 0, (R < 0.3 && n > 2000) ? 0 : 1};
+  return fjmi_bucket{0x7fffffff, FJMI_ENGINE_TILED, latency_prefers_coop(n, R) ? fjmi::kTiledCoopThreads : 64, 0,
+                     (R < 0.3 && n > 2000) ? 0 : 1};
 }
 
 LaunchArgs base_args() {
@@ -377,8 +387,12 @@ int cluster_one_device(fjmi_context *ctx, fjmi::rt::StreamState &st, int engine,
   }
   if (tiled) {
     void *params[] = {&a, &ts};
-    const void *fn = def ? fjmi::kTiledGenKernels[b->p4_global] : fjmi::kTiledKernels[b->p4_global];
-    HIPCHK(hipLaunchKernel(fn, dim3(1), dim3(64), params, 0, stream));
+    /* general definitions run one wave per event; a row of several waves
+     * per event runs the cooperative kernel */
+    const bool coop = !def && b->threads == fjmi::kTiledCoopThreads;
+    const void *fn = def ? fjmi::kTiledGenKernels[b->p4_global]
+                         : (coop ? fjmi::kTiledCoopKernels[b->p4_global] : fjmi::kTiledKernels[b->p4_global]);
+    HIPCHK(hipLaunchKernel(fn, dim3(1), dim3(coop ? fjmi::kTiledCoopThreads : 64), params, 0, stream));
   } else if ((rc = launch_dense(*kd, 1, dyn_lds_bytes(*kd), a, stream))) {
     return rc;
   }
@@ -415,8 +429,10 @@ int make_plan(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const d
   const bool gen = d_defs || def_uni;
   std::vector<fjmi_bucket> rows(ctx->buckets.begin(), ctx->buckets.end());
   if (gen)
-    for (fjmi_bucket &r : rows)
+    for (fjmi_bucket &r : rows) {
       if (r.engine != FJMI_ENGINE_TILED) r = fjmi_bucket{r.max_n, FJMI_ENGINE_TILED, 64, 0, 1};
+      r.threads = 64; /* the general kernels run one wave per event */
+    }
   pl.nb = nb;
   pl.runs.clear();
   pl.flat.clear();
@@ -467,8 +483,14 @@ int make_plan(fjmi_context *ctx, int n_events, const int64_t *h_offsets, const d
     BucketRun r;
     r.tiled = bk.engine == FJMI_ENGINE_TILED;
     r.kd = r.tiled ? nullptr : find_kernel(bk.threads, bk.slots, bk.p4_global);
-    r.fn = r.tiled ? (gen ? fjmi::kTiledGenKernels[bk.p4_global] : fjmi::kTiledKernels[bk.p4_global]) : r.kd->fn;
-    r.threads = r.tiled ? 64 : r.kd->T;
+    /* a row of several waves per event runs the cooperative kernel while its
+     * bucket holds at most one event per CU; with more events, one wave per
+     * event keeps more of them in flight (10 per CU), which a batch needs */
+    const bool coop = r.tiled && !gen && bk.threads == fjmi::kTiledCoopThreads && (int)lists[b].size() <= ctx->n_cu;
+    r.fn = r.tiled ? (gen ? fjmi::kTiledGenKernels[bk.p4_global]
+                          : (coop ? fjmi::kTiledCoopKernels[bk.p4_global] : fjmi::kTiledKernels[bk.p4_global]))
+                   : r.kd->fn;
+    r.threads = r.tiled ? (coop ? fjmi::kTiledCoopThreads : 64) : r.kd->T;
     r.lds = r.tiled ? 0 : dyn_lds_bytes(*r.kd);
     r.ts = r.tiled ? fjmi::tiled_scratch_carve(pl.tscratch.ptr, h_offsets[n_events], fjmi::kTiledPoolPerEvent)
                    : fjmi::TiledScratch{};
@@ -745,7 +767,14 @@ int fjmi_kernel_configs(fjmi_bucket *out, int cap) {
     r.p4_global = k;
     out[fjmi::kNumDenseKernels + k] = r;
   }
-  return fjmi::kNumDenseKernels + fjmi::kNumTiledKernels;
+  for (int k = 0; k < 2; ++k) { /* the cooperative kernels: 512 and 384 tiles, 8 waves per event */
+    if (fjmi::kNumDenseKernels + fjmi::kNumTiledKernels + k >= cap) break;
+    fjmi_bucket r = kTiledRow;
+    r.threads = fjmi::kTiledCoopThreads;
+    r.p4_global = k;
+    out[fjmi::kNumDenseKernels + fjmi::kNumTiledKernels + k] = r;
+  }
+  return fjmi::kNumDenseKernels + fjmi::kNumTiledKernels + 2;
 }
 
 int fjmi_create(int device, fjmi_context **out) {

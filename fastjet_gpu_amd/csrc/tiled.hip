@@ -679,6 +679,223 @@ __device__ int nn_scan(const SH &sh, const Gather &G, const Ev &ev, int ng, int 
   return __builtin_amdgcn_readlane(bp, ln);
 }
 
+/* ---- cooperative steps (tiled_coop_kernel): several waves per event ----
+ * For large events a step's gathers span many chunks of 64 members, and one
+ * wave issuing every chunk's instructions bounds the step.  In the cooperative
+ * kernels wave 0 runs the event exactly as the one-wave kernels do, but at the
+ * member pass and at each orphan's scan over a gather of several chunks it posts
+ * a command in LDS, and every wave of the workgroup takes chunks wid, wid + NW,
+ * ... of it.  Waves meet at two workgroup barriers per command (posted, done).
+ * The results are the one-wave kernel's: per-member updates do not depend on
+ * each other, the lists of changed entries and orphans are sets (their order
+ * decides nothing), and the winners are lexicographic (distance, label) minima
+ * over all candidates, whichever wave saw them. */
+constexpr int kCoopMaxWaves = 16;
+enum : int { COOP_EXIT = 0, COOP_MEMBERS = 1, COOP_SCAN = 2 };
+struct CoopCmd {
+  int op;
+  int ng, gsel;               /* the gather: tiles, and which (0: the step's, 1: the rescan's) */
+  int phys_hi, phys_lo, pt, o; /* the step's parents, the relabelled jet; the scanned orphan */
+  double nrap, nphi, nw, R2;  /* the merged jet */
+  double orap, ophi;          /* the scanned orphan */
+  JetRec *rec;                /* the event's views */
+  unsigned short *pool;
+  /* each wave's lexicographic nearest candidate */
+  double bd[kCoopMaxWaves], bw[kCoopMaxWaves];
+  int bl[kCoopMaxWaves], bp[kCoopMaxWaves];
+};
+
+/* Wave wid's share of a member pass over a gather of several chunks (the
+ * one-wave pass's multi-chunk path, chunk by chunk); its nearest candidate for
+ * the merged jet goes to slot wid. */
+template <class SH>
+__device__ void coop_members(SH &sh, const Gather &G, const Ev &ev, const CoopCmd &c, int wid, int nwaves, int lane,
+                             CoopCmd *out) {
+  const int total = G.pref[c.ng];
+  const int phys_hi = c.phys_hi, phys_lo = c.phys_lo, pt = c.pt;
+  double cbd = c.R2, cbw = 0.0;
+  int cbl = BIG, cbp = -1;
+  for (int j0 = wid * W; j0 < total; j0 += nwaves * W) {
+    const int q = gathered(sh, G, ev, c.ng, j0 + lane);
+    bool orph = false, chg = false;
+    int chg_key = 0, chg_tile = -1;
+    unsigned long long bits = ~0ull;
+    if (q >= 0 && q != phys_lo) {
+      const int4 mq = ev.rec[q].meta;
+      double dq = ev.rec[q].dij;
+      int nn = mq.x;
+      orph = (mq.x == phys_hi) || (phys_lo >= 0 && mq.x == phys_lo);
+      chg = q == pt && !orph; /* relabelled */
+      if (phys_lo >= 0) {
+        const double2 rq = ev.rec[q].rp;
+        const double2 wq = ev.rec[q].wn;
+        const double d = fjmi_dr2(rq.x, rq.y, c.nrap, c.nphi);
+        if (!orph && d < wq.y) {
+          dq = d * (c.nw < wq.x ? c.nw : wq.x);
+          ev.rec[q].meta.x = phys_lo;
+          ev.rec[q].wn = make_double2(wq.x, d);
+          ev.rec[q].dij = dq;
+          nn = phys_lo;
+          chg = true;
+        }
+        if (d < c.R2 && (d < cbd || (d == cbd && mq.y < cbl))) {
+          cbd = d;
+          cbl = mq.y;
+          cbp = q;
+          cbw = wq.x;
+        }
+      }
+      bits = (unsigned long long)__double_as_longlong(dq);
+      chg_key = tkey(mq.y, nn);
+      chg_tile = mq.z;
+    }
+    const unsigned long long cb = __ballot(chg);
+    if (cb) {
+      int cbase = 0;
+      if (lane == 0) cbase = atomicAdd(&sh.nchg, (int)__popcll(cb));
+      cbase = uni(cbase);
+      const int idx = cbase + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(cb >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((unsigned)cb, 0u));
+      if (chg && idx < W) {
+        sh.mt_d[idx] = bits;
+        sh.mt_l[idx] = chg_key;
+        sh.mt_p[idx] = q;
+        sh.mt_t[idx] = chg_tile;
+      }
+    }
+    const unsigned long long ob = __ballot(orph);
+    if (ob) {
+      int obase = 0;
+      if (lane == 0) obase = atomicAdd(&sh.norph, (int)__popcll(ob));
+      obase = uni(obase);
+      const int idx = obase + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(ob >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((unsigned)ob, 0u));
+      if (orph && idx < MAXO) sh.orph[idx] = q;
+    }
+  }
+  double md;
+  int ml;
+  const int ln = wave_argmin(cbd, cbl, &md, &ml);
+  const int bp = ln < 0 ? -1 : __builtin_amdgcn_readlane(cbp, ln);
+  const double bw = ln < 0 ? 0.0 : readlane_d(cbw, ln);
+  if (lane == 0) {
+    out->bd[wid] = md;
+    out->bl[wid] = ln < 0 ? BIG : ml;
+    out->bp[wid] = bp;
+    out->bw[wid] = bw;
+  }
+}
+
+/* Wave wid's share of an orphan's scan (nn_scan over chunks wid, wid + NW, ...
+ * in blocks of NCH chunks); its nearest candidate goes to slot wid. */
+template <int NCH, class SH>
+__device__ void coop_scan(const SH &sh, const Gather &G, const Ev &ev, const CoopCmd &c, int wid, int nwaves, int lane,
+                          CoopCmd *out) {
+  const int total = G.pref[c.ng];
+  double bd = c.R2, bw = 0.0;
+  int bl = BIG, bp = -1;
+  for (int j0 = wid * NCH * W; j0 < total; j0 += nwaves * NCH * W) {
+    int q[NCH];
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) q[k] = j0 + k * W < total ? gathered(sh, G, ev, c.ng, j0 + k * W + lane) : -1;
+    double2 rp[NCH];
+    double wq[NCH];
+    int lab[NCH];
+#pragma unroll
+    for (int k = 0; k < NCH; ++k)
+      if (q[k] >= 0) {
+        rp[k] = ev.rec[q[k]].rp;
+        wq[k] = ev.rec[q[k]].wn.x;
+        lab[k] = ev.rec[q[k]].meta.y;
+      }
+#pragma unroll
+    for (int k = 0; k < NCH; ++k)
+      if (q[k] >= 0 && q[k] != c.o) {
+        const double d = fjmi_dr2(c.orap, c.ophi, rp[k].x, rp[k].y);
+        if (d < c.R2 && (d < bd || (d == bd && lab[k] < bl))) {
+          bd = d;
+          bl = lab[k];
+          bp = q[k];
+          bw = wq[k];
+        }
+      }
+  }
+  double md;
+  int ml;
+  const int ln = wave_argmin(bd, bl, &md, &ml);
+  const int p = ln < 0 ? -1 : __builtin_amdgcn_readlane(bp, ln);
+  const double w = ln < 0 ? 0.0 : readlane_d(bw, ln);
+  if (lane == 0) {
+    out->bd[wid] = md;
+    out->bl[wid] = ln < 0 ? BIG : ml;
+    out->bp[wid] = p;
+    out->bw[wid] = w;
+  }
+}
+
+/* The lexicographic (distance, label) best of the waves' slots (wave 0, after
+ * the command's second barrier): its slot index, or -1 if no wave found one. */
+__device__ __forceinline__ int coop_best(const CoopCmd *cc, int nwaves) {
+  int best = -1;
+  for (int k = 0; k < nwaves; ++k) {
+    const double d = cc->bd[k];
+    const int l = cc->bl[k];
+    if (l != BIG && (best < 0 || d < cc->bd[best] || (d == cc->bd[best] && l < cc->bl[best]))) best = k;
+  }
+  return best;
+}
+
+/* Waves 1 .. NW-1 of a cooperative kernel: run wave 0's commands until it posts
+ * COOP_EXIT. */
+template <int NCH, class SH>
+__device__ void coop_helper(SH &sh, CoopCmd *cc, int wid, int nwaves, int lane) {
+  for (;;) {
+    __syncthreads(); /* posted */
+    const CoopCmd c = *cc;
+    if (c.op == COOP_EXIT) return;
+    Ev ev;
+    ev.rec = c.rec;
+    ev.pool = c.pool;
+    ev.p4 = nullptr;
+    ev.hist = nullptr;
+    ev.lab2 = nullptr;
+    ev.pool_size = 0;
+    if (c.op == COOP_MEMBERS)
+      coop_members(sh, sh.g[c.gsel], ev, c, wid, nwaves, lane, cc);
+    else
+      coop_scan<1>(sh, sh.g[c.gsel], ev, c, wid, nwaves, lane, cc);
+    __syncthreads(); /* done */
+  }
+}
+
+/* nn_scan by all waves of a cooperative kernel (called by wave 0), a chunk of
+ * 64 members per wave at a time; a gather of one chunk is scanned by wave 0
+ * alone (NCH chunks in flight), which costs no barrier. */
+template <int NCH, class SH>
+__device__ int nn_scan_coop(SH &sh, CoopCmd *cc, int nwaves, int gsel, const Ev &ev, int ng, int o, double orap,
+                            double ophi, double R2, int lane, double *best_d, double *best_w, int *best_l = nullptr) {
+  if (sh.g[gsel].pref[ng] <= W) return nn_scan<NCH>(sh, sh.g[gsel], ev, ng, o, orap, ophi, R2, lane, best_d, best_w, best_l);
+  if (lane == 0) {
+    cc->op = COOP_SCAN;
+    cc->ng = ng;
+    cc->gsel = gsel;
+    cc->o = o;
+    cc->orap = orap;
+    cc->ophi = ophi;
+    cc->R2 = R2;
+    cc->rec = ev.rec;
+    cc->pool = ev.pool;
+  }
+  __syncthreads(); /* posted */
+  coop_scan<1>(sh, sh.g[gsel], ev, *cc, 0, nwaves, lane, cc);
+  __syncthreads(); /* done */
+  const int k = coop_best(cc, nwaves);
+  *best_d = k < 0 ? R2 : cc->bd[k];
+  *best_w = k < 0 ? 0.0 : cc->bw[k];
+  if (best_l) *best_l = k < 0 ? BIG : cc->bl[k];
+  return k < 0 ? -1 : cc->bp[k];
+}
+
 /* Nearest neighbour (lexicographic (distance, label), within R) of jet o at
  * (oy, ophi) in tile otile over its 3x3 neighbourhood: its own tile first, then
  * only the neighbour tiles that could hold a jet at most as far as the best so
@@ -687,13 +904,20 @@ __device__ int nn_scan(const SH &sh, const Gather &G, const Ev &ev, int ng, int 
  * the rounding of the tile boundaries).  For dense events, whose 3x3
  * neighbourhoods span several chunks of 64; a sparse neighbourhood is one
  * chunk, scanned faster in one pass. */
-template <int NCH, class SH>
+template <int NCH, int NWV = 1, class SH>
 __device__ int nn_rescan(SH &sh, Gather &GO, const Geo &geo, const Ev &ev, int o, int otile, double oy, double ophi,
-                         double R2, int lane, unsigned &stampv, double *best_d, double *best_w) {
+                         double R2, int lane, unsigned &stampv, double *best_d, double *best_w,
+                         CoopCmd *cc = nullptr) {
+  /* NWV > 1: the scans are shared by the NWV waves of a cooperative kernel
+   * (GO is the rescan gather, sh.g[1]) */
   int ngo = gather_tiles(sh, GO, lane == 0 ? otile : -1, lane, stampv);
   double d1, w1;
   int l1;
-  const int n1 = nn_scan<NCH>(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &d1, &w1, &l1);
+  int n1;
+  if constexpr (NWV > 1)
+    n1 = nn_scan_coop<NCH>(sh, cc, NWV, 1, ev, ngo, o, oy, ophi, R2, lane, &d1, &w1, &l1);
+  else
+    n1 = nn_scan<NCH>(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &d1, &w1, &l1);
   int cand = -1;
   if (lane < 9 && lane != 4) {
     const int ty = geo.row(otile), tp = otile - ty * geo.nphi, dy = lane / 3 - 1, dp = lane % 3 - 1;
@@ -714,7 +938,11 @@ __device__ int nn_rescan(SH &sh, Gather &GO, const Geo &geo, const Ev &ev, int o
   ngo = gather_tiles(sh, GO, cand, lane, stampv);
   double d2, w2;
   int l2;
-  const int n2 = nn_scan<NCH>(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &d2, &w2, &l2);
+  int n2;
+  if constexpr (NWV > 1)
+    n2 = nn_scan_coop<NCH>(sh, cc, NWV, 1, ev, ngo, o, oy, ophi, R2, lane, &d2, &w2, &l2);
+  else
+    n2 = nn_scan<NCH>(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &d2, &w2, &l2);
   const bool take2 = n2 >= 0 && (n1 < 0 || d2 < d1 || (d2 == d1 && l2 < l1));
   *best_d = take2 ? d2 : d1;
   *best_w = take2 ? w2 : w1;
@@ -823,6 +1051,8 @@ __device__ __forceinline__ void serve_finish(const LaunchArgs &a, int slot, int6
 template <int MAXT, int WPE, bool SERVE>
 __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScratch ts) {
   constexpr bool GEN = false;
+  constexpr int NW = 1;
+  CoopCmd *const coop = nullptr;
 #include "tiled_body.inc"
 }
 
@@ -831,6 +1061,21 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
 template <int MAXT, int WPE>
 __global__ __launch_bounds__(64, WPE) void tiled_gen_kernel(LaunchArgs a, TiledScratch ts) {
   constexpr bool SERVE = false, GEN = true;
+  constexpr int NW = 1;
+  CoopCmd *const coop = nullptr;
+#include "tiled_body.inc"
+}
+
+/* large events: NW waves per event, 2 waves per SIMD (the full register
+ * budget), one event per CU at NW = 8; wave 0 runs the event and the others
+ * share its gathers of several chunks */
+template <int MAXT, int NW>
+__global__ __launch_bounds__(64 * NW, 2) void tiled_coop_kernel(LaunchArgs a, TiledScratch ts) {
+  static_assert(NW >= 2 && NW <= kCoopMaxWaves, "waves per event");
+  constexpr bool SERVE = false, GEN = false;
+  constexpr int WPE = 2;
+  __shared__ CoopCmd coop_cmd;
+  CoopCmd *const coop = &coop_cmd;
 #include "tiled_body.inc"
 }
 
@@ -851,6 +1096,11 @@ const void *const kTiledServeKernels[kNumTiledKernels] = {
 const void *const kTiledGenKernels[kNumTiledKernels] = {
     reinterpret_cast<const void *>(&tiled_gen_kernel<512, 2>), reinterpret_cast<const void *>(&tiled_gen_kernel<384, 3>),
     reinterpret_cast<const void *>(&tiled_gen_kernel<320, 3>), reinterpret_cast<const void *>(&tiled_gen_kernel<256, 3>)};
+/* large events, 8 waves per event: variants 0 and 1 (the other variants'
+ * rows take these too) */
+const void *const kTiledCoopKernels[kNumTiledKernels] = {
+    reinterpret_cast<const void *>(&tiled_coop_kernel<512, 8>), reinterpret_cast<const void *>(&tiled_coop_kernel<384, 8>),
+    reinterpret_cast<const void *>(&tiled_coop_kernel<384, 8>), reinterpret_cast<const void *>(&tiled_coop_kernel<384, 8>)};
 const int kTiledMaxTiles[kNumTiledKernels] = {512, 384, 320, 256};
 
 }  // namespace fjmi

@@ -38,30 +38,35 @@ inline double2 make_double2(double a, double b) { return double2{a, b}; }
 inline double4 make_double4(double a, double b, double c, double d) { return double4{a, b, c, d}; }
 
 namespace emu {
-/* Lanes are fibers on one OS thread.  A lane runs until it reaches a cross-lane
- * operation (or the end of the kernel), then parks; when every live lane is
- * parked at the same operation, the results are computed and the lanes resume
- * in order 0..63. */
-extern int lane_cur;
+/* Lanes are fibers on one OS thread, 64 per wavefront, one or more wavefronts
+ * per workgroup.  A lane runs until it reaches a cross-lane operation (or the
+ * end of the kernel), then parks; when every live lane of its wavefront is
+ * parked at the same operation, the results are computed from that
+ * wavefront's operands and its lanes resume in order 0..63.  A workgroup
+ * barrier waits for every wavefront. */
+extern int lane_cur; /* lane within the wavefront */
+extern int wave_cur; /* wavefront within the workgroup */
 void park(uint64_t a, uint64_t b, int op); /* stores the lane's operands, yields */
-extern uint64_t ra[64], rb[64];             /* every lane's operands at the last operation */
+const uint64_t *wave_ra();                   /* the current wavefront's operands at its last operation */
+const uint64_t *wave_rb();
 inline void sync() { park(0, 0, 1); }
 inline void all(uint64_t v, uint64_t *out) {
   park(v, 0, 2);
-  std::memcpy(out, ra, sizeof ra);
+  std::memcpy(out, wave_ra(), 64 * sizeof(uint64_t));
 }
 inline void all2(uint64_t v, uint64_t w, uint64_t *o1, uint64_t *o2) {
   park(v, w, 3);
-  std::memcpy(o1, ra, sizeof ra);
-  std::memcpy(o2, rb, sizeof rb);
+  std::memcpy(o1, wave_ra(), 64 * sizeof(uint64_t));
+  std::memcpy(o2, wave_rb(), 64 * sizeof(uint64_t));
 }
 inline void fence() { sync(); }
+inline void barrier() { park(0, 0, 4); }
 }  // namespace emu
 
 struct EmuThreadIdx {
   int x, y, z;
 };
-inline EmuThreadIdx emu_tid() { return EmuThreadIdx{emu::lane_cur, 0, 0}; }
+inline EmuThreadIdx emu_tid() { return EmuThreadIdx{emu::wave_cur * 64 + emu::lane_cur, 0, 0}; }
 #define threadIdx emu_tid()
 
 inline int __builtin_amdgcn_readfirstlane(int v) {
@@ -120,7 +125,7 @@ inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask,
 }
 #define __builtin_amdgcn_s_sleep(n) std::this_thread::yield()
 #define __builtin_amdgcn_fence(o, s) emu::fence()
-inline void __syncthreads() { emu::fence(); }
+inline void __syncthreads() { emu::barrier(); }
 inline void __threadfence_block() { std::atomic_thread_fence(std::memory_order_seq_cst); }
 inline void __threadfence_system() { std::atomic_thread_fence(std::memory_order_seq_cst); }
 inline unsigned long long wall_clock64() {

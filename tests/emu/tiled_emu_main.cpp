@@ -6,6 +6,7 @@
 //   tiled_emu qbound                                      queue indices outside [0, n_queue) leave at once
 //   tiled_emu gen WORKLOAD E0 E1 VARIANT P SCHEME R       general jet definition (real p, scheme)
 // FJMI_EMU_HEADLESS=1 launches without a queue head (the one-event form).
+// FJMI_EMU_COOP=1 runs tiled_coop_kernel (8 wavefronts per event) instead.
 // The tile pool of an event holds POOL_PER_N * N + POOL_EXTRA entries (default
 // 3 N + kTiledPoolPerEvent, as the library allocates); a small pool, with a
 // build with less segment slack (-DFJMI_TILED_SLACK_SHIFT=30 -DFJMI_TILED_SLACK_ADD=0),
@@ -25,10 +26,12 @@ static long g_rebuilds = 0;
 #include <functional>
 namespace emu {
 void run_wave(const std::function<void()> &fn);
+void run_block(int nwaves, const std::function<void()> &fn);
 }
 
 static int g_pool_extra = fjmi::kTiledPoolPerEvent, g_pool_per_n = 3;
 static bool g_headless = false;
+static bool g_coop = false; /* FJMI_EMU_COOP=1: tiled_coop_kernel, 8 waves per event (variants 0 and 1) */
 
 /* def: a general jet definition (tiled_gen_kernel, checked against the
  * oracle's *_def restatements), or null for p in {1, 0, -1} with the E-scheme */
@@ -70,6 +73,8 @@ static int run_event(const double *p4, int N, int p, double R, int tile_q, bool 
   if (def) {
     a.def_uni = *def;
     emu::run_wave([&] { tiled_gen_kernel<MAXT, WPE>(a, ts); });
+  } else if (g_coop) {
+    emu::run_block(8, [&] { tiled_coop_kernel<MAXT == 512 ? 512 : 384, 8>(a, ts); });
   } else {
     emu::run_wave([&] { tiled_kernel<MAXT, WPE, false>(a, ts); });
   }
@@ -224,6 +229,7 @@ static int run_gen(int workload, long e0, long e1, int variant, double pd, int s
 
 int main(int argc, char **argv) {
   if (const char *h = std::getenv("FJMI_EMU_HEADLESS")) g_headless = std::atoi(h) != 0;
+  if (const char *c = std::getenv("FJMI_EMU_COOP")) g_coop = std::atoi(c) != 0;
   if (argc >= 9 && std::strcmp(argv[1], "gen") == 0)
     return run_gen(std::atoi(argv[2]), std::atol(argv[3]), std::atol(argv[4]), std::atoi(argv[5]), std::atof(argv[6]),
                    std::atoi(argv[7]), std::atof(argv[8]));

@@ -153,3 +153,13 @@ def test_general_definitions(emu_bin, workload, e0, e1, variant, p, scheme, R):
     bit-exact against the oracle's restatement of the same definition."""
     out = run(emu_bin, "gen", workload, e0, e1, variant, p, scheme, R)
     assert "done: 0 mismatches" in out and "MISMATCH" not in out
+
+
+@pytest.mark.parametrize("workload,e0,e1,variant,p,R", [
+    (3, 0, 4, 0, -1, 1.0), (3, 4, 8, 1, 1, 1.0), (2, 0, 1, 1, 0, 1.5), (1, 0, 1, 0, -1, 2.0),
+])
+def test_cooperative_kernel(emu_bin, workload, e0, e1, variant, p, R):
+    """tiled_coop_kernel (8 wavefronts per event sharing gathers of several
+    chunks, emulated with workgroup barriers): bit-exact against the oracle."""
+    out = run(emu_bin, workload, e0, e1, variant, p, R, 0, env={"FJMI_EMU_COOP": "1"})
+    assert "done: 0 mismatches" in out
