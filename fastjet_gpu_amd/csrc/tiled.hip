@@ -203,6 +203,18 @@ __device__ __forceinline__ int wave_incl_scan(int v, int) {
   return v;
 }
 
+/* Inclusive prefix maximum over the wave of values >= 0 (as wave_incl_scan,
+ * with max for +). */
+__device__ __forceinline__ int wave_incl_max(int v) {
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false));
+  return v;
+}
+
 /* The workgroup is exactly one wavefront (64 threads), and a wavefront's
  * memory operations, global and LDS, take effect in program order for all its
  * lanes, so lanes hand data to each other through memory with no wait on the
@@ -569,14 +581,33 @@ __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &sta
   const int cnt = keep ? sh.seg_count[cand] : 0;
   const int st = keep ? sh.seg_start[cand] : 0;
   const int incl = wave_incl_scan(cnt, lane);
-  if (keep) {
-    G.tiles[idx] = cand;
-    G.pref[idx + 1] = incl;
-    for (int u = 0, j = incl - cnt; u < cnt && j < W; ++u, ++j) G.gpos[j] = st + u;
+  const int tot = __builtin_amdgcn_readlane(incl, 63);
+  if (tot <= W) { /* one chunk: each tile's lane lists its members' pool positions */
+    if (keep) {
+      G.tiles[idx] = cand;
+      G.pref[idx + 1] = incl;
+      for (int u = 0, j = incl - cnt; u < cnt && j < W; ++u, ++j) G.gpos[j] = st + u;
+    }
+    if (lane == 0) G.pref[0] = 0;
+  } else { /* several chunks, large tiles: lane j finds the tile of member j with a scan,
+            * not one tile's lane listing up to 64 positions one after another */
+    G.gpos[lane] = 0;
+    if (keep) {
+      G.tiles[idx] = cand;
+      G.pref[idx + 1] = incl;
+    }
+    if (lane == 0) G.pref[0] = 0;
+    wave_sync();
+    if (keep && cnt > 0 && incl - cnt < W) G.gpos[incl - cnt] = idx; /* the first member of each tile */
+    wave_sync();
+    const int own = wave_incl_max(G.gpos[lane]); /* the last tile starting at or before member `lane` */
+    const int t = G.tiles[own];
+    const int pos = sh.seg_start[t] + (lane - G.pref[own]);
+    wave_sync();
+    G.gpos[lane] = pos;
   }
-  if (lane == 0) G.pref[0] = 0;
   wave_sync();
-  if (total) *total = __builtin_amdgcn_readlane(incl, 63);
+  if (total) *total = tot;
   return (int)__popcll(kb);
 }
 
@@ -869,12 +900,14 @@ __device__ void coop_helper(SH &sh, CoopCmd *cc, int wid, int nwaves, int lane) 
 }
 
 /* nn_scan by all waves of a cooperative kernel (called by wave 0), a chunk of
- * 64 members per wave at a time; a gather of one chunk is scanned by wave 0
- * alone (NCH chunks in flight), which costs no barrier. */
+ * 64 members per wave at a time.  A gather of at most four chunks is scanned by
+ * wave 0 alone (NCH chunks in flight): sharing it would cost the two barriers'
+ * waits for outstanding stores, more than the chunks' own time. */
 template <int NCH, class SH>
 __device__ int nn_scan_coop(SH &sh, CoopCmd *cc, int nwaves, int gsel, const Ev &ev, int ng, int o, double orap,
                             double ophi, double R2, int lane, double *best_d, double *best_w, int *best_l = nullptr) {
-  if (sh.g[gsel].pref[ng] <= W) return nn_scan<NCH>(sh, sh.g[gsel], ev, ng, o, orap, ophi, R2, lane, best_d, best_w, best_l);
+  if (sh.g[gsel].pref[ng] <= 4 * W)
+    return nn_scan<NCH>(sh, sh.g[gsel], ev, ng, o, orap, ophi, R2, lane, best_d, best_w, best_l);
   if (lane == 0) {
     cc->op = COOP_SCAN;
     cc->ng = ng;
