@@ -397,8 +397,8 @@ template <class SH>
 __device__ __forceinline__ int gathered(const SH &sh, const Gather &G, const Ev &ev, int ng, int j);
 
 /* Recomputes the (diJ, label) minimum of tiles listed in G.tiles[0..ng) from
- * all their members. */
-template <class SH>
+ * all their members; gathers of up to NK chunks are loaded once. */
+template <int NK = 2, class SH>
 __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, int lane) {
   if (lane < ng) {
     const int t = G.tiles[lane];
@@ -410,15 +410,15 @@ __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, in
   const int total = G.pref[ng];
   /* three passes over all members: the diJ minimum, then the lowest key among
    * members at that minimum, then its index (each pass must see the final result
-   * of the previous one across every chunk of 64 members).  Up to two chunks
+   * of the previous one across every chunk of 64 members).  Up to NK chunks
    * are loaded once and kept in registers; larger gathers reload per pass. */
-  if (total <= 2 * W) {
-    unsigned long long bits[2];
-    int key[2], t[2], p[2];
+  if (total <= NK * W) {
+    unsigned long long bits[NK];
+    int key[NK], t[NK], p[NK];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) p[c] = gathered(sh, G, ev, ng, c * W + lane);
+    for (int c = 0; c < NK; ++c) p[c] = gathered(sh, G, ev, ng, c * W + lane);
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int c = 0; c < NK; ++c)
       if (p[c] >= 0) {
         const int4 m = ev.rec[p[c]].meta;
         t[c] = m.z;
@@ -426,15 +426,15 @@ __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, in
         bits[c] = (unsigned long long)__double_as_longlong(ev.rec[p[c]].dij);
       }
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int c = 0; c < NK; ++c)
       if (p[c] >= 0) atomicMin(&sh.tmin[t[c]].d, bits[c]);
     wave_sync();
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int c = 0; c < NK; ++c)
       if (p[c] >= 0 && sh.tmin[t[c]].d == bits[c]) atomicMin(&sh.tmin[t[c]].l, key[c]);
     wave_sync();
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int c = 0; c < NK; ++c)
       if (p[c] >= 0 && sh.tmin[t[c]].d == bits[c] && sh.tmin[t[c]].l == key[c]) sh.tmin[t[c]].p = p[c];
     wave_sync();
     return;
@@ -497,7 +497,7 @@ __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &sta
  * merged jet's weight is larger) or left the tile.  In every other tile the
  * old minimum still holds against the unchanged members, so the listed
  * entries enter the tiles' minima by comparison. */
-template <class SH>
+template <int NK = 2, class SH>
 __device__ void refresh_changed(SH &sh, const Gather &G, Gather &GF, const Ev &ev, int ng, int phys_hi,
                                 int lane, unsigned &stampv) {
   int cand = -1;
@@ -510,7 +510,7 @@ __device__ void refresh_changed(SH &sh, const Gather &G, Gather &GF, const Ev &e
     if (full) cand = t;
   }
   const int nf = gather_tiles(sh, GF, cand, lane, stampv);
-  if (nf > 0) refresh_minima(sh, GF, ev, nf, lane);
+  if (nf > 0) refresh_minima<NK>(sh, GF, ev, nf, lane);
   /* lane i holds listed entry i; the best entry of each tile (every lane
    * compares its entry with all listed ones) then meets the tile's minimum */
   const int nc = uni(sh.nchg);
