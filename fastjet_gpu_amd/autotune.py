@@ -225,7 +225,20 @@ def device_buffers(torch, dev, p4, off):
                 nj=torch.empty(len(off) - 1, dtype=torch.int32, device=dev))
 
 
+def coop_top_row(rows):
+    """The table with its last row (every larger event) on 8 waves per event (threads 512): a plan runs
+    that row cooperatively while its bucket holds at most one event per CU (large events, few of them)
+    and one wave per event otherwise, so the whole-batch stage's timings, whose largest events are in
+    full buckets or absent, hold for it too."""
+    rows = [tuple(r) for r in rows]
+    last = rows[-1]
+    if last[1] == 2:
+        rows[-1] = (last[0], 2, 512, last[3], last[4])
+    return rows
+
+
 def write_inc(rows, note):
+    rows = coop_top_row(rows)
     path = os.path.join(PKG, "csrc", "tune_table.inc")
     lines = ["/* Multiplicity buckets: {max_n, engine, threads, slots, four-momenta",
              " * in global memory} (engine 1 = dense; engine 2 = tiled, slots = tile side",

@@ -33,3 +33,10 @@ def test_simple_tables_put_dense_rows_before_the_given_tiled_row():
     for rows in tables.values():  # every candidate table is sorted and ends with an open tiled row
         assert [r[0] for r in rows] == sorted(r[0] for r in rows)
         assert rows[-1][0] == at.LAST_ROW and rows[-1][1] == 2
+
+
+def test_written_table_runs_its_last_row_on_eight_waves():
+    rows = at.coop_top_row([(4096, 2, 64, 4, 3), (at.LAST_ROW, 2, 64, 4, 1)])
+    assert rows == [(4096, 2, 64, 4, 3), (at.LAST_ROW, 2, 512, 4, 1)]
+    # a dense last row is left alone (merge always ends with a tiled row)
+    assert at.coop_top_row([(at.LAST_ROW, 1, 64, 4, 0)]) == [(at.LAST_ROW, 1, 64, 4, 0)]
