@@ -23,7 +23,8 @@ of the next.  Beside it the line reports:
                served events/s and per-event p50/p99 latency;
   configs      the single-event configurations (N = 354; N = 808 under three
                algorithms and two radii): the GPU's plain per-event time beside
-               the CPU oracle on one thread; the high-pileup batch on the GPU;
+               the CPU oracle on one thread; the high-pileup batch on the GPU
+               (device-resident, and through the host-memory path);
   roofline     the tiled kernel against MI355X peaks: algorithmic FP64 work of
                the tiled method (pair distances counted by the oracle) per
                launch, the dense-equivalent count, and rocprofv3 counters
@@ -35,6 +36,7 @@ of the next.  Beside it the line reports:
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --summarize DIR [DIR ...]   (rocprofv3 output -> profile summary)
+       python bench.py --highpileup                (the high-pileup batch alone, one JSON line)
 """
 import argparse
 import csv
@@ -297,6 +299,35 @@ def timed_plain(eng, d, ev_off, sizes, idx, p, R, stream, engine=0):
     return time.perf_counter() - t0, lat
 
 
+def highpileup_device(eng, fj, torch, p, R, launches=3):
+    """The high-pileup batch (64 events, N ~ U[5000, 20000]) from device memory, one graph launch per
+    batch on the default table (its top row: 8 waves per event while the bucket holds at most one event per
+    CU): ms per launch (mean of `launches` after one warm-up launch), the history digest and the bucket
+    statistics."""
+    hp4, hoff = fj.gen_workload(4, 64)
+    tot, ne = len(hp4), len(hoff) - 1
+    dev = torch.device("cuda:0")
+    d = dict(p4=torch.from_numpy(hp4).to(dev), off=torch.from_numpy(hoff).to(dev),
+             h1=torch.zeros(tot, dtype=torch.int32, device=dev), h2=torch.zeros(tot, dtype=torch.int32, device=dev),
+             hd=torch.zeros(tot, dtype=torch.float64, device=dev), j=torch.zeros(4 * tot, dtype=torch.float64, device=dev),
+             nj=torch.zeros(ne, dtype=torch.int32, device=dev))
+    b = eng.batch_create(hoff, d["p4"].data_ptr(), d["off"].data_ptr(), None, None, p, R, d["h1"].data_ptr(),
+                         d["h2"].data_ptr(), d["hd"].data_ptr(), d["j"].data_ptr(), d["nj"].data_ptr())
+    try:
+        b.launch()
+        b.sync()
+        t = time.perf_counter()
+        for _ in range(launches):
+            b.launch()
+        b.sync()
+        ms = (time.perf_counter() - t) / launches * 1e3
+        stats = b.stats()
+    finally:
+        b.close()
+    digest = hashlib.sha256(d["h1"].cpu().numpy().tobytes() + d["hd"].cpu().numpy().tobytes()).hexdigest()[:16]
+    return ms, digest, stats
+
+
 def serving_leg(rates, seconds):
     exe = os.path.join(ROOT, "fastjet_gpu_amd", "bin", "fjmi_serve")
     if not os.path.exists(exe):
@@ -337,6 +368,9 @@ def main():
                     help="before the timed region, fill and copy a 256 MiB buffer: reference kernels of known HBM "
                          "traffic for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--summarize", nargs="+", metavar="DIR", help="summarise rocprofv3 output directories and exit")
+    ap.add_argument("--highpileup", action="store_true",
+                    help="time only the high-pileup batch (64 events, N ~ U[5000, 20000], R = 0.4, kt and anti-kt) "
+                         "from device memory and print one JSON line")
     ap.add_argument("--events-file", metavar="FILE",
                     help="time the events of a binary event file (fjmi_events_write) instead of the generated "
                          "workload, with the file's per-event exponent and radius where it holds them")
@@ -351,6 +385,19 @@ def main():
 
     import torch
     import fastjet_gpu_amd as fj
+
+    if a.highpileup:
+        eng = fj.Engine(0)
+        out = {"workload": "high-pileup batch: 64 events, N ~ U[5000, 20000], R = 0.4, device-resident, "
+                           "one graph launch per batch, default table"}
+        for pp, name in ((fj.KT, "kt"), (fj.ANTIKT, "antikt")):
+            ms, digest, stats = highpileup_device(eng, fj, torch, pp, 0.4)
+            out[name] = {"ms_per_batch": ms, "events_per_s": 64e3 / ms, "history_sha": digest,
+                         "buckets": [{k: st[k] for k in ("max_n", "threads", "p4_global", "n_events", "grid")}
+                                     for st in stats]}
+        eng.close()
+        print(json.dumps(out), flush=True)
+        return 0
 
     p, R = fj.ANTIKT, 0.4
     pa = Ra = None
@@ -544,10 +591,14 @@ def main():
                 cfg[f"N808_{ {1: 'kt', 0: 'cam', -1: 'antikt'}[pp]}_{RR}"]["cpu_1thread_ms"] = cpu_event_ms(ev2, pp, RR, 100)
         hp4, hoff = fj.gen_workload(4, 64)
         for pp, name in ((1, "kt"), (-1, "antikt")):
+            ms, _, hst = highpileup_device(eng, fj, torch, pp, 0.4)
             eng.cluster_batch(hp4, hoff, pp, 0.4)
             th = time.perf_counter()
             eng.cluster_batch(hp4, hoff, pp, 0.4)
-            cfg[f"highpileup64_{name}_0.4"] = {"gpu_batch_events_per_s_host_path": 64 / (time.perf_counter() - th)}
+            cfg[f"highpileup64_{name}_0.4"] = {
+                "gpu_batch_ms": ms, "gpu_batch_events_per_s": 64e3 / ms,
+                "waves_per_event": max(st["threads"] for st in hst) // 64,
+                "gpu_batch_events_per_s_host_path": 64 / (time.perf_counter() - th)}
         extras["configs"] = cfg
         print("[bench] plain, floor, host path and configs done", file=sys.stderr, flush=True)
 
