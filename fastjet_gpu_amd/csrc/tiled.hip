@@ -400,6 +400,46 @@ __device__ __forceinline__ int gathered(const SH &sh, const Gather &G, const Ev 
  * all their members; gathers of up to NK chunks are loaded once. */
 template <int NK = 2, class SH>
 __device__ void refresh_minima(SH &sh, const Gather &G, const Ev &ev, int ng, int lane) {
+  if (ng == 1 && G.pref[1] > W) { /* one large tile: its minimum by a wave reduction, each member loaded once */
+    const int total = G.pref[1];
+    double bd = __builtin_inf();
+    int bk = BIG, bp = -1;
+    for (int j0 = 0; j0 < total; j0 += NK * W) {
+      int p[NK];
+#pragma unroll
+      for (int c = 0; c < NK; ++c) p[c] = gathered(sh, G, ev, 1, j0 + c * W + lane);
+      int4 m[NK];
+      double dq[NK];
+#pragma unroll
+      for (int c = 0; c < NK; ++c)
+        if (p[c] >= 0) {
+          m[c] = ev.rec[p[c]].meta;
+          dq[c] = ev.rec[p[c]].dij;
+        }
+#pragma unroll
+      for (int c = 0; c < NK; ++c)
+        if (p[c] >= 0) {
+          const int key = tkey(m[c].y, m[c].x);
+          if (dq[c] < bd || (dq[c] == bd && key < bk)) {
+            bd = dq[c];
+            bk = key;
+            bp = p[c];
+          }
+        }
+    }
+    double md;
+    int mk;
+    const int ln = wave_argmin(bd, bk, &md, &mk);
+    const int pw = ln < 0 ? -1 : __builtin_amdgcn_readlane(bp, ln);
+    if (lane == 0) {
+      const int t = G.tiles[0];
+      sh.tmin[t].d = ln < 0 ? ~0ull : (unsigned long long)__double_as_longlong(md);
+      sh.tmin[t].l = ln < 0 ? BIG : mk;
+      sh.tmin[t].p = pw;
+    }
+    wave_sync();
+    return;
+  }
   if (lane < ng) {
     const int t = G.tiles[lane];
     sh.tmin[t].d = ~0ull;
@@ -582,15 +622,17 @@ __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &sta
   const int st = keep ? sh.seg_start[cand] : 0;
   const int incl = wave_incl_scan(cnt, lane);
   const int tot = __builtin_amdgcn_readlane(incl, 63);
-  if (tot <= W) { /* one chunk: each tile's lane lists its members' pool positions */
+  const int ngt = (int)__popcll(kb);
+  if (tot <= W && tot <= 8 * ngt) { /* one chunk of small tiles: each tile's lane lists its members' positions */
     if (keep) {
       G.tiles[idx] = cand;
       G.pref[idx + 1] = incl;
       for (int u = 0, j = incl - cnt; u < cnt && j < W; ++u, ++j) G.gpos[j] = st + u;
     }
     if (lane == 0) G.pref[0] = 0;
-  } else { /* several chunks, large tiles: lane j finds the tile of member j with a scan,
-            * not one tile's lane listing up to 64 positions one after another */
+  } else { /* large tiles (over 8 members on average, or several chunks): lane j finds the tile
+            * of member j with a scan, not one tile's lane listing up to 64 positions one
+            * after another */
     G.gpos[lane] = 0;
     if (keep) {
       G.tiles[idx] = cand;
@@ -608,7 +650,7 @@ __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &sta
   }
   wave_sync();
   if (total) *total = tot;
-  return (int)__popcll(kb);
+  return ngt;
 }
 
 /* Member j of the gathered tiles: physical index (or -1 past the end). */
