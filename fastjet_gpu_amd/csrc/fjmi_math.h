@@ -132,6 +132,24 @@ FJMI_HD fjmi_dd fjmi_dd_make(double hi, double lo) {
   return r;
 }
 
+/* a + b for double-doubles whose high parts satisfy exponent(a.hi) >=
+ * exponent(b.hi), or a.hi == 0: the high parts' sum is exact by Fast2Sum and
+ * the low parts are added in double (relative error about 2^-104 when the sum
+ * does not cancel); 8 operations against fjmi_dd_add's 20.  Every use below
+ * states why its operands are ordered. */
+FJMI_HD fjmi_dd fjmi_dd_add_ordered(fjmi_dd a, fjmi_dd b) {
+  fjmi_dd s = fjmi_fast_two_sum(a.hi, b.hi);
+  s.lo += a.lo + b.lo;
+  return fjmi_fast_two_sum(s.hi, s.lo);
+}
+
+/* a + b for a double-double and a double with exponent(a.hi) >= exponent(b) */
+FJMI_HD fjmi_dd fjmi_dd_add_d_ordered(fjmi_dd a, double b) {
+  fjmi_dd s = fjmi_fast_two_sum(a.hi, b);
+  s.lo += a.lo;
+  return fjmi_fast_two_sum(s.hi, s.lo);
+}
+
 /* ---- natural logarithm ----
  * x = 2^e m, m in (sqrt(1/2), sqrt(2)];  j = round(128 m), INV = RN(128/j);
  * r = m INV - 1 is formed exactly in double-double (two_prod, then a Sterbenz
@@ -151,7 +169,8 @@ FJMI_HD fjmi_dd fjmi_log_core_dd(double x, int e0) {
   const int j = (int)(m * 128.0 + 0.5); /* 91..181 */
   const double *tab = fjmi_log_tab[j - FJMI_LOG_J0];
   const fjmi_dd pr = fjmi_two_prod(m, tab[0]);
-  const fjmi_dd r = fjmi_two_sum(pr.hi - 1.0, pr.lo); /* pr.hi - 1 is exact */
+  /* pr.hi - 1 is exact, and a nonzero multiple of ulp(pr.hi) >= 2 |pr.lo| */
+  const fjmi_dd r = fjmi_fast_two_sum(pr.hi - 1.0, pr.lo);
   const double rh = r.hi;
   /* P(r) = 1/3 - r/4 + r^2/5 - ... + r^8/11 */
   double P = __builtin_fma(rh, -1.0 / 11.0, 1.0 / 10.0);
@@ -165,13 +184,15 @@ FJMI_HD fjmi_dd fjmi_log_core_dd(double x, int e0) {
   fjmi_dd r2 = fjmi_two_prod(rh, rh);
   r2.lo += 2.0 * rh * r.lo;
   const double tail = (r2.hi * rh) * P;
-  fjmi_dd L = fjmi_dd_add(r, fjmi_dd_make(-0.5 * r2.hi, -0.5 * r2.lo));
-  L = fjmi_dd_add_d(L, tail);
+  /* |r| >= 2^8 r^2/2 and >> |tail|; with e != 0, |e ln2| >= 0.69 > 0.35 >= |log(INV)|;
+   * and |log(INV)| >= 2^-7 > |L| (|r| <= 1/(2 j)) unless INV = 1, where it is 0 */
+  fjmi_dd L = fjmi_dd_add_ordered(r, fjmi_dd_make(-0.5 * r2.hi, -0.5 * r2.lo));
+  L = fjmi_dd_add_d_ordered(L, tail);
   const fjmi_dd acc0 = fjmi_dd_make(tab[1], tab[2]);
-  const fjmi_dd acc1 = fjmi_dd_add(fjmi_dd_make((double)e * FJMI_LN2_HI, (double)e * FJMI_LN2_LO), acc0);
+  const fjmi_dd acc1 = fjmi_dd_add_ordered(fjmi_dd_make((double)e * FJMI_LN2_HI, (double)e * FJMI_LN2_LO), acc0);
   /* selects per component: a select of whole structs may be lowered through memory */
   const fjmi_dd acc = fjmi_dd_make(e != 0 ? acc1.hi : acc0.hi, e != 0 ? acc1.lo : acc0.lo);
-  return fjmi_dd_add(acc, L);
+  return fjmi_dd_add_ordered(acc, L);
 }
 
 /* the rounded logarithm: the high part of the double-double */
@@ -232,8 +253,9 @@ FJMI_HD double fjmi_atan2_core(double y, double x) {
   const double tl = __builtin_fma(-th, den, num) / den;
   const int j = (int)(th * 64.0 + 0.5); /* 0..64 */
   const double c = (double)j * 0.015625;
-  const fjmi_dd nu = fjmi_two_sum(th - c, tl); /* th - c is exact (Sterbenz) */
-  const fjmi_dd du = fjmi_dd_add_d(fjmi_dd_mul_d(fjmi_dd_make(th, tl), c), 1.0);
+  /* th - c is exact (Sterbenz), and 0 or a multiple of ulp(th) >= 2 |tl|; 1 >= t c */
+  const fjmi_dd nu = fjmi_fast_two_sum(th - c, tl);
+  const fjmi_dd du = fjmi_dd_add_ordered(fjmi_dd_make(1.0, 0.0), fjmi_dd_mul_d(fjmi_dd_make(th, tl), c));
   const double uh = nu.hi / du.hi;
   double rr = __builtin_fma(-uh, du.hi, nu.hi);
   rr = rr + nu.lo;
@@ -246,13 +268,15 @@ FJMI_HD double fjmi_atan2_core(double y, double x) {
   Q = __builtin_fma(w, Q, -1.0 / 7.0);
   Q = __builtin_fma(w, Q, 1.0 / 5.0);
   Q = __builtin_fma(w, Q, -1.0 / 3.0);
-  const fjmi_dd at = fjmi_dd_add_d(uu, (uu.hi * w) * Q);
-  const fjmi_dd th1 = fjmi_dd_add(fjmi_dd_make(fjmi_atan_tab[j][0], fjmi_atan_tab[j][1]), at);
+  /* |u| >> |u^3 Q|; atan(j/64) >= atan(1/64) > 2^-7 >= |atan(u)| for j >= 1 (0 for j = 0);
+   * pi/2 > th2 (<= pi/4) and pi > th2 (<= pi/2) */
+  const fjmi_dd at = fjmi_dd_add_d_ordered(uu, (uu.hi * w) * Q);
+  const fjmi_dd th1 = fjmi_dd_add_ordered(fjmi_dd_make(fjmi_atan_tab[j][0], fjmi_atan_tab[j][1]), at);
   /* selects per component: a select of whole structs may be lowered through memory */
   fjmi_dd th2 = fjmi_dd_make(j ? th1.hi : at.hi, j ? th1.lo : at.lo);
-  const fjmi_dd th3 = fjmi_dd_add(fjmi_dd_make(0x1.921fb54442d18p+0, 0x1.1a62633145c07p-54), fjmi_dd_neg(th2));
+  const fjmi_dd th3 = fjmi_dd_add_ordered(fjmi_dd_make(0x1.921fb54442d18p+0, 0x1.1a62633145c07p-54), fjmi_dd_neg(th2));
   th2 = fjmi_dd_make(swap ? th3.hi : th2.hi, swap ? th3.lo : th2.lo);
-  const fjmi_dd th4 = fjmi_dd_add(fjmi_dd_make(0x1.921fb54442d18p+1, 0x1.1a62633145c07p-53), fjmi_dd_neg(th2));
+  const fjmi_dd th4 = fjmi_dd_add_ordered(fjmi_dd_make(0x1.921fb54442d18p+1, 0x1.1a62633145c07p-53), fjmi_dd_neg(th2));
   const double hi = xneg ? th4.hi : th2.hi;
   return yneg ? -hi : hi;
 }
