@@ -756,21 +756,23 @@ __device__ int nn_scan(const SH &sh, const Gather &G, const Ev &ev, int ng, int 
  * For large events a step's gathers span many chunks of 64 members, and one
  * wave issuing every chunk's instructions bounds the step.  In the cooperative
  * kernels wave 0 runs the event exactly as the one-wave kernels do, but at the
- * member pass and at each orphan's scan over a gather of several chunks it posts
- * a command in LDS, and every wave of the workgroup takes chunks wid, wid + NW,
- * ... of it.  Waves meet at two workgroup barriers per command (posted, done).
- * The results are the one-wave kernel's: per-member updates do not depend on
- * each other, the lists of changed entries and orphans are sets (their order
- * decides nothing), and the winners are lexicographic (distance, label) minima
- * over all candidates, whichever wave saw them. */
+ * member pass over a gather of several chunks it posts a command in LDS, and
+ * every wave of the workgroup takes chunks wid, wid + NW, ... of it.  Waves
+ * meet at two workgroup barriers per command (posted, done).  The results are
+ * the one-wave kernel's: per-member updates do not depend on each other, the
+ * lists of changed entries and orphans are sets (their order decides nothing),
+ * and the winner is the lexicographic (distance, label) minimum over all
+ * candidates, whichever wave saw it.  The orphans' scans stay with wave 0: a
+ * scan does less per member than the member pass, and sharing one cost more in
+ * the barriers' waits than it saved (one event of N = 19996, R = 0.4: 190 ms
+ * with wave 0 alone, 210 ms shared). */
 constexpr int kCoopMaxWaves = 16;
-enum : int { COOP_EXIT = 0, COOP_MEMBERS = 1, COOP_SCAN = 2 };
+enum : int { COOP_EXIT = 0, COOP_MEMBERS = 1 };
 struct CoopCmd {
   int op;
-  int ng, gsel;               /* the gather: tiles, and which (0: the step's, 1: the rescan's) */
-  int phys_hi, phys_lo, pt, o; /* the step's parents, the relabelled jet; the scanned orphan */
+  int ng, gsel;               /* the gather: tiles, and which (0: the step's) */
+  int phys_hi, phys_lo, pt;   /* the step's parents, the relabelled jet */
   double nrap, nphi, nw, R2;  /* the merged jet */
-  double orap, ophi;          /* the scanned orphan */
   JetRec *rec;                /* the event's views */
   unsigned short *pool;
   /* each wave's lexicographic nearest candidate */
@@ -859,53 +861,6 @@ __device__ void coop_members(SH &sh, const Gather &G, const Ev &ev, const CoopCm
   }
 }
 
-/* Wave wid's share of an orphan's scan (nn_scan over chunks wid, wid + NW, ...
- * in blocks of NCH chunks); its nearest candidate goes to slot wid. */
-template <int NCH, class SH>
-__device__ void coop_scan(const SH &sh, const Gather &G, const Ev &ev, const CoopCmd &c, int wid, int nwaves, int lane,
-                          CoopCmd *out) {
-  const int total = G.pref[c.ng];
-  double bd = c.R2, bw = 0.0;
-  int bl = BIG, bp = -1;
-  for (int j0 = wid * NCH * W; j0 < total; j0 += nwaves * NCH * W) {
-    int q[NCH];
-#pragma unroll
-    for (int k = 0; k < NCH; ++k) q[k] = j0 + k * W < total ? gathered(sh, G, ev, c.ng, j0 + k * W + lane) : -1;
-    double2 rp[NCH];
-    double wq[NCH];
-    int lab[NCH];
-#pragma unroll
-    for (int k = 0; k < NCH; ++k)
-      if (q[k] >= 0) {
-        rp[k] = ev.rec[q[k]].rp;
-        wq[k] = ev.rec[q[k]].wn.x;
-        lab[k] = ev.rec[q[k]].meta.y;
-      }
-#pragma unroll
-    for (int k = 0; k < NCH; ++k)
-      if (q[k] >= 0 && q[k] != c.o) {
-        const double d = fjmi_dr2(c.orap, c.ophi, rp[k].x, rp[k].y);
-        if (d < c.R2 && (d < bd || (d == bd && lab[k] < bl))) {
-          bd = d;
-          bl = lab[k];
-          bp = q[k];
-          bw = wq[k];
-        }
-      }
-  }
-  double md;
-  int ml;
-  const int ln = wave_argmin(bd, bl, &md, &ml);
-  const int p = ln < 0 ? -1 : __builtin_amdgcn_readlane(bp, ln);
-  const double w = ln < 0 ? 0.0 : readlane_d(bw, ln);
-  if (lane == 0) {
-    out->bd[wid] = md;
-    out->bl[wid] = ln < 0 ? BIG : ml;
-    out->bp[wid] = p;
-    out->bw[wid] = w;
-  }
-}
-
 /* The lexicographic (distance, label) best of the waves' slots (wave 0, after
  * the command's second barrier): its slot index, or -1 if no wave found one. */
 __device__ __forceinline__ int coop_best(const CoopCmd *cc, int nwaves) {
@@ -920,7 +875,7 @@ __device__ __forceinline__ int coop_best(const CoopCmd *cc, int nwaves) {
 
 /* Waves 1 .. NW-1 of a cooperative kernel: run wave 0's commands until it posts
  * COOP_EXIT. */
-template <int NCH, class SH>
+template <class SH>
 __device__ void coop_helper(SH &sh, CoopCmd *cc, int wid, int nwaves, int lane) {
   for (;;) {
     __syncthreads(); /* posted */
@@ -933,42 +888,9 @@ __device__ void coop_helper(SH &sh, CoopCmd *cc, int wid, int nwaves, int lane) 
     ev.hist = nullptr;
     ev.lab2 = nullptr;
     ev.pool_size = 0;
-    if (c.op == COOP_MEMBERS)
-      coop_members(sh, sh.g[c.gsel], ev, c, wid, nwaves, lane, cc);
-    else
-      coop_scan<1>(sh, sh.g[c.gsel], ev, c, wid, nwaves, lane, cc);
+    coop_members(sh, sh.g[c.gsel], ev, c, wid, nwaves, lane, cc);
     __syncthreads(); /* done */
   }
-}
-
-/* nn_scan by all waves of a cooperative kernel (called by wave 0), a chunk of
- * 64 members per wave at a time.  A gather of at most four chunks is scanned by
- * wave 0 alone (NCH chunks in flight): sharing it would cost the two barriers'
- * waits for outstanding stores, more than the chunks' own time. */
-template <int NCH, class SH>
-__device__ int nn_scan_coop(SH &sh, CoopCmd *cc, int nwaves, int gsel, const Ev &ev, int ng, int o, double orap,
-                            double ophi, double R2, int lane, double *best_d, double *best_w, int *best_l = nullptr) {
-  if (sh.g[gsel].pref[ng] <= 4 * W)
-    return nn_scan<NCH>(sh, sh.g[gsel], ev, ng, o, orap, ophi, R2, lane, best_d, best_w, best_l);
-  if (lane == 0) {
-    cc->op = COOP_SCAN;
-    cc->ng = ng;
-    cc->gsel = gsel;
-    cc->o = o;
-    cc->orap = orap;
-    cc->ophi = ophi;
-    cc->R2 = R2;
-    cc->rec = ev.rec;
-    cc->pool = ev.pool;
-  }
-  __syncthreads(); /* posted */
-  coop_scan<1>(sh, sh.g[gsel], ev, *cc, 0, nwaves, lane, cc);
-  __syncthreads(); /* done */
-  const int k = coop_best(cc, nwaves);
-  *best_d = k < 0 ? R2 : cc->bd[k];
-  *best_w = k < 0 ? 0.0 : cc->bw[k];
-  if (best_l) *best_l = k < 0 ? BIG : cc->bl[k];
-  return k < 0 ? -1 : cc->bp[k];
 }
 
 /* Nearest neighbour (lexicographic (distance, label), within R) of jet o at
@@ -979,20 +901,13 @@ __device__ int nn_scan_coop(SH &sh, CoopCmd *cc, int nwaves, int gsel, const Ev 
  * the rounding of the tile boundaries).  For dense events, whose 3x3
  * neighbourhoods span several chunks of 64; a sparse neighbourhood is one
  * chunk, scanned faster in one pass. */
-template <int NCH, int NWV = 1, class SH>
+template <int NCH, class SH>
 __device__ int nn_rescan(SH &sh, Gather &GO, const Geo &geo, const Ev &ev, int o, int otile, double oy, double ophi,
-                         double R2, int lane, unsigned &stampv, double *best_d, double *best_w,
-                         CoopCmd *cc = nullptr) {
-  /* NWV > 1: the scans are shared by the NWV waves of a cooperative kernel
-   * (GO is the rescan gather, sh.g[1]) */
+                         double R2, int lane, unsigned &stampv, double *best_d, double *best_w) {
   int ngo = gather_tiles(sh, GO, lane == 0 ? otile : -1, lane, stampv);
   double d1, w1;
   int l1;
-  int n1;
-  if constexpr (NWV > 1)
-    n1 = nn_scan_coop<NCH>(sh, cc, NWV, 1, ev, ngo, o, oy, ophi, R2, lane, &d1, &w1, &l1);
-  else
-    n1 = nn_scan<NCH>(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &d1, &w1, &l1);
+  const int n1 = nn_scan<NCH>(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &d1, &w1, &l1);
   int cand = -1;
   if (lane < 9 && lane != 4) {
     const int ty = geo.row(otile), tp = otile - ty * geo.nphi, dy = lane / 3 - 1, dp = lane % 3 - 1;
@@ -1013,11 +928,7 @@ __device__ int nn_rescan(SH &sh, Gather &GO, const Geo &geo, const Ev &ev, int o
   ngo = gather_tiles(sh, GO, cand, lane, stampv);
   double d2, w2;
   int l2;
-  int n2;
-  if constexpr (NWV > 1)
-    n2 = nn_scan_coop<NCH>(sh, cc, NWV, 1, ev, ngo, o, oy, ophi, R2, lane, &d2, &w2, &l2);
-  else
-    n2 = nn_scan<NCH>(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &d2, &w2, &l2);
+  const int n2 = nn_scan<NCH>(sh, GO, ev, ngo, o, oy, ophi, R2, lane, &d2, &w2, &l2);
   const bool take2 = n2 >= 0 && (n1 < 0 || d2 < d1 || (d2 == d1 && l2 < l1));
   *best_d = take2 ? d2 : d1;
   *best_w = take2 ? w2 : w1;
