@@ -66,7 +66,7 @@ TILED_VARIANTS = {0: "tiled_kernel<512, 2, false>", 1: "tiled_kernel<384, 3, fal
 PROFILED_KERNEL = TILED_VARIANTS[3]  # the default table's kernel for the 4096-event batch
 KERNEL_SOURCES = ["fastjet_gpu_amd/csrc/tiled.hip", "fastjet_gpu_amd/csrc/tiled_body.inc", "fastjet_gpu_amd/csrc/fjmi_math.h",
                   "fastjet_gpu_amd/csrc/fjmi_math_tables.h", "fastjet_gpu_amd/csrc/fjmi_internal.h",
-                  "fastjet_gpu_amd/csrc/tune_table.inc", "fastjet_gpu_amd/csrc/runtime.hip"]
+                  "fastjet_gpu_amd/csrc/tune_table.inc", "fastjet_gpu_amd/csrc/runtime.hip", "fastjet_gpu_amd/build.py"]
 
 
 def source_sha():

@@ -45,10 +45,18 @@ def run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def hip_obj(src, extra=()):
-    out = os.path.join(OBJ, os.path.basename(src) + ".o")
+def hip_obj(src, extra=(), name=None):
+    out = os.path.join(OBJ, (name or os.path.basename(src)) + ".o")
     run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", out])
     return out
+
+
+# tiled.hip is built in two parts (FJMI_TILED_PART): the one-wave kernels with
+# LLVM's iterative occupancy-first machine scheduler (+1.6 % events/s on the
+# 4096-event batch, same registers), the cooperative kernels with the default
+# one (the occupancy-first schedule made them 1.5 % slower).
+TILED_ONE_WAVE_FLAGS = ["-DFJMI_TILED_PART=1", "-mllvm", "-amdgpu-sched-strategy=iterative-maxocc"]
+TILED_COOP_FLAGS = ["-DFJMI_TILED_PART=2"]
 
 
 def build(jobs=4, cli=True, oracle=True):
@@ -56,7 +64,8 @@ def build(jobs=4, cli=True, oracle=True):
         os.makedirs(d, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         f_dense = ex.submit(hip_obj, "dense.hip")
-        f_tiled = ex.submit(hip_obj, "tiled.hip")
+        f_tiled = ex.submit(hip_obj, "tiled.hip", TILED_ONE_WAVE_FLAGS)
+        f_coop = ex.submit(hip_obj, "tiled.hip", TILED_COOP_FLAGS, "tiled_coop.hip")
         f_rt = ex.submit(hip_obj, "runtime.hip")
         f_sv = ex.submit(hip_obj, "serve.hip")
         f_pp = ex.submit(hip_obj, "postproc.hip")
@@ -64,7 +73,8 @@ def build(jobs=4, cli=True, oracle=True):
         f_gen = ex.submit(run, ["gcc", *CC_FLAGS, "-shared", "-o", os.path.join(LIB, "libfjmi_gen.so"),
                                 os.path.join(CSRC, "fjmi_gen.c"), "-lm"])
         f_or = ex.submit(run, ["make", "-s", "-C", os.path.join(ROOT, "oracle")]) if oracle else None
-        objs = [f_dense.result(), f_tiled.result(), f_rt.result(), f_sv.result(), f_pp.result(), f_ef.result()]
+        objs = [f_dense.result(), f_tiled.result(), f_coop.result(), f_rt.result(), f_sv.result(), f_pp.result(),
+                f_ef.result()]
         f_gen.result()
         if f_or:
             f_or.result()

@@ -1065,6 +1065,14 @@ __global__ __launch_bounds__(64 * NW, 2) void tiled_coop_kernel(LaunchArgs a, Ti
 #include "tiled_body.inc"
 }
 
+/* The library builds this file twice: FJMI_TILED_PART 1 instantiates the
+ * one-wave kernels (batch, serving, general definitions) and 2 the cooperative
+ * ones, so that each part gets the machine-scheduler strategy that measured
+ * fastest for it; 0, the default, instantiates both. */
+#ifndef FJMI_TILED_PART
+#define FJMI_TILED_PART 0
+#endif
+#if FJMI_TILED_PART != 2
 /* Workgroups per CU, from 160 KiB of LDS and the register budget of the
  * waves per SIMD requested: 512 tiles, 2 waves (the full register budget):
  * 8 per CU; 384 tiles (25 rows of 15 at R = 0.4, |y| < 5), 3 waves: 10 per CU
@@ -1082,11 +1090,14 @@ const void *const kTiledServeKernels[kNumTiledKernels] = {
 const void *const kTiledGenKernels[kNumTiledKernels] = {
     reinterpret_cast<const void *>(&tiled_gen_kernel<512, 2>), reinterpret_cast<const void *>(&tiled_gen_kernel<384, 3>),
     reinterpret_cast<const void *>(&tiled_gen_kernel<320, 3>), reinterpret_cast<const void *>(&tiled_gen_kernel<256, 3>)};
+const int kTiledMaxTiles[kNumTiledKernels] = {512, 384, 320, 256};
+#endif
+#if FJMI_TILED_PART != 1
 /* large events, 8 waves per event: variants 0 and 1 (the other variants'
  * rows take these too) */
 const void *const kTiledCoopKernels[kNumTiledKernels] = {
     reinterpret_cast<const void *>(&tiled_coop_kernel<512, 8>), reinterpret_cast<const void *>(&tiled_coop_kernel<384, 8>),
     reinterpret_cast<const void *>(&tiled_coop_kernel<384, 8>), reinterpret_cast<const void *>(&tiled_coop_kernel<384, 8>)};
-const int kTiledMaxTiles[kNumTiledKernels] = {512, 384, 320, 256};
+#endif
 
 }  // namespace fjmi
