@@ -540,19 +540,7 @@ __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &sta
 template <int NK = 2, class SH>
 __device__ void refresh_changed(SH &sh, const Gather &G, Gather &GF, const Ev &ev, int ng, int phys_hi,
                                 int lane, unsigned &stampv) {
-  int cand = -1;
-  if (lane < ng) {
-    const int t = G.tiles[lane];
-    const int pm = sh.tmin[t].p;
-    bool full = pm >= 0 && pm == phys_hi;
-    const int nc = sh.nchg;
-    for (int i = 0; i < nc && !full; ++i) full = sh.mt_p[i] == pm;
-    if (full) cand = t;
-  }
-  const int nf = gather_tiles(sh, GF, cand, lane, stampv);
-  if (nf > 0) refresh_minima<NK>(sh, GF, ev, nf, lane);
-  /* lane i holds listed entry i; the best entry of each tile (every lane
-   * compares its entry with all listed ones) then meets the tile's minimum */
+  /* lane i holds listed entry i */
   const int nc = uni(sh.nchg);
   const bool mine = lane < nc;
   unsigned long long d = ~0ull;
@@ -563,6 +551,21 @@ __device__ void refresh_changed(SH &sh, const Gather &G, Gather &GF, const Ev &e
     t = sh.mt_t[lane];
     pe = sh.mt_p[lane];
   }
+  int cand = -1;
+  {
+    const int tg = lane < ng ? G.tiles[lane] : 0;
+    const int pm = lane < ng ? sh.tmin[tg].p : -1;
+    bool full = pm >= 0 && pm == phys_hi;
+    for (int i = 0; i < nc; ++i) {
+      const int pi = __builtin_amdgcn_readlane(pe, i);
+      full |= pm >= 0 && pi == pm;
+    }
+    if (lane < ng && full) cand = tg;
+  }
+  const int nf = gather_tiles(sh, GF, cand, lane, stampv);
+  if (nf > 0) refresh_minima<NK>(sh, GF, ev, nf, lane);
+  /* the best listed entry of each tile (every lane compares its entry with
+   * all listed ones) then meets the tile's minimum */
   bool best = mine;
   for (int i = 0; i < nc; ++i) {
     const int ti = __builtin_amdgcn_readlane(t, i);
@@ -614,6 +617,21 @@ __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &sta
   }
   stampv += 1;
   wave_sync(); /* the callers' LDS writes (segment counts) come first */
+  const unsigned long long cb = __ballot(cand >= 0);
+  if (__popcll(cb) == 1) { /* one tile (an orphan's own tile, a refreshed tile): nothing to deduplicate */
+    const int t = __builtin_amdgcn_readlane(cand, (int)__builtin_ctzll(cb));
+    const int cnt = uni(sh.seg_count[t]), st = uni(sh.seg_start[t]);
+    if (lane == 0) {
+      sh.stamp[t] = stampv;
+      G.tiles[0] = t;
+      G.pref[0] = 0;
+      G.pref[1] = cnt;
+    }
+    G.gpos[lane] = st + lane;
+    wave_sync();
+    if (total) *total = cnt;
+    return 1;
+  }
   bool keep = false;
   if (cand >= 0) keep = atomicExch(&sh.stamp[cand], stampv) != stampv;
   const unsigned long long kb = __ballot(keep);
