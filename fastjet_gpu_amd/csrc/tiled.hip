@@ -58,7 +58,6 @@ constexpr int BIG = 0x7fffffff;
 #ifndef FJMI_TILED_STAMP_LIMIT
 #define FJMI_TILED_STAMP_LIMIT 0xFFFFFF00u
 #endif
-constexpr unsigned kStampLimit = FJMI_TILED_STAMP_LIMIT;
 
 /* A tile minimum's tie-break key: the minimum jet's label in the high half and
  * its nearest neighbour's index (0xFFFF: none) in the low half, offset so that
@@ -339,9 +338,10 @@ __device__ __forceinline__ bool jetdef_ok(double pd, int scheme) {
 }
 
 /* Segment capacity = count + (count >> FJMI_TILED_SLACK_SHIFT) +
- * FJMI_TILED_SLACK_ADD: 50 % slack plus two.  Tests build with less slack (and
- * a small pool) so that segments relocate and the pool runs out;
- * FJMI_TILED_ON_REBUILD() lets them count the rebuilds that follow. */
+ * FJMI_TILED_SLACK_ADD: 50 % slack plus two.  The CPU emulation of this source
+ * defines these (and FJMI_TILED_STAMP_LIMIT) as its own run-time settings, and
+ * runs with less slack and a small pool so that segments relocate and the pool
+ * runs out; FJMI_TILED_ON_REBUILD() lets it count the rebuilds that follow. */
 #ifndef FJMI_TILED_SLACK_SHIFT
 #define FJMI_TILED_SLACK_SHIFT 1
 #endif
@@ -610,7 +610,7 @@ __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, i
  * returns the number of tiles and, through `total`, of members. */
 template <class SH>
 __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &stampv, int *total) {
-  if (stampv >= kStampLimit) { /* the counter starts again from zero before it could wrap */
+  if (stampv >= FJMI_TILED_STAMP_LIMIT) { /* the counter starts again from zero before it could wrap */
     wave_sync();
     for (int t = lane; t < (int)(sizeof sh.stamp / sizeof sh.stamp[0]); t += W) sh.stamp[t] = 0u;
     stampv = 0;

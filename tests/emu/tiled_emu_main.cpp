@@ -8,12 +8,23 @@
 // FJMI_EMU_HEADLESS=1 launches without a queue head (the one-event form).
 // FJMI_EMU_COOP=1 runs tiled_coop_kernel (8 wavefronts per event) instead.
 // The tile pool of an event holds POOL_PER_N * N + POOL_EXTRA entries (default
-// 3 N + kTiledPoolPerEvent, as the library allocates); a small pool, with a
-// build with less segment slack (-DFJMI_TILED_SLACK_SHIFT=30 -DFJMI_TILED_SLACK_ADD=0),
-// makes segments relocate until the pool is
-// exhausted and exercises the kernel's segment rebuild (counted and printed).
+// 3 N + kTiledPoolPerEvent, as the library allocates); a small pool, with less
+// segment slack (FJMI_EMU_SLACK_SHIFT=30 FJMI_EMU_SLACK_ADD=0), makes segments
+// relocate until the pool is exhausted and exercises the kernel's segment
+// rebuild (counted and printed).  FJMI_EMU_STAMP_LIMIT=n restarts the gather
+// counter every n gathers.
 static long g_rebuilds = 0;
 #define FJMI_TILED_ON_REBUILD() (++g_rebuilds)
+/* The kernel's gather-counter limit and segment slack, set at run time here
+ * (FJMI_EMU_STAMP_LIMIT, FJMI_EMU_SLACK_SHIFT, FJMI_EMU_SLACK_ADD) so that one
+ * build serves every test; the library's defaults otherwise. */
+namespace emu_knobs {
+static unsigned stamp_limit = 0xFFFFFF00u;
+static int slack_shift = 1, slack_add = 2;
+}  // namespace emu_knobs
+#define FJMI_TILED_STAMP_LIMIT (emu_knobs::stamp_limit)
+#define FJMI_TILED_SLACK_SHIFT (emu_knobs::slack_shift)
+#define FJMI_TILED_SLACK_ADD (emu_knobs::slack_add)
 #include "tiled.hip"
 
 #include <vector>
@@ -228,6 +239,9 @@ static int run_gen(int workload, long e0, long e1, int variant, double pd, int s
 }
 
 int main(int argc, char **argv) {
+  if (const char *v = std::getenv("FJMI_EMU_STAMP_LIMIT")) emu_knobs::stamp_limit = (unsigned)std::atol(v);
+  if (const char *v = std::getenv("FJMI_EMU_SLACK_SHIFT")) emu_knobs::slack_shift = std::atoi(v);
+  if (const char *v = std::getenv("FJMI_EMU_SLACK_ADD")) emu_knobs::slack_add = std::atoi(v);
   if (const char *h = std::getenv("FJMI_EMU_HEADLESS")) g_headless = std::atoi(h) != 0;
   if (const char *c = std::getenv("FJMI_EMU_COOP")) g_coop = std::atoi(c) != 0;
   if (argc >= 9 && std::strcmp(argv[1], "gen") == 0)

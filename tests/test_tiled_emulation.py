@@ -37,19 +37,14 @@ def emu_bin(tmp_path_factory):
     return _build_emu(tmp_path_factory, "tiled_emu")
 
 
-@pytest.fixture(scope="module")
-def emu_stamp_bin(tmp_path_factory):
-    """The gather counter restarts every 40 gathers (instead of near 2^32), so
-    every event restarts it many times, also between a step's gather and its
-    orphan rescans."""
-    return _build_emu(tmp_path_factory, "tiled_emu_stamp", ("-DFJMI_TILED_STAMP_LIMIT=40u",))
+# The gather counter restarts every 40 gathers (instead of near 2^32), so every
+# event restarts it many times, also between a step's gather and its orphan rescans.
+STAMP_ENV = {"FJMI_EMU_STAMP_LIMIT": "40"}
 
 
-@pytest.fixture(scope="module")
-def emu_tight_bin(tmp_path_factory):
-    """No segment slack: insertions relocate segments and a pool without spare
-    room runs out, so the kernel's segment rebuild runs."""
-    return _build_emu(tmp_path_factory, "tiled_emu_tight", ("-DFJMI_TILED_SLACK_SHIFT=30", "-DFJMI_TILED_SLACK_ADD=0"))
+# No segment slack: insertions relocate segments and a pool without spare room
+# runs out, so the kernel's segment rebuild runs.
+TIGHT_ENV = {"FJMI_EMU_SLACK_SHIFT": "30", "FJMI_EMU_SLACK_ADD": "0"}
 
 
 def run(emu_bin, *args, env=None):
@@ -62,9 +57,11 @@ def run(emu_bin, *args, env=None):
 @pytest.mark.parametrize("workload,e0,e1,variant,p,R", [
     (1, 0, 1, 0, -1, 0.4), (1, 0, 1, 0, 0, 0.4), (1, 0, 1, 0, 1, 0.4),
     (2, 0, 1, 0, 1, 1.0), (2, 1, 2, 3, 0, 0.8), (3, 0, 8, 1, -1, 0.4), (3, 8, 16, 2, 1, 0.6),
-    # high pileup: gathers of several chunks and the incremental tile-minimum refresh; for kt
-    # (event 17, N = 9867) a jet that takes the merged jet as its neighbour can get a larger diJ
-    (4, 1, 2, 0, -1, 0.4), (4, 17, 18, 0, 1, 0.4),
+    # high pileup: gathers of several chunks and the incremental tile-minimum refresh (event 29,
+    # N = 5194: about 150 members per gather); for kt (event 17, N = 9867, gathers beyond the
+    # four chunks a recomputation keeps in registers) a jet that takes the merged jet as its
+    # neighbour can get a larger diJ
+    (4, 29, 30, 0, -1, 0.4), (4, 17, 18, 0, 1, 0.4),
 ])
 def test_generated_events(emu_bin, workload, e0, e1, variant, p, R):
     out = run(emu_bin, workload, e0, e1, variant, p, R, 0)
@@ -109,10 +106,10 @@ def test_kernel_kinematics_bit_identical(emu_bin, seed):
 
 
 @pytest.mark.parametrize("workload,e0,e1,variant,p,R", [(3, 0, 6, 0, -1, 0.4), (3, 0, 6, 1, 1, 0.4)])
-def test_pool_exhaustion_rebuilds(emu_tight_bin, workload, e0, e1, variant, p, R):
+def test_pool_exhaustion_rebuilds(emu_bin, workload, e0, e1, variant, p, R):
     """A tile pool of exactly N entries (no room for relocated segments): every
     exhaustion lays all segments out again, and the results stay bit-exact."""
-    out = run(emu_tight_bin, workload, e0, e1, variant, p, R, 0, 1 << 30, 0, 1)
+    out = run(emu_bin, workload, e0, e1, variant, p, R, 0, 1 << 30, 0, 1, env=TIGHT_ENV)
     assert "done: 0 mismatches" in out
     rebuilds = int(out.split("rebuilds:")[1].split()[0])
     assert rebuilds > 0, out[-500:]
@@ -133,11 +130,11 @@ def test_headless_single_pass(emu_bin, variant, p):
     assert "done: 0 mismatches" in out
 
 
-@pytest.mark.parametrize("workload,e0,e1,variant,p,R", [(3, 0, 4, 1, -1, 0.4), (4, 1, 2, 0, 1, 0.4)])
-def test_gather_counter_restarts(emu_stamp_bin, workload, e0, e1, variant, p, R):
+@pytest.mark.parametrize("workload,e0,e1,variant,p,R", [(3, 0, 4, 1, -1, 0.4), (4, 29, 30, 0, 1, 0.4)])
+def test_gather_counter_restarts(emu_bin, workload, e0, e1, variant, p, R):
     """Restarting the per-event gather counter (all stamps cleared) keeps every
     result bit-exact, single-chunk and several-chunk gathers alike."""
-    out = run(emu_stamp_bin, workload, e0, e1, variant, p, R, 0)
+    out = run(emu_bin, workload, e0, e1, variant, p, R, 0, env=STAMP_ENV)
     assert "done: 0 mismatches" in out
 
 
@@ -146,7 +143,7 @@ def test_gather_counter_restarts(emu_stamp_bin, workload, e0, e1, variant, p, R)
     (1, 0, 1, 1, 0.5, 1, 0.7), (2, 0, 1, 0, 2.0, 2, 1.0), (3, 0, 3, 1, -1.0, 3, 0.4),
     (3, 3, 6, 2, 0.3, 4, 0.6), (1, 0, 1, 3, -0.5, 5, 0.4), (1, 0, 1, 1, 1.0, 6, 0.4),
     (3, 6, 9, 3, -2.5, 7, 0.5), (3, 9, 12, 0, 0.0, 8, 0.8),
-    (4, 1, 2, 0, 0.5, 1, 0.4),                                      # several-chunk gathers (N = 15716)
+    (4, 29, 30, 0, 0.5, 1, 0.4),                                    # several-chunk gathers (N = 5194)
 ])
 def test_general_definitions(emu_bin, workload, e0, e1, variant, p, scheme, R):
     """tiled_gen_kernel (any real genkt exponent, FastJet's recombination schemes)
