@@ -618,6 +618,12 @@ __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &sta
   stampv += 1;
   wave_sync(); /* the callers' LDS writes (segment counts) come first */
   const unsigned long long cb = __ballot(cand >= 0);
+  if (cb == 0) { /* no tile */
+    if (lane == 0) G.pref[0] = 0;
+    wave_sync();
+    if (total) *total = 0;
+    return 0;
+  }
   if (__popcll(cb) == 1) { /* one tile (an orphan's own tile, a refreshed tile): nothing to deduplicate */
     const int t = __builtin_amdgcn_readlane(cand, (int)__builtin_ctzll(cb));
     const int cnt = uni(sh.seg_count[t]), st = uni(sh.seg_start[t]);
@@ -637,35 +643,27 @@ __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &sta
   const unsigned long long kb = __ballot(keep);
   const int idx = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(kb >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)kb, 0u));
   const int cnt = keep ? sh.seg_count[cand] : 0;
-  const int st = keep ? sh.seg_start[cand] : 0;
   const int incl = wave_incl_scan(cnt, lane);
   const int tot = __builtin_amdgcn_readlane(incl, 63);
   const int ngt = (int)__popcll(kb);
-  if (tot <= W && tot <= 8 * ngt) { /* one chunk of small tiles: each tile's lane lists its members' positions */
-    if (keep) {
-      G.tiles[idx] = cand;
-      G.pref[idx + 1] = incl;
-      for (int u = 0, j = incl - cnt; u < cnt && j < W; ++u, ++j) G.gpos[j] = st + u;
-    }
-    if (lane == 0) G.pref[0] = 0;
-  } else { /* large tiles (over 8 members on average, or several chunks): lane j finds the tile
-            * of member j with a scan, not one tile's lane listing up to 64 positions one
-            * after another */
-    G.gpos[lane] = 0;
-    if (keep) {
-      G.tiles[idx] = cand;
-      G.pref[idx + 1] = incl;
-    }
-    if (lane == 0) G.pref[0] = 0;
-    wave_sync();
-    if (keep && cnt > 0 && incl - cnt < W) G.gpos[incl - cnt] = idx; /* the first member of each tile */
-    wave_sync();
-    const int own = wave_incl_max(G.gpos[lane]); /* the last tile starting at or before member `lane` */
-    const int t = G.tiles[own];
-    const int pos = sh.seg_start[t] + (lane - G.pref[own]);
-    wave_sync();
-    G.gpos[lane] = pos;
+  /* The first 64 members' pool positions: lane j finds the tile of member j with
+   * a prefix-maximum scan over the tiles' first members, rather than each tile's
+   * lane listing its members one after another (a loop as long as the largest
+   * tile, and divergent). */
+  G.gpos[lane] = 0;
+  if (keep) {
+    G.tiles[idx] = cand;
+    G.pref[idx + 1] = incl;
   }
+  if (lane == 0) G.pref[0] = 0;
+  wave_sync();
+  if (keep && cnt > 0 && incl - cnt < W) G.gpos[incl - cnt] = idx; /* the first member of each tile */
+  wave_sync();
+  const int own = wave_incl_max(G.gpos[lane]); /* the last tile starting at or before member `lane` */
+  const int t = G.tiles[own];
+  const int pos = sh.seg_start[t] + (lane - G.pref[own]);
+  wave_sync();
+  G.gpos[lane] = pos;
   wave_sync();
   if (total) *total = tot;
   return ngt;
