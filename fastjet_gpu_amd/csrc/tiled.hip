@@ -526,7 +526,7 @@ __device__ void refresh_from_regs(SH &sh, const Gather &G, int ng, int total, in
 }
 
 template <class SH>
-__device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &stampv, int *total = nullptr);
+__device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &stampv, int *total = nullptr, int *lpos = nullptr);
 
 /* Tile minima of a several-chunk gather, incrementally.  The step changed the
  * entries of the jets listed in the member table (nchg of them: jets that took
@@ -591,7 +591,7 @@ __device__ void refresh_changed(SH &sh, const Gather &G, Gather &GF, const Ev &e
  * written into every listed tile's stamp. */
 template <class SH>
 __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, int c2, int extra, int lane,
-                            unsigned &stampv, int *total = nullptr) {
+                            unsigned &stampv, int *total = nullptr, int *lpos = nullptr) {
   int cand = -1;
   if (lane < 27) {
     const int c = lane < 9 ? c0 : (lane < 18 ? c1 : c2);
@@ -603,13 +603,14 @@ __device__ int build_gather(SH &sh, Gather &G, const Geo &geo, int c0, int c1, i
   } else if (lane == 27) {
     cand = extra;
   }
-  return gather_tiles(sh, G, cand, lane, stampv, total);
+  return gather_tiles(sh, G, cand, lane, stampv, total, lpos);
 }
 
 /* The gather of the tiles the lanes propose (cand >= 0; duplicates dropped);
- * returns the number of tiles and, through `total`, of members. */
+ * returns the number of tiles and, through `total`, of members, and through
+ * `lpos` each lane's member's pool position (members 0..63, as G.gpos). */
 template <class SH>
-__device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &stampv, int *total) {
+__device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &stampv, int *total, int *lpos) {
   if (stampv >= FJMI_TILED_STAMP_LIMIT) { /* the counter starts again from zero before it could wrap */
     wave_sync();
     for (int t = lane; t < (int)(sizeof sh.stamp / sizeof sh.stamp[0]); t += W) sh.stamp[t] = 0u;
@@ -634,6 +635,7 @@ __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &sta
       G.pref[1] = cnt;
     }
     G.gpos[lane] = st + lane;
+    if (lpos) *lpos = st + lane;
     wave_sync();
     if (total) *total = cnt;
     return 1;
@@ -664,6 +666,7 @@ __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &sta
   const int pos = sh.seg_start[t] + (lane - G.pref[own]);
   wave_sync();
   G.gpos[lane] = pos;
+  if (lpos) *lpos = pos;
   wave_sync();
   if (total) *total = tot;
   return ngt;
