@@ -645,13 +645,15 @@ __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &sta
   const unsigned long long kb = __ballot(keep);
   const int idx = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(kb >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)kb, 0u));
   const int cnt = keep ? sh.seg_count[cand] : 0;
+  const int st = keep ? sh.seg_start[cand] : 0;
   const int incl = wave_incl_scan(cnt, lane);
   const int tot = __builtin_amdgcn_readlane(incl, 63);
   const int ngt = (int)__popcll(kb);
   /* The first 64 members' pool positions: lane j finds the tile of member j with
-   * a prefix-maximum scan over the tiles' first members, rather than each tile's
-   * lane listing its members one after another (a loop as long as the largest
-   * tile, and divergent). */
+   * a prefix-maximum scan over the tiles' first members (each marked with its
+   * proposing lane, plus one), rather than each tile's lane listing its members
+   * one after another (a loop as long as the largest tile, and divergent); the
+   * proposing lane's segment start then comes by a cross-lane read. */
   G.gpos[lane] = 0;
   if (keep) {
     G.tiles[idx] = cand;
@@ -659,11 +661,10 @@ __device__ int gather_tiles(SH &sh, Gather &G, int cand, int lane, unsigned &sta
   }
   if (lane == 0) G.pref[0] = 0;
   wave_sync();
-  if (keep && cnt > 0 && incl - cnt < W) G.gpos[incl - cnt] = idx; /* the first member of each tile */
+  if (keep && cnt > 0 && incl - cnt < W) G.gpos[incl - cnt] = lane + 1; /* the first member of each tile */
   wave_sync();
-  const int own = wave_incl_max(G.gpos[lane]); /* the last tile starting at or before member `lane` */
-  const int t = G.tiles[own];
-  const int pos = sh.seg_start[t] + (lane - G.pref[own]);
+  const int src = wave_incl_max(G.gpos[lane]) - 1; /* the lane of the last tile starting at or before member `lane` */
+  const int pos = __builtin_amdgcn_ds_bpermute(4 * (src < 0 ? 0 : src), st - (incl - cnt)) + lane;
   wave_sync();
   G.gpos[lane] = pos;
   if (lpos) *lpos = pos;

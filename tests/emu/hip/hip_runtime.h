@@ -87,6 +87,12 @@ inline unsigned long long __ballot(int p) {
   for (int i = 0; i < 64; ++i) m |= (o[i] ? 1ull : 0ull) << i;
   return m;
 }
+inline int __builtin_amdgcn_ds_bpermute(int addr, int src) { /* src of lane (addr / 4) mod 64 */
+  uint64_t oa[64], os[64];
+  emu::all2((uint32_t)addr, (uint32_t)src, oa, os);
+  const int s = (int)(((uint32_t)oa[emu::lane_cur] >> 2) & 63u);
+  return (int)(uint32_t)os[s];
+}
 inline unsigned __builtin_amdgcn_mbcnt_lo(unsigned m, unsigned add) {
   const int l = emu::lane_cur;
   return add + (unsigned)__builtin_popcount(l < 32 ? (m & ((1u << l) - 1u)) : m);
