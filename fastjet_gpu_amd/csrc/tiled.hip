@@ -802,16 +802,33 @@ struct CoopCmd {
 
 /* Wave wid's share of a member pass over a gather of several chunks (the
  * one-wave pass's multi-chunk path, chunk by chunk); its nearest candidate for
- * the merged jet goes to slot wid. */
+ * the merged jet goes to slot wid.  Member j's pool position: lane g < ng holds
+ * gathered tile g's first member index and segment start; the tile of member j
+ * is the last one starting at or before j, from the tiles starting before the
+ * chunk (a ballot) and a prefix maximum over the starts inside it, marked in
+ * the wave's 64 words of `mark`; the position then comes from that tile's lane
+ * (a binary search over the tiles for every member took five dependent LDS
+ * reads). */
 template <class SH>
 __device__ void coop_members(SH &sh, const Gather &G, const Ev &ev, const CoopCmd &c, int wid, int nwaves, int lane,
-                             CoopCmd *out) {
+                             CoopCmd *out, int *mark) {
   const int total = G.pref[c.ng];
   const int phys_hi = c.phys_hi, phys_lo = c.phys_lo, pt = c.pt;
   double cbd = c.R2, cbw = 0.0;
   int cbl = BIG, cbp = -1;
+  const int gs = lane < c.ng ? G.pref[lane] : BIG;                       /* tile `lane`'s first member */
+  const int gofs = lane < c.ng ? sh.seg_start[G.tiles[lane]] - gs : 0; /* its pool position less that index */
+  int *const wmark = mark + wid * W;
   for (int j0 = wid * W; j0 < total; j0 += nwaves * W) {
-    const int q = gathered(sh, G, ev, c.ng, j0 + lane);
+    const int g0 = (int)__popcll(__ballot(gs <= j0)) - 1; /* the tile holding member j0 */
+    wmark[lane] = 0;
+    wave_sync();
+    if (gs > j0 && gs < j0 + W) atomicMax(&wmark[gs - j0], lane + 1);
+    wave_sync();
+    const int gm = wave_incl_max(wmark[lane]) - 1;
+    const int own = gm > g0 ? gm : g0; /* the tile holding member j0 + lane */
+    const int pos = __builtin_amdgcn_ds_bpermute(4 * own, gofs) + j0 + lane;
+    const int q = j0 + lane < total ? (int)ev.pool[pos] : -1;
     bool orph = false, chg = false;
     int chg_key = 0, chg_tile = -1;
     unsigned long long bits = ~0ull;
@@ -896,7 +913,7 @@ __device__ __forceinline__ int coop_best(const CoopCmd *cc, int nwaves) {
 /* Waves 1 .. NW-1 of a cooperative kernel: run wave 0's commands until it posts
  * COOP_EXIT. */
 template <class SH>
-__device__ void coop_helper(SH &sh, CoopCmd *cc, int wid, int nwaves, int lane) {
+__device__ void coop_helper(SH &sh, CoopCmd *cc, int *mark, int wid, int nwaves, int lane) {
   for (;;) {
     __syncthreads(); /* posted */
     const CoopCmd c = *cc;
@@ -908,7 +925,7 @@ __device__ void coop_helper(SH &sh, CoopCmd *cc, int wid, int nwaves, int lane) 
     ev.hist = nullptr;
     ev.lab2 = nullptr;
     ev.pool_size = 0;
-    coop_members(sh, sh.g[c.gsel], ev, c, wid, nwaves, lane, cc);
+    coop_members(sh, sh.g[c.gsel], ev, c, wid, nwaves, lane, cc, mark);
     __syncthreads(); /* done */
   }
 }
@@ -1059,6 +1076,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_kernel(LaunchArgs a, TiledScrat
   constexpr bool GEN = false;
   constexpr int NW = 1;
   CoopCmd *const coop = nullptr;
+  int *const coop_mark = nullptr;
 #include "tiled_body.inc"
 }
 
@@ -1069,6 +1087,7 @@ __global__ __launch_bounds__(64, WPE) void tiled_gen_kernel(LaunchArgs a, TiledS
   constexpr bool SERVE = false, GEN = true;
   constexpr int NW = 1;
   CoopCmd *const coop = nullptr;
+  int *const coop_mark = nullptr;
 #include "tiled_body.inc"
 }
 
@@ -1081,7 +1100,9 @@ __global__ __launch_bounds__(64 * NW, 2) void tiled_coop_kernel(LaunchArgs a, Ti
   constexpr bool SERVE = false, GEN = false;
   constexpr int WPE = 2;
   __shared__ CoopCmd coop_cmd;
+  __shared__ int coop_marks[NW * W]; /* each wave's member-position marks */
   CoopCmd *const coop = &coop_cmd;
+  int *const coop_mark = coop_marks;
 #include "tiled_body.inc"
 }
 
